@@ -1,0 +1,144 @@
+"""ctypes bindings of the C ABI declared in include/ferrousalign_amd.h.
+
+The product path has no CPU fallback: if the HIP library is missing, or no
+gfx950 device can be opened, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libferrousalign_amd.so")
+SYNTH_PATH = os.path.join(_PKG, "libfa_synth.so")
+
+FA_OK = 0
+FA_ERR_ARG = -1
+FA_ERR_H0 = -2
+FA_ERR_QLEN = -3
+FA_ERR_CODE = -4
+FA_ERR_TLEN = -5
+FA_ERR_BAND = -6
+FA_ERR_SCORING = -7
+FA_ERR_DEVICE = -8
+FA_ERR_NODEVICE = -9
+FA_ERR_OVERFLOW = -10
+
+FA_MAX_QLEN = 255
+FA_MAX_TLEN = 2048
+
+FA_SCORE_I16 = 0x0
+FA_SCORE_U8 = 0x1
+FA_SCORE_AUTO = 0x2
+FA_PACK_LENGTH = 0x00
+FA_PACK_NONE = 0x10
+FA_PACK_COST = 0x20
+FA_LAUNCH_PLAIN = 0x000
+FA_LAUNCH_PERSISTENT = 0x100
+FA_LAUNCH_GRAPH = 0x200
+
+
+class FaError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = _strerror(code) if _lib is not None else str(code)
+        super().__init__("%s: %s (code %d)" % (what, msg, code) if what else "%s (code %d)" % (msg, code))
+
+
+class BandedPairwiseSW(ctypes.Structure):
+    """fa_banded_pairwise_sw == BandedPairWiseSW (banded_swa/types.rs:106-199)."""
+    _fields_ = [
+        ("m", ctypes.c_int32),
+        ("end_bonus", ctypes.c_int32),
+        ("zdrop", ctypes.c_int32),
+        ("pen_clip5", ctypes.c_int32),
+        ("pen_clip3", ctypes.c_int32),
+        ("o_del", ctypes.c_int32),
+        ("o_ins", ctypes.c_int32),
+        ("e_del", ctypes.c_int32),
+        ("e_ins", ctypes.c_int32),
+        ("mat", ctypes.c_int8 * 25),
+    ]
+
+
+# OutScore field order (banded_swa/types.rs:77-84)
+OUT_FIELDS = ("score", "target_end_pos", "gtarget_end_pos", "query_end_pos",
+              "global_score", "max_offset")
+
+_lib = None
+_synth = None
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+
+
+def _declare(lib):
+    lib.fa_strerror.restype = ctypes.c_char_p
+    lib.fa_strerror.argtypes = [ctypes.c_int]
+    lib.fa_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_P)]
+    lib.fa_ctx_destroy.argtypes = [_P]
+    lib.fa_ctx_describe.argtypes = [_P, ctypes.c_char_p, ctypes.c_size_t]
+    seqargs = [_I64, _P, _P, _P, _P, _P, _P, _P, _P]
+    lib.fa_execute_batch_simd_scoring.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
+    lib.fa_batch_upload.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [ctypes.POINTER(_P)]
+    lib.fa_batch_set_cost.argtypes = [_P, _P, _P]
+    lib.fa_batch_set_launch.argtypes = [_P, _P, _I32, _I32, _I32]
+    lib.fa_batch_run.argtypes = [_P, _P, _U32]
+    lib.fa_batch_time.argtypes = [_P, _P, _U32, _I32, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I32)]
+    lib.fa_batch_download.argtypes = [_P, _P, _P, _P]
+    lib.fa_batch_overflow_count.argtypes = [_P, _P, ctypes.POINTER(_I64)]
+    lib.fa_batch_free.argtypes = [_P, _P]
+    soa = [_P, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]
+    lib.fa_simd_banded_swa_batch_soa.argtypes = soa
+    lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
+    for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_execute_batch_simd_scoring",
+                 "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_run",
+                 "fa_batch_time", "fa_batch_download", "fa_batch_overflow_count", "fa_batch_free",
+                 "fa_simd_banded_swa_batch_soa", "fa_simd_banded_swa_batch_int16_soa"):
+        getattr(lib, name).restype = ctypes.c_int
+
+
+def lib():
+    """The loaded HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FaError(FA_ERR_NODEVICE, "libferrousalign_amd.so is not built (run __graft_entry__.build())")
+        l = ctypes.CDLL(LIB_PATH)
+        _declare(l)
+        _lib = l
+    return _lib
+
+
+def synth():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise FaError(FA_ERR_ARG, "libfa_synth.so is not built")
+        s = ctypes.CDLL(SYNTH_PATH)
+        s.fa_synth_reference.argtypes = [ctypes.c_uint64, _I64, _P]
+        s.fa_synth_reference.restype = None
+        s.fa_synth_extension_pairs.argtypes = [ctypes.c_int, ctypes.c_uint64, _P, _I64, _I64, _P, _I64, _P,
+                                               _P, _I64, _P, _P, _P, _I32, _I32, _I32, ctypes.c_int]
+        s.fa_synth_extension_pairs.restype = ctypes.c_int
+        s.fa_synth_mate_pairs.argtypes = [ctypes.c_uint64, _P, _I64, _I64, _P, _P, _P, ctypes.c_int]
+        s.fa_synth_mate_pairs.restype = ctypes.c_int
+        _synth = s
+    return _synth
+
+
+def _strerror(code):
+    return lib().fa_strerror(code).decode()
+
+
+def check(rc, what=""):
+    if rc != FA_OK:
+        raise FaError(rc, what)
+    return rc
+
+
+def ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)

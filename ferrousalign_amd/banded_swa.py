@@ -1,0 +1,371 @@
+"""Host-side mirror of the reference's banded seed-extension interface.
+
+Names, argument meaning and result records follow the Rust crate:
+
+* ``BandedPairWiseSW``            src/core/alignment/banded_swa/types.rs:106-199
+* ``OutScore``                    src/core/alignment/banded_swa/types.rs:77-84
+* ``bwa_fill_scmat``              src/core/alignment/banded_swa/scoring.rs:2-27
+* ``ExtensionJobBatch`` / ``BatchedExtensionJob`` / ``BatchExtensionResult``
+                                  src/pipelines/linear/batch_extension/types.rs:20-197
+* ``execute_batch_simd_scoring``  src/pipelines/linear/batch_extension/dispatch.rs:31-86
+* ``simd_banded_swa_batch{16,32,64}_soa`` and ``..._int16_soa``
+                                  src/core/alignment/banded_swa/shared.rs:161-389
+
+Every entry point computes the reference's ``ksw_extend2``
+(src/core/alignment/ksw_affine_gap.rs:952-1169) on the GPU for every job.
+Unlike the reference's SIMD dispatch, every job of a batch is scored (no
+first-W truncation, dispatch.rs:126-146) and lengths are not clamped to 127
+(shared.rs:69-72).  Where the reference would panic (h0 <= 0, qlen == 0,
+base code >= 5) the call raises :class:`FaError` with the matching code.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from enum import Enum
+from typing import List, Optional
+
+import numpy as np
+
+from . import _native as N
+from ._native import FaError
+
+Left = "Left"
+Right = "Right"
+
+
+class ExtensionDirection(Enum):
+    Left = 0
+    Right = 1
+
+
+def bwa_fill_scmat(match_score, mismatch_penalty, ambig_penalty):
+    """5x5 A,C,G,T,N matrix, scoring.rs:2-27 (mat[i*5+j])."""
+    mat = np.zeros(25, np.int8)
+    for i in range(4):
+        for j in range(4):
+            mat[i * 5 + j] = match_score if i == j else -mismatch_penalty
+        mat[i * 5 + 4] = ambig_penalty
+    mat[20:25] = ambig_penalty
+    return mat
+
+
+class BandedPairWiseSW:
+    """Parameter bundle, types.rs:106-199 (``new`` at :122-148)."""
+
+    def __init__(self, o_del, e_del, o_ins, e_ins, zdrop, end_bonus, pen_clip5, pen_clip3, mat,
+                 w_match=1, w_mismatch=4):
+        self.m = 5
+        self.o_del, self.e_del, self.o_ins, self.e_ins = int(o_del), int(e_del), int(o_ins), int(e_ins)
+        self.zdrop, self.end_bonus = int(zdrop), int(end_bonus)
+        self.pen_clip5, self.pen_clip3 = int(pen_clip5), int(pen_clip3)
+        self.mat = np.asarray(mat, np.int8).reshape(25).copy()
+
+    def alphabet_size(self):
+        return self.m
+
+    def scoring_matrix(self):
+        return self.mat
+
+    def to_c(self):
+        c = N.BandedPairwiseSW()
+        c.m = self.m
+        c.end_bonus, c.zdrop = self.end_bonus, self.zdrop
+        c.pen_clip5, c.pen_clip3 = self.pen_clip5, self.pen_clip3
+        c.o_del, c.o_ins, c.e_del, c.e_ins = self.o_del, self.o_ins, self.e_del, self.e_ins
+        for k in range(25):
+            c.mat[k] = int(self.mat[k])
+        return c
+
+
+@dataclass
+class OutScore:
+    score: int
+    target_end_pos: int
+    gtarget_end_pos: int
+    query_end_pos: int
+    global_score: int
+    max_offset: int
+
+
+@dataclass
+class BatchedExtensionJob:
+    read_idx: int
+    chain_idx: int
+    seed_idx: int
+    direction: ExtensionDirection
+    query_len: int
+    ref_len: int
+    h0: int
+    band_width: int
+    query_offset: int
+    ref_offset: int
+
+
+@dataclass
+class BatchExtensionResult:
+    read_idx: int
+    chain_idx: int
+    seed_idx: int
+    direction: ExtensionDirection
+    score: int
+    query_end: int
+    ref_end: int
+    gscore: int
+    gref_end: int
+    max_off: int
+
+
+class ExtensionJobBatch:
+    """Jobs with sequences in contiguous u8-code buffers (types.rs:67-197)."""
+
+    def __init__(self):
+        self.jobs: List[BatchedExtensionJob] = []
+        self._q = bytearray()
+        self._r = bytearray()
+
+    def add_job(self, read_idx, chain_idx, seed_idx, direction, query_seq, ref_seq, h0, band_width):
+        q = bytes(bytearray(query_seq))
+        r = bytes(bytearray(ref_seq))
+        self.jobs.append(BatchedExtensionJob(read_idx, chain_idx, seed_idx, direction, len(q), len(r),
+                                             int(h0), int(band_width), len(self._q), len(self._r)))
+        self._q += q
+        self._r += r
+
+    def __len__(self):
+        return len(self.jobs)
+
+    def is_empty(self):
+        return not self.jobs
+
+    def clear(self):
+        self.jobs.clear()
+        self._q = bytearray()
+        self._r = bytearray()
+
+    def get_query_seq(self, k):
+        j = self.jobs[k]
+        return bytes(self._q[j.query_offset:j.query_offset + j.query_len])
+
+    def get_ref_seq(self, k):
+        j = self.jobs[k]
+        return bytes(self._r[j.ref_offset:j.ref_offset + j.ref_len])
+
+    def arrays(self):
+        n = len(self.jobs)
+        q = np.frombuffer(bytes(self._q) + b"\0", np.uint8)
+        r = np.frombuffer(bytes(self._r) + b"\0", np.uint8)
+        qo = np.fromiter((j.query_offset for j in self.jobs), np.int64, n)
+        ro = np.fromiter((j.ref_offset for j in self.jobs), np.int64, n)
+        ql = np.fromiter((j.query_len for j in self.jobs), np.int32, n)
+        rl = np.fromiter((j.ref_len for j in self.jobs), np.int32, n)
+        h0 = np.fromiter((j.h0 for j in self.jobs), np.int32, n)
+        w = np.fromiter((j.band_width for j in self.jobs), np.int32, n)
+        return q, qo, ql, r, ro, rl, h0, w
+
+
+class Context:
+    """One GPU context (fa_ctx): device, stream, events.  Not shared across threads."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        N.check(N.lib().fa_ctx_create(int(device), ctypes.byref(self._h)), "fa_ctx_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(256)
+        N.check(N.lib().fa_ctx_describe(self._h, buf, 256), "fa_ctx_describe")
+        return buf.value.decode()
+
+    def close(self):
+        if self._h:
+            N.lib().fa_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def extend_arrays(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, h0, w, flags=0,
+                  ctx: Optional[Context] = None, want_cells=False, per_job_rc=False):
+    """Score every pair of a batch given as flat arrays; returns (out[n,6], cells, rc)."""
+    ctx = ctx or default_context()
+    n = len(qlen)
+    qseq = np.ascontiguousarray(qseq, np.uint8)
+    tseq = np.ascontiguousarray(tseq, np.uint8)
+    qoff = np.ascontiguousarray(qoff, np.int64)
+    toff = np.ascontiguousarray(toff, np.int64)
+    qlen = np.ascontiguousarray(qlen, np.int32)
+    tlen = np.ascontiguousarray(tlen, np.int32)
+    h0 = np.ascontiguousarray(h0, np.int32)
+    w = np.ascontiguousarray(w, np.int32)
+    out = np.zeros((n, 6), np.int32)
+    cells = np.zeros(n, np.uint32) if want_cells else None
+    rc = np.zeros(n, np.int32) if per_job_rc else None
+    c = sw.to_c()
+    code = N.lib().fa_execute_batch_simd_scoring(
+        ctx.handle, ctypes.byref(c), n, N.ptr(qseq), N.ptr(qoff), N.ptr(qlen), N.ptr(tseq), N.ptr(toff),
+        N.ptr(tlen), N.ptr(h0), N.ptr(w), flags, N.ptr(out), N.ptr(cells), N.ptr(rc))
+    if per_job_rc:
+        return out, cells, rc, code
+    N.check(code, "fa_execute_batch_simd_scoring")
+    return out, cells
+
+
+def execute_batch_simd_scoring(sw_params: BandedPairWiseSW, batch: ExtensionJobBatch, engine=None,
+                               ctx: Optional[Context] = None) -> List[BatchExtensionResult]:
+    """dispatch.rs:31-86: score every job of the batch; one result per job, in job order."""
+    if batch.is_empty():
+        return []
+    q, qo, ql, r, ro, rl, h0, w = batch.arrays()
+    out, _ = extend_arrays(sw_params, q, qo, ql, r, ro, rl, h0, w, N.FA_SCORE_AUTO, ctx)
+    res = []
+    for j, o in zip(batch.jobs, out):
+        res.append(BatchExtensionResult(j.read_idx, j.chain_idx, j.seed_idx, j.direction,
+                                        score=int(o[0]), query_end=int(o[3]), ref_end=int(o[1]),
+                                        gscore=int(o[4]), gref_end=int(o[2]), max_off=int(o[5])))
+    return res
+
+
+@dataclass
+class SoAInputs:
+    """shared.rs:12-22 with int32 per-lane parameters (the i8 lanes clamp at 127)."""
+    query_soa: np.ndarray
+    target_soa: np.ndarray
+    qlen: np.ndarray
+    tlen: np.ndarray
+    w: np.ndarray
+    h0: np.ndarray
+    lanes: int
+    max_qlen: int = 0
+    max_tlen: int = 0
+
+
+def soa_transform(batch, lanes, max_len):
+    """shared.rs:87-111: lane-interleaved buffers padded with 0xFF.
+
+    batch: list of (qlen, query, tlen, target, w, h0) tuples (pad_batch order,
+    shared.rs:40-82)."""
+    q = np.full(max_len * lanes, 0xFF, np.uint8)
+    t = np.full(max_len * lanes, 0xFF, np.uint8)
+    for i, (ql, qs, tl, ts, _w, _h) in enumerate(batch[:lanes]):
+        q[np.arange(min(ql, max_len)) * lanes + i] = np.asarray(qs[:max_len], np.uint8)
+        t[np.arange(min(tl, max_len)) * lanes + i] = np.asarray(ts[:max_len], np.uint8)
+    return q, t
+
+
+def _soa_call(fn, inputs: SoAInputs, num_jobs, o_del, e_del, o_ins, e_ins, zdrop, mat, m, end_bonus, dtype,
+              ctx=None):
+    ctx = ctx or default_context()
+    out = np.zeros((max(num_jobs, 1), 6), np.int32)
+    qsoa = np.ascontiguousarray(inputs.query_soa, dtype)
+    tsoa = np.ascontiguousarray(inputs.target_soa, dtype)
+    ql = np.ascontiguousarray(inputs.qlen, np.int32)
+    tl = np.ascontiguousarray(inputs.tlen, np.int32)
+    h0 = np.ascontiguousarray(inputs.h0, np.int32)
+    w = np.ascontiguousarray(inputs.w, np.int32)
+    mat = np.ascontiguousarray(mat, np.int8)
+    N.check(fn(ctx.handle, int(inputs.lanes), N.ptr(qsoa), N.ptr(tsoa), N.ptr(ql), N.ptr(tl), N.ptr(h0), N.ptr(w),
+               int(num_jobs), o_del, e_del, o_ins, e_ins, zdrop, end_bonus, N.ptr(mat), m, N.ptr(out)),
+            fn.__name__)
+    return [OutScore(*map(int, o)) for o in out[:num_jobs]]
+
+
+def _make_soa_entry(lanes, int16):
+    def entry(inputs: SoAInputs, num_jobs, o_del, e_del, o_ins, e_ins, zdrop, mat, m, end_bonus=5, ctx=None):
+        if inputs.lanes != lanes:
+            raise FaError(N.FA_ERR_ARG, "SoA lanes %d != %d" % (inputs.lanes, lanes))
+        fn = N.lib().fa_simd_banded_swa_batch_int16_soa if int16 else N.lib().fa_simd_banded_swa_batch_soa
+        return _soa_call(fn, inputs, num_jobs, o_del, e_del, o_ins, e_ins, zdrop, mat, m, end_bonus,
+                         np.int16 if int16 else np.uint8, ctx)
+    entry.__name__ = "simd_banded_swa_batch%d%s_soa" % (lanes, "_int16" if int16 else "")
+    entry.__doc__ = ("GPU replacement of %s (shared.rs:161-389); returns one OutScore per job "
+                     "with ksw_extend2 semantics." % entry.__name__)
+    return entry
+
+
+simd_banded_swa_batch16_soa = _make_soa_entry(16, False)
+simd_banded_swa_batch32_soa = _make_soa_entry(32, False)
+simd_banded_swa_batch64_soa = _make_soa_entry(64, False)
+simd_banded_swa_batch8_int16_soa = _make_soa_entry(8, True)
+simd_banded_swa_batch16_int16_soa = _make_soa_entry(16, True)
+simd_banded_swa_batch32_int16_soa = _make_soa_entry(32, True)
+
+
+class DeviceBatch:
+    """A batch resident in HBM (fa_batch): upload once, run many times."""
+
+    def __init__(self, sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, h0, w, ctx=None):
+        self.ctx = ctx or default_context()
+        self.n = len(qlen)
+        self._keep = [np.ascontiguousarray(x, dt) for x, dt in
+                      ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32), (tseq, np.uint8),
+                       (toff, np.int64), (tlen, np.int32), (h0, np.int32), (w, np.int32))]
+        self._h = ctypes.c_void_p()
+        c = sw.to_c()
+        k = self._keep
+        N.check(N.lib().fa_batch_upload(self.ctx.handle, ctypes.byref(c), self.n, *[N.ptr(x) for x in k],
+                                        ctypes.byref(self._h)), "fa_batch_upload")
+        self._keep = None
+
+    def set_cost(self, cost):
+        cost = np.ascontiguousarray(cost, np.float32)
+        self._cost = cost
+        N.check(N.lib().fa_batch_set_cost(self.ctx.handle, self._h, N.ptr(cost)), "fa_batch_set_cost")
+
+    def set_launch(self, lanes_per_pair=8, chunk_pairs=0, waves_per_cu=0):
+        N.check(N.lib().fa_batch_set_launch(self.ctx.handle, self._h, lanes_per_pair, chunk_pairs, waves_per_cu),
+                "fa_batch_set_launch")
+
+    def run(self, flags=0):
+        N.check(N.lib().fa_batch_run(self.ctx.handle, self._h, flags), "fa_batch_run")
+
+    def time(self, flags=0, reps=10):
+        ms = ctypes.c_double()
+        dom = ctypes.c_double()
+        nl = ctypes.c_int32()
+        N.check(N.lib().fa_batch_time(self.ctx.handle, self._h, flags, reps, ctypes.byref(ms), ctypes.byref(dom),
+                                      ctypes.byref(nl)), "fa_batch_time")
+        return ms.value, dom.value, nl.value
+
+    def download(self):
+        out = np.zeros((self.n, 6), np.int32)
+        cells = np.zeros(self.n, np.uint32)
+        N.check(N.lib().fa_batch_download(self.ctx.handle, self._h, N.ptr(out), N.ptr(cells)), "fa_batch_download")
+        return out, cells
+
+    def overflow_count(self):
+        c = ctypes.c_int64()
+        N.check(N.lib().fa_batch_overflow_count(self.ctx.handle, self._h, ctypes.byref(c)), "overflow_count")
+        return c.value
+
+    def free(self):
+        if self._h:
+            N.lib().fa_batch_free(self.ctx.handle, self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

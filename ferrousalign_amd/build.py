@@ -1,0 +1,78 @@
+"""Build recipe for the native parts of ferrousalign_amd (gfx950 only).
+
+Produces, in-tree:
+  ferrousalign_amd/libferrousalign_amd.so  HIP kernels + C ABI (hipcc)
+  ferrousalign_amd/libfa_synth.so          seeded workload generator (g++)
+and, as test infrastructure, oracle/liboracle.so (make -C oracle).
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    # packed fp32 VALU forms are kept out of every code object
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+HIP_SOURCES = ["extend_kernel.hip", "fa_api.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
+                    and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build_hip_library():
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ, src + ".o")
+        objs.append(obj)
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        list(ex.map(_run, cmds))
+    lib = os.path.join(PKG, "libferrousalign_amd.so")
+    tmp = lib + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_synth_library():
+    lib = os.path.join(PKG, "libfa_synth.so")
+    tmp = lib + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-pthread", "-o", tmp,
+          os.path.join(CSRC, "synth.cpp")])
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_oracle():
+    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
+    return os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def build_all(verbose=True):
+    libs = [build_synth_library(), build_oracle(), build_hip_library()]
+    if verbose:
+        for l in libs:
+            print("built", os.path.relpath(l, ROOT))
+    return libs
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

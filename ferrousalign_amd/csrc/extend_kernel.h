@@ -1,0 +1,45 @@
+// Kernel-side argument block and launch table of the banded seed-extension
+// kernels (internal to the library; not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fa {
+
+// Device layout of a batch (all arrays in HBM):
+//   seq2  : 2-bit packed bases, 16 per 32-bit word, base p of a sequence
+//           starting at word `off` is (seq2[off + p/16] >> 2*(p%16)) & 3;
+//           every sequence starts on an even word (32-base boundary)
+//   seqn  : N mask, 1 bit per base, the mask of that sequence starts at word
+//           off/2: base p is N when (seqn[off/2 + p/32] >> p%32) & 1
+//   qo/to : per-pair word offsets of query and target in seq2
+//   qlen, tlen, h0 : per pair; w : per pair, already capped by end_bonus
+//   order : permutation of pair ids giving the packing into lane groups
+//   out   : 6 int32 per pair in OutScore field order; cells : DP cells
+struct ExtArgs {
+    const uint32_t *seq2;
+    const uint32_t *seqn;
+    const uint32_t *qo;
+    const uint32_t *to;
+    const int32_t *qlen;
+    const int32_t *tlen;
+    const int32_t *h0;
+    const int32_t *w;
+    const uint32_t *order;
+    int32_t *out;
+    uint32_t *cells;
+    uint32_t *queue;     // persistent launch: work counter (one per launch)
+    uint32_t *ovf;       // 8-bit path: per-pair overflow flags
+    uint32_t profw[5];   // per query code q: bytes mat[b*5+q]+128 for target b = 0..3
+    uint32_t nrow;       // mat[20]+128 (the N target row is uniform)
+    int32_t o_del, e_del, o_ins, e_ins, zdrop;
+    int32_t tpad;        // LDS bytes of staged target per pair (multiple of 16)
+    int32_t chunk;       // persistent launch: pairs per queue pull (multiple of 64/L)
+};
+
+// Smallest instantiated slot count S with L*S >= qlen+1, or -1.
+int extend_slot_count(int L, int qlen);
+hipError_t extend_launch_plain(int L, int S, const ExtArgs &a, uint32_t first, uint32_t count,
+                               hipStream_t st);
+
+}  // namespace fa

@@ -1,0 +1,674 @@
+// Host side of the C ABI (include/ferrousalign_amd.h): validation, 2-bit
+// packing, length/cost packing into lane groups, device upload, kernel
+// launches and result download.  Replaces the reference's CPU dispatch
+// (src/pipelines/linear/batch_extension/dispatch.rs:31-268 and
+// src/core/alignment/banded_swa/shared.rs:161-389) for one GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ferrousalign_amd.h"
+#include "extend_kernel.h"
+
+struct fa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string arch;
+    int cus = 0;
+};
+
+namespace {
+
+struct Bucket {
+    int S;
+    uint32_t first, count;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct fa_batch {
+    int64_t n = 0;
+    // host copies kept for re-packing
+    std::vector<int32_t> qlen, tlen, h0, w;
+    std::vector<float> cost;
+    // device
+    uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
+    int32_t *d_qlen = nullptr, *d_tlen = nullptr, *d_h0 = nullptr, *d_w = nullptr;
+    uint32_t *d_order = nullptr;
+    int32_t *d_out = nullptr;
+    uint32_t *d_cells = nullptr;
+    uint32_t *d_queue = nullptr;
+    fa::ExtArgs args{};
+    // launch plan for the current packing
+    uint32_t packed_flags = 0xFFFFFFFFu;
+    std::vector<Bucket> buckets;
+    int lanes = 8;
+    int chunk = 0;
+    int waves_per_cu = 0;
+    int tmax = 0;
+};
+
+namespace {
+
+int hip_fail(hipError_t e, const char *what) {
+    if (e == hipSuccess) return FA_OK;
+    std::fprintf(stderr, "ferrousalign_amd: %s failed: %s\n", what, hipGetErrorString(e));
+    return FA_ERR_DEVICE;
+}
+#define FA_HIP(call)                                   \
+    do {                                               \
+        int _rc = hip_fail((call), #call);             \
+        if (_rc != FA_OK) return _rc;                  \
+    } while (0)
+
+// Rust `f64 as i32`: truncation toward zero, saturating, NaN -> 0.
+int32_t f64_as_i32(double x) {
+    if (std::isnan(x)) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return INT32_MIN;
+    return (int32_t)x;
+}
+
+// Band width after end-bonus capping, ksw_affine_gap.rs:1001-1018.
+int32_t capped_w(const fa_banded_pairwise_sw &sw, int32_t qlen, int32_t w) {
+    int32_t max_score = 0;
+    for (int i = 0; i < 25; ++i)
+        if (sw.mat[i] > (int8_t)max_score) max_score = sw.mat[i];
+    int32_t max_ins = f64_as_i32(((double)qlen * max_score + sw.end_bonus - sw.o_ins) /
+                                     (double)sw.e_ins + 1.0);
+    max_ins = std::max(max_ins, 1);
+    w = std::min(w, max_ins);
+    int32_t max_del = f64_as_i32(((double)qlen * max_score + sw.end_bonus - sw.o_del) /
+                                     (double)sw.e_del + 1.0);
+    max_del = std::max(max_del, 1);
+    return std::min(w, max_del);
+}
+
+int check_scoring(const fa_banded_pairwise_sw *sw) {
+    if (!sw) return FA_ERR_ARG;
+    if (sw->m != 5) return FA_ERR_SCORING;
+    if (sw->o_del < 0 || sw->e_del < 0 || sw->o_ins < 0 || sw->e_ins < 0) return FA_ERR_SCORING;
+    if (sw->o_del > 4096 || sw->e_del > 4096 || sw->o_ins > 4096 || sw->e_ins > 4096)
+        return FA_ERR_SCORING;
+    for (int q = 1; q < 5; ++q)
+        if (sw->mat[20 + q] != sw->mat[20]) return FA_ERR_SCORING;
+    return FA_OK;
+}
+
+template <class F>
+void parallel_for(int64_t n, F &&f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > 16) nt = 16;
+    if (n < 65536 || nt == 1) {
+        f(0, n, 0u);
+        return;
+    }
+    std::vector<std::thread> th;
+    int64_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        int64_t lo = per * t, hi = std::min<int64_t>(lo + per, n);
+        if (lo >= hi) break;
+        th.emplace_back([=, &f] { f(lo, hi, t); });
+    }
+    for (auto &x : th) x.join();
+}
+
+inline uint32_t seq_words(int32_t len) { return (uint32_t)((len + 31) / 32) * 2u; }
+
+int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q,
+                 const uint8_t *t) {
+    if (h0 <= 0) return FA_ERR_H0;
+    if (ql <= 0 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
+    if (tl < 0 || tl > FA_MAX_TLEN) return FA_ERR_TLEN;
+    if (w < 0) return FA_ERR_BAND;
+    if (h0 > (1 << 19)) return FA_ERR_SCORING;
+    for (int32_t j = 0; j < ql; ++j)
+        if (q[j] > 4) return FA_ERR_CODE;
+    for (int32_t j = 0; j < tl; ++j)
+        if (t[j] > 4) return FA_ERR_CODE;
+    return FA_OK;
+}
+
+void pack_seq(const uint8_t *s, int32_t len, uint32_t *w2, uint32_t *wn) {
+    const uint32_t nw = seq_words(len);
+    for (uint32_t k = 0; k < nw; ++k) w2[k] = 0;
+    for (uint32_t k = 0; k < nw / 2; ++k) wn[k] = 0;
+    for (int32_t p = 0; p < len; ++p) {
+        uint32_t c = s[p];
+        if (c == 4) {
+            wn[p >> 5] |= 1u << (p & 31);
+            c = 0;
+        }
+        w2[p >> 4] |= (c & 3u) << ((p & 15) * 2);
+    }
+}
+
+template <class T>
+int dev_alloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hip_fail(hipMalloc((void **)p, count * sizeof(T)), "hipMalloc");
+}
+
+template <class T>
+int dev_upload(T **p, const T *src, size_t count, hipStream_t st) {
+    int rc = dev_alloc(p, count);
+    if (rc) return rc;
+    if (count) FA_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return FA_OK;
+}
+
+void batch_release(fa_batch *b) {
+    if (!b) return;
+    void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
+                    b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+// Build the pair order and the per-slot-count buckets for `flags`.
+int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
+    const uint32_t pack = flags & 0xF0u;
+    const int L = b->lanes;
+    if (b->packed_flags == (pack | (uint32_t)L << 16)) return FA_OK;
+    const int64_t n = b->n;
+    std::vector<uint32_t> order((size_t)n);
+    std::vector<Bucket> buckets;
+    if (pack == FA_PACK_NONE) {
+        int maxq = 1;
+        for (int64_t k = 0; k < n; ++k) {
+            order[(size_t)k] = (uint32_t)k;
+            maxq = std::max(maxq, b->qlen[(size_t)k]);
+        }
+        int S = fa::extend_slot_count(L, maxq);
+        if (S < 0) return FA_ERR_QLEN;
+        if (n) buckets.push_back({S, 0u, (uint32_t)n});
+    } else {
+        // bucket by slot count, then sort inside a bucket by (qlen, tlen) or by cost
+        std::vector<int> skey((size_t)n);
+        for (int64_t k = 0; k < n; ++k) {
+            int S = fa::extend_slot_count(L, b->qlen[(size_t)k]);
+            if (S < 0) return FA_ERR_QLEN;
+            skey[(size_t)k] = S;
+        }
+        for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
+        if (pack == FA_PACK_COST) {
+            if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
+            const float *cost = b->cost.data();
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+                if (skey[x] != skey[y]) return skey[x] < skey[y];
+                return cost[x] < cost[y];
+            });
+        } else {
+            const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+                if (skey[x] != skey[y]) return skey[x] < skey[y];
+                if (ql[x] != ql[y]) return ql[x] < ql[y];
+                return tl[x] < tl[y];
+            });
+        }
+        for (int64_t k = 0; k < n;) {
+            int S = skey[order[(size_t)k]];
+            int64_t e = k;
+            while (e < n && skey[order[(size_t)e]] == S) ++e;
+            buckets.push_back({S, (uint32_t)k, (uint32_t)(e - k)});
+            k = e;
+        }
+    }
+    FA_HIP(hipMemcpyAsync(b->d_order, order.data(), order.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    b->buckets.swap(buckets);
+    b->packed_flags = pack | (uint32_t)L << 16;
+    return FA_OK;
+}
+
+int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
+    int rc = plan_packing(ctx, b, flags);
+    if (rc) return rc;
+    int launches = 0;
+    for (const Bucket &bk : b->buckets) {
+        FA_HIP(fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count, ctx->stream));
+        ++launches;
+    }
+    if (nlaunch) *nlaunch = launches;
+    return FA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fa_strerror(int code) {
+    switch (code) {
+        case FA_OK: return "ok";
+        case FA_ERR_ARG: return "invalid argument";
+        case FA_ERR_H0: return "h0 must be positive";
+        case FA_ERR_QLEN: return "query length out of range (1..FA_MAX_QLEN)";
+        case FA_ERR_CODE: return "base code out of range (0..4)";
+        case FA_ERR_TLEN: return "target length out of range (0..FA_MAX_TLEN)";
+        case FA_ERR_BAND: return "band width must be non-negative";
+        case FA_ERR_SCORING: return "unsupported scoring parameters";
+        case FA_ERR_DEVICE: return "HIP device error";
+        case FA_ERR_NODEVICE: return "no usable gfx950 device";
+        case FA_ERR_OVERFLOW: return "8-bit score overflow";
+        default: return "unknown error";
+    }
+}
+
+int fa_ctx_create(int device, fa_ctx **out) {
+    if (!out) return FA_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FA_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return FA_ERR_NODEVICE;
+    hipDeviceProp_t prop;
+    FA_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return FA_ERR_NODEVICE;
+    FA_HIP(hipSetDevice(device));
+    fa_ctx *c = new (std::nothrow) fa_ctx();
+    if (!c) return FA_ERR_ARG;
+    c->device = device;
+    c->arch = prop.gcnArchName;
+    c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return FA_ERR_DEVICE;
+    }
+    *out = c;
+    return FA_OK;
+}
+
+int fa_ctx_destroy(fa_ctx *ctx) {
+    if (!ctx) return FA_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FA_OK;
+}
+
+int fa_ctx_describe(fa_ctx *ctx, char *buf, size_t cap) {
+    if (!ctx || !buf || cap == 0) return FA_ERR_ARG;
+    std::snprintf(buf, cap, "device=%d arch=%s cus=%d", ctx->device, ctx->arch.c_str(), ctx->cus);
+    return FA_OK;
+}
+
+int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                    const uint8_t *query_seqs, const int64_t *query_offset,
+                    const int32_t *query_len, const uint8_t *ref_seqs, const int64_t *ref_offset,
+                    const int32_t *ref_len, const int32_t *h0, const int32_t *band_width,
+                    fa_batch **out) {
+    if (!ctx || !out || njobs < 0) return FA_ERR_ARG;
+    *out = nullptr;
+    int rc = check_scoring(sw);
+    if (rc) return rc;
+    if (njobs > 0 && (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset ||
+                      !ref_len || !h0 || !band_width))
+        return FA_ERR_ARG;
+    if (njobs > (int64_t)0x7FFFFFFF) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+
+    fa_batch *b = new (std::nothrow) fa_batch();
+    if (!b) return FA_ERR_ARG;
+    const int64_t n = njobs;
+    b->n = n;
+    b->qlen.assign(query_len, query_len + n);
+    b->tlen.assign(ref_len, ref_len + n);
+    b->h0.assign(h0, h0 + n);
+    b->w.resize((size_t)n);
+
+    // validate + offsets
+    std::vector<uint32_t> qo((size_t)n), to((size_t)n);
+    uint64_t words = 0;
+    int first_err = FA_OK;
+    int tmax = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                             query_seqs + query_offset[k], ref_seqs + ref_offset[k]);
+        if (r != FA_OK) {
+            first_err = r;
+            break;
+        }
+        qo[(size_t)k] = (uint32_t)words;
+        words += seq_words(query_len[k]);
+        to[(size_t)k] = (uint32_t)words;
+        words += seq_words(ref_len[k]);
+        if (words > 0xFFFFFFF0ull) {
+            first_err = FA_ERR_ARG;
+            break;
+        }
+        b->w[(size_t)k] = capped_w(*sw, query_len[k], band_width[k]);
+        tmax = std::max(tmax, ref_len[k]);
+    }
+    if (first_err != FA_OK) {
+        delete b;
+        return first_err;
+    }
+    std::vector<uint32_t> seq2((size_t)words), seqn((size_t)(words / 2));
+    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
+        for (int64_t k = lo; k < hi; ++k) {
+            pack_seq(query_seqs + query_offset[k], query_len[k], &seq2[qo[(size_t)k]],
+                     &seqn[qo[(size_t)k] / 2]);
+            pack_seq(ref_seqs + ref_offset[k], ref_len[k], &seq2[to[(size_t)k]],
+                     &seqn[to[(size_t)k] / 2]);
+        }
+    });
+
+    hipStream_t st = ctx->stream;
+    if ((rc = dev_upload(&b->d_seq2, seq2.data(), seq2.size(), st)) ||
+        (rc = dev_upload(&b->d_seqn, seqn.data(), seqn.size(), st)) ||
+        (rc = dev_upload(&b->d_qo, qo.data(), qo.size(), st)) ||
+        (rc = dev_upload(&b->d_to, to.data(), to.size(), st)) ||
+        (rc = dev_upload(&b->d_qlen, b->qlen.data(), b->qlen.size(), st)) ||
+        (rc = dev_upload(&b->d_tlen, b->tlen.data(), b->tlen.size(), st)) ||
+        (rc = dev_upload(&b->d_h0, b->h0.data(), b->h0.size(), st)) ||
+        (rc = dev_upload(&b->d_w, b->w.data(), b->w.size(), st)) ||
+        (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
+        (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64))) {
+        batch_release(b);
+        return rc;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        batch_release(b);
+        return FA_ERR_DEVICE;
+    }
+
+    fa::ExtArgs &a = b->args;
+    a.seq2 = b->d_seq2;
+    a.seqn = b->d_seqn;
+    a.qo = b->d_qo;
+    a.to = b->d_to;
+    a.qlen = b->d_qlen;
+    a.tlen = b->d_tlen;
+    a.h0 = b->d_h0;
+    a.w = b->d_w;
+    a.order = b->d_order;
+    a.out = b->d_out;
+    a.cells = b->d_cells;
+    a.queue = b->d_queue;
+    a.ovf = nullptr;
+    for (int q = 0; q < 5; ++q) {
+        uint32_t v = 0;
+        for (int t = 0; t < 4; ++t) v |= (uint32_t)(uint8_t)(sw->mat[t * 5 + q] + 128) << (8 * t);
+        a.profw[q] = v;
+    }
+    a.nrow = (uint32_t)(uint8_t)(sw->mat[20] + 128);
+    a.o_del = sw->o_del;
+    a.e_del = sw->e_del;
+    a.o_ins = sw->o_ins;
+    a.e_ins = sw->e_ins;
+    a.zdrop = sw->zdrop;
+    b->tmax = tmax;
+    a.tpad = ((tmax + 16) / 16) * 16;
+    a.chunk = 0;
+    *out = b;
+    return FA_OK;
+}
+
+int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost) {
+    if (!ctx || !b || (!cost && b->n > 0)) return FA_ERR_ARG;
+    b->cost.assign(cost, cost + b->n);
+    b->packed_flags = 0xFFFFFFFFu;
+    return FA_OK;
+}
+
+int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_t chunk_pairs,
+                        int32_t waves_per_cu) {
+    if (!ctx || !b) return FA_ERR_ARG;
+    if (lanes_per_pair == 0) lanes_per_pair = 8;
+    if (lanes_per_pair != 4 && lanes_per_pair != 8 && lanes_per_pair != 16) return FA_ERR_ARG;
+    if (chunk_pairs < 0 || waves_per_cu < 0) return FA_ERR_ARG;
+    for (int32_t q : b->qlen)
+        if (fa::extend_slot_count(lanes_per_pair, q) < 0) return FA_ERR_QLEN;
+    if (b->lanes != lanes_per_pair) b->packed_flags = 0xFFFFFFFFu;
+    b->lanes = lanes_per_pair;
+    b->chunk = chunk_pairs;
+    b->waves_per_cu = waves_per_cu;
+    return FA_OK;
+}
+
+int fa_batch_run(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
+    if (!ctx || !b) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = run_batch(ctx, b, flags, nullptr);
+    if (rc) return rc;
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    return FA_OK;
+}
+
+int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double *ms_per_run,
+                  double *ms_dominant_launch, int32_t *launches_per_run) {
+    if (!ctx || !b || reps <= 0) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = plan_packing(ctx, b, flags);
+    if (rc) return rc;
+    FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    int nl = 0;
+    for (int r = 0; r < reps; ++r) {
+        rc = run_batch(ctx, b, flags, &nl);
+        if (rc) return rc;
+    }
+    FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    FA_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    FA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ms_per_run) *ms_per_run = ms / reps;
+    if (launches_per_run) *launches_per_run = nl;
+    if (ms_dominant_launch) {
+        // time each launch of one run separately; report the longest
+        double best = 0.0;
+        for (const Bucket &bk : b->buckets) {
+            FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+            FA_HIP(fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count,
+                                           ctx->stream));
+            FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+            FA_HIP(hipEventSynchronize(ctx->ev1));
+            float t = 0.f;
+            FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+            best = std::max(best, (double)t);
+        }
+        *ms_dominant_launch = best;
+    }
+    return FA_OK;
+}
+
+int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cells) {
+    if (!ctx || !b) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    static_assert(sizeof(fa_out_score) == 24, "OutScore layout");
+    if (out && b->n)
+        FA_HIP(hipMemcpyAsync(out, b->d_out, (size_t)b->n * sizeof(fa_out_score),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    if (cells && b->n)
+        FA_HIP(hipMemcpyAsync(cells, b->d_cells, (size_t)b->n * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    return FA_OK;
+}
+
+int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count) {
+    if (!ctx || !b || !count) return FA_ERR_ARG;
+    *count = 0;
+    return FA_OK;
+}
+
+int fa_batch_free(fa_ctx *ctx, fa_batch *b) {
+    if (!ctx || !b) return FA_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    batch_release(b);
+    return FA_OK;
+}
+
+int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                                  const uint8_t *query_seqs, const int64_t *query_offset,
+                                  const int32_t *query_len, const uint8_t *ref_seqs,
+                                  const int64_t *ref_offset, const int32_t *ref_len,
+                                  const int32_t *h0, const int32_t *band_width, uint32_t flags,
+                                  fa_out_score *out, uint32_t *cells, int32_t *per_job_rc) {
+    if (!ctx || njobs < 0 || (njobs > 0 && !out)) return FA_ERR_ARG;
+    if (njobs == 0) return check_scoring(sw);
+    int rc = check_scoring(sw);
+    if (rc) return rc;
+    if (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset || !ref_len ||
+        !h0 || !band_width)
+        return FA_ERR_ARG;
+    // per-job validation: invalid jobs are reported and left out of the launch
+    std::vector<int64_t> keep;
+    keep.reserve((size_t)njobs);
+    int first_err = FA_OK;
+    for (int64_t k = 0; k < njobs; ++k) {
+        int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                             query_seqs + query_offset[k], ref_seqs + ref_offset[k]);
+        if (per_job_rc) per_job_rc[k] = r;
+        if (r == FA_OK)
+            keep.push_back(k);
+        else if (first_err == FA_OK)
+            first_err = r;
+        if (r != FA_OK) {
+            std::memset(&out[k], 0, sizeof(fa_out_score));
+            if (cells) cells[k] = 0;
+        }
+    }
+    if (!keep.empty()) {
+        const bool all = (int64_t)keep.size() == njobs;
+        std::vector<int64_t> qo2, to2;
+        std::vector<int32_t> ql2, tl2, h02, w2;
+        const int64_t *qo = query_offset, *to = ref_offset;
+        const int32_t *ql = query_len, *tl = ref_len, *hh = h0, *ww = band_width;
+        if (!all) {
+            for (int64_t k : keep) {
+                qo2.push_back(query_offset[k]);
+                to2.push_back(ref_offset[k]);
+                ql2.push_back(query_len[k]);
+                tl2.push_back(ref_len[k]);
+                h02.push_back(h0[k]);
+                w2.push_back(band_width[k]);
+            }
+            qo = qo2.data();
+            to = to2.data();
+            ql = ql2.data();
+            tl = tl2.data();
+            hh = h02.data();
+            ww = w2.data();
+        }
+        fa_batch *b = nullptr;
+        rc = fa_batch_upload(ctx, sw, (int64_t)keep.size(), query_seqs, qo, ql, ref_seqs, to, tl,
+                             hh, ww, &b);
+        if (rc) return rc;
+        rc = fa_batch_run(ctx, b, flags);
+        std::vector<fa_out_score> o2;
+        std::vector<uint32_t> c2;
+        if (!rc) {
+            if (all) {
+                rc = fa_batch_download(ctx, b, out, cells);
+            } else {
+                o2.resize(keep.size());
+                c2.resize(keep.size());
+                rc = fa_batch_download(ctx, b, o2.data(), c2.data());
+                if (!rc)
+                    for (size_t x = 0; x < keep.size(); ++x) {
+                        out[keep[x]] = o2[x];
+                        if (cells) cells[keep[x]] = c2[x];
+                    }
+            }
+        }
+        fa_batch_free(ctx, b);
+        if (rc) return rc;
+    }
+    return first_err;
+}
+
+static int soa_common(fa_ctx *ctx, int32_t lanes, int32_t num_jobs, const int32_t *qlen,
+                      const int32_t *tlen, const int32_t *h0, const int32_t *w, int32_t o_del,
+                      int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t zdrop,
+                      int32_t end_bonus, const int8_t *mat, int32_t m, fa_out_score *out,
+                      std::vector<uint8_t> &q, std::vector<uint8_t> &t) {
+    fa_banded_pairwise_sw sw{};
+    sw.m = m;
+    sw.end_bonus = end_bonus;
+    sw.zdrop = zdrop;
+    sw.o_del = o_del;
+    sw.e_del = e_del;
+    sw.o_ins = o_ins;
+    sw.e_ins = e_ins;
+    std::memcpy(sw.mat, mat, 25);
+    std::vector<int64_t> qo(num_jobs), to(num_jobs);
+    int64_t qacc = 0, tacc = 0;
+    for (int32_t k = 0; k < num_jobs; ++k) {
+        qo[k] = qacc;
+        qacc += std::max(qlen[k], 0);
+        to[k] = tacc;
+        tacc += std::max(tlen[k], 0);
+    }
+    (void)lanes;
+    q.push_back(0);  // keep data() non-null for empty jobs; offsets never reach it
+    t.push_back(0);
+    return fa_execute_batch_simd_scoring(ctx, &sw, num_jobs, q.data(), qo.data(), qlen, t.data(),
+                                         to.data(), tlen, h0, w, FA_SCORE_I16 | FA_PACK_LENGTH,
+                                         out, nullptr, nullptr);
+}
+
+int fa_simd_banded_swa_batch_soa(fa_ctx *ctx, int32_t lanes, const uint8_t *query_soa,
+                                 const uint8_t *target_soa, const int32_t *qlen,
+                                 const int32_t *tlen, const int32_t *h0, const int32_t *w,
+                                 int32_t num_jobs, int32_t o_del, int32_t e_del, int32_t o_ins,
+                                 int32_t e_ins, int32_t zdrop, int32_t end_bonus,
+                                 const int8_t *mat, int32_t m, fa_out_score *out) {
+    if (!ctx || lanes <= 0 || num_jobs < 0 || num_jobs > lanes || !mat) return FA_ERR_ARG;
+    if (num_jobs > 0 && (!query_soa || !target_soa || !qlen || !tlen || !h0 || !w || !out))
+        return FA_ERR_ARG;
+    std::vector<uint8_t> q(1, 0), t(1, 0);
+    q.clear();
+    t.clear();
+    for (int32_t k = 0; k < num_jobs; ++k) {
+        for (int32_t p = 0; p < qlen[k]; ++p) q.push_back(query_soa[(size_t)p * lanes + k]);
+        for (int32_t p = 0; p < tlen[k]; ++p) t.push_back(target_soa[(size_t)p * lanes + k]);
+    }
+    return soa_common(ctx, lanes, num_jobs, qlen, tlen, h0, w, o_del, e_del, o_ins, e_ins, zdrop,
+                      end_bonus, mat, m, out, q, t);
+}
+
+int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t *query_soa,
+                                       const int16_t *target_soa, const int32_t *qlen,
+                                       const int32_t *tlen, const int32_t *h0, const int32_t *w,
+                                       int32_t num_jobs, int32_t o_del, int32_t e_del,
+                                       int32_t o_ins, int32_t e_ins, int32_t zdrop,
+                                       int32_t end_bonus, const int8_t *mat, int32_t m,
+                                       fa_out_score *out) {
+    if (!ctx || lanes <= 0 || num_jobs < 0 || num_jobs > lanes || !mat) return FA_ERR_ARG;
+    if (num_jobs > 0 && (!query_soa || !target_soa || !qlen || !tlen || !h0 || !w || !out))
+        return FA_ERR_ARG;
+    std::vector<uint8_t> q, t;
+    for (int32_t k = 0; k < num_jobs; ++k) {
+        for (int32_t p = 0; p < qlen[k]; ++p) {
+            int16_t v = query_soa[(size_t)p * lanes + k];
+            q.push_back(v < 0 || v > 255 ? 255 : (uint8_t)v);
+        }
+        for (int32_t p = 0; p < tlen[k]; ++p) {
+            int16_t v = target_soa[(size_t)p * lanes + k];
+            t.push_back(v < 0 || v > 255 ? 255 : (uint8_t)v);
+        }
+    }
+    return soa_common(ctx, lanes, num_jobs, qlen, tlen, h0, w, o_del, e_del, o_ins, e_ins, zdrop,
+                      end_bonus, mat, m, out, q, t);
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+/*
+ * ferrousalign_amd — C ABI of the MI355X (gfx950) banded seed-extension path.
+ *
+ * Every entry point computes, for each extension pair, the result of the
+ * reference's `ksw_extend2` (src/core/alignment/ksw_affine_gap.rs:952-1169):
+ * banded affine-gap extension from score h0 with adaptive band, z-drop and
+ * end-bonus width capping, returning score, qle, tle, gtle, gscore, max_off.
+ * Plain pointers and sizes only; no C++ or framework types cross this ABI.
+ *
+ * Threading: every call is synchronous; a context may be used by one host
+ * thread at a time (give each thread its own context, as each rayon worker
+ * holds its own workspace in the reference, workspace.rs:155-160).
+ * Errors: every call returns FA_OK (0) or a negative FA_ERR_* code and never
+ * aborts the process.
+ */
+#ifndef FERROUSALIGN_AMD_H
+#define FERROUSALIGN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------- */
+#define FA_OK 0
+#define FA_ERR_ARG (-1)      /* null pointer, negative count, bad flag */
+#define FA_ERR_H0 (-2)       /* h0 <= 0 (the reference asserts h0 > 0, ksw_affine_gap.rs:968) */
+#define FA_ERR_QLEN (-3)     /* qlen <= 0 (reference panics, :993) or qlen > FA_MAX_QLEN */
+#define FA_ERR_CODE (-4)     /* base code > 4 (reference indexes out of bounds, :985) */
+#define FA_ERR_TLEN (-5)     /* tlen < 0 or tlen > FA_MAX_TLEN */
+#define FA_ERR_BAND (-6)     /* w < 0 (reference indexes eh[] out of bounds, :1041-1107) */
+#define FA_ERR_SCORING (-7)  /* m != 5, negative penalty, N row of mat not uniform, score range */
+#define FA_ERR_DEVICE (-8)   /* HIP runtime error */
+#define FA_ERR_NODEVICE (-9) /* no usable gfx950 device */
+#define FA_ERR_OVERFLOW (-10) /* 8-bit path only: a pair exceeded 8-bit range (FA_SCORE_U8 without rerun) */
+
+#define FA_MAX_QLEN 255
+#define FA_MAX_TLEN 2048
+
+/* ---- flags -------------------------------------------------------------- */
+/* score path: the reference's i8 (kernel.rs) vs i16 (kernel_i16.rs) kernels */
+#define FA_SCORE_I16 0x0u       /* 16-bit score path */
+#define FA_SCORE_U8 0x1u        /* 8-bit score path; pairs that overflow 8 bits are rerun in 16-bit */
+#define FA_SCORE_AUTO 0x2u      /* 8-bit when every pair provably fits, else per-pair split */
+/* packing of pairs into lane groups (output order never changes) */
+#define FA_PACK_LENGTH 0x00u    /* sort by (qlen, tlen): the plain kernel */
+#define FA_PACK_NONE 0x10u      /* input order */
+#define FA_PACK_COST 0x20u      /* sort by a caller-supplied per-pair cost (fa_batch_set_cost) */
+/* launch strategy */
+#define FA_LAUNCH_PLAIN 0x000u  /* one launch per query-length bucket */
+#define FA_LAUNCH_PERSISTENT 0x100u /* persistent waves pulling chunks from a device queue */
+#define FA_LAUNCH_GRAPH 0x200u  /* replay the launch sequence as a captured hipGraph */
+
+/* ---- records ------------------------------------------------------------ */
+/* BandedPairWiseSW, src/core/alignment/banded_swa/types.rs:106-199 */
+typedef struct {
+    int32_t m;          /* alphabet size; must be 5 (A,C,G,T,N) */
+    int32_t end_bonus;
+    int32_t zdrop;
+    int32_t pen_clip5;  /* carried for the caller; not used by extension */
+    int32_t pen_clip3;
+    int32_t o_del;
+    int32_t o_ins;
+    int32_t e_del;
+    int32_t e_ins;
+    int8_t mat[25];     /* mat[target_base * 5 + query_base] */
+} fa_banded_pairwise_sw;
+
+/* OutScore, src/core/alignment/banded_swa/types.rs:77-84 (same field order).
+ * target_end_pos = tle, gtarget_end_pos = gtle, query_end_pos = qle,
+ * global_score = gscore, max_offset = max_off of KswExtendResult
+ * (ksw_affine_gap.rs:902-916). */
+typedef struct {
+    int32_t score;
+    int32_t target_end_pos;
+    int32_t gtarget_end_pos;
+    int32_t query_end_pos;
+    int32_t global_score;
+    int32_t max_offset;
+} fa_out_score;
+
+typedef struct fa_ctx fa_ctx;
+typedef struct fa_batch fa_batch;
+
+/* ---- context ------------------------------------------------------------ */
+/* Opens `device` (HIP ordinal).  Replaces the CPU-SIMD engine selection of
+ * ComputeBackend::Gpu (src/core/compute/mod.rs:67-128, :141-167). */
+int fa_ctx_create(int device, fa_ctx **out);
+int fa_ctx_destroy(fa_ctx *ctx);
+const char *fa_strerror(int code);
+/* Library and device description (null-terminated, <= cap bytes). */
+int fa_ctx_describe(fa_ctx *ctx, char *buf, size_t cap);
+
+/* ---- pipeline-level entry point ----------------------------------------- */
+/* Replaces execute_batch_simd_scoring (src/pipelines/linear/batch_extension/
+ * dispatch.rs:31-86) over an ExtensionJobBatch (batch_extension/types.rs:
+ * 20-197): job k's query is query_seqs[query_offset[k] .. + query_len[k]]
+ * and its target ref_seqs[ref_offset[k] .. + ref_len[k]], u8 codes
+ * A=0 C=1 G=2 T=3 N=4 (src/core/compute/encoding.rs:208-216).  Every job is
+ * scored (no first-W truncation, no 127 clamp).  out[k] receives job k's
+ * OutScore; cells[k] (optional) the DP cells its row loop visited.
+ * per_job_rc (optional) receives each job's validation code; invalid jobs get
+ * a zeroed record and the call returns the first error code. */
+int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw,
+                                  int64_t njobs, const uint8_t *query_seqs,
+                                  const int64_t *query_offset, const int32_t *query_len,
+                                  const uint8_t *ref_seqs, const int64_t *ref_offset,
+                                  const int32_t *ref_len, const int32_t *h0,
+                                  const int32_t *band_width, uint32_t flags,
+                                  fa_out_score *out, uint32_t *cells, int32_t *per_job_rc);
+
+/* ---- kernel-level entry points (the banded_swa batch surface) ----------- */
+/* Replace simd_banded_swa_batch{16,32,64}_soa (8-bit; isa_sse_neon.rs:10-16,
+ * isa_avx2.rs:64-70, isa_avx512_int8.rs:7-13) and
+ * simd_banded_swa_batch{8,16,32}_int16_soa (16-bit; isa_sse_neon.rs:26-32,
+ * isa_avx2.rs:72-78, isa_avx512_int16.rs:9-15), generated by
+ * banded_swa/shared.rs:161-389.  Inputs use the SoAInputs layout
+ * (shared.rs:12-35): query_soa[pos * lanes + lane], target_soa likewise,
+ * padded with 0xFF past each lane's length; per-lane lengths, h0 and w are
+ * int32 here (the reference's i8 lanes clamp at 127).  end_bonus is passed
+ * explicitly (the reference carries it only in BandedPairWiseSW).
+ * num_jobs <= lanes; out[0..num_jobs). */
+int fa_simd_banded_swa_batch_soa(fa_ctx *ctx, int32_t lanes, const uint8_t *query_soa,
+                                 const uint8_t *target_soa, const int32_t *qlen,
+                                 const int32_t *tlen, const int32_t *h0, const int32_t *w,
+                                 int32_t num_jobs, int32_t o_del, int32_t e_del, int32_t o_ins,
+                                 int32_t e_ins, int32_t zdrop, int32_t end_bonus,
+                                 const int8_t *mat, int32_t m, fa_out_score *out);
+int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t *query_soa,
+                                       const int16_t *target_soa, const int32_t *qlen,
+                                       const int32_t *tlen, const int32_t *h0,
+                                       const int32_t *w, int32_t num_jobs, int32_t o_del,
+                                       int32_t e_del, int32_t o_ins, int32_t e_ins,
+                                       int32_t zdrop, int32_t end_bonus, const int8_t *mat,
+                                       int32_t m, fa_out_score *out);
+
+/* ---- device-resident batches --------------------------------------------- */
+/* Validate, 2-bit pack and upload a batch once; run it many times with the
+ * sequences resident in HBM; fetch results.  Same job layout as
+ * fa_execute_batch_simd_scoring.  The batch keeps its results on the device
+ * until fa_batch_download. */
+int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                    const uint8_t *query_seqs, const int64_t *query_offset,
+                    const int32_t *query_len, const uint8_t *ref_seqs,
+                    const int64_t *ref_offset, const int32_t *ref_len, const int32_t *h0,
+                    const int32_t *band_width, fa_batch **out);
+/* Per-pair packing cost (e.g. predicted DP cells) used by FA_PACK_COST. */
+int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost);
+/* Launch configuration: lanes per pair (4, 8 or 16; 0 = default) and the
+ * persistent-queue chunk size in pairs (0 = default). */
+int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_t chunk_pairs,
+                        int32_t waves_per_cu);
+int fa_batch_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
+/* Runs the batch `reps` times back to back and reports the mean time of one
+ * run measured with HIP events on the batch's stream (kernel-only, ms) and the
+ * mean duration of the dominant kernel launch (ms). */
+int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double *ms_per_run,
+                  double *ms_dominant_launch, int32_t *launches_per_run);
+int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cells);
+/* Number of pairs that the 8-bit path handed to the 16-bit rerun on the last
+ * run. */
+int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count);
+int fa_batch_free(fa_ctx *ctx, fa_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FERROUSALIGN_AMD_H */

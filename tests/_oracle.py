@@ -1,0 +1,70 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) -- test infrastructure."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        l = ctypes.CDLL(LIB)
+        l.ora_ksw_extend2.argtypes = [_I32, _P, _I32, _P, _I32, _P] + [_I32] * 8 + [_P, _P]
+        l.ora_ksw_extend2.restype = ctypes.c_int
+        l.ora_extend_batch.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P] + [_I32] * 6 + [_P, _P, _P,
+                                                                                                        ctypes.c_int]
+        l.ora_extend_batch.restype = _I64
+        _lib = l
+    return _lib
+
+
+def P(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def ksw_extend2(query, target, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0, m=5):
+    """Single pair; returns (rc, (score, qle, tle, gtle, gscore, max_off), cells)."""
+    q = np.ascontiguousarray(query, np.uint8)
+    t = np.ascontiguousarray(target, np.uint8)
+    if q.size == 0:
+        q = np.zeros(1, np.uint8)[:0]
+    mat = np.ascontiguousarray(mat, np.int8)
+    out = np.zeros(6, np.int32)
+    cells = np.zeros(1, np.uint64)
+    qb = np.concatenate([q, np.zeros(1, np.uint8)])
+    tb = np.concatenate([t, np.zeros(1, np.uint8)])
+    rc = lib().ora_ksw_extend2(len(query), P(qb), len(target), P(tb), m, P(mat), o_del, e_del, o_ins, e_ins, w,
+                               end_bonus, zdrop, h0, P(out), P(cells))
+    return rc, tuple(int(x) for x in out), int(cells[0])
+
+
+def extend_batch(sw, qseq, qoff, qlen, tseq, toff, tlen, h0, w, nthreads=0):
+    """Oracle over a batch; returns (out[n,6] in KswExtendResult order, cells, rc)."""
+    n = len(qlen)
+    out = np.zeros((n, 6), np.int32)
+    cells = np.zeros(n, np.uint64)
+    rc = np.zeros(n, np.int32)
+    arrs = [np.ascontiguousarray(x, dt) for x, dt in ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (tseq, np.uint8), (toff, np.int64), (tlen, np.int32),
+                                                      (h0, np.int32), (w, np.int32))]
+    mat = np.ascontiguousarray(sw.mat, np.int8)
+    lib().ora_extend_batch(n, *[P(a) for a in arrs], 5, P(mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins,
+                           sw.end_bonus, sw.zdrop, P(out), P(cells), P(rc), nthreads)
+    return out, cells, rc
+
+
+def to_outscore_order(ora_out):
+    """KswExtendResult (score,qle,tle,gtle,gscore,max_off) -> OutScore field order
+    (score, target_end_pos=tle, gtarget_end_pos=gtle, query_end_pos=qle, global_score, max_offset)."""
+    o = np.asarray(ora_out)
+    return np.stack([o[:, 0], o[:, 2], o[:, 3], o[:, 1], o[:, 4], o[:, 5]], axis=1)

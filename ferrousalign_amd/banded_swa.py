@@ -348,6 +348,24 @@ class DeviceBatch:
                                       ctypes.byref(nl)), "fa_batch_time")
         return ms.value, dom.value, nl.value
 
+    def profile_launches(self, flags=0, cap=64):
+        nl = ctypes.c_int32()
+        slots = np.zeros(cap, np.int32)
+        first = np.zeros(cap, np.int64)
+        count = np.zeros(cap, np.int64)
+        ms = np.zeros(cap, np.float64)
+        N.check(N.lib().fa_batch_profile_launches(self.ctx.handle, self._h, flags, cap, ctypes.byref(nl),
+                                                  N.ptr(slots), N.ptr(first), N.ptr(count), N.ptr(ms)),
+                "fa_batch_profile_launches")
+        k = min(nl.value, cap)
+        return [dict(slots=int(slots[i]), first=int(first[i]), count=int(count[i]), ms=float(ms[i]))
+                for i in range(k)]
+
+    def order(self, flags=0):
+        o = np.zeros(self.n, np.uint32)
+        N.check(N.lib().fa_batch_get_order(self.ctx.handle, self._h, flags, N.ptr(o)), "fa_batch_get_order")
+        return o
+
     def download(self):
         out = np.zeros((self.n, 6), np.int32)
         cells = np.zeros(self.n, np.uint32)

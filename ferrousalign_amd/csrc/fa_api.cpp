@@ -55,7 +55,7 @@ struct fa_batch {
     // launch plan for the current packing
     uint32_t packed_flags = 0xFFFFFFFFu;
     std::vector<Bucket> buckets;
-    int lanes = 8;
+    int lanes = 16;
     int chunk = 0;
     int waves_per_cu = 0;
     int tmax = 0;
@@ -271,6 +271,23 @@ const char *fa_strerror(int code) {
     }
 }
 
+int fa_pack_2bit(const uint8_t *codes, int32_t len, uint32_t *w2, uint32_t *wn) {
+    if (len < 0 || (len > 0 && (!codes || !w2 || !wn))) return FA_ERR_ARG;
+    for (int32_t p = 0; p < len; ++p)
+        if (codes[p] > 4) return FA_ERR_CODE;
+    pack_seq(codes, len, w2, wn);
+    return FA_OK;
+}
+
+int fa_unpack_2bit(const uint32_t *w2, const uint32_t *wn, int32_t len, uint8_t *codes) {
+    if (len < 0 || (len > 0 && (!codes || !w2 || !wn))) return FA_ERR_ARG;
+    for (int32_t p = 0; p < len; ++p) {
+        uint32_t c = (w2[p >> 4] >> ((p & 15) * 2)) & 3u;
+        codes[p] = ((wn[p >> 5] >> (p & 31)) & 1u) ? 4 : (uint8_t)c;
+    }
+    return FA_OK;
+}
+
 int fa_ctx_create(int device, fa_ctx **out) {
     if (!out) return FA_ERR_ARG;
     *out = nullptr;
@@ -433,7 +450,7 @@ int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost) {
 int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_t chunk_pairs,
                         int32_t waves_per_cu) {
     if (!ctx || !b) return FA_ERR_ARG;
-    if (lanes_per_pair == 0) lanes_per_pair = 8;
+    if (lanes_per_pair == 0) lanes_per_pair = 16;
     if (lanes_per_pair != 4 && lanes_per_pair != 8 && lanes_per_pair != 16) return FA_ERR_ARG;
     if (chunk_pairs < 0 || waves_per_cu < 0) return FA_ERR_ARG;
     for (int32_t q : b->qlen)
@@ -487,6 +504,44 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
         }
         *ms_dominant_launch = best;
     }
+    return FA_OK;
+}
+
+int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t cap,
+                              int32_t *nlaunch, int32_t *slots, int64_t *first, int64_t *count,
+                              double *ms) {
+    if (!ctx || !b || !nlaunch || cap < 0) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = plan_packing(ctx, b, flags);
+    if (rc) return rc;
+    int k = 0;
+    for (const Bucket &bk : b->buckets) {
+        FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        FA_HIP(fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count, ctx->stream));
+        FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        FA_HIP(hipEventSynchronize(ctx->ev1));
+        float t = 0.f;
+        FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        if (k < cap) {
+            if (slots) slots[k] = bk.S;
+            if (first) first[k] = bk.first;
+            if (count) count[k] = bk.count;
+            if (ms) ms[k] = t;
+        }
+        ++k;
+    }
+    *nlaunch = k;
+    return FA_OK;
+}
+
+int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order) {
+    if (!ctx || !b || (!order && b->n)) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = plan_packing(ctx, b, flags);
+    if (rc) return rc;
+    if (b->n)
+        FA_HIP(hipMemcpy(order, b->d_order, (size_t)b->n * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
     return FA_OK;
 }
 

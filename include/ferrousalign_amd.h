@@ -136,6 +136,15 @@ int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t
                                        int32_t zdrop, int32_t end_bonus, const int8_t *mat,
                                        int32_t m, fa_out_score *out);
 
+/* ---- sequence layout in HBM --------------------------------------------- */
+/* 2-bit packing used on the device: base p of a sequence is
+ * (w2[p/16] >> 2*(p%16)) & 3, or N (code 4) when (wn[p/32] >> p%32) & 1.
+ * w2 must hold 2*ceil(len/32) words and wn ceil(len/32) words.  The 2-bit
+ * order inside a word differs from the reference's .pac (MSB-first, 4 bases
+ * per byte, index/bntseq.rs:563-575); N has no 2-bit code there. */
+int fa_pack_2bit(const uint8_t *codes, int32_t len, uint32_t *w2, uint32_t *wn);
+int fa_unpack_2bit(const uint32_t *w2, const uint32_t *wn, int32_t len, uint8_t *codes);
+
 /* ---- device-resident batches --------------------------------------------- */
 /* Validate, 2-bit pack and upload a batch once; run it many times with the
  * sequences resident in HBM; fetch results.  Same job layout as
@@ -158,6 +167,15 @@ int fa_batch_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
  * mean duration of the dominant kernel launch (ms). */
 int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double *ms_per_run,
                   double *ms_dominant_launch, int32_t *launches_per_run);
+/* Runs each kernel launch of one run once, timed with HIP events on the
+ * batch's stream: launch k covers packed positions [first[k], first[k] +
+ * count[k]) of the pair order with slot count slots[k].  Fills at most cap
+ * entries; *nlaunch receives the number of launches. */
+int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t cap,
+                              int32_t *nlaunch, int32_t *slots, int64_t *first, int64_t *count,
+                              double *ms);
+/* The pair order (packed position -> job index) used under `flags`. */
+int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order);
 int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cells);
 /* Number of pairs that the 8-bit path handed to the 16-bit rerun on the last
  * run. */

@@ -337,6 +337,9 @@ class DeviceBatch:
         N.check(N.lib().fa_batch_set_launch(self.ctx.handle, self._h, lanes_per_pair, chunk_pairs, waves_per_cu),
                 "fa_batch_set_launch")
 
+    def set_streams(self, streams):
+        N.check(N.lib().fa_batch_set_streams(self.ctx.handle, self._h, int(streams)), "fa_batch_set_streams")
+
     def run(self, flags=0):
         N.check(N.lib().fa_batch_run(self.ctx.handle, self._h, flags), "fa_batch_run")
 

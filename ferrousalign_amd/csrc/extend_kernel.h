@@ -41,5 +41,12 @@ struct ExtArgs {
 int extend_slot_count(int L, int qlen);
 hipError_t extend_launch_plain(int L, int S, const ExtArgs &a, uint32_t first, uint32_t count,
                                hipStream_t st);
+// Persistent launch over packed positions [first, first+count) using work
+// counter a.queue[qidx] (must be zero); grid = resident blocks per CU (capped
+// by blocks_per_cu when > 0) times `cus`.
+hipError_t extend_launch_persistent(int L, int S, const ExtArgs &a, uint32_t first,
+                                    uint32_t count, uint32_t qidx, int cus, int blocks_per_cu,
+                                    hipStream_t st);
+hipError_t queue_reset(uint32_t *q, uint32_t n, hipStream_t st);
 
 }  // namespace fa

@@ -308,6 +308,38 @@ __global__ __launch_bounds__(256) void extend_plain_kernel(ExtArgs a, uint32_t f
     extend_wave<L, S>(a, first + slot0, first + count, lds_wave);
 }
 
+// Persistent form: a grid sized to the resident capacity of the GPU; every
+// wavefront repeatedly claims `a.chunk` wave tasks (64/L consecutive pairs of
+// the packed order each) from the device work counter a.queue[qidx] and
+// exits once the counter passes the last task.  The counter only grows, so
+// every wave reaches the exit.
+template <int L, int S>
+__global__ __launch_bounds__(256) void extend_persistent_kernel(ExtArgs a, uint32_t first,
+                                                                uint32_t count, uint32_t qidx) {
+    extern __shared__ uint8_t lds[];
+    constexpr int G = 64 / L;
+    const int wave = (int)(threadIdx.x >> 6);
+    uint8_t *lds_wave = lds + wave * G * a.tpad;
+    const uint32_t ntasks = (count + G - 1) / G;
+    const uint32_t chunk = (uint32_t)a.chunk;
+    uint32_t *counter = a.queue + qidx;
+    for (;;) {
+        uint32_t t0 = 0;
+        if ((threadIdx.x & 63) == 0) t0 = atomicAdd(counter, chunk);
+        t0 = __builtin_amdgcn_readfirstlane(t0);
+        if (t0 >= ntasks) break;
+        const uint32_t t1 = t0 + chunk < ntasks ? t0 + chunk : ntasks;
+        for (uint32_t t = t0; t < t1; ++t) extend_wave<L, S>(a, first + t * G, first + count, lds_wave);
+    }
+}
+
+// Clears the work counters of a run (a kernel, so that the sequence can be
+// captured in a graph without memset nodes).
+__global__ void queue_reset_kernel(uint32_t *q, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = 0u;
+}
+
 // ---- host-side launch table ------------------------------------------------
 
 template <int L, int S>
@@ -323,13 +355,42 @@ static hipError_t launch_plain(const ExtArgs &a, uint32_t first, uint32_t count,
     return hipGetLastError();
 }
 
-#define FA_SLOT_LIST(X, L) X(L, 1) X(L, 2) X(L, 3) X(L, 4) X(L, 6) X(L, 8) X(L, 10) X(L, 12) \
-    X(L, 16) X(L, 20) X(L, 24) X(L, 28) X(L, 32)
+template <int L, int S>
+static hipError_t launch_persistent(const ExtArgs &a, uint32_t first, uint32_t count,
+                                    uint32_t qidx, int cus, int blocks_per_cu, hipStream_t st) {
+    constexpr int G = 64 / L;
+    if (count == 0) return hipSuccess;
+    const size_t lds = (size_t)4 * G * a.tpad;
+    int occ = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void *>(&extend_persistent_kernel<L, S>), 256, lds);
+    if (e != hipSuccess) return e;
+    if (occ < 1) occ = 1;
+    if (blocks_per_cu > 0 && blocks_per_cu < occ) occ = blocks_per_cu;
+    uint32_t blocks = (uint32_t)occ * (uint32_t)cus;
+    const uint32_t need = (count + 4u * G - 1) / (4u * G);
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL((extend_persistent_kernel<L, S>), dim3(blocks), dim3(256), lds, st, a,
+                       first, count, qidx);
+    return hipGetLastError();
+}
+
+#define FA_SLOT_LIST_16(X) X(16, 1) X(16, 2) X(16, 3) X(16, 4) X(16, 6) X(16, 8) X(16, 10) \
+    X(16, 12) X(16, 16)
+#define FA_SLOT_LIST_8(X) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 6) X(8, 8) X(8, 10) X(8, 12) \
+    X(8, 16) X(8, 20) X(8, 24) X(8, 28) X(8, 32)
+#define FA_SLOT_LIST_4(X) X(4, 2) X(4, 4) X(4, 6) X(4, 8) X(4, 12) X(4, 16) X(4, 20) X(4, 24) \
+    X(4, 28) X(4, 32)
 
 int extend_slot_count(int L, int qlen) {
-    static const int slots[] = {1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 28, 32};
-    for (int s : slots)
-        if (L * s >= qlen + 1) return s;
+    static const int s16[] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
+    static const int s8[] = {1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 28, 32};
+    static const int s4[] = {2, 4, 6, 8, 12, 16, 20, 24, 28, 32};
+    const int *sl = L == 16 ? s16 : (L == 8 ? s8 : s4);
+    const int n = L == 16 ? 9 : (L == 8 ? 13 : 10);
+    if (L != 16 && L != 8 && L != 4) return -1;
+    for (int k = 0; k < n; ++k)
+        if (L * sl[k] >= qlen + 1) return sl[k];
     return -1;
 }
 
@@ -337,11 +398,30 @@ hipError_t extend_launch_plain(int L, int S, const ExtArgs &a, uint32_t first, u
                                hipStream_t st) {
 #define FA_CASE(LL, SS) \
     if (L == LL && S == SS) return launch_plain<LL, SS>(a, first, count, st);
-    FA_SLOT_LIST(FA_CASE, 8)
-    FA_SLOT_LIST(FA_CASE, 16)
-    FA_SLOT_LIST(FA_CASE, 4)
+    FA_SLOT_LIST_16(FA_CASE)
+    FA_SLOT_LIST_8(FA_CASE)
+    FA_SLOT_LIST_4(FA_CASE)
 #undef FA_CASE
     return hipErrorInvalidValue;
+}
+
+hipError_t extend_launch_persistent(int L, int S, const ExtArgs &a, uint32_t first,
+                                    uint32_t count, uint32_t qidx, int cus, int blocks_per_cu,
+                                    hipStream_t st) {
+#define FA_CASE(LL, SS)                                                                   \
+    if (L == LL && S == SS)                                                               \
+        return launch_persistent<LL, SS>(a, first, count, qidx, cus, blocks_per_cu, st);
+    FA_SLOT_LIST_16(FA_CASE)
+    FA_SLOT_LIST_8(FA_CASE)
+    FA_SLOT_LIST_4(FA_CASE)
+#undef FA_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t queue_reset(uint32_t *q, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, st, q, n);
+    return hipGetLastError();
 }
 
 }  // namespace fa

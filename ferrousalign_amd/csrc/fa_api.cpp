@@ -19,7 +19,9 @@
 
 struct fa_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;           // main stream of the context
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};  // fork/join streams (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string arch;
     int cus = 0;
@@ -30,6 +32,7 @@ namespace {
 struct Bucket {
     int S;
     uint32_t first, count;
+    double work;  // estimated lane-slot work: sum over pairs of S * tlen (or of the cost)
 };
 
 struct DevBuf {
@@ -58,7 +61,11 @@ struct fa_batch {
     int lanes = 16;
     int chunk = 0;
     int waves_per_cu = 0;
+    int streams = 4;
     int tmax = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_key = 0;
+    int graph_wpc = 0;
 };
 
 namespace {
@@ -172,8 +179,14 @@ int dev_upload(T **p, const T *src, size_t count, hipStream_t st) {
     return FA_OK;
 }
 
+void drop_graph(fa_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+}
+
 void batch_release(fa_batch *b) {
     if (!b) return;
+    drop_graph(b);
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue};
     for (void *p : ptrs)
@@ -181,7 +194,10 @@ void batch_release(fa_batch *b) {
     delete b;
 }
 
-// Build the pair order and the per-slot-count buckets for `flags`.
+// Build the pair order and the per-slot-count buckets for `flags`.  Buckets
+// run widest first; inside a bucket pairs run from the most to the least
+// expensive (by (qlen, tlen), or by the caller's cost), so that lane groups
+// of one wavefront hold pairs of similar cost and the long pairs start first.
 int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t pack = flags & 0xF0u;
     const int L = b->lanes;
@@ -189,64 +205,128 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const int64_t n = b->n;
     std::vector<uint32_t> order((size_t)n);
     std::vector<Bucket> buckets;
+    for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
     if (pack == FA_PACK_NONE) {
         int maxq = 1;
-        for (int64_t k = 0; k < n; ++k) {
-            order[(size_t)k] = (uint32_t)k;
-            maxq = std::max(maxq, b->qlen[(size_t)k]);
-        }
+        for (int64_t k = 0; k < n; ++k) maxq = std::max(maxq, b->qlen[(size_t)k]);
         int S = fa::extend_slot_count(L, maxq);
         if (S < 0) return FA_ERR_QLEN;
-        if (n) buckets.push_back({S, 0u, (uint32_t)n});
+        if (n) buckets.push_back({S, 0u, (uint32_t)n, 1.0});
     } else {
-        // bucket by slot count, then sort inside a bucket by (qlen, tlen) or by cost
         std::vector<int> skey((size_t)n);
         for (int64_t k = 0; k < n; ++k) {
             int S = fa::extend_slot_count(L, b->qlen[(size_t)k]);
             if (S < 0) return FA_ERR_QLEN;
             skey[(size_t)k] = S;
         }
-        for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
         if (pack == FA_PACK_COST) {
             if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
             const float *cost = b->cost.data();
             std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-                if (skey[x] != skey[y]) return skey[x] < skey[y];
-                return cost[x] < cost[y];
+                if (skey[x] != skey[y]) return skey[x] > skey[y];
+                return cost[x] > cost[y];
             });
         } else {
             const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
             std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-                if (skey[x] != skey[y]) return skey[x] < skey[y];
-                if (ql[x] != ql[y]) return ql[x] < ql[y];
-                return tl[x] < tl[y];
+                if (skey[x] != skey[y]) return skey[x] > skey[y];
+                if (ql[x] != ql[y]) return ql[x] > ql[y];
+                return tl[x] > tl[y];
             });
         }
         for (int64_t k = 0; k < n;) {
             int S = skey[order[(size_t)k]];
             int64_t e = k;
             while (e < n && skey[order[(size_t)e]] == S) ++e;
-            buckets.push_back({S, (uint32_t)k, (uint32_t)(e - k)});
+            double work = 0.0;
+            for (int64_t x = k; x < e; ++x) {
+                const uint32_t p = order[(size_t)x];
+                work += pack == FA_PACK_COST ? (double)b->cost[p]
+                                             : (double)S * (double)std::max(b->tlen[p], 1);
+            }
+            buckets.push_back({S, (uint32_t)k, (uint32_t)(e - k), work});
             k = e;
         }
     }
+    if (buckets.size() > 64) return FA_ERR_ARG;
     FA_HIP(hipMemcpyAsync(b->d_order, order.data(), order.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
     b->buckets.swap(buckets);
     b->packed_flags = pack | (uint32_t)L << 16;
+    drop_graph(b);
+    return FA_OK;
+}
+
+hipError_t launch_bucket(fa_ctx *ctx, fa_batch *b, uint32_t flags, size_t k, hipStream_t st) {
+    const Bucket &bk = b->buckets[k];
+    if (flags & FA_LAUNCH_PERSISTENT)
+        return fa::extend_launch_persistent(b->lanes, bk.S, b->args, bk.first, bk.count,
+                                            (uint32_t)k, ctx->cus, b->waves_per_cu / 4, st);
+    return fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count, st);
+}
+
+// Enqueue one run of the batch (no synchronisation).  Buckets are spread over
+// up to 4 streams (the main stream and three side streams forked from it and
+// joined back), heaviest bucket first on the least-loaded stream, so that
+// the drain of one bucket overlaps the start of the next.  Under graph
+// capture the fork/join becomes parallel graph branches.
+int enqueue_run(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
+    if (flags & FA_LAUNCH_PERSISTENT)
+        FA_HIP(fa::queue_reset(b->d_queue, (uint32_t)b->buckets.size(), ctx->stream));
+    const int ns = std::max(1, std::min(4, b->streams));
+    const size_t nb = b->buckets.size();
+    if (ns == 1 || nb <= 1) {
+        for (size_t k = 0; k < nb; ++k) FA_HIP(launch_bucket(ctx, b, flags, k, ctx->stream));
+        return FA_OK;
+    }
+    hipStream_t st[4] = {ctx->stream, ctx->side[0], ctx->side[1], ctx->side[2]};
+    FA_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    for (int j = 1; j < ns; ++j) FA_HIP(hipStreamWaitEvent(st[j], ctx->ev_fork, 0));
+    std::vector<size_t> idx(nb);
+    for (size_t k = 0; k < nb; ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](size_t x, size_t y) { return b->buckets[x].work > b->buckets[y].work; });
+    double load[4] = {0, 0, 0, 0};
+    for (size_t k : idx) {
+        int j = 0;
+        for (int q = 1; q < ns; ++q)
+            if (load[q] < load[j]) j = q;
+        load[j] += b->buckets[k].work;
+        FA_HIP(launch_bucket(ctx, b, flags, k, st[j]));
+    }
+    for (int j = 1; j < ns; ++j) {
+        FA_HIP(hipEventRecord(ctx->ev_join[j - 1], st[j]));
+        FA_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join[j - 1], 0));
+    }
     return FA_OK;
 }
 
 int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
-    int launches = 0;
-    for (const Bucket &bk : b->buckets) {
-        FA_HIP(fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count, ctx->stream));
-        ++launches;
+    b->args.chunk = std::max(1, b->chunk / (64 / b->lanes));
+    if (nlaunch) *nlaunch = (int)b->buckets.size();
+    if (!(flags & FA_LAUNCH_GRAPH)) return enqueue_run(ctx, b, flags);
+    const uint32_t key = (flags & 0xFF0u) | (uint32_t)b->lanes << 16 | (uint32_t)b->args.chunk << 21;
+    if (!b->graph_exec || b->graph_key != key || b->graph_wpc != b->waves_per_cu * 8 + b->streams) {
+        drop_graph(b);
+        FA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_run(ctx, b, flags & ~FA_LAUNCH_GRAPH);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        FA_HIP(e);
+        e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FA_HIP(e);
+        b->graph_key = key;
+        b->graph_wpc = b->waves_per_cu * 8 + b->streams;
     }
-    if (nlaunch) *nlaunch = launches;
+    FA_HIP(hipGraphLaunch(b->graph_exec, ctx->stream));
     return FA_OK;
 }
 
@@ -303,9 +383,14 @@ int fa_ctx_create(int device, fa_ctx **out) {
     c->device = device;
     c->arch = prop.gcnArchName;
     c->cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int j = 0; ok && j < 3; ++j)
+        ok = hipStreamCreateWithFlags(&c->side[j], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_join[j], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        fa_ctx_destroy(c);
         return FA_ERR_DEVICE;
     }
     *out = c;
@@ -318,6 +403,11 @@ int fa_ctx_destroy(fa_ctx *ctx) {
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    for (int j = 0; j < 3; ++j) {
+        if (ctx->side[j]) (void)hipStreamDestroy(ctx->side[j]);
+        if (ctx->ev_join[j]) (void)hipEventDestroy(ctx->ev_join[j]);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     delete ctx;
     return FA_OK;
 }
@@ -444,6 +534,7 @@ int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost) {
     if (!ctx || !b || (!cost && b->n > 0)) return FA_ERR_ARG;
     b->cost.assign(cost, cost + b->n);
     b->packed_flags = 0xFFFFFFFFu;
+    drop_graph(b);
     return FA_OK;
 }
 
@@ -456,9 +547,17 @@ int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_
     for (int32_t q : b->qlen)
         if (fa::extend_slot_count(lanes_per_pair, q) < 0) return FA_ERR_QLEN;
     if (b->lanes != lanes_per_pair) b->packed_flags = 0xFFFFFFFFu;
+    drop_graph(b);
     b->lanes = lanes_per_pair;
     b->chunk = chunk_pairs;
     b->waves_per_cu = waves_per_cu;
+    return FA_OK;
+}
+
+int fa_batch_set_streams(fa_ctx *ctx, fa_batch *b, int32_t streams) {
+    if (!ctx || !b || streams < 1 || streams > 4) return FA_ERR_ARG;
+    b->streams = streams;
+    drop_graph(b);
     return FA_OK;
 }
 
@@ -490,18 +589,12 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
     if (ms_per_run) *ms_per_run = ms / reps;
     if (launches_per_run) *launches_per_run = nl;
     if (ms_dominant_launch) {
-        // time each launch of one run separately; report the longest
+        int32_t nl2 = 0;
+        double ms_l[64];
+        rc = fa_batch_profile_launches(ctx, b, flags, 64, &nl2, nullptr, nullptr, nullptr, ms_l);
+        if (rc) return rc;
         double best = 0.0;
-        for (const Bucket &bk : b->buckets) {
-            FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-            FA_HIP(fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count,
-                                           ctx->stream));
-            FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-            FA_HIP(hipEventSynchronize(ctx->ev1));
-            float t = 0.f;
-            FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
-            best = std::max(best, (double)t);
-        }
+        for (int k = 0; k < nl2 && k < 64; ++k) best = std::max(best, ms_l[k]);
         *ms_dominant_launch = best;
     }
     return FA_OK;
@@ -515,9 +608,12 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
     int k = 0;
+    b->args.chunk = std::max(1, b->chunk / (64 / b->lanes));
+    if (flags & FA_LAUNCH_PERSISTENT)
+        FA_HIP(fa::queue_reset(b->d_queue, (uint32_t)b->buckets.size(), ctx->stream));
     for (const Bucket &bk : b->buckets) {
         FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        FA_HIP(fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count, ctx->stream));
+        FA_HIP(launch_bucket(ctx, b, flags, (size_t)k, ctx->stream));
         FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         FA_HIP(hipEventSynchronize(ctx->ev1));
         float t = 0.f;

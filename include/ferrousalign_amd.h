@@ -161,6 +161,9 @@ int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost);
  * persistent-queue chunk size in pairs (0 = default). */
 int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_t chunk_pairs,
                         int32_t waves_per_cu);
+/* Number of streams (1..4, default 4) the query-length buckets of one run
+ * are spread over. */
+int fa_batch_set_streams(fa_ctx *ctx, fa_batch *b, int32_t streams);
 int fa_batch_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
 /* Runs the batch `reps` times back to back and reports the mean time of one
  * run measured with HIP events on the batch's stream (kernel-only, ms) and the

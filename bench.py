@@ -4,13 +4,23 @@
 One step = one pass of the GPU path over the whole batch, with the 2-bit
 packed sequences already resident in HBM.  Prints one JSON line.
 
-Workloads (seeded, synthetic; scoring A=1 B=4 O=6 E=1 w=100 zdrop=100
-end_bonus=5):
-  illumina_150bp_1m  1,048,576 right-extension pairs from 150 bp reads
-                     (substitutions 0.2% -> 2% along the read, 0.1% indels),
-                     16-bit scores, length-sorted packing: the plain kernel
-  uneven_mix_4m      4,194,304 pairs, reads {100,151,250} bp, 5% random
-                     targets, 10% w=200
+Default workload (uneven_mix_4m): 4,194,304 right-extension pairs, read
+lengths {100,151,250} bp (weights .3/.5/.2), substitutions rising 0.2%->2%
+along the read, 0.1% indels, 5% non-homologous targets, 10% band w=200;
+scoring A=1 B=4 O=6 E=1 w=100 zdrop=100 end_bonus=5.  Measured:
+  * value: the learned-packed path -- pairs ordered by the work predictor's
+    predicted DP cells inside query-length buckets, launch form (bucket
+    launches over 4 streams, or persistent waves on a device work queue),
+    lanes per pair and queue chunk chosen by the autotuner, replayed as a
+    hipGraph;
+  * plain: one lane group per pair, length-sorted packing, single launch;
+  * length-sorted: the learned path's launch configuration with the
+    predictor replaced by length sorting (what the network gains).
+The predictor is trained first, on the GPU with the library's kernels, on
+cells that the plain kernel records for a separate 16,777,216-pair training
+mix drawn the same way (other seed).
+--workload illumina_150bp_1m times the plain kernel on 1,048,576 pairs from
+150 bp reads instead.
 """
 import argparse
 import hashlib
@@ -24,9 +34,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X public peaks
-VALU_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12   # 39.3 T int32 lane-ops/s
-OPS_PER_CELL = 16                          # integer ops per DP cell of ksw_extend2's inner loop
+# MI355X public figure: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz
+VALU_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
+OPS_PER_CELL = 16  # integer ops per DP cell of ksw_extend2's inner loop (ksw_affine_gap.rs:1061-1103)
 
 
 def log(*a):
@@ -34,27 +44,51 @@ def log(*a):
 
 
 def cpu_baseline(ps, sw, budget_s=15.0):
-    """The CPU oracle (C restatement of ksw_extend2, all host cores) on a
+    """The CPU oracle (C restatement of ksw_extend2, all host threads) on a
     bounded sample of the same workload."""
     from tests import _oracle as O
     threads = os.cpu_count() or 1
-    n = min(ps.n, 16384)
-    t0 = time.perf_counter()
+    n = min(ps.n, 32768)
     sub = ps.subset(np.arange(n))
+    t0 = time.perf_counter()
     _, cells, _ = O.extend_batch(sw, *sub.flat(), nthreads=threads)
     dt = time.perf_counter() - t0
-    # grow the sample to roughly budget_s of CPU work
-    target = int(min(ps.n, n * max(1.0, budget_s / max(dt, 1e-3))))
+    target = int(min(ps.n, n * max(1.0, 0.8 * budget_s / max(dt, 1e-3))))
     if target > n:
-        sub = ps.subset(np.arange(target))
+        n = target
+        sub = ps.subset(np.arange(n))
         t0 = time.perf_counter()
         _, cells, _ = O.extend_batch(sw, *sub.flat(), nthreads=threads)
         dt = time.perf_counter() - t0
-        n = target
-    return {"value": n / dt, "unit": "pairs/s", "gcups": float(cells.sum()) / dt / 1e9, "cores": threads,
-            "kind": "port",
-            "sample": "first %d pairs of the workload, C restatement of ksw_extend2 (ksw_affine_gap.rs:952-1169), "
-                      "%d host threads, %.1f s" % (n, threads, dt)}
+    return {"value": round(n / dt, 1), "unit": "pairs/s", "gcups": round(float(cells.sum()) / dt / 1e9, 3),
+            "cores": threads, "kind": "port",
+            "sample": "first %d pairs of the workload through the C restatement of ksw_extend2 "
+                      "(ksw_affine_gap.rs:952-1169) on %d host threads: %.1f s" % (n, threads, dt)}
+
+
+def timed(db, flags, steps, warmup):
+    for _ in range(warmup):
+        db.run(flags)
+    t0 = time.perf_counter()
+    ms_ev, _, nlaunch = db.time(flags, steps)  # K runs enqueued back to back, synchronised at both ends
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / steps, ms_ev, nlaunch
+
+
+def roofline(db, flags, cells, kernel_name):
+    launches = db.profile_launches(flags)
+    order = db.order(flags)
+    dom = max(launches, key=lambda x: x["ms"])
+    dom_cells = float(cells[order[dom["first"]:dom["first"] + dom["count"]]].astype(np.float64).sum())
+    tops = dom_cells * OPS_PER_CELL / (dom["ms"] / 1e3) / 1e12
+    return {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_INT32_TOPS, 1), "unit": "TOPS",
+            "frac": round(tops / VALU_INT32_TOPS, 4), "traffic": None,
+            "kernel": kernel_name % dom["slots"], "launch_ms": round(dom["ms"], 4),
+            "launch_cells": dom_cells, "ops_per_cell": OPS_PER_CELL}
+
+
+def digest_of(out, cells):
+    return hashlib.sha256(out.tobytes() + cells.astype(np.uint32).tobytes()).hexdigest()
 
 
 def main():
@@ -62,74 +96,134 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="illumina_150bp_1m")
-    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--workload", default="uneven_mix_4m", choices=["uneven_mix_4m", "illumina_150bp_1m"])
+    ap.add_argument("--train-pairs", type=int, default=16777216)
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the extension pairs shard independently across GPUs "
-              "(no collective on the path); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: extension pairs shard independently across GPUs with no "
+              "collective on the path, so only --gpus 1 is run here", file=sys.stderr)
         sys.exit(2)
 
-    from ferrousalign_amd import synth
+    from ferrousalign_amd import synth, autotune, _native as N
     from ferrousalign_amd.banded_swa import DeviceBatch, Context
+    from ferrousalign_amd.predictor import WorkPredictor, mae
 
-    t0 = time.perf_counter()
-    ps = synth.workload(args.workload)
-    sw = synth.default_scoring()
-    log("generated %d pairs in %.1f s" % (ps.n, time.perf_counter() - t0))
     ctx = Context(0)
-    db = DeviceBatch(sw, *ps.flat(), ctx=ctx)
-    db.set_launch(args.lanes)
-    flags = 0
-    for _ in range(args.warmup):
-        db.run(flags)
-    log("warmup done")
-    t0 = time.perf_counter()
-    ms_ev, _, nlaunch = db.time(flags, args.steps)   # K steps, synchronised at both ends
-    wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / args.steps
-    out, cells = db.download()
-    total_cells = float(cells.astype(np.float64).sum())
-    pairs_s = ps.n / (ms_step / 1e3)
-    gcups = total_cells / (ms_step / 1e3) / 1e9
-    digest = hashlib.sha256(out.tobytes() + cells.astype(np.uint32).tobytes()).hexdigest()
-    golden = None
+    sw = synth.default_scoring()
+    golden = {}
     gpath = os.path.join(ROOT, "tests", "golden", "digests.json")
     if os.path.exists(gpath):
-        golden = json.load(open(gpath)).get(args.workload)
-    bit_exact = None if golden is None else (golden == digest)
+        golden = json.load(open(gpath))
 
-    # dominant kernel launch: HIP events around each launch on the batch's stream
-    launches = db.profile_launches(flags)
-    order = db.order(flags)
-    dom = max(launches, key=lambda x: x["ms"])
-    dom_cells = float(cells[order[dom["first"]:dom["first"] + dom["count"]]].astype(np.float64).sum())
-    dom_tops = dom_cells * OPS_PER_CELL / (dom["ms"] / 1e3) / 1e12
-    roofline = {"bound": "valu", "achieved": round(dom_tops, 3), "peak": round(VALU_INT32_TOPS, 1),
-                "unit": "TOPS", "frac": round(dom_tops / VALU_INT32_TOPS, 4), "traffic": None,
-                "kernel": "extend_plain_kernel<%d,%d>" % (args.lanes, dom["slots"]),
-                "launch_ms": round(dom["ms"], 4), "launch_cells": dom_cells}
+    if args.workload == "illumina_150bp_1m":
+        ps = synth.workload(args.workload)
+        db = DeviceBatch(sw, *ps.flat(), ctx=ctx)
+        db.set_launch(16)
+        flags = N.FA_LAUNCH_SINGLE
+        ms, ms_ev, nl = timed(db, flags, args.steps, args.warmup)
+        out, cells = db.download()
+        res = {"metric": "extension pairs/s (ksw_extend2-exact banded seed extension)",
+               "value": round(ps.n / (ms / 1e3), 1), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+               "config": {"workload": args.workload, "pairs": ps.n, "lanes_per_pair": 16,
+                          "packing": "length-sorted", "launch": "single"},
+               "gcups": round(float(cells.astype(np.float64).sum()) / (ms / 1e3) / 1e9, 2),
+               "bit_exact_vs_golden": golden.get(args.workload) == digest_of(out, cells),
+               "roofline": roofline(db, flags, cells, "extend_plain_kernel<16,%d>")}
+        if not args.no_cpu:
+            res["cpu_baseline"] = cpu_baseline(ps, sw)
+        print(json.dumps(res), flush=True)
+        return
+
+    # -- 1. train the work predictor on a separate training mix ---------------------
+    t0 = time.perf_counter()
+    tr = synth.workload("uneven_mix_train_16m", n=args.train_pairs)
+    dbt = DeviceBatch(sw, *tr.flat(), ctx=ctx)
+    ntrain = tr.n
+    del tr
+    dbt.set_launch(8)
+    dbt.run(N.FA_LAUNCH_GRAPH)  # records the cell labels
+    pred = WorkPredictor(ctx, seed=0)
+    t1 = time.perf_counter()
+    losses = pred.fit(dbt, epochs=args.epochs, minibatch=16384, lr=3e-3, log=log)
+    train_s = time.perf_counter() - t1
+    dbt.free()
+    log("predictor trained on %d pairs in %.2f s (setup %.1f s)" % (ntrain, train_s, t1 - t0))
+
+    # -- 2. evaluation batch -----------------------------------------------------------
+    ps = synth.workload(args.workload)
+    db = DeviceBatch(sw, *ps.flat(), ctx=ctx)
+
+    # plain GPU kernel: one lane group (16 lanes) per pair, length-sorted, single launch
+    db.set_launch(16)
+    db.set_streams(1)
+    plain_flags = N.FA_PACK_LENGTH | N.FA_LAUNCH_SINGLE
+    plain_steps = max(2, args.steps // 2)
+    ms_plain, _, _ = timed(db, plain_flags, plain_steps, 1)
+    out_plain, cells = db.download()
+    log("plain single-launch: %.2f ms/step" % ms_plain)
+
+    # learned packing
+    cpred = pred.predict(db, set_cost=True)
+    pred_mae = mae(cpred, cells)
+    if args.no_tune:
+        best = dict(lanes=8, chunk=0, wpc=0, streams=4, launch=N.FA_LAUNCH_GRAPH)
+        db.set_launch(8)
+        db.set_streams(4)
+        flags = N.FA_PACK_COST | N.FA_LAUNCH_GRAPH
+    else:
+        best, flags, _ = autotune.tune(db, ps.qlen, ps.tlen, ps.w, pack_flags=N.FA_PACK_COST, log=log,
+                                       use_cache=False)
+    ms, ms_ev, nlaunch = timed(db, flags, args.steps, args.warmup)
+    out, cells2 = db.download()
+    total_cells = float(cells2.astype(np.float64).sum())
+    kname = ("extend_persistent_kernel<%d," % best["lanes"] if flags & N.FA_LAUNCH_PERSISTENT
+             else "extend_plain_kernel<%d," % best["lanes"]) + "%d>"
+    roof = roofline(db, flags, cells2, kname)
+
+    # ablation: same launch configuration, length-sorted instead of learned
+    lflags = (flags & ~0xF0) | N.FA_PACK_LENGTH
+    ms_len, _, _ = timed(db, lflags, max(2, args.steps // 2), 1)
+    out_len, cells_len = db.download()
+
+    dg = digest_of(out, cells2)
+    exact = (dg == golden.get(args.workload)) if golden.get(args.workload) else None
+    same = bool(np.array_equal(out, out_plain) and np.array_equal(cells2, cells)
+                and np.array_equal(out_len, out_plain) and np.array_equal(cells_len, cells))
     res = {
-        "metric": "extension pairs/s (ksw_extend2-exact banded seed extension)",
-        "value": round(pairs_s, 1),
+        "metric": "extension pairs/s + GCUPS, learned-packed vs plain GPU kernel, bit-exact",
+        "value": round(ps.n / (ms / 1e3), 1),
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "ms_per_step_events": round(ms_ev, 4),
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "pairs": ps.n, "lanes_per_pair": args.lanes,
-                   "packing": "length-sorted", "launches_per_step": nlaunch},
-        "gcups": round(gcups, 2),
-        "bit_exact_vs_golden": bit_exact,
-        "digest": digest,
-        "roofline": roofline,
+        "config": {"workload": args.workload, "pairs": ps.n, "packing": "learned (predicted cells)",
+                   "lanes_per_pair": best["lanes"], "chunk_pairs": best["chunk"], "streams": best.get("streams", 4),
+                   "launch": ("persistent queue" if flags & N.FA_LAUNCH_PERSISTENT else "bucket launches")
+                   + " + hipGraph", "launches_per_step": nlaunch},
+        "gcups": round(total_cells / (ms / 1e3) / 1e9, 2),
+        "plain_pairs_per_s": round(ps.n / (ms_plain / 1e3), 1),
+        "plain_gcups": round(total_cells / (ms_plain / 1e3) / 1e9, 2),
+        "speedup_vs_plain": round(ms_plain / ms, 3),
+        "length_sorted_pairs_per_s": round(ps.n / (ms_len / 1e3), 1),
+        "learned_gain_vs_length_sorted": round(ms_len / ms, 3),
+        "bit_exact_vs_golden": exact,
+        "identical_across_packings": same,
+        "digest": dg,
+        "predictor": {"train_pairs": ntrain, "epochs": args.epochs, "train_s": round(train_s, 2),
+                      "final_train_loss": round(losses[-1], 4), "heldout_mae_cells": round(pred_mae, 1),
+                      "mean_cells": round(float(cells.mean()), 1)},
+        "roofline": roof,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(ps, sw)

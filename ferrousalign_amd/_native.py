@@ -36,6 +36,7 @@ FA_PACK_COST = 0x20
 FA_LAUNCH_PLAIN = 0x000
 FA_LAUNCH_PERSISTENT = 0x100
 FA_LAUNCH_GRAPH = 0x200
+FA_LAUNCH_SINGLE = 0x400
 
 
 class FaError(RuntimeError):

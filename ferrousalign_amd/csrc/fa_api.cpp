@@ -16,6 +16,7 @@
 
 #include "../../include/ferrousalign_amd.h"
 #include "extend_kernel.h"
+#include "predictor.h"
 
 struct fa_ctx {
     int device = 0;
@@ -63,6 +64,8 @@ struct fa_batch {
     int waves_per_cu = 0;
     int streams = 4;
     int tmax = 0;
+    float *d_feat = nullptr;       // predictor features [n][6]
+    float *d_pred = nullptr;       // predicted cells [n]
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_key = 0;
     int graph_wpc = 0;
@@ -188,7 +191,7 @@ void batch_release(fa_batch *b) {
     if (!b) return;
     drop_graph(b);
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
-                    b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue};
+                    b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -200,8 +203,9 @@ void batch_release(fa_batch *b) {
 // of one wavefront hold pairs of similar cost and the long pairs start first.
 int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t pack = flags & 0xF0u;
+    const uint32_t single = flags & FA_LAUNCH_SINGLE;
     const int L = b->lanes;
-    if (b->packed_flags == (pack | (uint32_t)L << 16)) return FA_OK;
+    if (b->packed_flags == (pack | single | (uint32_t)L << 16)) return FA_OK;
     const int64_t n = b->n;
     std::vector<uint32_t> order((size_t)n);
     std::vector<Bucket> buckets;
@@ -248,12 +252,19 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             k = e;
         }
     }
+    if (single && buckets.size() > 1) {
+        // one launch at the widest slot count, keeping the sorted order
+        double work = 0.0;
+        for (const Bucket &bk : buckets) work += bk.work;
+        const int S = buckets.front().S;  // buckets run widest first
+        buckets.assign(1, Bucket{S, 0u, (uint32_t)n, work});
+    }
     if (buckets.size() > 64) return FA_ERR_ARG;
     FA_HIP(hipMemcpyAsync(b->d_order, order.data(), order.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
     b->buckets.swap(buckets);
-    b->packed_flags = pack | (uint32_t)L << 16;
+    b->packed_flags = pack | single | (uint32_t)L << 16;
     drop_graph(b);
     return FA_OK;
 }
@@ -447,9 +458,15 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     uint64_t words = 0;
     int first_err = FA_OK;
     int tmax = 0;
+    std::vector<int8_t> vrc((size_t)n);
+    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
+        for (int64_t k = lo; k < hi; ++k)
+            vrc[(size_t)k] = (int8_t)validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                                                  query_seqs + query_offset[k],
+                                                  ref_seqs + ref_offset[k]);
+    });
     for (int64_t k = 0; k < n; ++k) {
-        int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
-                             query_seqs + query_offset[k], ref_seqs + ref_offset[k]);
+        int r = vrc[(size_t)k];
         if (r != FA_OK) {
             first_err = r;
             break;
@@ -820,6 +837,190 @@ int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t
     }
     return soa_common(ctx, lanes, num_jobs, qlen, tlen, h0, w, o_del, e_del, o_ins, e_ins, zdrop,
                       end_bonus, mat, m, out, q, t);
+}
+
+// ---- learned work predictor ---------------------------------------------------
+
+struct fa_mlp {
+    float *master = nullptr, *m = nullptr, *v = nullptr, *norm = nullptr;
+    void *wh = nullptr;  // fp16 weights
+    float *partial = nullptr;
+    uint32_t partial_blocks = 0;
+    float *grad = nullptr, *loss = nullptr;
+    uint32_t *perm = nullptr;
+    int64_t perm_n = 0;
+    int64_t step = 0;
+};
+
+static void mlp_release(fa_mlp *m) {
+    void *ptrs[] = {m->master, m->m, m->v, m->norm, m->wh, m->partial, m->grad, m->loss, m->perm};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete m;
+}
+
+int fa_mlp_create(fa_ctx *ctx, fa_mlp **out) {
+    if (!ctx || !out) return FA_ERR_ARG;
+    *out = nullptr;
+    FA_HIP(hipSetDevice(ctx->device));
+    fa_mlp *m = new (std::nothrow) fa_mlp();
+    if (!m) return FA_ERR_ARG;
+    int rc;
+    if ((rc = dev_alloc(&m->master, fa::kMlpParams)) || (rc = dev_alloc(&m->m, fa::kMlpParams)) ||
+        (rc = dev_alloc(&m->v, fa::kMlpParams)) || (rc = dev_alloc(&m->norm, 2 * fa::kMlpFeatures)) ||
+        (rc = dev_alloc((uint16_t **)&m->wh, fa::kMlpParams)) ||
+        (rc = dev_alloc(&m->grad, fa::kMlpParams)) || (rc = dev_alloc(&m->loss, 1))) {
+        mlp_release(m);
+        return rc;
+    }
+    *out = m;
+    return FA_OK;
+}
+
+int fa_mlp_destroy(fa_ctx *ctx, fa_mlp *m) {
+    if (!ctx || !m) return FA_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    mlp_release(m);
+    return FA_OK;
+}
+
+int fa_mlp_set_params(fa_ctx *ctx, fa_mlp *m, const float *params, const float *norm) {
+    if (!ctx || !m || !params || !norm) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    FA_HIP(hipMemcpyAsync(m->master, params, fa::kMlpParams * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    FA_HIP(hipMemcpyAsync(m->norm, norm, 2 * fa::kMlpFeatures * sizeof(float),
+                          hipMemcpyHostToDevice, ctx->stream));
+    FA_HIP(hipMemsetAsync(m->m, 0, fa::kMlpParams * sizeof(float), ctx->stream));
+    FA_HIP(hipMemsetAsync(m->v, 0, fa::kMlpParams * sizeof(float), ctx->stream));
+    FA_HIP(fa::mlp_launch_to_half(m->master, m->wh, fa::kMlpParams, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    m->step = 0;
+    return FA_OK;
+}
+
+int fa_mlp_get_params(fa_ctx *ctx, fa_mlp *m, float *params) {
+    if (!ctx || !m || !params) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    FA_HIP(hipMemcpy(params, m->master, fa::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    return FA_OK;
+}
+
+static int ensure_features(fa_ctx *ctx, fa_batch *b) {
+    if (!b->d_feat) {
+        int rc = dev_alloc(&b->d_feat, (size_t)b->n * fa::kMlpFeatures);
+        if (rc) return rc;
+        FA_HIP(fa::mlp_launch_features(b->args, (uint32_t)b->n, b->d_feat, ctx->stream));
+    }
+    return FA_OK;
+}
+
+int fa_batch_features(fa_ctx *ctx, fa_batch *b, float *feat) {
+    if (!ctx || !b) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_features(ctx, b);
+    if (rc) return rc;
+    if (feat && b->n)
+        FA_HIP(hipMemcpyAsync(feat, b->d_feat, (size_t)b->n * fa::kMlpFeatures * sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    return FA_OK;
+}
+
+int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32_t set_cost) {
+    if (!ctx || !m || !b) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_features(ctx, b);
+    if (rc) return rc;
+    if (!b->d_pred && (rc = dev_alloc(&b->d_pred, (size_t)b->n))) return rc;
+    FA_HIP(fa::mlp_launch_forward(m->wh, m->norm, b->d_feat, (uint32_t)b->n, b->d_pred, nullptr,
+                                  ctx->stream));
+    std::vector<float> host((size_t)b->n);
+    if (b->n)
+        FA_HIP(hipMemcpyAsync(host.data(), b->d_pred, (size_t)b->n * sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    if (cells_pred) std::memcpy(cells_pred, host.data(), host.size() * sizeof(float));
+    if (set_cost) return fa_batch_set_cost(ctx, b, host.data());
+    return FA_OK;
+}
+
+static int mlp_ensure_partial(fa_mlp *m, uint32_t blocks) {
+    if (m->partial_blocks >= blocks) return FA_OK;
+    if (m->partial) (void)hipFree(m->partial);
+    m->partial = nullptr;
+    m->partial_blocks = 0;
+    int rc = dev_alloc(&m->partial, (size_t)blocks * (fa::kMlpParams + 1));
+    if (rc) return rc;
+    m->partial_blocks = blocks;
+    return FA_OK;
+}
+
+static int mlp_set_perm(fa_ctx *ctx, fa_mlp *m, const uint32_t *perm, int64_t n) {
+    if (m->perm_n < n) {
+        if (m->perm) (void)hipFree(m->perm);
+        m->perm = nullptr;
+        m->perm_n = 0;
+        int rc = dev_alloc(&m->perm, (size_t)n);
+        if (rc) return rc;
+        m->perm_n = n;
+    }
+    FA_HIP(hipMemcpyAsync(m->perm, perm, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    return FA_OK;
+}
+
+int fa_mlp_gradient(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *pairs, int64_t count,
+                    float *grad, float *mean_loss) {
+    if (!ctx || !m || !b || !pairs || count <= 0 || count > b->n) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_features(ctx, b);
+    if (rc) return rc;
+    const uint32_t blocks = (uint32_t)((count + fa::kMlpBlock - 1) / fa::kMlpBlock);
+    if ((rc = mlp_ensure_partial(m, blocks)) || (rc = mlp_set_perm(ctx, m, pairs, count))) return rc;
+    FA_HIP(fa::mlp_launch_backward(m->wh, m->norm, b->d_feat, b->d_cells, m->perm, 0,
+                                   (uint32_t)count, m->partial, ctx->stream));
+    FA_HIP(fa::mlp_launch_reduce_adam(m->partial, blocks, m->grad, m->loss, m->master, m->m, m->v,
+                                      m->wh, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0, ctx->stream));
+    float lsum = 0.f;
+    if (grad)
+        FA_HIP(hipMemcpyAsync(grad, m->grad, fa::kMlpParams * sizeof(float), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    FA_HIP(hipMemcpyAsync(&lsum, m->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    if (mean_loss) *mean_loss = lsum / (float)count;
+    return FA_OK;
+}
+
+int fa_mlp_train(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *order, int64_t count,
+                 int32_t minibatch, float lr, float *mean_loss) {
+    if (!ctx || !m || !b || !order || count <= 0 || count > b->n || minibatch <= 0) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_features(ctx, b);
+    if (rc) return rc;
+    const uint32_t mb_blocks = (uint32_t)((minibatch + fa::kMlpBlock - 1) / fa::kMlpBlock);
+    if ((rc = mlp_ensure_partial(m, mb_blocks)) || (rc = mlp_set_perm(ctx, m, order, count)))
+        return rc;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    std::vector<float> losses;
+    double total = 0.0;
+    for (int64_t first = 0; first < count; first += minibatch) {
+        const uint32_t cnt = (uint32_t)std::min<int64_t>(minibatch, count - first);
+        const uint32_t blocks = (cnt + fa::kMlpBlock - 1) / fa::kMlpBlock;
+        m->step += 1;
+        const float bc1 = 1.f - std::pow(b1, (float)m->step);
+        const float bc2 = 1.f - std::pow(b2, (float)m->step);
+        FA_HIP(fa::mlp_launch_backward(m->wh, m->norm, b->d_feat, b->d_cells, m->perm,
+                                       (uint32_t)first, cnt, m->partial, ctx->stream));
+        FA_HIP(fa::mlp_launch_reduce_adam(m->partial, blocks, nullptr, m->loss, m->master, m->m,
+                                          m->v, m->wh, lr, b1, b2, eps, bc1, bc2, 1, ctx->stream));
+        float lsum = 0.f;
+        FA_HIP(hipMemcpyAsync(&lsum, m->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        FA_HIP(hipStreamSynchronize(ctx->stream));
+        total += lsum;
+    }
+    if (mean_loss) *mean_loss = (float)(total / (double)count);
+    return FA_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,280 @@
+// Learned work predictor: a 6-32-32-1 MLP that predicts log(1 + DP cells) of
+// an extension pair from six features, used to pack pairs of similar cost
+// into the lane groups of one wavefront.  Hand-written gfx950 kernels for the
+// features, the forward pass, the backward pass (per-block partial
+// gradients) and the gradient reduction + Adam step.  FP16 storage of
+// weights and activations, FP32 accumulation.  The predictor has no
+// counterpart in the reference; it never changes any alignment result.
+//
+// Parameter vector (1,313 floats): W1[32][6], b1[32], W2[32][32], b2[32],
+// W3[32], b3.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "extend_kernel.h"
+#include "predictor.h"
+
+namespace fa {
+
+namespace {
+
+constexpr int NIN = 6, NH = 32;
+constexpr int OFF_B1 = NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH;
+static_assert(OFF_B3 + 1 == kMlpParams, "parameter layout");
+
+__device__ __forceinline__ float r16(float v) { return __half2float(__float2half(v)); }
+
+__device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *seqn, uint32_t off,
+                                        int p) {
+    const uint32_t c = (seq2[off + (p >> 4)] >> ((p & 15) * 2)) & 3u;
+    return ((seqn[(off >> 1) + (p >> 5)] >> (p & 31)) & 1u) ? 4 : (int)c;
+}
+
+// Forward of one pair with weights `w` (fp16, LDS).  Fills x, h1, h2 (fp16
+// values held in float) and the pre-activations' signs.
+__device__ __forceinline__ float mlp_forward1(const __half *w, const float *raw, const float *norm,
+                                              float *x, float *h1, float *h2, uint32_t &pos1,
+                                              uint32_t &pos2) {
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) x[k] = r16((raw[k] - norm[k]) * norm[NIN + k]);
+    pos1 = 0;
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+        float a = __half2float(w[OFF_B1 + i]);
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) a += __half2float(w[i * NIN + k]) * x[k];
+        pos1 |= (a > 0.f ? 1u : 0u) << i;
+        h1[i] = r16(a > 0.f ? a : 0.f);
+    }
+    pos2 = 0;
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+        float a = __half2float(w[OFF_B2 + i]);
+#pragma unroll
+        for (int k = 0; k < NH; ++k) a += __half2float(w[OFF_W2 + i * NH + k]) * h1[k];
+        pos2 |= (a > 0.f ? 1u : 0u) << i;
+        h2[i] = r16(a > 0.f ? a : 0.f);
+    }
+    float y = __half2float(w[OFF_B3]);
+#pragma unroll
+    for (int k = 0; k < NH; ++k) y += __half2float(w[OFF_W3 + k]) * h2[k];
+    return y;
+}
+
+}  // namespace
+
+// Raw features per pair from the batch's 2-bit sequences in HBM.
+__global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n, float *feat) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int qlen = a.qlen[k], tlen = a.tlen[k];
+    const uint32_t qo = a.qo[k], to = a.to[k];
+    int lim = qlen < tlen ? qlen : tlen;
+    if (lim > 32) lim = 32;
+    int mm = 0;
+    for (int j = 0; j < lim; ++j)
+        mm += seq_code(a.seq2, a.seqn, qo, j) != seq_code(a.seq2, a.seqn, to, j);
+    // C (01) and G (10) are the 2-bit codes whose two bits differ; N is 00
+    int gc = 0;
+    const int nw = (qlen + 15) >> 4;
+    for (int x = 0; x < nw; ++x) {
+        const uint32_t w2 = a.seq2[qo + x];
+        gc += __popc((w2 ^ (w2 >> 1)) & 0x55555555u);
+    }
+    float *o = feat + (size_t)k * NIN;
+    o[0] = (float)qlen;
+    o[1] = (float)tlen;
+    o[2] = (float)a.h0[k];
+    o[3] = (float)a.w[k];
+    o[4] = (float)mm;
+    o[5] = qlen > 0 ? (float)gc / (float)qlen : 0.f;
+}
+
+// Predicted cells (expm1 of the network output) per pair.
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *wg, const float *normg,
+                                                          const float *feat, uint32_t n,
+                                                          float *cells_pred, float *yout) {
+    __shared__ __half w[kMlpParams];
+    __shared__ float norm[2 * NIN];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = wg[i];
+    if (threadIdx.x < 2 * NIN) norm[threadIdx.x] = normg[threadIdx.x];
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float raw[NIN], x[NIN], h1[NH], h2[NH];
+    uint32_t p1, p2;
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) raw[i] = feat[(size_t)k * NIN + i];
+    const float y = mlp_forward1(w, raw, norm, x, h1, h2, p1, p2);
+    if (yout) yout[k] = y;
+    float c = expm1f(y);
+    cells_pred[k] = c > 0.f ? c : 0.f;
+}
+
+// One minibatch slice of kMlpBlock pairs per block: forward, backward, and the
+// block's partial gradient sum (already divided by the minibatch size) in
+// partial[blockIdx.x][0 .. kMlpParams), block loss sum in partial[..][kMlpParams].
+__global__ __launch_bounds__(kMlpBlock) void mlp_backward_kernel(
+    const __half *wg, const float *normg, const float *feat, const uint32_t *cells,
+    const uint32_t *perm, uint32_t first, uint32_t count, float inv_count, float *partial) {
+    __shared__ __half w[kMlpParams];
+    __shared__ float norm[2 * NIN];
+    // per pair: x[6], h1[32], h2[32], d1[32], d2[32], d3  (fp32)
+    constexpr int REC = NIN + 4 * NH + 1;
+    __shared__ float rec[kMlpBlock * REC];
+    __shared__ float lsum[kMlpBlock];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = wg[i];
+    if (threadIdx.x < 2 * NIN) norm[threadIdx.x] = normg[threadIdx.x];
+    __syncthreads();
+
+    const uint32_t slot = blockIdx.x * kMlpBlock + threadIdx.x;
+    float *r = rec + threadIdx.x * REC;
+    float loss = 0.f;
+    if (slot < count) {
+        const uint32_t k = perm[first + slot];
+        float raw[NIN], x[NIN], h1[NH], h2[NH];
+        uint32_t p1, p2;
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) raw[i] = feat[(size_t)k * NIN + i];
+        const float y = mlp_forward1(w, raw, norm, x, h1, h2, p1, p2);
+        const float t = log1pf((float)cells[k]);
+        const float d = y - t;
+        loss = 0.5f * d * d;
+        const float d3 = d * inv_count;
+        float d2[NH];
+#pragma unroll
+        for (int j = 0; j < NH; ++j)
+            d2[j] = ((p2 >> j) & 1u) ? d3 * __half2float(w[OFF_W3 + j]) : 0.f;
+#pragma unroll
+        for (int kk = 0; kk < NH; ++kk) {
+            float acc = 0.f;
+#pragma unroll
+            for (int j = 0; j < NH; ++j) acc += __half2float(w[OFF_W2 + j * NH + kk]) * d2[j];
+            r[NIN + 2 * NH + kk] = ((p1 >> kk) & 1u) ? acc : 0.f;  // d1
+        }
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) r[i] = x[i];
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            r[NIN + j] = h1[j];
+            r[NIN + NH + j] = h2[j];
+            r[NIN + 3 * NH + j] = d2[j];
+        }
+        r[NIN + 4 * NH] = d3;
+    } else {
+        for (int i = 0; i < REC; ++i) r[i] = 0.f;
+    }
+    lsum[threadIdx.x] = loss;
+    __syncthreads();
+
+    float *out = partial + (size_t)blockIdx.x * (kMlpParams + 1);
+    for (int p = threadIdx.x; p < kMlpParams; p += blockDim.x) {
+        float g = 0.f;
+        if (p < OFF_B1) {  // W1[i][m] : d1[i] * x[m]
+            const int i = p / NIN, m = p % NIN;
+            for (int q = 0; q < kMlpBlock; ++q) g += rec[q * REC + NIN + 2 * NH + i] * rec[q * REC + m];
+        } else if (p < OFF_W2) {  // b1
+            const int i = p - OFF_B1;
+            for (int q = 0; q < kMlpBlock; ++q) g += rec[q * REC + NIN + 2 * NH + i];
+        } else if (p < OFF_B2) {  // W2[j][k] : d2[j] * h1[k]
+            const int j = (p - OFF_W2) / NH, kk = (p - OFF_W2) % NH;
+            for (int q = 0; q < kMlpBlock; ++q)
+                g += rec[q * REC + NIN + 3 * NH + j] * rec[q * REC + NIN + kk];
+        } else if (p < OFF_W3) {  // b2
+            const int j = p - OFF_B2;
+            for (int q = 0; q < kMlpBlock; ++q) g += rec[q * REC + NIN + 3 * NH + j];
+        } else if (p < OFF_B3) {  // W3[j] : d3 * h2[j]
+            const int j = p - OFF_W3;
+            for (int q = 0; q < kMlpBlock; ++q) g += rec[q * REC + NIN + 4 * NH] * rec[q * REC + NIN + NH + j];
+        } else {  // b3
+            for (int q = 0; q < kMlpBlock; ++q) g += rec[q * REC + NIN + 4 * NH];
+        }
+        out[p] = g;
+    }
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int q = 0; q < kMlpBlock; ++q) s += lsum[q];
+        out[kMlpParams] = s;
+    }
+}
+
+// Sums the per-block partial gradients in block order (deterministic), then
+// optionally takes one Adam step on the fp32 master weights and refreshes the
+// fp16 copy.  grad_out / loss_out receive the summed gradient and loss sum.
+__global__ __launch_bounds__(256) void mlp_reduce_adam_kernel(const float *partial, uint32_t nblocks,
+                                                              float *grad_out, float *loss_out,
+                                                              float *master, float *m, float *v,
+                                                              __half *wh, float lr, float b1,
+                                                              float b2, float eps, float bc1,
+                                                              float bc2, int apply) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpParams) return;
+    float g = 0.f;
+    for (uint32_t b = 0; b < nblocks; ++b) g += partial[(size_t)b * (kMlpParams + 1) + p];
+    if (p == kMlpParams) {
+        if (loss_out) *loss_out = g;
+        return;
+    }
+    if (grad_out) grad_out[p] = g;
+    if (!apply) return;
+    const float mm = b1 * m[p] + (1.f - b1) * g;
+    const float vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    const float upd = lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    const float nw = master[p] - upd;
+    master[p] = nw;
+    wh[p] = __float2half(nw);
+}
+
+__global__ void mlp_to_half_kernel(const float *src, __half *dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = __float2half(src[i]);
+}
+
+// ---- host launchers ----------------------------------------------------------
+
+hipError_t mlp_launch_features(const ExtArgs &a, uint32_t n, float *feat, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n, feat);
+    return hipGetLastError();
+}
+
+hipError_t mlp_launch_forward(const void *wh, const float *norm, const float *feat, uint32_t n,
+                              float *cells_pred, float *y, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+                       (const __half *)wh, norm, feat, n, cells_pred, y);
+    return hipGetLastError();
+}
+
+hipError_t mlp_launch_backward(const void *wh, const float *norm, const float *feat,
+                               const uint32_t *cells, const uint32_t *perm, uint32_t first,
+                               uint32_t count, float *partial, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    const uint32_t blocks = (count + kMlpBlock - 1) / kMlpBlock;
+    hipLaunchKernelGGL(mlp_backward_kernel, dim3(blocks), dim3(kMlpBlock), 0, st,
+                       (const __half *)wh, norm, feat, cells, perm, first, count,
+                       1.0f / (float)count, partial);
+    return hipGetLastError();
+}
+
+hipError_t mlp_launch_reduce_adam(const float *partial, uint32_t nblocks, float *grad_out,
+                                  float *loss_out, float *master, float *m, float *v, void *wh,
+                                  float lr, float b1, float b2, float eps, float bc1, float bc2,
+                                  int apply, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_reduce_adam_kernel, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0,
+                       st, partial, nblocks, grad_out, loss_out, master, m, v, (__half *)wh, lr,
+                       b1, b2, eps, bc1, bc2, apply);
+    return hipGetLastError();
+}
+
+hipError_t mlp_launch_to_half(const float *src, void *dst, int n, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_to_half_kernel, dim3((n + 255) / 256), dim3(256), 0, st, src,
+                       (__half *)dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace fa

@@ -52,6 +52,8 @@ extern "C" {
 #define FA_LAUNCH_PLAIN 0x000u  /* one launch per query-length bucket */
 #define FA_LAUNCH_PERSISTENT 0x100u /* persistent waves pulling chunks from a device queue */
 #define FA_LAUNCH_GRAPH 0x200u  /* replay the launch sequence as a captured hipGraph */
+#define FA_LAUNCH_SINGLE 0x400u /* one launch for the whole batch at the slot count of its
+                                   longest query (no query-length buckets) */
 
 /* ---- records ------------------------------------------------------------ */
 /* BandedPairWiseSW, src/core/alignment/banded_swa/types.rs:106-199 */
@@ -184,6 +186,34 @@ int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cel
  * run. */
 int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count);
 int fa_batch_free(fa_ctx *ctx, fa_batch *b);
+
+/* ---- learned work predictor (no counterpart in the reference) ------------ */
+/* A 6-32-32-1 MLP (FP16 storage, FP32 accumulation) predicting log(1 + DP
+ * cells) of each pair from: qlen, tlen, h0, w (after end-bonus capping),
+ * mismatches among the first min(32, qlen, tlen) positions of the ungapped
+ * diagonal, and query GC fraction.  Used only to order pairs for packing
+ * (FA_PACK_COST); it never changes an alignment result.  Parameters: 1,313
+ * floats W1[32][6], b1[32], W2[32][32], b2[32], W3[32], b3; norm: 6 feature
+ * means then 6 inverse standard deviations. */
+typedef struct fa_mlp fa_mlp;
+int fa_mlp_create(fa_ctx *ctx, fa_mlp **out);
+int fa_mlp_destroy(fa_ctx *ctx, fa_mlp *m);
+int fa_mlp_set_params(fa_ctx *ctx, fa_mlp *m, const float *params, const float *norm);
+int fa_mlp_get_params(fa_ctx *ctx, fa_mlp *m, float *params);
+/* Raw features of every pair of a resident batch, computed on the device
+ * from its 2-bit sequences; copies them to feat[n][6] when feat != NULL. */
+int fa_batch_features(fa_ctx *ctx, fa_batch *b, float *feat);
+/* Predicted cells per pair (to cells_pred when non-NULL); with set_cost the
+ * predictions become the batch's packing cost. */
+int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32_t set_cost);
+/* Mean loss 0.5*(y - log1p(cells))^2 over pairs[0..count) of the batch and
+ * its gradient (labels: the cells recorded by the batch's last run). */
+int fa_mlp_gradient(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *pairs, int64_t count,
+                    float *grad, float *mean_loss);
+/* One pass of Adam over order[0..count) in minibatches (labels: the cells
+ * recorded by the batch's last run); *mean_loss = mean training loss. */
+int fa_mlp_train(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *order, int64_t count,
+                 int32_t minibatch, float lr, float *mean_loss);
 
 #ifdef __cplusplus
 }

@@ -68,3 +68,52 @@ def to_outscore_order(ora_out):
     (score, target_end_pos=tle, gtarget_end_pos=gtle, query_end_pos=qle, global_score, max_offset)."""
     o = np.asarray(ora_out)
     return np.stack([o[:, 0], o[:, 2], o[:, 3], o[:, 1], o[:, 4], o[:, 5]], axis=1)
+
+
+def _mlp_lib():
+    l = lib()
+    if not getattr(l, "_mlp", False):
+        l.ora_round16.argtypes = [ctypes.c_float]
+        l.ora_round16.restype = ctypes.c_float
+        l.ora_mlp_forward.argtypes = [_P, _P, _P, _I64, ctypes.c_int, _P]
+        l.ora_mlp_forward.restype = None
+        l.ora_mlp_grad.argtypes = [_P, _P, _P, _P, _I64, ctypes.c_int, _P]
+        l.ora_mlp_grad.restype = ctypes.c_double
+        l.ora_pair_features.argtypes = [_P, _I32, _P, _I32, _I32, _I32, _P]
+        l.ora_pair_features.restype = None
+        l._mlp = True
+    return l
+
+
+# Every array handed to C is bound to a local name first, so that it stays
+# alive for the duration of the call.
+
+def mlp_forward(params, norm, feat, fp16=True):
+    feat = np.ascontiguousarray(feat, np.float32)
+    params = np.ascontiguousarray(params, np.float32)
+    norm = np.ascontiguousarray(norm, np.float32)
+    y = np.zeros(len(feat), np.float32)
+    _mlp_lib().ora_mlp_forward(P(params), P(norm), P(feat), len(feat), int(fp16), P(y))
+    return y
+
+
+def mlp_grad(params, norm, feat, target, fp16=True):
+    feat = np.ascontiguousarray(feat, np.float32)
+    params = np.ascontiguousarray(params, np.float32)
+    norm = np.ascontiguousarray(norm, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    g = np.zeros(1313, np.float64)
+    loss = _mlp_lib().ora_mlp_grad(P(params), P(norm), P(feat), P(target), len(feat), int(fp16), P(g))
+    return loss, g
+
+
+def pair_features(q, t, h0, w):
+    qb = np.concatenate([np.asarray(q, np.uint8), np.zeros(1, np.uint8)])
+    tb = np.concatenate([np.asarray(t, np.uint8), np.zeros(1, np.uint8)])
+    out = np.zeros(6, np.float32)
+    _mlp_lib().ora_pair_features(P(qb), len(qb) - 1, P(tb), len(tb) - 1, h0, w, P(out))
+    return out
+
+
+def round16(x):
+    return _mlp_lib().ora_round16(float(x))

@@ -17,7 +17,7 @@ the parallel reformulation against each other.  Pure Python: small inputs.
 
 
 def capped_w(qlen, mat, o_del, e_del, o_ins, e_ins, w, end_bonus):
-    mx = max(0, max(mat))
+    mx = max(0, max(int(v) for v in mat))
 
     def as_i32(x):
         if x != x:

@@ -163,7 +163,7 @@ def main():
     db.set_streams(1)
     plain_flags = N.FA_PACK_LENGTH | N.FA_LAUNCH_SINGLE
     plain_steps = max(2, args.steps // 2)
-    ms_plain, _, _ = timed(db, plain_flags, plain_steps, 1)
+    ms_plain, ms_plain_ev, _ = timed(db, plain_flags, plain_steps, 1)
     out_plain, cells = db.download()
     log("plain single-launch: %.2f ms/step" % ms_plain)
 
@@ -187,7 +187,7 @@ def main():
 
     # ablation: same launch configuration, length-sorted instead of learned
     lflags = (flags & ~0xF0) | N.FA_PACK_LENGTH
-    ms_len, _, _ = timed(db, lflags, max(2, args.steps // 2), 1)
+    ms_len, ms_len_ev, _ = timed(db, lflags, max(2, args.steps // 2), 1)
     out_len, cells_len = db.download()
 
     dg = digest_of(out, cells2)
@@ -202,6 +202,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "ms_per_step_hip_events": round(ms_ev, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -213,6 +214,8 @@ def main():
                    + " + hipGraph", "launches_per_step": nlaunch},
         "gcups": round(total_cells / (ms / 1e3) / 1e9, 2),
         "plain_pairs_per_s": round(ps.n / (ms_plain / 1e3), 1),
+        "plain_ms_per_step": round(ms_plain, 4),
+        "length_sorted_ms_per_step": round(ms_len, 4),
         "plain_gcups": round(total_cells / (ms_plain / 1e3) / 1e9, 2),
         "speedup_vs_plain": round(ms_plain / ms, 3),
         "length_sorted_pairs_per_s": round(ps.n / (ms_len / 1e3), 1),

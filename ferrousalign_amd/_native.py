@@ -37,6 +37,7 @@ FA_LAUNCH_PLAIN = 0x000
 FA_LAUNCH_PERSISTENT = 0x100
 FA_LAUNCH_GRAPH = 0x200
 FA_LAUNCH_SINGLE = 0x400
+FA_RUN_ALIGN2 = 0x1000
 
 
 class FaError(RuntimeError):
@@ -88,6 +89,10 @@ def _declare(lib):
     lib.fa_batch_set_launch.argtypes = [_P, _P, _I32, _I32, _I32]
     lib.fa_batch_run.argtypes = [_P, _P, _U32]
     lib.fa_batch_set_streams.argtypes = [_P, _P, _I32]
+    lib.fa_batch_set_align2.argtypes = [_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _U32]
+    lib.fa_batch_download_align2.argtypes = [_P, _P, _P, _P]
+    lib.fa_ksw_align2_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32, _U32,
+                                        _P, _P]
     lib.fa_batch_time.argtypes = [_P, _P, _U32, _I32, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I32)]
     lib.fa_batch_download.argtypes = [_P, _P, _P, _P]
@@ -99,7 +104,8 @@ def _declare(lib):
     lib.fa_simd_banded_swa_batch_soa.argtypes = soa
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
     for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_execute_batch_simd_scoring",
-                 "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run",
+                 "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
+                 "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",
                  "fa_simd_banded_swa_batch_soa", "fa_simd_banded_swa_batch_int16_soa"):
         getattr(lib, name).restype = ctypes.c_int

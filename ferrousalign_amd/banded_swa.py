@@ -343,13 +343,28 @@ class DeviceBatch:
     def run(self, flags=0):
         N.check(N.lib().fa_batch_run(self.ctx.handle, self._h, flags), "fa_batch_run")
 
-    def time(self, flags=0, reps=10):
+    def time(self, flags=0, reps=10, dominant=False):
+        """(ms per run over `reps` back-to-back runs from HIP events, longest
+        single launch in ms or None, launches per run)."""
         ms = ctypes.c_double()
         dom = ctypes.c_double()
         nl = ctypes.c_int32()
-        N.check(N.lib().fa_batch_time(self.ctx.handle, self._h, flags, reps, ctypes.byref(ms), ctypes.byref(dom),
-                                      ctypes.byref(nl)), "fa_batch_time")
-        return ms.value, dom.value, nl.value
+        N.check(N.lib().fa_batch_time(self.ctx.handle, self._h, flags, reps, ctypes.byref(ms),
+                                      ctypes.byref(dom) if dominant else None, ctypes.byref(nl)), "fa_batch_time")
+        return ms.value, (dom.value if dominant else None), nl.value
+
+    def set_align2(self, mat, o_del, e_del, o_ins, e_ins, xtra, m=5):
+        """Attach mate-rescue local SW parameters (run with FA_RUN_ALIGN2)."""
+        mat = np.ascontiguousarray(mat, np.int8)
+        N.check(N.lib().fa_batch_set_align2(self.ctx.handle, self._h, N.ptr(mat), m, o_del, e_del, o_ins, e_ins,
+                                            xtra), "fa_batch_set_align2")
+
+    def download_align2(self):
+        out = np.zeros((self.n, 7), np.int32)
+        cells = np.zeros(self.n, np.uint32)
+        N.check(N.lib().fa_batch_download_align2(self.ctx.handle, self._h, N.ptr(out), N.ptr(cells)),
+                "fa_batch_download_align2")
+        return out, cells
 
     def profile_launches(self, flags=0, cap=64):
         nl = ctypes.c_int32()
@@ -390,3 +405,19 @@ class DeviceBatch:
             self.free()
         except Exception:
             pass
+
+
+def ksw_align2_batch(qseq, qoff, qlen, tseq, toff, tlen, mat, o_del, e_del, o_ins, e_ins, xtra, m=5, ctx=None):
+    """Mate-rescue local SW for a batch: one Kswr record (score, te, qe,
+    score2, te2, tb, qb) per job, exactly ksw_align2's 16-bit mode
+    (ksw_affine_gap.rs:773-861).  Returns (out[n,7], cells[n])."""
+    ctx = ctx or default_context()
+    n = len(qlen)
+    arrs = [np.ascontiguousarray(x, dt) for x, dt in ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (tseq, np.uint8), (toff, np.int64), (tlen, np.int32))]
+    mat = np.ascontiguousarray(mat, np.int8)
+    out = np.zeros((n, 7), np.int32)
+    cells = np.zeros(n, np.uint32)
+    N.check(N.lib().fa_ksw_align2_batch(ctx.handle, n, *[N.ptr(a) for a in arrs], m, N.ptr(mat), o_del, e_del, o_ins,
+                                        e_ins, xtra, N.ptr(out), N.ptr(cells)), "fa_ksw_align2_batch")
+    return out, cells

@@ -17,6 +17,7 @@
 #include "../../include/ferrousalign_amd.h"
 #include "extend_kernel.h"
 #include "predictor.h"
+#include "mate_kernel.h"
 
 struct fa_ctx {
     int device = 0;
@@ -64,6 +65,13 @@ struct fa_batch {
     int waves_per_cu = 0;
     int streams = 4;
     int tmax = 0;
+    // mate-rescue local SW (FA_RUN_ALIGN2)
+    bool align2_set = false;
+    fa::MateArgs margs{};
+    int32_t *d_mout = nullptr;     // 7 int32 per pair
+    uint16_t *d_imax_log = nullptr;
+    uint32_t log_slots = 0;
+    std::vector<Bucket> mbuckets;
     float *d_feat = nullptr;       // predictor features [n][6]
     float *d_pred = nullptr;       // predicted cells [n]
     hipGraphExec_t graph_exec = nullptr;
@@ -190,8 +198,10 @@ void drop_graph(fa_batch *b) {
 void batch_release(fa_batch *b) {
     if (!b) return;
     drop_graph(b);
+    if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
-                    b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred};
+                    b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
+                    b->d_mout, b->d_imax_log};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -313,7 +323,24 @@ int enqueue_run(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     return FA_OK;
 }
 
+constexpr uint32_t kMateChunk = 262144;  // pairs per mate launch (row-log scratch reuse)
+
+int run_align2(fa_ctx *ctx, fa_batch *b, int *nlaunch) {
+    if (!b->align2_set) return FA_ERR_ARG;
+    int launches = 0;
+    for (const Bucket &bk : b->mbuckets) {
+        for (uint32_t off = 0; off < bk.count; off += kMateChunk) {
+            const uint32_t cnt = std::min<uint32_t>(kMateChunk, bk.count - off);
+            FA_HIP(fa::mate_launch(bk.S, b->margs, bk.first + off, cnt, ctx->stream));
+            ++launches;
+        }
+    }
+    if (nlaunch) *nlaunch = launches;
+    return FA_OK;
+}
+
 int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
+    if (flags & FA_RUN_ALIGN2) return run_align2(ctx, b, nlaunch);
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
     b->args.chunk = std::max(1, b->chunk / (64 / b->lanes));
@@ -591,7 +618,7 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
                   double *ms_dominant_launch, int32_t *launches_per_run) {
     if (!ctx || !b || reps <= 0) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
-    int rc = plan_packing(ctx, b, flags);
+    int rc = (flags & FA_RUN_ALIGN2) ? FA_OK : plan_packing(ctx, b, flags);
     if (rc) return rc;
     FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     int nl = 0;
@@ -605,7 +632,9 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
     FA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (ms_per_run) *ms_per_run = ms / reps;
     if (launches_per_run) *launches_per_run = nl;
-    if (ms_dominant_launch) {
+    if (ms_dominant_launch && (flags & FA_RUN_ALIGN2)) {
+        *ms_dominant_launch = ms / reps;
+    } else if (ms_dominant_launch) {
         int32_t nl2 = 0;
         double ms_l[64];
         rc = fa_batch_profile_launches(ctx, b, flags, 64, &nl2, nullptr, nullptr, nullptr, ms_l);
@@ -622,6 +651,30 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
                               double *ms) {
     if (!ctx || !b || !nlaunch || cap < 0) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
+    if (flags & FA_RUN_ALIGN2) {
+        if (!b->align2_set) return FA_ERR_ARG;
+        int k = 0;
+        for (const Bucket &bk : b->mbuckets) {
+            for (uint32_t off = 0; off < bk.count; off += kMateChunk) {
+                const uint32_t cnt = std::min<uint32_t>(kMateChunk, bk.count - off);
+                FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+                FA_HIP(fa::mate_launch(bk.S, b->margs, bk.first + off, cnt, ctx->stream));
+                FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+                FA_HIP(hipEventSynchronize(ctx->ev1));
+                float t = 0.f;
+                FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+                if (k < cap) {
+                    if (slots) slots[k] = bk.S;
+                    if (first) first[k] = bk.first + off;
+                    if (count) count[k] = cnt;
+                    if (ms) ms[k] = t;
+                }
+                ++k;
+            }
+        }
+        *nlaunch = k;
+        return FA_OK;
+    }
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
     int k = 0;
@@ -650,6 +703,13 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
 int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order) {
     if (!ctx || !b || (!order && b->n)) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
+    if (flags & FA_RUN_ALIGN2) {
+        if (!b->align2_set) return FA_ERR_ARG;
+        if (b->n)
+            FA_HIP(hipMemcpy(order, b->margs.order, (size_t)b->n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+        return FA_OK;
+    }
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
     if (b->n)
@@ -837,6 +897,123 @@ int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t
     }
     return soa_common(ctx, lanes, num_jobs, qlen, tlen, h0, w, o_del, e_del, o_ins, e_ins, zdrop,
                       end_bonus, mat, m, out, q, t);
+}
+
+// ---- mate-rescue local SW (ksw_align2, 16-bit mode) --------------------------------
+
+int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, int32_t o_del,
+                        int32_t e_del, int32_t o_ins, int32_t e_ins, uint32_t xtra) {
+    if (!ctx || !b || !mat) return FA_ERR_ARG;
+    if (m != 5) return FA_ERR_SCORING;
+    if (xtra & fa::kKswXByte) return FA_ERR_SCORING;  // 8-bit mode is not provided on the GPU
+    if (o_del < 0 || e_del < 0 || o_ins < 0 || e_ins < 0 || o_del > 4096 || o_ins > 4096 ||
+        e_del > 4096 || e_ins > 4096)
+        return FA_ERR_SCORING;
+    for (int q = 0; q < 5; ++q)
+        if (mat[20 + q] != 0) return FA_ERR_SCORING;  // the N target row must score 0
+    int qmax = 0;
+    for (int k = 0; k < 25; ++k) qmax = std::max(qmax, (int)mat[k]);
+    if (qmax < 1) return FA_ERR_SCORING;
+    if (b->tmax > 2032) return FA_ERR_TLEN;
+    FA_HIP(hipSetDevice(ctx->device));
+    const int64_t n = b->n;
+    // bucket by segments per lane; longest first
+    std::vector<int> sk((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+        sk[(size_t)k] = fa::mate_slen_bucket(b->qlen[(size_t)k]);
+        if (sk[(size_t)k] < 0) return FA_ERR_QLEN;
+    }
+    std::vector<uint32_t> order((size_t)n);
+    for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return sk[x] > sk[y]; });
+    std::vector<Bucket> bk;
+    for (int64_t k = 0; k < n;) {
+        int S = sk[order[(size_t)k]];
+        int64_t e = k;
+        while (e < n && sk[order[(size_t)e]] == S) ++e;
+        bk.push_back({S, (uint32_t)k, (uint32_t)(e - k), (double)(e - k)});
+        k = e;
+    }
+    int rc;
+    if (!b->d_mout && (rc = dev_alloc(&b->d_mout, (size_t)n * 7))) return rc;
+    const uint32_t stride = (uint32_t)std::max(16, ((b->tmax + 15) / 16) * 16);
+    const uint32_t slots = (uint32_t)std::min<int64_t>(n, kMateChunk);
+    if (b->log_slots < slots || !b->d_imax_log) {
+        if (b->d_imax_log) (void)hipFree(b->d_imax_log);
+        b->d_imax_log = nullptr;
+        if ((rc = dev_alloc(&b->d_imax_log, (size_t)slots * stride + 64))) return rc;
+        b->log_slots = slots;
+    }
+    // the order buffer of the extension path is reused: the mate order is its own upload
+    uint32_t *d_morder = nullptr;
+    if ((rc = dev_upload(&d_morder, order.data(), order.size(), ctx->stream))) return rc;
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
+    fa::MateArgs &a = b->margs;
+    a.seq2 = b->d_seq2;
+    a.seqn = b->d_seqn;
+    a.qo = b->d_qo;
+    a.to = b->d_to;
+    a.qlen = b->d_qlen;
+    a.tlen = b->d_tlen;
+    a.order = d_morder;
+    a.out = b->d_mout;
+    a.cells = b->d_cells;
+    a.imax_log = b->d_imax_log;
+    a.log_stride = stride;
+    for (int q = 0; q < 5; ++q) {
+        uint32_t v = 0;
+        for (int t = 0; t < 4; ++t) v |= (uint32_t)(uint8_t)(mat[t * 5 + q] + 128) << (8 * t);
+        a.profw[q] = v;
+    }
+    a.o_del = o_del;
+    a.e_del = e_del;
+    a.o_ins = o_ins;
+    a.e_ins = e_ins;
+    a.xtra = xtra;
+    a.qmax = qmax;
+    a.tpad = ((b->tmax + 16) / 16) * 16;
+    b->mbuckets.swap(bk);
+    b->align2_set = true;
+    return FA_OK;
+}
+
+int fa_batch_download_align2(fa_ctx *ctx, fa_batch *b, fa_kswr *out, uint32_t *cells) {
+    if (!ctx || !b || !b->align2_set) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    static_assert(sizeof(fa_kswr) == 28, "Kswr layout");
+    if (out && b->n)
+        FA_HIP(hipMemcpyAsync(out, b->d_mout, (size_t)b->n * sizeof(fa_kswr), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    if (cells && b->n)
+        FA_HIP(hipMemcpyAsync(cells, b->d_cells, (size_t)b->n * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    return FA_OK;
+}
+
+int fa_ksw_align2_batch(fa_ctx *ctx, int64_t n, const uint8_t *query_seqs,
+                        const int64_t *query_offset, const int32_t *query_len,
+                        const uint8_t *target_seqs, const int64_t *target_offset,
+                        const int32_t *target_len, int32_t m, const int8_t *mat, int32_t o_del,
+                        int32_t e_del, int32_t o_ins, int32_t e_ins, uint32_t xtra, fa_kswr *out,
+                        uint32_t *cells) {
+    if (!ctx || n < 0 || (n > 0 && !out)) return FA_ERR_ARG;
+    if (n == 0) return FA_OK;
+    fa_banded_pairwise_sw sw{};
+    sw.m = 5;
+    sw.e_del = sw.e_ins = 1;
+    std::vector<int32_t> ones((size_t)n, 1), zeros((size_t)n, 0);
+    fa_batch *b = nullptr;
+    int rc = fa_batch_upload(ctx, &sw, n, query_seqs, query_offset, query_len, target_seqs,
+                             target_offset, target_len, ones.data(), zeros.data(), &b);
+    if (rc) return rc;
+    rc = fa_batch_set_align2(ctx, b, mat, m, o_del, e_del, o_ins, e_ins, xtra);
+    if (!rc) rc = fa_batch_run(ctx, b, FA_RUN_ALIGN2);
+    if (!rc) rc = fa_batch_download_align2(ctx, b, out, cells);
+    fa_batch_free(ctx, b);
+    return rc;
 }
 
 // ---- learned work predictor ---------------------------------------------------

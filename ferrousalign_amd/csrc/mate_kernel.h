@@ -1,0 +1,40 @@
+// Mate-rescue local SW kernel (ksw_align2, 16-bit mode): argument block and
+// launcher (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fa {
+
+constexpr uint32_t kKswXByte = 0x10000u;
+constexpr uint32_t kKswXStop = 0x20000u;
+constexpr uint32_t kKswXSubo = 0x40000u;
+constexpr uint32_t kKswXStart = 0x80000u;
+
+struct MateArgs {
+    const uint32_t *seq2;
+    const uint32_t *seqn;
+    const uint32_t *qo;
+    const uint32_t *to;
+    const int32_t *qlen;
+    const int32_t *tlen;
+    const uint32_t *order;
+    int32_t *out;         // 7 int32 per pair: score, te, qe, score2, te2, tb, qb
+    uint32_t *cells;
+    uint16_t *imax_log;   // per launch slot: log_stride row maxima
+    uint32_t log_stride;
+    uint32_t profw[5];    // per query code q: bytes mat[b*5+q]+128 for target b = 0..3
+    int32_t o_del, e_del, o_ins, e_ins;
+    uint32_t xtra;
+    int32_t qmax;         // largest matrix entry (>= 1)
+    int32_t tpad;         // LDS bytes of staged target per pair (multiple of 16)
+};
+
+// Slot bucket (segments per lane) for a query length, or -1.
+int mate_slen_bucket(int qlen);
+// Launch over packed positions [first, first+count): 32 pairs per block;
+// imax_log must hold count * log_stride entries.
+hipError_t mate_launch(int SLEN, const MateArgs &a, uint32_t first, uint32_t count,
+                       hipStream_t st);
+
+}  // namespace fa

@@ -26,6 +26,16 @@ WORKLOADS = {
     "uneven_mix_train_16m": (2, 16777216, 0xFA03),  # predictor training pairs, same model, other seed
 }
 
+MATE_SEED = 0xFA04
+MATE_PAIRS = 2097152
+MATE_QLEN = 151
+MATE_TLEN = 1500
+# Mate-rescue scoring (src/pipelines/linear/paired/mate_rescue.rs:116-122): match 1,
+# mismatch -4, N scores 0; gaps 6/1 (:642-645); xtra = KSW_XSUBO|KSW_XSTART|20.
+MATE_MATRIX = np.array([1, -4, -4, -4, 0, -4, 1, -4, -4, 0, -4, -4, 1, -4, 0, -4, -4, -4, 1, 0,
+                        0, 0, 0, 0, 0], np.int8)
+MATE_XTRA = 0x40000 | 0x80000 | 20
+
 
 def default_scoring():
     return BandedPairWiseSW(6, 1, 6, 1, 100, 5, 5, 5, bwa_fill_scmat(1, 4, -1))
@@ -86,3 +96,35 @@ def extension_pairs(kind, n, seed, w1=100, w2=200, threads=0):
 def workload(name, n=None, threads=0):
     kind, count, seed = WORKLOADS[name]
     return extension_pairs(kind, count if n is None else n, seed, threads=threads)
+
+
+@dataclass
+class MatePairSet:
+    q: np.ndarray          # [n, 151]
+    t: np.ndarray          # [n, 1500]
+    has_mate: np.ndarray
+
+    @property
+    def n(self):
+        return len(self.has_mate)
+
+    def flat(self):
+        n = self.n
+        return (self.q.reshape(-1), np.arange(n, dtype=np.int64) * MATE_QLEN, np.full(n, MATE_QLEN, np.int32),
+                self.t.reshape(-1), np.arange(n, dtype=np.int64) * MATE_TLEN, np.full(n, MATE_TLEN, np.int32))
+
+    def subset(self, idx):
+        return MatePairSet(self.q[idx], self.t[idx], self.has_mate[idx])
+
+
+def mate_pairs(n=MATE_PAIRS, seed=MATE_SEED, threads=0):
+    """Mate-rescue workload: 151 bp queries against 1,500 bp windows."""
+    ref = reference()
+    q = np.zeros((n, MATE_QLEN), np.uint8)
+    t = np.zeros((n, MATE_TLEN), np.uint8)
+    hm = np.zeros(n, np.uint8)
+    rc = N.synth().fa_synth_mate_pairs(ctypes.c_uint64(seed), N.ptr(ref), ref.size, n, N.ptr(q), N.ptr(t),
+                                       N.ptr(hm), threads)
+    if rc != 0:
+        raise ValueError("fa_synth_mate_pairs failed")
+    return MatePairSet(q, t, hm)

@@ -52,6 +52,7 @@ extern "C" {
 #define FA_LAUNCH_PLAIN 0x000u  /* one launch per query-length bucket */
 #define FA_LAUNCH_PERSISTENT 0x100u /* persistent waves pulling chunks from a device queue */
 #define FA_LAUNCH_GRAPH 0x200u  /* replay the launch sequence as a captured hipGraph */
+#define FA_RUN_ALIGN2 0x1000u  /* run/time the batch's mate-rescue local SW (fa_batch_set_align2) */
 #define FA_LAUNCH_SINGLE 0x400u /* one launch for the whole batch at the slot count of its
                                    longest query (no query-length buckets) */
 
@@ -186,6 +187,31 @@ int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cel
  * run. */
 int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count);
 int fa_batch_free(fa_ctx *ctx, fa_batch *b);
+
+/* ---- mate-rescue local Smith-Waterman (ksw_align2, 16-bit mode) --------- */
+/* Kswr, src/core/alignment/ksw_affine_gap.rs:51-61. */
+typedef struct {
+    int32_t score, te, qe, score2, te2, tb, qb;
+} fa_kswr;
+/* Replaces the per-job ksw_align2 call of execute_mate_rescue_batch_scalar
+ * (src/pipelines/linear/paired/mate_rescue.rs:640-685; ksw_affine_gap.rs:
+ * 773-861) for a whole batch: out[k] is exactly what ksw_align2 with the
+ * reference's 8-lane SSE engine returns for job k, including score2/te2 and
+ * the XSTART reverse pass.  16-bit mode only (xtra without KSW_XBYTE); the
+ * N row of mat must score 0 (true of the mate-rescue matrix,
+ * mate_rescue.rs:116-122); 1 <= qlen <= 256, tlen <= 2032.  cells[k]
+ * (optional): target rows run times query length over both passes. */
+int fa_ksw_align2_batch(fa_ctx *ctx, int64_t n, const uint8_t *query_seqs,
+                        const int64_t *query_offset, const int32_t *query_len,
+                        const uint8_t *target_seqs, const int64_t *target_offset,
+                        const int32_t *target_len, int32_t m, const int8_t *mat, int32_t o_del,
+                        int32_t e_del, int32_t o_ins, int32_t e_ins, uint32_t xtra, fa_kswr *out,
+                        uint32_t *cells);
+/* Device-resident form: attach mate-rescue parameters to an uploaded batch,
+ * then fa_batch_run / fa_batch_time with FA_RUN_ALIGN2. */
+int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, int32_t o_del,
+                        int32_t e_del, int32_t o_ins, int32_t e_ins, uint32_t xtra);
+int fa_batch_download_align2(fa_ctx *ctx, fa_batch *b, fa_kswr *out, uint32_t *cells);
 
 /* ---- learned work predictor (no counterpart in the reference) ------------ */
 /* A 6-32-32-1 MLP (FP16 storage, FP32 accumulation) predicting log(1 + DP

@@ -117,3 +117,42 @@ def pair_features(q, t, h0, w):
 
 def round16(x):
     return _mlp_lib().ora_round16(float(x))
+
+
+def _align_lib():
+    l = lib()
+    if not getattr(l, "_align", False):
+        l.ora_ksw_align2_i16.argtypes = [_I32, _P, _I32, _P, _I32, _P, _I32, _I32, _I32, _I32, ctypes.c_uint32, _P, _P]
+        l.ora_ksw_align2_i16.restype = ctypes.c_int
+        l.ora_ksw_align2_batch.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32,
+                                           ctypes.c_uint32, _P, _P, _P, ctypes.c_int]
+        l.ora_ksw_align2_batch.restype = None
+        l._align = True
+    return l
+
+
+KSW_XSTOP, KSW_XSUBO, KSW_XSTART, KSW_XBYTE = 0x20000, 0x40000, 0x80000, 0x10000
+
+
+def ksw_align2_i16(query, target, mat, o_del, e_del, o_ins, e_ins, xtra):
+    qb = np.concatenate([np.asarray(query, np.uint8), np.zeros(1, np.uint8)])
+    tb = np.concatenate([np.asarray(target, np.uint8), np.zeros(1, np.uint8)])
+    mat = np.ascontiguousarray(mat, np.int8)
+    out = np.zeros(7, np.int32)
+    cells = np.zeros(1, np.uint64)
+    rc = _align_lib().ora_ksw_align2_i16(len(qb) - 1, P(qb), len(tb) - 1, P(tb), 5, P(mat), o_del, e_del, o_ins,
+                                         e_ins, xtra, P(out), P(cells))
+    return rc, tuple(int(x) for x in out), int(cells[0])
+
+
+def ksw_align2_batch(q, qoff, qlen, t, toff, tlen, mat, o_del, e_del, o_ins, e_ins, xtra, nthreads=0):
+    n = len(qlen)
+    arrs = [np.ascontiguousarray(x, dt) for x, dt in ((q, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (t, np.uint8), (toff, np.int64), (tlen, np.int32))]
+    mat = np.ascontiguousarray(mat, np.int8)
+    out = np.zeros((n, 7), np.int32)
+    cells = np.zeros(n, np.uint64)
+    rc = np.zeros(n, np.int32)
+    _align_lib().ora_ksw_align2_batch(n, *[P(a) for a in arrs], 5, P(mat), o_del, e_del, o_ins, e_ins, xtra,
+                                      P(out), P(cells), P(rc), nthreads)
+    return out, cells, rc

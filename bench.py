@@ -91,12 +91,64 @@ def digest_of(out, cells):
     return hashlib.sha256(out.tobytes() + cells.astype(np.uint32).tobytes()).hexdigest()
 
 
+def mate_bench(args, ctx, golden):
+    """Mate-rescue local SW (ksw_align2, 16-bit mode) on 2,097,152 pairs of a
+    151 bp query against a 1,500 bp window."""
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd.banded_swa import DeviceBatch
+    from tests import _oracle as O
+    ps = synth.mate_pairs()
+    n = ps.n
+    db = DeviceBatch(synth.default_scoring(), *ps.flat(), np.ones(n, np.int32), np.zeros(n, np.int32), ctx=ctx)
+    db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA)
+    flags = N.FA_RUN_ALIGN2
+    steps = max(1, min(args.steps, 3))
+    ms, ms_ev, nl = timed(db, flags, steps, 1)
+    out, cells = db.download_align2()
+    total = float(cells.astype(np.float64).sum())
+    from tests.golden.make_golden_mate import digest
+    gm = {}
+    gpath = os.path.join(ROOT, "tests", "golden", "mate_digests.json")
+    if os.path.exists(gpath):
+        gm = json.load(open(gpath))
+    key = "mate_%d" % n
+    exact = (digest(out, cells) == gm[key]) if key in gm else None
+    launches = db.profile_launches(flags)
+    dom = max(launches, key=lambda x: x["ms"])
+    dom_cells = float(cells[dom["first"]:dom["first"] + dom["count"]].astype(np.float64).sum()) \
+        if len(launches) == 1 else total * dom["count"] / n
+    tops = dom_cells * OPS_PER_CELL / (dom["ms"] / 1e3) / 1e12
+    res = {"metric": "mate-rescue pairs/s (ksw_align2 16-bit, bit-exact)", "value": round(n / (ms / 1e3), 1),
+           "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
+           "ms_per_step_hip_events": round(ms_ev, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int16 (saturating, in int32 lanes)", "data": "synthetic",
+           "config": {"workload": "mate_rescue_2m", "pairs": n, "qlen": 151, "tlen": 1500, "xtra": hex(synth.MATE_XTRA),
+                      "launches_per_step": nl},
+           "gcups": round(total / (ms / 1e3) / 1e9, 2), "bit_exact_vs_golden": exact,
+           "roofline": {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_INT32_TOPS, 1), "unit": "TOPS",
+                        "frac": round(tops / VALU_INT32_TOPS, 4), "traffic": None, "kernel": "mate_kernel<19>",
+                        "launch_ms": round(dom["ms"], 3), "ops_per_cell": OPS_PER_CELL}}
+    if not args.no_cpu:
+        threads = os.cpu_count() or 1
+        m = 16384
+        sub = ps.subset(np.arange(m))
+        t0 = time.perf_counter()
+        _, oc, _ = O.ksw_align2_batch(*sub.flat(), synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA, nthreads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "pairs/s", "gcups": round(float(oc.sum()) / dt / 1e9, 3),
+                               "cores": threads, "kind": "port",
+                               "sample": "first %d pairs through the C restatement of ksw_align2 16-bit "
+                                         "(ksw_affine_gap.rs:503-861) on %d host threads: %.1f s" % (m, threads, dt)}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="uneven_mix_4m", choices=["uneven_mix_4m", "illumina_150bp_1m"])
+    ap.add_argument("--workload", default="uneven_mix_4m",
+                    choices=["uneven_mix_4m", "illumina_150bp_1m", "mate_rescue_2m"])
     ap.add_argument("--train-pairs", type=int, default=16777216)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -117,6 +169,10 @@ def main():
     gpath = os.path.join(ROOT, "tests", "golden", "digests.json")
     if os.path.exists(gpath):
         golden = json.load(open(gpath))
+
+    if args.workload == "mate_rescue_2m":
+        mate_bench(args, ctx, golden)
+        return
 
     if args.workload == "illumina_150bp_1m":
         ps = synth.workload(args.workload)

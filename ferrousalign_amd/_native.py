@@ -30,6 +30,7 @@ FA_MAX_TLEN = 2048
 FA_SCORE_I16 = 0x0
 FA_SCORE_U8 = 0x1
 FA_SCORE_AUTO = 0x2
+FA_SCORE_I32 = 0x4
 FA_PACK_LENGTH = 0x00
 FA_PACK_NONE = 0x10
 FA_PACK_COST = 0x20

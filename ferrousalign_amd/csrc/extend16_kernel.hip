@@ -1,0 +1,467 @@
+// 16-bit banded seed extension for gfx950: two pairs per lane group.
+//
+// Same exact row-parallel formulation as extend_kernel.hip (ksw_extend2,
+// src/core/alignment/ksw_affine_gap.rs:952-1169), but every column register
+// holds two 16-bit values: pair A in the low half, pair B in the high half,
+// and the arithmetic runs on packed v_pk_* instructions, so one lane group
+// of L lanes advances two pairs per instruction.  Band tests are sign bits of
+// packed differences (column - bound), selects are v_bfi on those masks.
+//
+// 16-bit range: the host routes a pair here only when every value provably
+// fits: h0 + qlen*max(mat) < 8192 and (qlen+1)*e_ins, o+e < 8192, so H, E,
+// F, G stay in (-32768, 16384) (see extend16_eligible); other pairs run the
+// 32-bit kernel.
+//
+// Differences from the 32-bit kernel that keep results identical:
+//  * M = H ? H + s : 0 becomes min(H + s, min(H, 127) << 8): equal when H > 0,
+//    and <= 0 when H == 0, which every later use (max with E >= 0, max with
+//    0) treats like 0;
+//  * the F prefix identity is -16384 (F(beg) stays below E >= 0 without
+//    wrapping);
+//  * E' = max(E -sat e_del, M - oe_del): E >= 0, so the unsigned saturating
+//    subtract equals max(E - e_del, 0).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "extend_kernel.h"
+
+namespace fa {
+
+namespace {
+
+typedef short s2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t bits(s2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t bits(u2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s2 vs(uint32_t v) { return __builtin_bit_cast(s2, v); }
+__device__ __forceinline__ u2 vu(uint32_t v) { return __builtin_bit_cast(u2, v); }
+
+__device__ __forceinline__ uint32_t pk(int lo, int hi) {
+    return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
+}
+__device__ __forceinline__ int lo16(uint32_t x) { return (int)(short)(x & 0xFFFFu); }
+__device__ __forceinline__ int hi16(uint32_t x) { return (int)(short)(x >> 16); }
+__device__ __forceinline__ uint32_t smax(uint32_t a, uint32_t b) {
+    return bits(__builtin_elementwise_max(vs(a), vs(b)));
+}
+__device__ __forceinline__ uint32_t smin(uint32_t a, uint32_t b) {
+    return bits(__builtin_elementwise_min(vs(a), vs(b)));
+}
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) {
+    return bits(__builtin_elementwise_max(vu(a), vu(b)));
+}
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) {
+    return bits(__builtin_elementwise_min(vu(a), vu(b)));
+}
+__device__ __forceinline__ uint32_t add16(uint32_t a, uint32_t b) { return bits(vu(a) + vu(b)); }
+__device__ __forceinline__ uint32_t sub16(uint32_t a, uint32_t b) { return bits(vu(a) - vu(b)); }
+__device__ __forceinline__ uint32_t mul16(uint32_t a, uint32_t b) { return bits(vu(a) * vu(b)); }
+__device__ __forceinline__ uint32_t subsat(uint32_t a, uint32_t b) {
+    return bits(__builtin_elementwise_sub_sat(vu(a), vu(b)));
+}
+__device__ __forceinline__ uint32_t shl8(uint32_t a) { return bits(vu(a) << (u2){8, 8}); }
+// Sign of each 16-bit half spread over the half (0xFFFF if negative).  Kept
+// as one VALU instruction: written in C, the compiler rewrites the
+// mask-and-select idiom into per-half compares and selects.
+__device__ __forceinline__ uint32_t sgnmask(uint32_t d) {
+    uint32_t r;
+    asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(r) : "v"(d));
+    return r;
+}
+// 0xFFFF in each half where a < b (|a - b| < 32768)
+__device__ __forceinline__ uint32_t ltmask(uint32_t a, uint32_t b) { return sgnmask(sub16(a, b)); }
+__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) {
+    return (m & a) | (~m & b);
+}
+
+constexpr uint32_t kNeg2 = 0xC000C000u;  // (-16384, -16384)
+
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
+constexpr int kRowShr = 0x110;
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xF, 0xF, false);
+}
+template <int L>
+__device__ __forceinline__ int allmax_i(int x) {
+    x = max(x, (int)dpp<kQuadXor1>((uint32_t)x, (uint32_t)x));
+    if constexpr (L >= 4) x = max(x, (int)dpp<kQuadXor2>((uint32_t)x, (uint32_t)x));
+    if constexpr (L >= 8) x = max(x, (int)dpp<kRowHalfMirror>((uint32_t)x, (uint32_t)x));
+    if constexpr (L >= 16) x = max(x, (int)dpp<kRowMirror>((uint32_t)x, (uint32_t)x));
+    return x;
+}
+template <int L>
+__device__ __forceinline__ uint32_t allmax_u2(uint32_t x) {
+    x = umax(x, dpp<kQuadXor1>(x, x));
+    if constexpr (L >= 4) x = umax(x, dpp<kQuadXor2>(x, x));
+    if constexpr (L >= 8) x = umax(x, dpp<kRowHalfMirror>(x, x));
+    if constexpr (L >= 16) x = umax(x, dpp<kRowMirror>(x, x));
+    return x;
+}
+// y from lane l-K of the group for a max-scan step: a lane without such a
+// source gets its own value back, which leaves the maximum unchanged
+template <int L, int K>
+__device__ __forceinline__ uint32_t shr_self(uint32_t y, int l) {
+    uint32_t z = dpp<kRowShr + K>(y, y);
+    if constexpr (L < 16) z = (l >= K) ? z : y;
+    return z;
+}
+// exclusive packed signed max-scan over the L lanes of a group
+template <int L>
+__device__ __forceinline__ uint32_t excl_scan_s2(uint32_t x, int l) {
+    uint32_t y = dpp<kRowShr + 1>(kNeg2, x);
+    y = (l >= 1) ? y : kNeg2;
+    if constexpr (L > 1) y = smax(y, shr_self<L, 1>(y, l));
+    if constexpr (L > 2) y = smax(y, shr_self<L, 2>(y, l));
+    if constexpr (L > 4) y = smax(y, shr_self<L, 4>(y, l));
+    if constexpr (L > 8) y = smax(y, shr_self<L, 8>(y, l));
+    return y;
+}
+
+__device__ __forceinline__ int decode_base(const uint32_t *seq2, const uint32_t *seqn,
+                                           uint32_t off, int pos) {
+    const uint32_t w2 = seq2[off + (pos >> 4)];
+    const uint32_t wn = seqn[(off >> 1) + (pos >> 5)];
+    const int code = (int)((w2 >> ((pos & 15) * 2)) & 3u);
+    return ((wn >> (pos & 31)) & 1u) ? 4 : code;
+}
+
+__device__ __forceinline__ void stage_target(const ExtArgs &a, uint8_t *tcodes, int tlen,
+                                             uint32_t to, int l, int L) {
+    const int nw = (tlen + 15) >> 4;
+    for (int k = l; k < nw; k += L) {
+        const uint32_t w2 = a.seq2[to + k];
+        const uint32_t wn = (a.seqn[(to >> 1) + (k >> 1)] >> ((k & 1) * 16)) & 0xFFFFu;
+        uint32_t v[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const int p = b * 4 + y;
+                uint32_t c = (w2 >> (p * 2)) & 3u;
+                c = ((wn >> p) & 1u) ? 4u : c;
+                x |= c << (y * 8);
+            }
+            v[b] = x;
+        }
+        uint32_t *dst = reinterpret_cast<uint32_t *>(tcodes + k * 16);
+        dst[0] = v[0];
+        dst[1] = v[1];
+        dst[2] = v[2];
+        dst[3] = v[3];
+    }
+}
+
+// Per-pair scalar state of the reference's row loop (replicated in every
+// lane of the group).
+struct PairState {
+    bool valid, alive;
+    uint32_t pid;
+    int qlen, tlen, h0, w;
+    int beg, end, best, max_i, max_j, max_ie, gscore, max_off;
+    uint32_t ncell;
+};
+
+// End of row i for one pair (:1110-1158), written with selects only so that
+// the lane groups of a wavefront never diverge.
+__device__ __forceinline__ void pair_row_end(const ExtArgs &a, PairState &P, int i, int rb, int re,
+                                             int key, int hend, int fenc, int lastp1) {
+    const bool alive = P.alive;
+    P.ncell += alive ? (uint32_t)max(re - rb, 0) : 0u;
+    const bool g = alive && re == P.qlen && P.gscore < hend;  // :1110-1120
+    P.gscore = g ? hend : P.gscore;
+    P.max_ie = g ? i : P.max_ie;
+    const int m_row = key >> 10;
+    const int mj = 1023 - (key & 1023);
+    const bool zero = m_row == 0;                          // :1122
+    const bool better = !zero && m_row > P.best;           // :1126-1133
+    const int di = i - P.max_i, dj = mj - P.max_j;         // :1134-1145
+    const int drop = di > dj ? P.best - m_row - (di - dj) * a.e_del
+                             : P.best - m_row - (dj - di) * a.e_ins;
+    const bool zbrk = !zero && !better && a.zdrop > 0 && drop > a.zdrop;
+    const bool brk = zero || zbrk;
+    const bool upd = alive && better;
+    P.best = upd ? m_row : P.best;
+    P.max_i = upd ? i : P.max_i;
+    P.max_j = upd ? mj : P.max_j;
+    const int off = mj > i ? mj - i : i - mj;
+    P.max_off = upd ? max(P.max_off, off) : P.max_off;
+    const int nb = fenc ? 1023 - fenc : re;                 // :1148-1158
+    const int ne = max(nb, lastp1 - 1);
+    const bool band = alive && !brk;
+    P.beg = band ? nb : P.beg;
+    P.end = band ? min(ne + 2, P.qlen) : P.end;
+    P.alive = alive && !brk && i + 1 < P.tlen;
+}
+
+}  // namespace
+
+// One wavefront: 64/L lane groups, each running pairs order[slot0 + 2g] (low
+// halves) and order[slot0 + 2g + 1] (high halves).
+template <int L, int S>
+__device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, uint32_t nslots,
+                                              uint8_t *lds_wave) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane / L;
+    const int l = lane % L;
+    PairState P[2];
+    uint32_t qo[2];
+    uint8_t *tc[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
+        P[h].valid = slot < nslots;
+        P[h].pid = P[h].valid ? a.order[slot] : 0u;
+        P[h].qlen = P[h].valid ? a.qlen[P[h].pid] : 1;
+        P[h].tlen = P[h].valid ? a.tlen[P[h].pid] : 0;
+        P[h].h0 = P[h].valid ? a.h0[P[h].pid] : 1;
+        P[h].w = P[h].valid ? a.w[P[h].pid] : 0;
+        qo[h] = P[h].valid ? a.qo[P[h].pid] : 0u;
+        const uint32_t to = P[h].valid ? a.to[P[h].pid] : 0u;
+        tc[h] = lds_wave + (2 * g + h) * a.tpad;
+        stage_target(a, tc[h], P[h].tlen, to, l, L);
+        P[h].beg = 0;
+        P[h].end = P[h].qlen;
+        P[h].best = P[h].h0;
+        P[h].max_i = P[h].max_j = P[h].max_ie = -1;
+        P[h].gscore = -1;
+        P[h].max_off = 0;
+        P[h].ncell = 0;
+        P[h].alive = P[h].valid && P[h].tlen > 0;
+    }
+
+    const int oe_ins = a.o_ins + a.e_ins, oe_del = a.o_del + a.e_del;
+    const uint32_t einsp = pk(a.e_ins, a.e_ins), edelp = pk(a.e_del, a.e_del);
+    const uint32_t oeinsp = pk(oe_ins, oe_ins), oedelp = pk(oe_del, oe_del);
+    const uint32_t nvalp = a.nrow | (a.nrow << 16);  // biased N-row score in both halves
+    const int cb = l * S;
+    const uint32_t cbp = pk(cb, cb);
+
+    uint32_t profA[S], profB[S], Hh[S], Ee[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int c = cb + s;
+        int hv[2];
+        uint32_t pw[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int code = 4;
+            if (P[h].valid && c < P[h].qlen) code = decode_base(a.seq2, a.seqn, qo[h], c);
+            pw[h] = a.profw[code];
+            const int h0 = P[h].h0;
+            const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
+            const int x = c == 0 ? h0 : max(v1 - (c - 1) * a.e_ins, 0);  // :992-998
+            hv[h] = c <= P[h].qlen ? x : 0;
+        }
+        profA[s] = pw[0];
+        profB[s] = pw[1];
+        Hh[s] = pk(hv[0], hv[1]);
+        Ee[s] = 0u;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+
+    for (int i = 0;; ++i) {
+        if (__builtin_amdgcn_ballot_w64(P[0].alive || P[1].alive) == 0) break;
+        int rb[2], re[2], h1i[2], tb[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // rows past the end are never used; the read stays inside the pair's region
+            const int ti = i < P[h].tlen ? i : 0;
+            const int t = tc[h][ti];
+            tb[h] = P[h].alive ? t : 0;
+            rb[h] = max(P[h].beg, i - P[h].w);  // :1038-1046
+            re[h] = min(min(P[h].end, i + P[h].w + 1), P[h].qlen);
+            if (!P[h].alive) {
+                rb[h] = 4000;
+                re[h] = -3;
+            }
+            h1i[h] = rb[h] == 0 ? max(P[h].h0 - (a.o_del + a.e_del * (i + 1)), 0) : 0;  // :1049-1054
+        }
+        const uint32_t rbp = pk(rb[0], rb[1]), rep = pk(re[0], re[1]);
+        const uint32_t rb1p = pk(rb[0] + 1, rb[1] + 1), re1p = pk(re[0] + 1, re[1] + 1);
+        const uint32_t h1ip = pk(h1i[0], h1i[1]);
+        const uint32_t sel = (uint32_t)(tb[0] & 3) | 0x0C00u | ((uint32_t)(4 + (tb[1] & 3)) << 16) |
+                             0x0C000000u;
+        const uint32_t nmask = (tb[0] == 4 ? 0xFFFFu : 0u) | (tb[1] == 4 ? 0xFFFF0000u : 0u);
+
+        // pass 1: M and the lane-local exclusive prefix of G
+        uint32_t Mv[S], pre[S];
+        uint32_t run = kNeg2;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const uint32_t col = add16(cbp, (uint32_t)s * 0x00010001u);
+            uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
+            sb = bfi(nmask, nvalp, sb);
+            uint32_t M = sub16(add16(Hh[s], sb), 0x00800080u);
+            M = smin(M, shl8(umin(Hh[s], 0x007F007Fu)));
+            Mv[s] = M;
+            uint32_t G = add16(smax(sub16(M, oeinsp), 0u), mul16(col, einsp));
+            G = bfi(ltmask(col, rbp), kNeg2, G);
+            pre[s] = run;
+            run = smax(run, G);
+        }
+        const uint32_t carry = excl_scan_s2<L>(run, l);
+
+        // pass 2: H, E, row-max keys, write-back, next band
+        int kA = 0, kB = 0;
+        uint32_t hend = 0, ffirst = 0, flast = 0;
+        uint32_t Hprev = 0, H0v = 0, mle_rb0 = 0, wr0 = 0, eqre0 = 0, inb0 = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = cb + s;
+            const uint32_t col = add16(cbp, (uint32_t)s * 0x00010001u);
+            const uint32_t ce = mul16(col, einsp);
+            const uint32_t F = add16(sub16(smax(carry, pre[s]), ce), einsp);  // - (c-1)*e_ins
+            const uint32_t H = smax(smax(Mv[s], Ee[s]), F);
+            const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+            const uint32_t mlt_rb = ltmask(col, rbp), mle_rb = ltmask(col, rb1p);
+            const uint32_t mlt_re = ltmask(col, rep), mle_re = ltmask(col, re1p);
+            const uint32_t inb = mlt_re & ~mlt_rb, wr = mle_re & ~mlt_rb, eqre = mle_re & ~mlt_re;
+            const uint32_t Hm = H & inb;
+            const int kc = 1023 - c;
+            kA = max(kA, (int)(((Hm & 0xFFFFu) << 10) | (uint32_t)kc));
+            kB = max(kB, (int)(((Hm >> 16) << 10) | (uint32_t)kc));
+            Ee[s] = bfi(inb, En, Ee[s] & ~eqre);
+            if (s == 0) {
+                H0v = H;
+                mle_rb0 = mle_rb;
+                wr0 = wr;
+                eqre0 = eqre;
+                inb0 = inb;
+            } else {
+                const uint32_t hv = bfi(mle_rb, h1ip, Hprev);  // eh[c].h <- H(i, c-1) or h1
+                Hh[s] = bfi(wr, hv, Hh[s]);
+                hend = bfi(eqre, Hh[s], hend);
+                const uint32_t x = Hh[s] | Ee[s];
+                const uint32_t nzm = sgnmask(sub16(0u, x));
+                ffirst = umax(ffirst, bfi(nzm & inb, pk(1023 - c, 1023 - c), 0u));
+                flast = umax(flast, bfi(nzm & wr, add16(col, 0x00010001u), 0u));
+            }
+            Hprev = H;
+        }
+        {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
+            const uint32_t hl = dpp<kRowShr + 1>(0u, Hprev);
+            const uint32_t hv = bfi(mle_rb0, h1ip, hl);
+            Hh[0] = bfi(wr0, hv, Hh[0]);
+            hend = bfi(eqre0, Hh[0], hend);
+            const uint32_t x = Hh[0] | Ee[0];
+            const uint32_t nzm = sgnmask(sub16(0u, x));
+            ffirst = umax(ffirst, bfi(nzm & inb0, pk(1023 - cb, 1023 - cb), 0u));
+            flast = umax(flast, bfi(nzm & wr0, add16(cbp, 0x00010001u), 0u));
+            (void)H0v;
+        }
+        kA = allmax_i<L>(kA);
+        kB = allmax_i<L>(kB);
+        hend = allmax_u2<L>(hend);
+        ffirst = allmax_u2<L>(ffirst);
+        flast = allmax_u2<L>(flast);
+        pair_row_end(a, P[0], i, rb[0], re[0], kA, lo16(hend), (int)(ffirst & 0xFFFFu),
+                     (int)(flast & 0xFFFFu));
+        pair_row_end(a, P[1], i, rb[1], re[1], kB, hi16(hend), (int)(ffirst >> 16),
+                     (int)(flast >> 16));
+    }
+
+    if (l == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!P[h].valid) continue;
+            int32_t *o = a.out + (size_t)P[h].pid * 6;
+            o[0] = P[h].best;
+            o[1] = P[h].max_i + 1;
+            o[2] = P[h].max_ie + 1;
+            o[3] = P[h].max_j + 1;
+            o[4] = P[h].gscore;
+            o[5] = P[h].max_off;
+            if (a.cells) a.cells[P[h].pid] = P[h].ncell;
+        }
+    }
+}
+
+template <int L, int S>
+__global__ __launch_bounds__(256) void extend16_plain_kernel(ExtArgs a, uint32_t first,
+                                                             uint32_t count) {
+    extern __shared__ uint8_t lds[];
+    constexpr int G = 64 / L;
+    const int wave = (int)(threadIdx.x >> 6);
+    const uint32_t slot0 = (blockIdx.x * 4u + (uint32_t)wave) * (2u * G);
+    uint8_t *lds_wave = lds + wave * 2 * G * a.tpad;
+    if (slot0 >= count) return;
+    extend16_wave<L, S>(a, first + slot0, first + count, lds_wave);
+}
+
+template <int L, int S>
+__global__ __launch_bounds__(256) void extend16_persistent_kernel(ExtArgs a, uint32_t first,
+                                                                  uint32_t count, uint32_t qidx) {
+    extern __shared__ uint8_t lds[];
+    constexpr int G = 64 / L;
+    const int wave = (int)(threadIdx.x >> 6);
+    uint8_t *lds_wave = lds + wave * 2 * G * a.tpad;
+    const uint32_t ntasks = (count + 2 * G - 1) / (2 * G);
+    const uint32_t chunk = (uint32_t)a.chunk;
+    uint32_t *counter = a.queue + qidx;
+    for (;;) {
+        uint32_t t0 = 0;
+        if ((threadIdx.x & 63) == 0) t0 = atomicAdd(counter, chunk);
+        t0 = __builtin_amdgcn_readfirstlane(t0);
+        if (t0 >= ntasks) break;
+        const uint32_t t1 = t0 + chunk < ntasks ? t0 + chunk : ntasks;
+        for (uint32_t t = t0; t < t1; ++t)
+            extend16_wave<L, S>(a, first + t * 2 * G, first + count, lds_wave);
+    }
+}
+
+template <int L, int S>
+static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, bool persistent,
+                           uint32_t qidx, int cus, int blocks_per_cu, hipStream_t st) {
+    constexpr int G = 64 / L;
+    if (count == 0) return hipSuccess;
+    const size_t lds = (size_t)4 * 2 * G * a.tpad;
+    if (lds > 65536) return hipErrorInvalidValue;
+    const uint32_t per_block = 4u * 2u * G;
+    const uint32_t need = (count + per_block - 1) / per_block;
+    if (!persistent) {
+        hipLaunchKernelGGL((extend16_plain_kernel<L, S>), dim3(need), dim3(256), lds, st, a,
+                           first, count);
+        return hipGetLastError();
+    }
+    int occ = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void *>(&extend16_persistent_kernel<L, S>), 256, lds);
+    if (e != hipSuccess) return e;
+    if (occ < 1) occ = 1;
+    if (blocks_per_cu > 0 && blocks_per_cu < occ) occ = blocks_per_cu;
+    uint32_t blocks = (uint32_t)occ * (uint32_t)cus;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL((extend16_persistent_kernel<L, S>), dim3(blocks), dim3(256), lds, st, a,
+                       first, count, qidx);
+    return hipGetLastError();
+}
+
+#define FA16_LIST(X, L) X(L, 1) X(L, 2) X(L, 3) X(L, 4) X(L, 6) X(L, 8) X(L, 10) X(L, 12) X(L, 16)
+
+int extend16_slot_count(int L, int qlen) {
+    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
+    if (L != 4 && L != 8 && L != 16) return -1;
+    for (int s : sl)
+        if (L * s >= qlen + 1) return s;
+    return -1;
+}
+
+hipError_t extend16_launch(int L, int S, const ExtArgs &a, uint32_t first, uint32_t count,
+                           bool persistent, uint32_t qidx, int cus, int blocks_per_cu,
+                           hipStream_t st) {
+#define FA_CASE(LL, SS)                                                                       \
+    if (L == LL && S == SS)                                                                   \
+        return launch16<LL, SS>(a, first, count, persistent, qidx, cus, blocks_per_cu, st);
+    FA16_LIST(FA_CASE, 16)
+    FA16_LIST(FA_CASE, 8)
+    FA16_LIST(FA_CASE, 4)
+#undef FA_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fa

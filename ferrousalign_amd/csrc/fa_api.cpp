@@ -34,7 +34,9 @@ namespace {
 struct Bucket {
     int S;
     uint32_t first, count;
-    double work;  // estimated lane-slot work: sum over pairs of S * tlen (or of the cost)
+    double work;   // estimated work: sum over pairs of qlen * tlen (or of the cost)
+    int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group)
+    int L = 16;    // lanes per pair of this bucket
 };
 
 struct DevBuf {
@@ -48,6 +50,7 @@ struct fa_batch {
     int64_t n = 0;
     // host copies kept for re-packing
     std::vector<int32_t> qlen, tlen, h0, w;
+    std::vector<uint8_t> elig16;  // scores provably fit the 16-bit kernel
     std::vector<float> cost;
     // device
     uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
@@ -214,77 +217,117 @@ void batch_release(fa_batch *b) {
 int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t pack = flags & 0xF0u;
     const uint32_t single = flags & FA_LAUNCH_SINGLE;
+    const uint32_t wide = flags & FA_SCORE_I32;
     const int L = b->lanes;
-    if (b->packed_flags == (pack | single | (uint32_t)L << 16)) return FA_OK;
+    const uint32_t key = pack | single | wide | (uint32_t)L << 16;
+    if (b->packed_flags == key) return FA_OK;
     const int64_t n = b->n;
     std::vector<uint32_t> order((size_t)n);
-    std::vector<Bucket> buckets;
     for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
-    if (pack == FA_PACK_NONE) {
-        int maxq = 1;
-        for (int64_t k = 0; k < n; ++k) maxq = std::max(maxq, b->qlen[(size_t)k]);
-        int S = fa::extend_slot_count(L, maxq);
-        if (S < 0) return FA_ERR_QLEN;
-        if (n) buckets.push_back({S, 0u, (uint32_t)n, 1.0});
-    } else {
-        std::vector<int> skey((size_t)n);
-        for (int64_t k = 0; k < n; ++k) {
-            int S = fa::extend_slot_count(L, b->qlen[(size_t)k]);
-            if (S < 0) return FA_ERR_QLEN;
-            skey[(size_t)k] = S;
-        }
-        if (pack == FA_PACK_COST) {
-            if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
-            const float *cost = b->cost.data();
-            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-                if (skey[x] != skey[y]) return skey[x] > skey[y];
-                return cost[x] > cost[y];
-            });
-        } else {
-            const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
-            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-                if (skey[x] != skey[y]) return skey[x] > skey[y];
-                if (ql[x] != ql[y]) return ql[x] > ql[y];
-                return tl[x] > tl[y];
-            });
-        }
-        for (int64_t k = 0; k < n;) {
-            int S = skey[order[(size_t)k]];
-            int64_t e = k;
-            while (e < n && skey[order[(size_t)e]] == S) ++e;
-            double work = 0.0;
-            for (int64_t x = k; x < e; ++x) {
-                const uint32_t p = order[(size_t)x];
-                work += pack == FA_PACK_COST ? (double)b->cost[p]
-                                             : (double)S * (double)std::max(b->tlen[p], 1);
+    // kernel of each pair: (kind, lanes, slots) packed in one sortable key
+    std::vector<int> skey((size_t)n);
+    auto pair_key = [&](int64_t k, int &kind, int &lanes, int &S) -> bool {
+        const int q = b->qlen[(size_t)k];
+        if (!wide && b->elig16[(size_t)k]) {
+            lanes = fa::extend16_slot_count(L, q) > 0 ? L : 16;
+            S = fa::extend16_slot_count(lanes, q);
+            if (S > 0) {
+                kind = 1;
+                return true;
             }
-            buckets.push_back({S, (uint32_t)k, (uint32_t)(e - k), work});
-            k = e;
+        }
+        kind = 0;
+        lanes = fa::extend_slot_count(L, q) > 0 ? L : 8;
+        S = fa::extend_slot_count(lanes, q);
+        return S > 0;
+    };
+    for (int64_t k = 0; k < n; ++k) {
+        int kind, lanes, S;
+        if (!pair_key(k, kind, lanes, S)) return FA_ERR_QLEN;
+        skey[(size_t)k] = (kind << 16) | (lanes << 8) | S;
+    }
+    if (pack == FA_PACK_NONE || single) {
+        // one bucket per kernel kind at its widest slot count, input or sorted order
+        for (int64_t k = 0; k < n; ++k) {
+            const int kind = skey[(size_t)k] >> 16;
+            skey[(size_t)k] = kind << 16;
         }
     }
-    if (single && buckets.size() > 1) {
-        // one launch at the widest slot count, keeping the sorted order
+    if (pack == FA_PACK_COST) {
+        if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
+        const float *cost = b->cost.data();
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            if (skey[x] != skey[y]) return skey[x] > skey[y];
+            return cost[x] > cost[y];
+        });
+    } else if (pack != FA_PACK_NONE) {
+        const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            if (skey[x] != skey[y]) return skey[x] > skey[y];
+            if (ql[x] != ql[y]) return ql[x] > ql[y];
+            return tl[x] > tl[y];
+        });
+    } else {
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t x, uint32_t y) { return skey[x] > skey[y]; });
+    }
+    std::vector<Bucket> buckets;
+    for (int64_t k = 0; k < n;) {
+        const int key0 = skey[order[(size_t)k]];
+        int64_t e = k;
+        int maxq = 1;
         double work = 0.0;
-        for (const Bucket &bk : buckets) work += bk.work;
-        const int S = buckets.front().S;  // buckets run widest first
-        buckets.assign(1, Bucket{S, 0u, (uint32_t)n, work});
+        while (e < n && skey[order[(size_t)e]] == key0) {
+            const uint32_t p = order[(size_t)e];
+            maxq = std::max(maxq, b->qlen[p]);
+            work += pack == FA_PACK_COST ? (double)b->cost[p]
+                                         : (double)b->qlen[p] * (double)std::max(b->tlen[p], 1);
+            ++e;
+        }
+        Bucket bk;
+        bk.kind = key0 >> 16;
+        bk.first = (uint32_t)k;
+        bk.count = (uint32_t)(e - k);
+        bk.work = work;
+        if ((key0 & 0xFFFF) == 0) {  // merged bucket: widest slot count of its pairs
+            int kind = bk.kind;
+            if (kind == 1) {
+                bk.L = fa::extend16_slot_count(L, maxq) > 0 ? L : 16;
+                bk.S = fa::extend16_slot_count(bk.L, maxq);
+            } else {
+                bk.L = fa::extend_slot_count(L, maxq) > 0 ? L : 8;
+                bk.S = fa::extend_slot_count(bk.L, maxq);
+            }
+            if (bk.S < 0) return FA_ERR_QLEN;
+        } else {
+            bk.L = (key0 >> 8) & 0xFF;
+            bk.S = key0 & 0xFF;
+        }
+        buckets.push_back(bk);
+        k = e;
     }
     if (buckets.size() > 64) return FA_ERR_ARG;
     FA_HIP(hipMemcpyAsync(b->d_order, order.data(), order.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
     b->buckets.swap(buckets);
-    b->packed_flags = pack | single | (uint32_t)L << 16;
+    b->packed_flags = key;
     drop_graph(b);
     return FA_OK;
 }
 
 hipError_t launch_bucket(fa_ctx *ctx, fa_batch *b, uint32_t flags, size_t k, hipStream_t st) {
     const Bucket &bk = b->buckets[k];
+    fa::ExtArgs a = b->args;
+    a.chunk = std::max(1, b->chunk / (64 / bk.L) / (bk.kind == 1 ? 2 : 1));
+    if (bk.kind == 1)
+        return fa::extend16_launch(bk.L, bk.S, a, bk.first, bk.count,
+                                   (flags & FA_LAUNCH_PERSISTENT) != 0, (uint32_t)k, ctx->cus,
+                                   b->waves_per_cu / 4, st);
     if (flags & FA_LAUNCH_PERSISTENT)
-        return fa::extend_launch_persistent(b->lanes, bk.S, b->args, bk.first, bk.count,
-                                            (uint32_t)k, ctx->cus, b->waves_per_cu / 4, st);
-    return fa::extend_launch_plain(b->lanes, bk.S, b->args, bk.first, bk.count, st);
+        return fa::extend_launch_persistent(bk.L, bk.S, a, bk.first, bk.count, (uint32_t)k,
+                                            ctx->cus, b->waves_per_cu / 4, st);
+    return fa::extend_launch_plain(bk.L, bk.S, a, bk.first, bk.count, st);
 }
 
 // Enqueue one run of the batch (no synchronisation).  Buckets are spread over
@@ -346,7 +389,7 @@ int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
     b->args.chunk = std::max(1, b->chunk / (64 / b->lanes));
     if (nlaunch) *nlaunch = (int)b->buckets.size();
     if (!(flags & FA_LAUNCH_GRAPH)) return enqueue_run(ctx, b, flags);
-    const uint32_t key = (flags & 0xFF0u) | (uint32_t)b->lanes << 16 | (uint32_t)b->args.chunk << 21;
+    const uint32_t key = (flags & 0xFF4u) | (uint32_t)b->lanes << 16 | (uint32_t)b->chunk << 21;
     if (!b->graph_exec || b->graph_key != key || b->graph_wpc != b->waves_per_cu * 8 + b->streams) {
         drop_graph(b);
         FA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -479,6 +522,7 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     b->tlen.assign(ref_len, ref_len + n);
     b->h0.assign(h0, h0 + n);
     b->w.resize((size_t)n);
+    b->elig16.resize((size_t)n);
 
     // validate + offsets
     std::vector<uint32_t> qo((size_t)n), to((size_t)n);
@@ -507,6 +551,14 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             break;
         }
         b->w[(size_t)k] = capped_w(*sw, query_len[k], band_width[k]);
+        {
+            int mx = 0;
+            for (int x = 0; x < 25; ++x) mx = std::max(mx, (int)sw->mat[x]);
+            const int64_t bound = (int64_t)h0[k] + (int64_t)query_len[k] * mx;
+            b->elig16[(size_t)k] =
+                bound < 8192 && (int64_t)(query_len[k] + 1) * sw->e_ins < 8192 &&
+                sw->o_ins + sw->e_ins < 8192 && sw->o_del + sw->e_del < 8192;
+        }
         tmax = std::max(tmax, ref_len[k]);
     }
     if (first_err != FA_OK) {
@@ -588,8 +640,6 @@ int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_
     if (lanes_per_pair == 0) lanes_per_pair = 16;
     if (lanes_per_pair != 4 && lanes_per_pair != 8 && lanes_per_pair != 16) return FA_ERR_ARG;
     if (chunk_pairs < 0 || waves_per_cu < 0) return FA_ERR_ARG;
-    for (int32_t q : b->qlen)
-        if (fa::extend_slot_count(lanes_per_pair, q) < 0) return FA_ERR_QLEN;
     if (b->lanes != lanes_per_pair) b->packed_flags = 0xFFFFFFFFu;
     drop_graph(b);
     b->lanes = lanes_per_pair;

@@ -40,10 +40,15 @@ extern "C" {
 #define FA_MAX_TLEN 2048
 
 /* ---- flags -------------------------------------------------------------- */
-/* score path: the reference's i8 (kernel.rs) vs i16 (kernel_i16.rs) kernels */
-#define FA_SCORE_I16 0x0u       /* 16-bit score path */
-#define FA_SCORE_U8 0x1u        /* 8-bit score path; pairs that overflow 8 bits are rerun in 16-bit */
-#define FA_SCORE_AUTO 0x2u      /* 8-bit when every pair provably fits, else per-pair split */
+/* score path.  FA_SCORE_I16 (default): pairs whose scores provably fit run
+ * the packed 16-bit kernel (two pairs per lane group); the others fall back
+ * to the 32-bit kernel.  CDNA4 has no packed 8-bit integer ALU, so the 8-bit
+ * entry points (FA_SCORE_U8, the reference's i8 kernels) take the same
+ * 16-bit/32-bit route; every path returns the same exact records. */
+#define FA_SCORE_I16 0x0u
+#define FA_SCORE_U8 0x1u
+#define FA_SCORE_AUTO 0x2u
+#define FA_SCORE_I32 0x4u       /* force the 32-bit kernel for every pair */
 /* packing of pairs into lane groups (output order never changes) */
 #define FA_PACK_LENGTH 0x00u    /* sort by (qlen, tlen): the plain kernel */
 #define FA_PACK_NONE 0x10u      /* input order */

@@ -53,3 +53,66 @@ def test_parity_lane_groups(lanes):
     print("lanes", lanes, ps.n, flush=True)
     ora, oc, gpu, gc = _run_both(ps, sw, lanes=lanes)
     _assert_same(ora, oc, gpu, gc, "lanes=%d" % lanes)
+
+
+def test_parity_wide_kernel():
+    from ferrousalign_amd import synth, _native as N
+    ps = synth.workload("uneven_mix_4m", n=8192)
+    sw = synth.default_scoring()
+    print("wide kernel", flush=True)
+    ora, oc, gpu, gc = _run_both(ps, sw, flags=N.FA_SCORE_I32)
+    _assert_same(ora, oc, gpu, gc, "32-bit kernel")
+
+
+def test_overflow_fallback_mixed():
+    """Pairs whose score bound does not fit the 16-bit kernel run the 32-bit
+    kernel in the same call; results stay exact."""
+    from ferrousalign_amd import synth
+    ps = synth.workload("illumina_150bp_1m", n=6000)
+    big = np.arange(0, ps.n, 7)
+    ps.h0[big] = 8100 + (big % 50)  # h0 + qlen exceeds the 16-bit bound
+    sw = synth.default_scoring()
+    print("overflow fallback", flush=True)
+    ora, oc, gpu, gc = _run_both(ps, sw)
+    _assert_same(ora, oc, gpu, gc, "mixed 16/32-bit")
+
+
+def _random_case_batch(rng, n, qmax=60, tmax=90):
+    from ferrousalign_amd.synth import PairSet
+    q = np.zeros((n, 256), np.uint8)
+    t = np.zeros((n, 352), np.uint8)
+    ql = rng.integers(1, qmax + 1, n).astype(np.int32)
+    tl = rng.integers(0, tmax + 1, n).astype(np.int32)
+    for k in range(n):
+        alpha = 5 if rng.random() < 0.3 else 4
+        q[k, :ql[k]] = rng.integers(0, alpha, ql[k])
+        if rng.random() < 0.6 and tl[k] > 0:  # mutated copy of the query
+            m = min(ql[k], tl[k])
+            t[k, :m] = q[k, :m]
+            flip = rng.random(m) < 0.1
+            t[k, :m][flip] = rng.integers(0, alpha, flip.sum())
+            t[k, m:tl[k]] = rng.integers(0, alpha, tl[k] - m)
+        else:
+            t[k, :tl[k]] = rng.integers(0, alpha, tl[k])
+    h0 = rng.integers(1, 80, n).astype(np.int32)
+    w = rng.choice([0, 1, 2, 3, 5, 10, 100], n).astype(np.int32)
+    return PairSet(q, t, ql, tl, h0, w)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_parity_random_edge_cases(seed):
+    from ferrousalign_amd.banded_swa import BandedPairWiseSW, bwa_fill_scmat
+    from ferrousalign_amd import _native as N
+    rng = np.random.default_rng(seed)
+    ps = _random_case_batch(rng, 3000)
+    configs = [
+        BandedPairWiseSW(6, 1, 6, 1, 100, 5, 5, 5, bwa_fill_scmat(1, 4, -1)),
+        BandedPairWiseSW(5, 2, 3, 1, 0, 0, 5, 5, bwa_fill_scmat(2, 3, -1)),     # zdrop off
+        BandedPairWiseSW(0, 1, 0, 1, 10, 5, 5, 5, bwa_fill_scmat(1, 1, 0)),     # zero gap open
+        BandedPairWiseSW(12, 3, 9, 2, 30, 20, 5, 5, bwa_fill_scmat(3, 5, -2)),
+    ]
+    for ci, sw in enumerate(configs):
+        for flags in (0, N.FA_SCORE_I32):
+            print("random", seed, ci, flags, flush=True)
+            ora, oc, gpu, gc = _run_both(ps, sw, flags=flags)
+            _assert_same(ora, oc, gpu, gc, "random seed=%d cfg=%d flags=%d" % (seed, ci, flags))

@@ -75,16 +75,21 @@ def timed(db, flags, steps, warmup):
     return wall * 1e3 / steps, ms_ev, nlaunch
 
 
-def roofline(db, flags, cells, kernel_name):
-    launches = db.profile_launches(flags)
+def roofline(db, flags, cells, reps=3):
+    """Dominant kernel: every launch of one run is replayed alone on the
+    batch's stream and timed with HIP events (reps times, mean); achieved =
+    16 ops x the cells that launch computes / its mean duration."""
+    runs = [db.profile_launches(flags) for _ in range(reps)]
+    ms = np.mean([[x["ms"] for x in r] for r in runs], axis=0)
+    launches = runs[0]
+    k = int(np.argmax(ms))
+    dom = launches[k]
     order = db.order(flags)
-    dom = max(launches, key=lambda x: x["ms"])
     dom_cells = float(cells[order[dom["first"]:dom["first"] + dom["count"]]].astype(np.float64).sum())
-    tops = dom_cells * OPS_PER_CELL / (dom["ms"] / 1e3) / 1e12
+    tops = dom_cells * OPS_PER_CELL / (ms[k] / 1e3) / 1e12
     return {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_INT32_TOPS, 1), "unit": "TOPS",
-            "frac": round(tops / VALU_INT32_TOPS, 4), "traffic": None,
-            "kernel": kernel_name % dom["slots"], "launch_ms": round(dom["ms"], 4),
-            "launch_cells": dom_cells, "ops_per_cell": OPS_PER_CELL}
+            "frac": round(tops / VALU_INT32_TOPS, 4), "traffic": None, "kernel": dom["kernel"],
+            "launch_ms": round(float(ms[k]), 4), "launch_cells": dom_cells, "ops_per_cell": OPS_PER_CELL}
 
 
 def digest_of(out, cells):
@@ -113,11 +118,7 @@ def mate_bench(args, ctx, golden):
         gm = json.load(open(gpath))
     key = "mate_%d" % n
     exact = (digest(out, cells) == gm[key]) if key in gm else None
-    launches = db.profile_launches(flags)
-    dom = max(launches, key=lambda x: x["ms"])
-    dom_cells = float(cells[dom["first"]:dom["first"] + dom["count"]].astype(np.float64).sum()) \
-        if len(launches) == 1 else total * dom["count"] / n
-    tops = dom_cells * OPS_PER_CELL / (dom["ms"] / 1e3) / 1e12
+    roof = roofline(db, flags, cells)
     res = {"metric": "mate-rescue pairs/s (ksw_align2 16-bit, bit-exact)", "value": round(n / (ms / 1e3), 1),
            "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
            "ms_per_step_hip_events": round(ms_ev, 3), "higher_is_better": True, "scaling": "weak",
@@ -125,9 +126,7 @@ def mate_bench(args, ctx, golden):
            "config": {"workload": "mate_rescue_2m", "pairs": n, "qlen": 151, "tlen": 1500, "xtra": hex(synth.MATE_XTRA),
                       "launches_per_step": nl},
            "gcups": round(total / (ms / 1e3) / 1e9, 2), "bit_exact_vs_golden": exact,
-           "roofline": {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_INT32_TOPS, 1), "unit": "TOPS",
-                        "frac": round(tops / VALU_INT32_TOPS, 4), "traffic": None, "kernel": "mate_kernel<19>",
-                        "launch_ms": round(dom["ms"], 3), "ops_per_cell": OPS_PER_CELL}}
+           "roofline": roof}
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         m = 16384
@@ -153,6 +152,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="stop after the roofline measurement (isolated launches); for rocprofv3 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: extension pairs shard independently across GPUs with no "
@@ -189,7 +190,7 @@ def main():
                           "packing": "length-sorted", "launch": "single"},
                "gcups": round(float(cells.astype(np.float64).sum()) / (ms / 1e3) / 1e9, 2),
                "bit_exact_vs_golden": golden.get(args.workload) == digest_of(out, cells),
-               "roofline": roofline(db, flags, cells, "extend_plain_kernel<16,%d>")}
+               "roofline": roofline(db, flags, cells)}
         if not args.no_cpu:
             res["cpu_baseline"] = cpu_baseline(ps, sw)
         print(json.dumps(res), flush=True)
@@ -214,6 +215,17 @@ def main():
     ps = synth.workload(args.workload)
     db = DeviceBatch(sw, *ps.flat(), ctx=ctx)
 
+    if args.profile:
+        # isolated launches of the learned-packed configuration only (for rocprofv3)
+        pred.predict(db, set_cost=True)
+        db.set_launch(8)
+        db.set_streams(1)
+        flags = N.FA_PACK_COST
+        db.run(flags)
+        _, pcells = db.download()
+        print(json.dumps({"roofline": roofline(db, flags, pcells)}), flush=True)
+        return
+
     # plain GPU kernel: one lane group (16 lanes) per pair, length-sorted, single launch
     db.set_launch(16)
     db.set_streams(1)
@@ -237,9 +249,7 @@ def main():
     ms, ms_ev, nlaunch = timed(db, flags, args.steps, args.warmup)
     out, cells2 = db.download()
     total_cells = float(cells2.astype(np.float64).sum())
-    kname = ("extend_persistent_kernel<%d," % best["lanes"] if flags & N.FA_LAUNCH_PERSISTENT
-             else "extend_plain_kernel<%d," % best["lanes"]) + "%d>"
-    roof = roofline(db, flags, cells2, kname)
+    roof = roofline(db, flags, cells2)
 
     # ablation: same launch configuration, length-sorted instead of learned
     lflags = (flags & ~0xF0) | N.FA_PACK_LENGTH

@@ -97,7 +97,7 @@ def _declare(lib):
     lib.fa_batch_time.argtypes = [_P, _P, _U32, _I32, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I32)]
     lib.fa_batch_download.argtypes = [_P, _P, _P, _P]
-    lib.fa_batch_profile_launches.argtypes = [_P, _P, _U32, _I32, ctypes.POINTER(_I32), _P, _P, _P, _P]
+    lib.fa_batch_profile_launches.argtypes = [_P, _P, _U32, _I32, ctypes.POINTER(_I32), _P, _P, _P, _P, _P]
     lib.fa_batch_get_order.argtypes = [_P, _P, _U32, _P]
     lib.fa_batch_overflow_count.argtypes = [_P, _P, ctypes.POINTER(_I64)]
     lib.fa_batch_free.argtypes = [_P, _P]

@@ -367,17 +367,30 @@ class DeviceBatch:
         return out, cells
 
     def profile_launches(self, flags=0, cap=64):
+        """Each launch of one run, alone and timed: list of dicts with the
+        kernel name, slots, lanes, packed range and ms."""
         nl = ctypes.c_int32()
         slots = np.zeros(cap, np.int32)
+        kern = np.zeros(cap, np.int32)
         first = np.zeros(cap, np.int64)
         count = np.zeros(cap, np.int64)
         ms = np.zeros(cap, np.float64)
         N.check(N.lib().fa_batch_profile_launches(self.ctx.handle, self._h, flags, cap, ctypes.byref(nl),
-                                                  N.ptr(slots), N.ptr(first), N.ptr(count), N.ptr(ms)),
+                                                  N.ptr(slots), N.ptr(kern), N.ptr(first), N.ptr(count), N.ptr(ms)),
                 "fa_batch_profile_launches")
-        k = min(nl.value, cap)
-        return [dict(slots=int(slots[i]), first=int(first[i]), count=int(count[i]), ms=float(ms[i]))
-                for i in range(k)]
+        names = {0: "extend_plain_kernel<%d,%d>", 1: "extend16_plain_kernel<%d,%d>", 2: "mate_kernel<%d>"}
+        out = []
+        for i in range(min(nl.value, cap)):
+            kind, lanes = int(kern[i]) >> 8, int(kern[i]) & 0xFF
+            if kind == 2:
+                name = names[2] % int(slots[i])
+            else:
+                name = names[kind] % (lanes, int(slots[i]))
+                if flags & N.FA_LAUNCH_PERSISTENT:
+                    name = name.replace("_plain_", "_persistent_")
+            out.append(dict(kernel=name, slots=int(slots[i]), lanes=lanes, first=int(first[i]),
+                            count=int(count[i]), ms=float(ms[i])))
+        return out
 
     def order(self, flags=0):
         o = np.zeros(self.n, np.uint32)

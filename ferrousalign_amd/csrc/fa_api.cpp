@@ -687,7 +687,8 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
     } else if (ms_dominant_launch) {
         int32_t nl2 = 0;
         double ms_l[64];
-        rc = fa_batch_profile_launches(ctx, b, flags, 64, &nl2, nullptr, nullptr, nullptr, ms_l);
+        rc = fa_batch_profile_launches(ctx, b, flags, 64, &nl2, nullptr, nullptr, nullptr, nullptr,
+                                       ms_l);
         if (rc) return rc;
         double best = 0.0;
         for (int k = 0; k < nl2 && k < 64; ++k) best = std::max(best, ms_l[k]);
@@ -697,8 +698,8 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
 }
 
 int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t cap,
-                              int32_t *nlaunch, int32_t *slots, int64_t *first, int64_t *count,
-                              double *ms) {
+                              int32_t *nlaunch, int32_t *slots, int32_t *kernel, int64_t *first,
+                              int64_t *count, double *ms) {
     if (!ctx || !b || !nlaunch || cap < 0) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     if (flags & FA_RUN_ALIGN2) {
@@ -715,6 +716,7 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
                 FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
                 if (k < cap) {
                     if (slots) slots[k] = bk.S;
+                    if (kernel) kernel[k] = 2 << 8 | 8;  // mate kernel, 8 lanes per pair
                     if (first) first[k] = bk.first + off;
                     if (count) count[k] = cnt;
                     if (ms) ms[k] = t;
@@ -740,6 +742,7 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
         FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
         if (k < cap) {
             if (slots) slots[k] = bk.S;
+            if (kernel) kernel[k] = bk.kind << 8 | bk.L;
             if (first) first[k] = bk.first;
             if (count) count[k] = bk.count;
             if (ms) ms[k] = t;

@@ -178,13 +178,16 @@ int fa_batch_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
  * mean duration of the dominant kernel launch (ms). */
 int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double *ms_per_run,
                   double *ms_dominant_launch, int32_t *launches_per_run);
-/* Runs each kernel launch of one run once, timed with HIP events on the
- * batch's stream: launch k covers packed positions [first[k], first[k] +
- * count[k]) of the pair order with slot count slots[k].  Fills at most cap
- * entries; *nlaunch receives the number of launches. */
+/* Runs each kernel launch of one run once, alone on the batch's stream and
+ * timed with HIP events: launch k covers packed positions [first[k],
+ * first[k] + count[k]) of the pair order with slot count slots[k] and kernel
+ * kernel[k] = (kind << 8) | lanes per pair, kind 0 = 32-bit extension,
+ * 1 = packed 16-bit extension, 2 = mate-rescue local SW.  Fills at most cap
+ * entries (any array may be NULL); *nlaunch receives the number of
+ * launches. */
 int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t cap,
-                              int32_t *nlaunch, int32_t *slots, int64_t *first, int64_t *count,
-                              double *ms);
+                              int32_t *nlaunch, int32_t *slots, int32_t *kernel, int64_t *first,
+                              int64_t *count, double *ms);
 /* The pair order (packed position -> job index) used under `flags`. */
 int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order);
 int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cells);

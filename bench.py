@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="with --profile: also run one device copy of this many MiB (counter calibration)")
     ap.add_argument("--profile", action="store_true",
                     help="stop after the roofline measurement (isolated launches); for rocprofv3 runs")
     args = ap.parse_args()
@@ -224,6 +226,14 @@ def main():
         db.run(flags)
         _, pcells = db.download()
         print(json.dumps({"roofline": roofline(db, flags, pcells)}), flush=True)
+        if args.copy_mib:
+            # calibration for HBM byte counters: one device copy of known size
+            import torch
+            src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            print(json.dumps({"calibration_copy_bytes": args.copy_mib << 20}), flush=True)
         return
 
     # plain GPU kernel: one lane group (16 lanes) per pair, length-sorted, single launch

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "extend_kernel.h"
 
 namespace fa {
@@ -263,6 +265,10 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
+    const uint32_t cbe = mul16(cbp, einsp);  // cb * e_ins in both halves
+    uint32_t se[S];                          // s * e_ins (wave-uniform)
+#pragma unroll
+    for (int s = 0; s < S; ++s) se[s] = pk(s * a.e_ins, s * a.e_ins);
 
     for (int i = 0;; ++i) {
         if (__builtin_amdgcn_ballot_w64(P[0].alive || P[1].alive) == 0) break;
@@ -288,46 +294,65 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
                              0x0C000000u;
         const uint32_t nmask = (tb[0] == 4 ? 0xFFFFu : 0u) | (tb[1] == 4 ? 0xFFFF0000u : 0u);
 
+        // Band tests are made lane-relative: with t = bound - cb, column cb+s
+        // is below the bound iff s - t < 0, and c <= bound at slot s is
+        // c < bound at slot s-1.
+        const uint32_t trb = sub16(rbp, cbp), tre = sub16(rep, cbp);
+        const bool anyN = __builtin_amdgcn_ballot_w64(tb[0] == 4 || tb[1] == 4) != 0;
+
         // pass 1: M and the lane-local exclusive prefix of G
         uint32_t Mv[S], pre[S];
         uint32_t run = kNeg2;
+        auto pass1 = [&](auto hasN) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const uint32_t col = add16(cbp, (uint32_t)s * 0x00010001u);
-            uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
-            sb = bfi(nmask, nvalp, sb);
-            uint32_t M = sub16(add16(Hh[s], sb), 0x00800080u);
-            M = smin(M, shl8(umin(Hh[s], 0x007F007Fu)));
-            Mv[s] = M;
-            uint32_t G = add16(smax(sub16(M, oeinsp), 0u), mul16(col, einsp));
-            G = bfi(ltmask(col, rbp), kNeg2, G);
-            pre[s] = run;
-            run = smax(run, G);
-        }
+            for (int s = 0; s < S; ++s) {
+                uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
+                if constexpr (decltype(hasN)::value) sb = bfi(nmask, nvalp, sb);
+                uint32_t M = sub16(add16(Hh[s], sb), 0x00800080u);
+                M = smin(M, shl8(umin(Hh[s], 0x007F007Fu)));
+                Mv[s] = M;
+                uint32_t G = add16(smax(sub16(M, oeinsp), 0u), add16(cbe, se[s]));
+                G = bfi(sgnmask(sub16((uint32_t)s * 0x00010001u, trb)), kNeg2, G);
+                pre[s] = run;
+                run = smax(run, G);
+            }
+        };
+        if (anyN)
+            pass1(std::true_type{});
+        else
+            pass1(std::false_type{});
         const uint32_t carry = excl_scan_s2<L>(run, l);
 
         // pass 2: H, E, row-max keys, write-back, next band
         int kA = 0, kB = 0;
-        uint32_t hend = 0, ffirst = 0, flast = 0;
-        uint32_t Hprev = 0, H0v = 0, mle_rb0 = 0, wr0 = 0, eqre0 = 0, inb0 = 0;
+        uint32_t pkey = 0;  // S <= 8: per half (Hm << 3) | (7 - s)
+        uint32_t hend = 0, frel = 0, lrel = 0;
+        uint32_t Hprev = 0, mle_rb0 = 0, wr0 = 0, eqre0 = 0, inb0 = 0;
+        uint32_t mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
+        uint32_t mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
+        uint32_t ce_prev = sub16(cbe, einsp);                      // (cb-1)*e_ins
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int c = cb + s;
-            const uint32_t col = add16(cbp, (uint32_t)s * 0x00010001u);
-            const uint32_t ce = mul16(col, einsp);
-            const uint32_t F = add16(sub16(smax(carry, pre[s]), ce), einsp);  // - (c-1)*e_ins
+            const uint32_t sp = (uint32_t)s * 0x00010001u;
+            const uint32_t F = sub16(smax(carry, pre[s]), ce_prev);  // - (c-1)*e_ins
+            ce_prev = add16(cbe, se[s]);
             const uint32_t H = smax(smax(Mv[s], Ee[s]), F);
             const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
-            const uint32_t mlt_rb = ltmask(col, rbp), mle_rb = ltmask(col, rb1p);
-            const uint32_t mlt_re = ltmask(col, rep), mle_re = ltmask(col, re1p);
+            const uint32_t mlt_rb = sgnmask(sub16(sp, trb)), mle_rb = mlt_rb_prev;
+            const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
+            mlt_rb_prev = mlt_rb;
+            mlt_re_prev = mlt_re;
             const uint32_t inb = mlt_re & ~mlt_rb, wr = mle_re & ~mlt_rb, eqre = mle_re & ~mlt_re;
             const uint32_t Hm = H & inb;
-            const int kc = 1023 - c;
-            kA = max(kA, (int)(((Hm & 0xFFFFu) << 10) | (uint32_t)kc));
-            kB = max(kB, (int)(((Hm >> 16) << 10) | (uint32_t)kc));
+            if constexpr (S <= 8) {
+                pkey = umax(pkey, bits(vu(Hm) << (u2){3, 3}) | ((uint32_t)(7 - s) * 0x00010001u));
+            } else {
+                const int kc = 1023 - (cb + s);
+                kA = max(kA, (int)(((Hm & 0xFFFFu) << 10) | (uint32_t)kc));
+                kB = max(kB, (int)(((Hm >> 16) << 10) | (uint32_t)kc));
+            }
             Ee[s] = bfi(inb, En, Ee[s] & ~eqre);
             if (s == 0) {
-                H0v = H;
                 mle_rb0 = mle_rb;
                 wr0 = wr;
                 eqre0 = eqre;
@@ -336,10 +361,9 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
                 const uint32_t hv = bfi(mle_rb, h1ip, Hprev);  // eh[c].h <- H(i, c-1) or h1
                 Hh[s] = bfi(wr, hv, Hh[s]);
                 hend = bfi(eqre, Hh[s], hend);
-                const uint32_t x = Hh[s] | Ee[s];
-                const uint32_t nzm = sgnmask(sub16(0u, x));
-                ffirst = umax(ffirst, bfi(nzm & inb, pk(1023 - c, 1023 - c), 0u));
-                flast = umax(flast, bfi(nzm & wr, add16(col, 0x00010001u), 0u));
+                const uint32_t nzm = sgnmask(sub16(0u, Hh[s] | Ee[s]));
+                frel = umax(frel, bfi(nzm & inb, (uint32_t)(64 - s) * 0x00010001u, 0u));
+                lrel = umax(lrel, bfi(nzm & wr, sp + 0x00010001u, 0u));
             }
             Hprev = H;
         }
@@ -348,12 +372,18 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             const uint32_t hv = bfi(mle_rb0, h1ip, hl);
             Hh[0] = bfi(wr0, hv, Hh[0]);
             hend = bfi(eqre0, Hh[0], hend);
-            const uint32_t x = Hh[0] | Ee[0];
-            const uint32_t nzm = sgnmask(sub16(0u, x));
-            ffirst = umax(ffirst, bfi(nzm & inb0, pk(1023 - cb, 1023 - cb), 0u));
-            flast = umax(flast, bfi(nzm & wr0, add16(cbp, 0x00010001u), 0u));
-            (void)H0v;
+            const uint32_t nzm = sgnmask(sub16(0u, Hh[0] | Ee[0]));
+            frel = umax(frel, bfi(nzm & inb0, 64u * 0x00010001u, 0u));
+            lrel = umax(lrel, bfi(nzm & wr0, 0x00010001u, 0u));
         }
+        if constexpr (S <= 8) {  // lane-relative keys -> (H << 10) | (1023 - c)
+            const int ka = (int)(pkey & 0xFFFFu), kb = (int)(pkey >> 16);
+            kA = ((ka >> 3) << 10) | (1023 - (cb + 7 - (ka & 7)));
+            kB = ((kb >> 3) << 10) | (1023 - (cb + 7 - (kb & 7)));
+        }
+        // first non-zero column as 1023 - c, last as c + 1, 0 when none
+        uint32_t ffirst = add16(frel, pk(959 - cb, 959 - cb)) & sgnmask(sub16(0u, frel));
+        uint32_t flast = add16(lrel, cbp) & sgnmask(sub16(0u, lrel));
         kA = allmax_i<L>(kA);
         kB = allmax_i<L>(kB);
         hend = allmax_u2<L>(hend);

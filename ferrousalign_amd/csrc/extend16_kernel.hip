@@ -8,14 +8,15 @@
 // packed differences (column - bound), selects are v_bfi on those masks.
 //
 // 16-bit range: the host routes a pair here only when every value provably
-// fits: h0 + qlen*max(mat) < 8192 and (qlen+1)*e_ins, o+e < 8192, so H, E,
-// F, G stay in (-32768, 16384) (see extend16_eligible); other pairs run the
-// 32-bit kernel.
+// fits: with Hmax = h0 + qlen*max(mat), Hmax < 4096 and Hmax*K < 32768
+// (K = max(2, 1 + max(mat))), e_ins < 32 and o+e < 8192, so H, E, F, G stay
+// in (-32768, 16384) and the row-argmax key (H << 4) | slot fits 16 bits;
+// other pairs run the 32-bit kernel.
 //
 // Differences from the 32-bit kernel that keep results identical:
-//  * M = H ? H + s : 0 becomes min(H + s, min(H, 127) << 8): equal when H > 0,
-//    and <= 0 when H == 0, which every later use (max with E >= 0, max with
-//    0) treats like 0;
+//  * M = H ? H + s : 0 becomes min(H + s, H*K): equal when H > 0, and <= 0
+//    when H == 0, which every later use (max with E >= 0, max with 0) treats
+//    like 0;
 //  * the F prefix identity is -16384 (F(beg) stays below E >= 0 without
 //    wrapping);
 //  * E' = max(E -sat e_del, M - oe_del): E >= 0, so the unsigned saturating
@@ -62,7 +63,6 @@ __device__ __forceinline__ uint32_t mul16(uint32_t a, uint32_t b) { return bits(
 __device__ __forceinline__ uint32_t subsat(uint32_t a, uint32_t b) {
     return bits(__builtin_elementwise_sub_sat(vu(a), vu(b)));
 }
-__device__ __forceinline__ uint32_t shl8(uint32_t a) { return bits(vu(a) << (u2){8, 8}); }
 // Sign of each 16-bit half spread over the half (0xFFFF if negative).  Kept
 // as one VALU instruction: written in C, the compiler rewrites the
 // mask-and-select idiom into per-half compares and selects.
@@ -71,10 +71,39 @@ __device__ __forceinline__ uint32_t sgnmask(uint32_t d) {
     asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(r) : "v"(d));
     return r;
 }
-// 0xFFFF in each half where a < b (|a - b| < 32768)
-__device__ __forceinline__ uint32_t ltmask(uint32_t a, uint32_t b) { return sgnmask(sub16(a, b)); }
+// min(x, 1) per half: 1 where the half is non-zero
+__device__ __forceinline__ uint32_t nz01(uint32_t x) {
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, 1 op_sel_hi:[1,0]" : "=v"(r) : "v"(x));
+    return r;
+}
+// a * 2 + b per half
+__device__ __forceinline__ uint32_t dbl_add(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_pk_mad_u16 %0, %1, 2, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// a * MUL + b per half, MUL an inline constant (0..64), b wave-uniform
+template <int MUL>
+__device__ __forceinline__ uint32_t mad_imm(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_pk_mad_u16 %0, %1, %2, %3 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "I"(MUL), "s"(b));
+    return r;
+}
+// a * m + b per half, m wave-uniform
+__device__ __forceinline__ uint32_t mad_s(uint32_t a, uint32_t m, uint32_t b) {
+    uint32_t r;
+    asm("v_pk_mad_u16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(m), "v"(b));
+    return r;
+}
 __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) {
     return (m & a) | (~m & b);
+}
+// bfi kept as one v_bfi_b32 where the compiler would split it
+__device__ __forceinline__ uint32_t bfi1(uint32_t m, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
 }
 
 constexpr uint32_t kNeg2 = 0xC000C000u;  // (-16384, -16384)
@@ -203,15 +232,27 @@ __device__ __forceinline__ void pair_row_end(const ExtArgs &a, PairState &P, int
 
 // One wavefront: 64/L lane groups, each running pairs order[slot0 + 2g] (low
 // halves) and order[slot0 + 2g + 1] (high halves).
+//
+// Column layout: each pair is right-aligned in its group, column c sits in
+// lane (c + pad) / S, slot (c + pad) % S with pad = L*S - 1 - qlen, so column
+// qlen (whose eh.h after a row ending at qlen is H(i, qlen-1), the gscore
+// candidate) is always the group's last slot.  Positions with c < 0 and
+// columns left of the band are dead: the band start rb never decreases
+// (rb = max(beg, i - w) with beg non-decreasing), so those registers may hold
+// anything; every reader masks them (G in pass 1, the argmax, the non-zero
+// scan) or overwrites them before use (eh.h at c = rb is h1).
 template <int L, int S>
 __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, uint32_t nslots,
                                               uint8_t *lds_wave) {
+    constexpr int KB = S <= 8 ? 3 : 4;  // row-argmax key per half: (H << KB) | (KM - s)
+    constexpr uint32_t KM = (1u << KB) - 1u;
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane / L;
     const int l = lane % L;
     PairState P[2];
     uint32_t qo[2];
     uint8_t *tc[2];
+    int cbh[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
@@ -233,30 +274,31 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         P[h].max_off = 0;
         P[h].ncell = 0;
         P[h].alive = P[h].valid && P[h].tlen > 0;
+        cbh[h] = l * S - (L * S - 1 - P[h].qlen);  // column of slot 0 in this lane
     }
 
     const int oe_ins = a.o_ins + a.e_ins, oe_del = a.o_del + a.e_del;
     const uint32_t einsp = pk(a.e_ins, a.e_ins), edelp = pk(a.e_del, a.e_del);
-    const uint32_t oeinsp = pk(oe_ins, oe_ins), oedelp = pk(oe_del, oe_del);
+    const uint32_t oedelp = pk(oe_del, oe_del);
     const uint32_t nvalp = a.nrow | (a.nrow << 16);  // biased N-row score in both halves
-    const int cb = l * S;
-    const uint32_t cbp = pk(cb, cb);
+    const uint32_t kp = pk(a.mguard, a.mguard);
+    const uint32_t cbp = pk(cbh[0], cbh[1]);
 
     uint32_t profA[S], profB[S], Hh[S], Ee[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const int c = cb + s;
         int hv[2];
         uint32_t pw[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
+            const int c = cbh[h] + s;
             int code = 4;
-            if (P[h].valid && c < P[h].qlen) code = decode_base(a.seq2, a.seqn, qo[h], c);
+            if (P[h].valid && c >= 0 && c < P[h].qlen) code = decode_base(a.seq2, a.seqn, qo[h], c);
             pw[h] = a.profw[code];
             const int h0 = P[h].h0;
             const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
             const int x = c == 0 ? h0 : max(v1 - (c - 1) * a.e_ins, 0);  // :992-998
-            hv[h] = c <= P[h].qlen ? x : 0;
+            hv[h] = c >= 0 ? x : 0;
         }
         profA[s] = pw[0];
         profB[s] = pw[1];
@@ -265,10 +307,22 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    const uint32_t cbe = mul16(cbp, einsp);  // cb * e_ins in both halves
-    uint32_t se[S];                          // s * e_ins (wave-uniform)
+    // G(c) = max(M - oe_ins, 0) + c*e_ins is split into the lane's cb*e_ins
+    // and a lane-local part max(M + (s*e_ins - oe_ins), s*e_ins) whose
+    // constants are wave-uniform.
+    const uint32_t cbe = mul16(cbp, einsp);  // cb * e_ins per half
+    uint32_t se[S], seo[S];                  // s*e_ins, s*e_ins - oe_ins (wave-uniform)
 #pragma unroll
-    for (int s = 0; s < S; ++s) se[s] = pk(s * a.e_ins, s * a.e_ins);
+    for (int s = 0; s < S; ++s) {
+        se[s] = pk(s * a.e_ins, s * a.e_ins);
+        seo[s] = pk(s * a.e_ins - oe_ins, s * a.e_ins - oe_ins);
+    }
+    const uint32_t se_m1 = pk(-a.e_ins, -a.e_ins);
+    // identity of masked G: any value <= -e_ins leaves F(c >= rb) unchanged
+    // (F <= 0 there while E >= 0); kept in a scalar register so the select
+    // is one v_bfi_b32
+    uint32_t negs;
+    asm("s_mov_b32 %0, 0xf000f000" : "=s"(negs));
 
     for (int i = 0;; ++i) {
         if (__builtin_amdgcn_ballot_w64(P[0].alive || P[1].alive) == 0) break;
@@ -288,31 +342,29 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             h1i[h] = rb[h] == 0 ? max(P[h].h0 - (a.o_del + a.e_del * (i + 1)), 0) : 0;  // :1049-1054
         }
         const uint32_t rbp = pk(rb[0], rb[1]), rep = pk(re[0], re[1]);
-        const uint32_t rb1p = pk(rb[0] + 1, rb[1] + 1), re1p = pk(re[0] + 1, re[1] + 1);
         const uint32_t h1ip = pk(h1i[0], h1i[1]);
         const uint32_t sel = (uint32_t)(tb[0] & 3) | 0x0C00u | ((uint32_t)(4 + (tb[1] & 3)) << 16) |
                              0x0C000000u;
         const uint32_t nmask = (tb[0] == 4 ? 0xFFFFu : 0u) | (tb[1] == 4 ? 0xFFFF0000u : 0u);
 
-        // Band tests are made lane-relative: with t = bound - cb, column cb+s
-        // is below the bound iff s - t < 0, and c <= bound at slot s is
-        // c < bound at slot s-1.
+        // Band tests are lane-relative: with t = bound - cb, column cb+s is
+        // below the bound iff s - t < 0, and c <= bound at slot s is c < bound
+        // at slot s-1.
         const uint32_t trb = sub16(rbp, cbp), tre = sub16(rep, cbp);
         const bool anyN = __builtin_amdgcn_ballot_w64(tb[0] == 4 || tb[1] == 4) != 0;
 
-        // pass 1: M and the lane-local exclusive prefix of G
-        uint32_t Mv[S], pre[S];
-        uint32_t run = kNeg2;
+        // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
+        uint32_t Mv[S], pre[S], mrb[S];
+        uint32_t run = negs;
         auto pass1 = [&](auto hasN) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
                 if constexpr (decltype(hasN)::value) sb = bfi(nmask, nvalp, sb);
-                uint32_t M = sub16(add16(Hh[s], sb), 0x00800080u);
-                M = smin(M, shl8(umin(Hh[s], 0x007F007Fu)));
+                const uint32_t M = smin(sub16(add16(Hh[s], sb), 0x00800080u), mul16(Hh[s], kp));
                 Mv[s] = M;
-                uint32_t G = add16(smax(sub16(M, oeinsp), 0u), add16(cbe, se[s]));
-                G = bfi(sgnmask(sub16((uint32_t)s * 0x00010001u, trb)), kNeg2, G);
+                mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
+                const uint32_t G = bfi(mrb[s], negs, smax(add16(M, seo[s]), se[s]));
                 pre[s] = run;
                 run = smax(run, G);
             }
@@ -321,71 +373,68 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             pass1(std::true_type{});
         else
             pass1(std::false_type{});
-        const uint32_t carry = excl_scan_s2<L>(run, l);
+        // carry relative to this lane's cb*e_ins
+        const uint32_t carry = sub16(excl_scan_s2<L>(add16(run, cbe), l), cbe);
 
-        // pass 2: H, E, row-max keys, write-back, next band
-        int kA = 0, kB = 0;
-        uint32_t pkey = 0;  // S <= 8: per half (Hm << 3) | (7 - s)
-        uint32_t hend = 0, frel = 0, lrel = 0;
-        uint32_t Hprev = 0, mle_rb0 = 0, wr0 = 0, eqre0 = 0, inb0 = 0;
+        // pass 2: H, E, row-max key, write-back, non-zero columns
+        uint32_t pkey = 0, acc = 0, Hprev = 0;
+        uint32_t mle_rb0 = 0, mle_re0 = 0, wr0 = 0;
         uint32_t mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
         uint32_t mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
-        uint32_t ce_prev = sub16(cbe, einsp);                      // (cb-1)*e_ins
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const uint32_t sp = (uint32_t)s * 0x00010001u;
-            const uint32_t F = sub16(smax(carry, pre[s]), ce_prev);  // - (c-1)*e_ins
-            ce_prev = add16(cbe, se[s]);
+            const uint32_t F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
             const uint32_t H = smax(smax(Mv[s], Ee[s]), F);
             const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
-            const uint32_t mlt_rb = sgnmask(sub16(sp, trb)), mle_rb = mlt_rb_prev;
+            const uint32_t mlt_rb = mrb[s], mle_rb = mlt_rb_prev;
             const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
             mlt_rb_prev = mlt_rb;
             mlt_re_prev = mlt_re;
-            const uint32_t inb = mlt_re & ~mlt_rb, wr = mle_re & ~mlt_rb, eqre = mle_re & ~mlt_re;
-            const uint32_t Hm = H & inb;
-            if constexpr (S <= 8) {
-                pkey = umax(pkey, bits(vu(Hm) << (u2){3, 3}) | ((uint32_t)(7 - s) * 0x00010001u));
-            } else {
-                const int kc = 1023 - (cb + s);
-                kA = max(kA, (int)(((Hm & 0xFFFFu) << 10) | (uint32_t)kc));
-                kB = max(kB, (int)(((Hm >> 16) << 10) | (uint32_t)kc));
-            }
-            Ee[s] = bfi(inb, En, Ee[s] & ~eqre);
+            const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
+            pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+            Ee[s] = bfi1(mle_re, En & mlt_re, Ee[s]);  // E' in band, 0 at c = re (:1106-1107)
+            const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
             if (s == 0) {
                 mle_rb0 = mle_rb;
+                mle_re0 = mle_re;
                 wr0 = wr;
-                eqre0 = eqre;
-                inb0 = inb;
             } else {
-                const uint32_t hv = bfi(mle_rb, h1ip, Hprev);  // eh[c].h <- H(i, c-1) or h1
-                Hh[s] = bfi(wr, hv, Hh[s]);
-                hend = bfi(eqre, Hh[s], hend);
-                const uint32_t nzm = sgnmask(sub16(0u, Hh[s] | Ee[s]));
-                frel = umax(frel, bfi(nzm & inb, (uint32_t)(64 - s) * 0x00010001u, 0u));
-                lrel = umax(lrel, bfi(nzm & wr, sp + 0x00010001u, 0u));
+                const uint32_t hv = bfi1(mle_rb, h1ip, Hprev);  // eh[c].h <- H(i, c-1) or h1
+                Hh[s] = bfi1(mle_re, hv, Hh[s]);
+                const uint32_t x = (Hh[s] | Ee[s]) & wr;
+                acc = dbl_add(acc, nz01(x));
             }
             Hprev = H;
         }
         {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
             const uint32_t hl = dpp<kRowShr + 1>(0u, Hprev);
-            const uint32_t hv = bfi(mle_rb0, h1ip, hl);
-            Hh[0] = bfi(wr0, hv, Hh[0]);
-            hend = bfi(eqre0, Hh[0], hend);
-            const uint32_t nzm = sgnmask(sub16(0u, Hh[0] | Ee[0]));
-            frel = umax(frel, bfi(nzm & inb0, 64u * 0x00010001u, 0u));
-            lrel = umax(lrel, bfi(nzm & wr0, 0x00010001u, 0u));
+            const uint32_t hv = bfi1(mle_rb0, h1ip, hl);
+            Hh[0] = bfi1(mle_re0, hv, Hh[0]);
+            const uint32_t x = (Hh[0] | Ee[0]) & wr0;
+            acc = mad_s(nz01(x), (1u << (S - 1)) * 0x00010001u, acc);
         }
-        if constexpr (S <= 8) {  // lane-relative keys -> (H << 10) | (1023 - c)
-            const int ka = (int)(pkey & 0xFFFFu), kb = (int)(pkey >> 16);
-            kA = ((ka >> 3) << 10) | (1023 - (cb + 7 - (ka & 7)));
-            kB = ((kb >> 3) << 10) | (1023 - (cb + 7 - (kb & 7)));
+        // acc: bit S-1-s of each half set where column cb+s in [rb, re] has
+        // eh.h or eh.e non-zero after the row.  First such column as 1023 - c,
+        // last as c + 1, 0 when none (:1148-1158).
+        uint32_t fenc[2], lenc[2];
+        int kk[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t bitsh = h ? (acc >> 16) : (acc & 0xFFFFu);
+            const int clz = __builtin_clz(bitsh | 1u);
+            const int ctz = __builtin_ctz(bitsh | 0x10000u);
+            fenc[h] = bitsh ? (uint32_t)(1023 - (cbh[h] + S - 32 + clz)) : 0u;
+            lenc[h] = bitsh ? (uint32_t)(cbh[h] + S - ctz) : 0u;
+            const int key = (int)(h ? (pkey >> 16) : (pkey & 0xFFFFu));
+            const int c = max(cbh[h] + (int)KM - (key & (int)KM), 0);
+            kk[h] = ((key >> KB) << 10) | (1023 - c);
         }
-        // first non-zero column as 1023 - c, last as c + 1, 0 when none
-        uint32_t ffirst = add16(frel, pk(959 - cb, 959 - cb)) & sgnmask(sub16(0u, frel));
-        uint32_t flast = add16(lrel, cbp) & sgnmask(sub16(0u, lrel));
-        kA = allmax_i<L>(kA);
-        kB = allmax_i<L>(kB);
+        uint32_t ffirst = fenc[0] | (fenc[1] << 16);
+        uint32_t flast = lenc[0] | (lenc[1] << 16);
+        uint32_t hend = l == L - 1 ? Hh[S - 1] : 0u;  // column qlen: H(i, qlen-1) when re == qlen
+        const int kA = allmax_i<L>(kk[0]);
+        const int kB = allmax_i<L>(kk[1]);
         hend = allmax_u2<L>(hend);
         ffirst = allmax_u2<L>(ffirst);
         flast = allmax_u2<L>(flast);
@@ -394,7 +443,6 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         pair_row_end(a, P[1], i, rb[1], re[1], kB, hi16(hend), (int)(ffirst >> 16),
                      (int)(flast >> 16));
     }
-
     if (l == 0) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {

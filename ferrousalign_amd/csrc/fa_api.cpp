@@ -555,9 +555,9 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             int mx = 0;
             for (int x = 0; x < 25; ++x) mx = std::max(mx, (int)sw->mat[x]);
             const int64_t bound = (int64_t)h0[k] + (int64_t)query_len[k] * mx;
-            b->elig16[(size_t)k] =
-                bound < 8192 && (int64_t)(query_len[k] + 1) * sw->e_ins < 8192 &&
-                sw->o_ins + sw->e_ins < 8192 && sw->o_del + sw->e_del < 8192;
+            const int64_t K = std::max(2, 1 + mx);
+            b->elig16[(size_t)k] = bound < 4096 && bound * K < 32768 && sw->e_ins < 32 &&
+                                   sw->o_ins + sw->e_ins < 8192 && sw->o_del + sw->e_del < 8192;
         }
         tmax = std::max(tmax, ref_len[k]);
     }
@@ -614,6 +614,11 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         a.profw[q] = v;
     }
     a.nrow = (uint32_t)(uint8_t)(sw->mat[20] + 128);
+    {
+        int mx = 0;
+        for (int x = 0; x < 25; ++x) mx = std::max(mx, (int)sw->mat[x]);
+        a.mguard = std::max(2, 1 + mx);
+    }
     a.o_del = sw->o_del;
     a.e_del = sw->e_del;
     a.o_ins = sw->o_ins;

@@ -1,6 +1,7 @@
 """Launch autotuner for the extension path.
 
-Searches lanes per pair (4, 8, 16), persistent-queue chunk size and launch
+Searches lanes per pair (4, 8, 16, or the narrowest group per query
+length), persistent-queue chunk size and launch
 form (one launch per query-length bucket, or persistent waves pulling from a
 device work counter with the launch sequence replayed as a hipGraph) by
 timing the actual batch on the device, and caches the winner per workload
@@ -30,10 +31,10 @@ def signature(qlen, tlen, w, pack_flags, n):
 
 def candidates(max_qlen):
     out = []
-    for lanes in (16, 8, 4):
-        if lanes * 32 < max_qlen + 1 or (lanes == 16 and 16 * 16 < max_qlen + 1):
+    for lanes in (0, 16, 8, 4):  # 0: narrowest lane group per query length
+        if lanes and (lanes * 32 < max_qlen + 1 or (lanes == 16 and 16 * 16 < max_qlen + 1)):
             continue
-        g = 64 // lanes
+        g = 64 // (lanes or 4)
         for streams in (1, 4):
             out.append(dict(lanes=lanes, chunk=0, wpc=0, streams=streams, launch=N.FA_LAUNCH_GRAPH))
         for chunk_tasks in (1, 4):

@@ -519,10 +519,11 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
     return hipGetLastError();
 }
 
-#define FA16_LIST(X, L) X(L, 1) X(L, 2) X(L, 3) X(L, 4) X(L, 6) X(L, 8) X(L, 10) X(L, 12) X(L, 16)
+#define FA16_LIST(X, L) \
+    X(L, 1) X(L, 2) X(L, 3) X(L, 4) X(L, 6) X(L, 8) X(L, 10) X(L, 12) X(L, 14) X(L, 16)
 
 int extend16_slot_count(int L, int qlen) {
-    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
+    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16};
     if (L != 4 && L != 8 && L != 16) return -1;
     for (int s : sl)
         if (L * s >= qlen + 1) return s;

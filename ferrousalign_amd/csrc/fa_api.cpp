@@ -63,7 +63,7 @@ struct fa_batch {
     // launch plan for the current packing
     uint32_t packed_flags = 0xFFFFFFFFu;
     std::vector<Bucket> buckets;
-    int lanes = 16;
+    int lanes = 0;  // 0: narrowest lane group per query length
     int chunk = 0;
     int waves_per_cu = 0;
     int streams = 4;
@@ -226,10 +226,26 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
     // kernel of each pair: (kind, lanes, slots) packed in one sortable key
     std::vector<int> skey((size_t)n);
+    // lanes per pair of a query length: the configured L when its slot
+    // counts reach, else the next wider group; L == 0 picks the narrowest
+    // group that holds the query (the fewest lanes per pair amortise the
+    // per-row reductions over the most pairs per wavefront)
+    auto lanes16 = [&](int q) {
+        if (L == 0) {
+            for (int c : {4, 8, 16})
+                if (fa::extend16_slot_count(c, q) > 0) return c;
+            return 16;
+        }
+        return fa::extend16_slot_count(L, q) > 0 ? L : 16;
+    };
+    auto lanes32 = [&](int q) {
+        const int c = L ? L : 8;
+        return fa::extend_slot_count(c, q) > 0 ? c : 8;
+    };
     auto pair_key = [&](int64_t k, int &kind, int &lanes, int &S) -> bool {
         const int q = b->qlen[(size_t)k];
         if (!wide && b->elig16[(size_t)k]) {
-            lanes = fa::extend16_slot_count(L, q) > 0 ? L : 16;
+            lanes = lanes16(q);
             S = fa::extend16_slot_count(lanes, q);
             if (S > 0) {
                 kind = 1;
@@ -237,7 +253,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             }
         }
         kind = 0;
-        lanes = fa::extend_slot_count(L, q) > 0 ? L : 8;
+        lanes = lanes32(q);
         S = fa::extend_slot_count(lanes, q);
         return S > 0;
     };
@@ -292,10 +308,10 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         if ((key0 & 0xFFFF) == 0) {  // merged bucket: widest slot count of its pairs
             int kind = bk.kind;
             if (kind == 1) {
-                bk.L = fa::extend16_slot_count(L, maxq) > 0 ? L : 16;
+                bk.L = lanes16(maxq);
                 bk.S = fa::extend16_slot_count(bk.L, maxq);
             } else {
-                bk.L = fa::extend_slot_count(L, maxq) > 0 ? L : 8;
+                bk.L = lanes32(maxq);
                 bk.S = fa::extend_slot_count(bk.L, maxq);
             }
             if (bk.S < 0) return FA_ERR_QLEN;
@@ -386,7 +402,6 @@ int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
     if (flags & FA_RUN_ALIGN2) return run_align2(ctx, b, nlaunch);
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
-    b->args.chunk = std::max(1, b->chunk / (64 / b->lanes));
     if (nlaunch) *nlaunch = (int)b->buckets.size();
     if (!(flags & FA_LAUNCH_GRAPH)) return enqueue_run(ctx, b, flags);
     const uint32_t key = (flags & 0xFF4u) | (uint32_t)b->lanes << 16 | (uint32_t)b->chunk << 21;
@@ -642,8 +657,8 @@ int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost) {
 int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_t chunk_pairs,
                         int32_t waves_per_cu) {
     if (!ctx || !b) return FA_ERR_ARG;
-    if (lanes_per_pair == 0) lanes_per_pair = 16;
-    if (lanes_per_pair != 4 && lanes_per_pair != 8 && lanes_per_pair != 16) return FA_ERR_ARG;
+    if (lanes_per_pair != 0 && lanes_per_pair != 4 && lanes_per_pair != 8 && lanes_per_pair != 16)
+        return FA_ERR_ARG;
     if (chunk_pairs < 0 || waves_per_cu < 0) return FA_ERR_ARG;
     if (b->lanes != lanes_per_pair) b->packed_flags = 0xFFFFFFFFu;
     drop_graph(b);
@@ -735,7 +750,6 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
     int k = 0;
-    b->args.chunk = std::max(1, b->chunk / (64 / b->lanes));
     if (flags & FA_LAUNCH_PERSISTENT)
         FA_HIP(fa::queue_reset(b->d_queue, (uint32_t)b->buckets.size(), ctx->stream));
     for (const Bucket &bk : b->buckets) {

@@ -165,8 +165,10 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                     const int32_t *band_width, fa_batch **out);
 /* Per-pair packing cost (e.g. predicted DP cells) used by FA_PACK_COST. */
 int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost);
-/* Launch configuration: lanes per pair (4, 8 or 16; 0 = default) and the
- * persistent-queue chunk size in pairs (0 = default). */
+/* Launch configuration: lanes per pair (4, 8 or 16, widened per query length
+ * where that group is too narrow; 0 = the narrowest group that holds each
+ * query length, the default) and the persistent-queue chunk size in pairs
+ * (0 = default). */
 int fa_batch_set_launch(fa_ctx *ctx, fa_batch *b, int32_t lanes_per_pair, int32_t chunk_pairs,
                         int32_t waves_per_cu);
 /* Number of streams (1..4, default 4) the query-length buckets of one run

@@ -87,7 +87,7 @@ def test_autotuner_picks_a_valid_config(mix, tmp_path):
     ps, sw, ora, oc = mix
     db = DeviceBatch(sw, *ps.flat())
     best, flags, table = autotune.tune(db, ps.qlen, ps.tlen, ps.w, cache_path=str(tmp_path / "c.json"), reps=1)
-    assert table and best["lanes"] in (4, 8, 16)
+    assert table and best["lanes"] in (0, 4, 8, 16)
     db.run(flags)
     _check(db, ora, oc, "autotuned")
     # second call hits the cache

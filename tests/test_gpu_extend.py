@@ -43,12 +43,13 @@ def test_parity_workloads(name, n):
     _assert_same(ora, oc, gpu, gc, name)
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [0, 4, 8, 16])
 def test_parity_lane_groups(lanes):
     from ferrousalign_amd import synth
     ps = synth.workload("uneven_mix_4m", n=8192)
-    # a lane group of L lanes holds at most 32 columns per lane
-    ps = ps.subset(np.nonzero(ps.qlen + 1 <= lanes * 32)[0])
+    # a lane group of L lanes holds at most 32 columns per lane (0: narrowest group per pair)
+    if lanes:
+        ps = ps.subset(np.nonzero(ps.qlen + 1 <= lanes * 32)[0])
     sw = synth.default_scoring()
     print("lanes", lanes, ps.n, flush=True)
     ora, oc, gpu, gc = _run_both(ps, sw, lanes=lanes)

@@ -34,9 +34,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X public figure: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz
+# MI355X public figures: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz; packed
+# 16-bit VALU instructions (v_pk_*) do two ops per lane
 VALU_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
+VALU_INT16X2_TOPS = 2 * VALU_INT32_TOPS
 OPS_PER_CELL = 16  # integer ops per DP cell of ksw_extend2's inner loop (ksw_affine_gap.rs:1061-1103)
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.md")
 
 
 def log(*a):
@@ -75,10 +78,27 @@ def timed(db, flags, steps, warmup):
     return wall * 1e3 / steps, ms_ev, nlaunch
 
 
-def roofline(db, flags, cells, reps=3):
+def traffic_of(kernel, cells):
+    """HBM bytes per launch of `kernel` over `cells` DP cells, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (rows 'traffic: <kernel>
+    cells=<n> bytes=<b>'), or None when that launch was not profiled."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 4 and p[0] == "traffic:" and p[1] == kernel and p[2] == "cells=%d" % cells:
+                    return int(p[3].split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def roofline(db, flags, cells, ctx=None, reps=3):
     """Dominant kernel: every launch of one run is replayed alone on the
     batch's stream and timed with HIP events (reps times, mean); achieved =
-    16 ops x the cells that launch computes / its mean duration."""
+    16 ops x the cells that launch computes / its mean duration, against the
+    VALU peak of the arithmetic it runs in (packed 16-bit for extend16_*,
+    32-bit otherwise), published and measured on this device."""
     runs = [db.profile_launches(flags) for _ in range(reps)]
     ms = np.mean([[x["ms"] for x in r] for r in runs], axis=0)
     launches = runs[0]
@@ -87,9 +107,17 @@ def roofline(db, flags, cells, reps=3):
     order = db.order(flags)
     dom_cells = float(cells[order[dom["first"]:dom["first"] + dom["count"]]].astype(np.float64).sum())
     tops = dom_cells * OPS_PER_CELL / (ms[k] / 1e3) / 1e12
-    return {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_INT32_TOPS, 1), "unit": "TOPS",
-            "frac": round(tops / VALU_INT32_TOPS, 4), "traffic": None, "kernel": dom["kernel"],
-            "launch_ms": round(float(ms[k]), 4), "launch_cells": dom_cells, "ops_per_cell": OPS_PER_CELL}
+    packed = dom["kernel"].startswith("extend16")
+    peak = VALU_INT16X2_TOPS if packed else VALU_INT32_TOPS
+    res = {"bound": "valu", "achieved": round(tops, 3), "peak": round(peak, 1), "unit": "TOPS",
+           "frac": round(tops / peak, 4), "traffic": traffic_of(dom["kernel"], int(dom_cells)),
+           "kernel": dom["kernel"], "peak_dtype": "int16x2" if packed else "int32",
+           "launch_ms": round(float(ms[k]), 4), "launch_cells": dom_cells, "ops_per_cell": OPS_PER_CELL}
+    if ctx is not None:
+        m = ctx.valu_peak(packed)
+        res["peak_measured"] = round(m, 2)
+        res["frac_of_measured"] = round(tops / m, 4)
+    return res
 
 
 def digest_of(out, cells):
@@ -118,7 +146,7 @@ def mate_bench(args, ctx, golden):
         gm = json.load(open(gpath))
     key = "mate_%d" % n
     exact = (digest(out, cells) == gm[key]) if key in gm else None
-    roof = roofline(db, flags, cells)
+    roof = roofline(db, flags, cells, ctx)
     res = {"metric": "mate-rescue pairs/s (ksw_align2 16-bit, bit-exact)", "value": round(n / (ms / 1e3), 1),
            "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
            "ms_per_step_hip_events": round(ms_ev, 3), "higher_is_better": True, "scaling": "weak",
@@ -153,7 +181,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="with --profile: also run one device copy of this many MiB (counter calibration)")
+                    help="with --profile: also run a device copy of this many MiB (counter calibration)")
     ap.add_argument("--profile", action="store_true",
                     help="stop after the roofline measurement (isolated launches); for rocprofv3 runs")
     args = ap.parse_args()
@@ -187,12 +215,12 @@ def main():
         res = {"metric": "extension pairs/s (ksw_extend2-exact banded seed extension)",
                "value": round(ps.n / (ms / 1e3), 1), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-               "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+               "vs_baseline": None, "dtype": "int16x2", "data": "synthetic",
                "config": {"workload": args.workload, "pairs": ps.n, "lanes_per_pair": 16,
                           "packing": "length-sorted", "launch": "single"},
                "gcups": round(float(cells.astype(np.float64).sum()) / (ms / 1e3) / 1e9, 2),
                "bit_exact_vs_golden": golden.get(args.workload) == digest_of(out, cells),
-               "roofline": roofline(db, flags, cells)}
+               "roofline": roofline(db, flags, cells, ctx)}
         if not args.no_cpu:
             res["cpu_baseline"] = cpu_baseline(ps, sw)
         print(json.dumps(res), flush=True)
@@ -220,20 +248,18 @@ def main():
     if args.profile:
         # isolated launches of the learned-packed configuration only (for rocprofv3)
         pred.predict(db, set_cost=True)
-        db.set_launch(8)
+        db.set_launch(0)
         db.set_streams(1)
         flags = N.FA_PACK_COST
         db.run(flags)
         _, pcells = db.download()
-        print(json.dumps({"roofline": roofline(db, flags, pcells)}), flush=True)
+        print(json.dumps({"roofline": roofline(db, flags, pcells, ctx)}), flush=True)
         if args.copy_mib:
-            # calibration for HBM byte counters: one device copy of known size
-            import torch
-            src = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
-            dst = torch.empty_like(src)
-            dst.copy_(src)
-            torch.cuda.synchronize()
-            print(json.dumps({"calibration_copy_bytes": args.copy_mib << 20}), flush=True)
+            # calibration for HBM byte counters: one copy kernel that reads
+            # and writes copy_mib MiB (launched twice; the second is timed)
+            gbps = ctx.hbm_copy(args.copy_mib << 20)
+            print(json.dumps({"calibration_copy_bytes": args.copy_mib << 20, "copy_gbps": round(gbps, 1)}),
+                  flush=True)
         return
 
     # plain GPU kernel: one lane group (16 lanes) per pair, length-sorted, single launch
@@ -259,7 +285,7 @@ def main():
     ms, ms_ev, nlaunch = timed(db, flags, args.steps, args.warmup)
     out, cells2 = db.download()
     total_cells = float(cells2.astype(np.float64).sum())
-    roof = roofline(db, flags, cells2)
+    roof = roofline(db, flags, cells2, ctx)
 
     # ablation: same launch configuration, length-sorted instead of learned
     lflags = (flags & ~0xF0) | N.FA_PACK_LENGTH
@@ -282,10 +308,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16x2",
         "data": "synthetic",
         "config": {"workload": args.workload, "pairs": ps.n, "packing": "learned (predicted cells)",
-                   "lanes_per_pair": best["lanes"], "chunk_pairs": best["chunk"], "streams": best.get("streams", 4),
+                   "lanes_per_pair": best["lanes"] or "narrowest group per query length",
+                   "chunk_pairs": best["chunk"], "streams": best.get("streams", 4),
                    "launch": ("persistent queue" if flags & N.FA_LAUNCH_PERSISTENT else "bucket launches")
                    + " + hipGraph", "launches_per_step": nlaunch},
         "gcups": round(total_cells / (ms / 1e3) / 1e9, 2),

@@ -83,6 +83,8 @@ def _declare(lib):
     lib.fa_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_P)]
     lib.fa_ctx_destroy.argtypes = [_P]
     lib.fa_ctx_describe.argtypes = [_P, ctypes.c_char_p, ctypes.c_size_t]
+    lib.fa_valu_peak.argtypes = [_P, _I32, ctypes.POINTER(ctypes.c_double)]
+    lib.fa_hbm_copy.argtypes = [_P, _I64, ctypes.POINTER(ctypes.c_double)]
     seqargs = [_I64, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.fa_execute_batch_simd_scoring.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
     lib.fa_batch_upload.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [ctypes.POINTER(_P)]
@@ -104,7 +106,7 @@ def _declare(lib):
     soa = [_P, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]
     lib.fa_simd_banded_swa_batch_soa.argtypes = soa
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
-    for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_execute_batch_simd_scoring",
+    for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_valu_peak", "fa_hbm_copy", "fa_execute_batch_simd_scoring",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
                  "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",

@@ -178,6 +178,19 @@ class Context:
         N.check(N.lib().fa_ctx_describe(self._h, buf, 256), "fa_ctx_describe")
         return buf.value.decode()
 
+    def valu_peak(self, packed16=True):
+        """Measured integer add/max VALU rate of this device, 10^12 lane-ops/s
+        (packed 16-bit counts two ops per lane per instruction)."""
+        v = ctypes.c_double()
+        N.check(N.lib().fa_valu_peak(self._h, 1 if packed16 else 0, ctypes.byref(v)), "fa_valu_peak")
+        return v.value
+
+    def hbm_copy(self, nbytes):
+        """Read+write GB/s of one device copy of `nbytes` (multiple of 16)."""
+        v = ctypes.c_double()
+        N.check(N.lib().fa_hbm_copy(self._h, int(nbytes), ctypes.byref(v)), "fa_hbm_copy")
+        return v.value
+
     def close(self):
         if self._h:
             N.lib().fa_ctx_destroy(self._h)

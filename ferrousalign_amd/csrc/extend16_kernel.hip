@@ -257,13 +257,19 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     for (int h = 0; h < 2; ++h) {
         const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
         P[h].valid = slot < nslots;
-        P[h].pid = P[h].valid ? a.order[slot] : 0u;
-        P[h].qlen = P[h].valid ? a.qlen[P[h].pid] : 1;
-        P[h].tlen = P[h].valid ? a.tlen[P[h].pid] : 0;
-        P[h].h0 = P[h].valid ? a.h0[P[h].pid] : 1;
-        P[h].w = P[h].valid ? a.w[P[h].pid] : 0;
-        qo[h] = P[h].valid ? a.qo[P[h].pid] : 0u;
-        const uint32_t to = P[h].valid ? a.to[P[h].pid] : 0u;
+        uint4 m0 = {0u, 0u, 0u, 1u}, m1 = {0u, 1u, 0u, 0u};  // empty slot: qlen 1, tlen 0, h0 1
+        if (P[h].valid) {
+            const uint4 *mp = reinterpret_cast<const uint4 *>(a.meta) + 2 * (size_t)slot;
+            m0 = mp[0];
+            m1 = mp[1];
+        }
+        P[h].pid = m0.x;
+        P[h].qlen = (int)m0.w;
+        P[h].tlen = (int)m1.x;
+        P[h].h0 = (int)m1.y;
+        P[h].w = (int)m1.z;
+        qo[h] = m0.y;
+        const uint32_t to = m0.z;
         tc[h] = lds_wave + (2 * g + h) * a.tpad;
         stage_target(a, tc[h], P[h].tlen, to, l, L);
         P[h].beg = 0;
@@ -457,6 +463,21 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             if (a.cells) a.cells[P[h].pid] = P[h].ncell;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void meta_gather_kernel(ExtArgs a, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = a.order[k];
+    uint4 *mp = reinterpret_cast<uint4 *>(a.meta) + 2 * (size_t)k;
+    mp[0] = make_uint4(p, a.qo[p], a.to[p], (uint32_t)a.qlen[p]);
+    mp[1] = make_uint4((uint32_t)a.tlen[p], (uint32_t)a.h0[p], (uint32_t)a.w[p], 0u);
+}
+
+hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(meta_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    return hipGetLastError();
 }
 
 template <int L, int S>

@@ -26,6 +26,7 @@ struct ExtArgs {
     const int32_t *h0;
     const int32_t *w;
     const uint32_t *order;
+    uint32_t *meta;      // 8 words per packed position: pid, qo, to, qlen, tlen, h0, w, 0
     int32_t *out;
     uint32_t *cells;
     uint32_t *queue;     // persistent launch: work counter (one per launch)
@@ -49,6 +50,11 @@ hipError_t extend_launch_persistent(int L, int S, const ExtArgs &a, uint32_t fir
                                     uint32_t count, uint32_t qidx, int cus, int blocks_per_cu,
                                     hipStream_t st);
 hipError_t queue_reset(uint32_t *q, uint32_t n, hipStream_t st);
+
+// Fills a.meta for packed positions [0, n) from a.order and the per-pair
+// arrays, so that a wavefront reads its pairs' parameters as one contiguous
+// block.
+hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st);
 
 // 16-bit packed kernel (two pairs per lane group): slot count for L lanes,
 // or -1, and its launcher (plain or persistent).

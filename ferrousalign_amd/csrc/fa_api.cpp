@@ -18,6 +18,7 @@
 #include "extend_kernel.h"
 #include "predictor.h"
 #include "mate_kernel.h"
+#include "valu_probe.h"
 
 struct fa_ctx {
     int device = 0;
@@ -56,6 +57,7 @@ struct fa_batch {
     uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
     int32_t *d_qlen = nullptr, *d_tlen = nullptr, *d_h0 = nullptr, *d_w = nullptr;
     uint32_t *d_order = nullptr;
+    uint32_t *d_meta = nullptr;  // per packed slot: pid, qo, to, qlen, tlen, h0, w, 0
     int32_t *d_out = nullptr;
     uint32_t *d_cells = nullptr;
     uint32_t *d_queue = nullptr;
@@ -203,7 +205,7 @@ void batch_release(fa_batch *b) {
     drop_graph(b);
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
-                    b->d_w, b->d_order, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
+                    b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
                     b->d_mout, b->d_imax_log};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -325,6 +327,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     if (buckets.size() > 64) return FA_ERR_ARG;
     FA_HIP(hipMemcpyAsync(b->d_order, order.data(), order.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, ctx->stream));
+    FA_HIP(fa::meta_gather(b->args, (uint32_t)n, ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
     b->buckets.swap(buckets);
     b->packed_flags = key;
@@ -599,7 +602,8 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         (rc = dev_upload(&b->d_tlen, b->tlen.data(), b->tlen.size(), st)) ||
         (rc = dev_upload(&b->d_h0, b->h0.data(), b->h0.size(), st)) ||
         (rc = dev_upload(&b->d_w, b->w.data(), b->w.size(), st)) ||
-        (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
+        (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
+        (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
         (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64))) {
         batch_release(b);
         return rc;
@@ -619,6 +623,7 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     a.h0 = b->d_h0;
     a.w = b->d_w;
     a.order = b->d_order;
+    a.meta = b->d_meta;
     a.out = b->d_out;
     a.cells = b->d_cells;
     a.queue = b->d_queue;
@@ -769,6 +774,49 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
         ++k;
     }
     *nlaunch = k;
+    return FA_OK;
+}
+
+int fa_valu_peak(fa_ctx *ctx, int32_t packed16, double *tera_ops) {
+    if (!ctx || !tera_ops) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    const int blocks = ctx->cus * 8;  // 32 waves per CU
+    const int iters = 512;
+    uint32_t *out = nullptr;
+    FA_HIP(hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint32_t)));
+    hipError_t e = fa::valu_probe_launch(packed16 != 0, out, blocks, 8, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) e = fa::valu_probe_launch(packed16 != 0, out, blocks, iters, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(out);
+    if (e != hipSuccess) return FA_ERR_DEVICE;
+    const double instr = (double)blocks * 256.0 * iters * fa::kValuProbeUnroll * 16.0;
+    *tera_ops = instr * (packed16 ? 2.0 : 1.0) / (ms * 1e-3) / 1e12;
+    return FA_OK;
+}
+
+int fa_hbm_copy(fa_ctx *ctx, int64_t bytes, double *gbytes_per_s) {
+    if (!ctx || !gbytes_per_s || bytes < 16 || bytes % 16) return FA_ERR_ARG;
+    FA_HIP(hipSetDevice(ctx->device));
+    void *src = nullptr, *dst = nullptr;
+    FA_HIP(hipMalloc(&src, (size_t)bytes));
+    hipError_t e = hipMalloc(&dst, (size_t)bytes);
+    const int blocks = ctx->cus * 16;
+    if (e == hipSuccess) e = hipMemsetAsync(src, 1, (size_t)bytes, ctx->stream);
+    if (e == hipSuccess) e = fa::hbm_copy_launch(src, dst, (size_t)bytes, blocks, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) e = fa::hbm_copy_launch(src, dst, (size_t)bytes, blocks, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e != hipSuccess) return FA_ERR_DEVICE;
+    *gbytes_per_s = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
     return FA_OK;
 }
 

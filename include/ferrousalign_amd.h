@@ -190,6 +190,18 @@ int fa_batch_time(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t reps, double
 int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t cap,
                               int32_t *nlaunch, int32_t *slots, int32_t *kernel, int64_t *first,
                               int64_t *count, double *ms);
+/* VALU throughput probe (no counterpart in the reference; measurement only):
+ * the integer add/max rate this device sustains in packed 16-bit form
+ * (packed16 != 0, two 16-bit ops per lane per instruction) or 32-bit form,
+ * in 10^12 lane-ops/s, the measured ceiling beside the published VALU peak
+ * that the extension kernels' roofline is quoted against. */
+int fa_valu_peak(fa_ctx *ctx, int32_t packed16, double *tera_ops);
+/* HBM copy probe (no counterpart in the reference; measurement only): copies
+ * `bytes` (a multiple of 16) between two device buffers with one vector-copy
+ * kernel launch, twice, and reports the second launch's read+write rate in
+ * GB/s.  Its per-dispatch FETCH_SIZE/WRITE_SIZE counters are the reference
+ * point that scales the HBM byte counters of the other kernels. */
+int fa_hbm_copy(fa_ctx *ctx, int64_t bytes, double *gbytes_per_s);
 /* The pair order (packed position -> job index) used under `flags`. */
 int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order);
 int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cells);

@@ -7,6 +7,7 @@ from .banded_swa import (  # noqa: F401
     BatchedExtensionJob,
     Context,
     DeviceBatch,
+    DeviceReference,
     ExtensionDirection,
     ExtensionJobBatch,
     OutScore,
@@ -14,6 +15,7 @@ from .banded_swa import (  # noqa: F401
     bwa_fill_scmat,
     execute_batch_simd_scoring,
     extend_arrays,
+    pack_pac,
     simd_banded_swa_batch8_int16_soa,
     simd_banded_swa_batch16_int16_soa,
     simd_banded_swa_batch16_soa,

@@ -23,6 +23,7 @@ FA_ERR_SCORING = -7
 FA_ERR_DEVICE = -8
 FA_ERR_NODEVICE = -9
 FA_ERR_OVERFLOW = -10
+FA_ERR_SEGMENT = -11
 
 FA_MAX_QLEN = 255
 FA_MAX_TLEN = 2048
@@ -85,6 +86,11 @@ def _declare(lib):
     lib.fa_ctx_describe.argtypes = [_P, ctypes.c_char_p, ctypes.c_size_t]
     lib.fa_valu_peak.argtypes = [_P, _I32, ctypes.POINTER(ctypes.c_double)]
     lib.fa_hbm_copy.argtypes = [_P, _I64, ctypes.POINTER(ctypes.c_double)]
+    lib.fa_ref_upload.argtypes = [_P, _P, _I64, ctypes.POINTER(_P)]
+    lib.fa_ref_free.argtypes = [_P, _P]
+    lib.fa_batch_upload_ref.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW), _I64, _P, _P, _P, _P, _P, _P, _P,
+                                        _P, _P, ctypes.POINTER(_P)]
+    lib.fa_batch_download_targets.argtypes = [_P, _P, _P, _P]
     seqargs = [_I64, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.fa_execute_batch_simd_scoring.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
     lib.fa_batch_upload.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [ctypes.POINTER(_P)]
@@ -106,7 +112,8 @@ def _declare(lib):
     soa = [_P, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]
     lib.fa_simd_banded_swa_batch_soa.argtypes = soa
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
-    for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_valu_peak", "fa_hbm_copy", "fa_execute_batch_simd_scoring",
+    for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_valu_peak", "fa_hbm_copy", "fa_ref_upload", "fa_ref_free",
+                 "fa_batch_upload_ref", "fa_batch_download_targets", "fa_execute_batch_simd_scoring",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
                  "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",
@@ -135,7 +142,7 @@ def synth():
         s.fa_synth_reference.argtypes = [ctypes.c_uint64, _I64, _P]
         s.fa_synth_reference.restype = None
         s.fa_synth_extension_pairs.argtypes = [ctypes.c_int, ctypes.c_uint64, _P, _I64, _I64, _P, _I64, _P,
-                                               _P, _I64, _P, _P, _P, _I32, _I32, _I32, ctypes.c_int]
+                                               _P, _I64, _P, _P, _P, _I32, _I32, _I32, ctypes.c_int, _P]
         s.fa_synth_extension_pairs.restype = ctypes.c_int
         s.fa_synth_mate_pairs.argtypes = [ctypes.c_uint64, _P, _I64, _I64, _P, _P, _P, ctypes.c_int]
         s.fa_synth_mate_pairs.restype = ctypes.c_int

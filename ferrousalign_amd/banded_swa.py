@@ -325,8 +325,82 @@ simd_banded_swa_batch16_int16_soa = _make_soa_entry(16, True)
 simd_banded_swa_batch32_int16_soa = _make_soa_entry(32, True)
 
 
+def pack_pac(codes):
+    """2-bit .pac bytes of a base-code sequence (codes 0-3; 4 bases per byte,
+    first base in the two most significant bits, src/pipelines/linear/index/
+    bntseq.rs:559-575)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    if codes.size and codes.max() > 3:
+        raise ValueError("a .pac holds codes 0-3 only")
+    n = codes.size
+    pad = np.zeros((n + 3) // 4 * 4, np.uint8)
+    pad[:n] = codes
+    q = pad.reshape(-1, 4).astype(np.uint8)
+    return ((q[:, 0] << 6) | (q[:, 1] << 4) | (q[:, 2] << 2) | q[:, 3]).astype(np.uint8)
+
+
+class DeviceReference:
+    """A 2-bit reference (.pac bytes, l_pac bases) resident in HBM (fa_ref)."""
+
+    def __init__(self, pac, l_pac, ctx=None):
+        self.ctx = ctx or default_context()
+        self.l_pac = int(l_pac)
+        pac = np.ascontiguousarray(pac, np.uint8)
+        if pac.size < (self.l_pac + 3) // 4:
+            raise ValueError("pac holds fewer than l_pac bases")
+        self._h = ctypes.c_void_p()
+        N.check(N.lib().fa_ref_upload(self.ctx.handle, N.ptr(pac), self.l_pac, ctypes.byref(self._h)),
+                "fa_ref_upload")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def free(self):
+        if self._h:
+            N.lib().fa_ref_free(self.ctx.handle, self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceBatch:
     """A batch resident in HBM (fa_batch): upload once, run many times."""
+
+    @classmethod
+    def from_reference(cls, sw, qseq, qoff, qlen, ref, seg_beg, seg_len, seg_rev, h0, w, ctx=None):
+        """Batch whose targets are segments of a DeviceReference, read on the
+        device: target j = [seg_beg[j], seg_beg[j] + seg_len[j]) of the
+        bidirectional coordinates [0, 2 l_pac), backwards when seg_rev[j]."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or ref.ctx
+        self.n = len(qlen)
+        k = [np.ascontiguousarray(x, dt) for x, dt in
+             ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32))]
+        seg = [np.ascontiguousarray(x, dt) for x, dt in
+               ((seg_beg, np.int64), (seg_len, np.int32), (seg_rev, np.uint8), (h0, np.int32), (w, np.int32))]
+        self._h = ctypes.c_void_p()
+        c = sw.to_c()
+        N.check(N.lib().fa_batch_upload_ref(self.ctx.handle, ctypes.byref(c), self.n, *[N.ptr(x) for x in k],
+                                            ref.handle, *[N.ptr(x) for x in seg], ctypes.byref(self._h)),
+                "fa_batch_upload_ref")
+        self._tlen = seg[1]
+        return self
+
+    def download_targets(self, tlen=None):
+        """The batch's targets as one array of codes (target j at the offset
+        of the running sum of lengths) and those offsets."""
+        tlen = np.asarray(tlen if tlen is not None else self._tlen, np.int64)
+        off = np.zeros(len(tlen), np.int64)
+        off[1:] = np.cumsum(tlen)[:-1]
+        codes = np.zeros(int(tlen.sum()), np.uint8)
+        N.check(N.lib().fa_batch_download_targets(self.ctx.handle, self._h, N.ptr(codes), N.ptr(off)),
+                "fa_batch_download_targets")
+        return codes, off
 
     def __init__(self, sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, h0, w, ctx=None):
         self.ctx = ctx or default_context()
@@ -339,6 +413,7 @@ class DeviceBatch:
         k = self._keep
         N.check(N.lib().fa_batch_upload(self.ctx.handle, ctypes.byref(c), self.n, *[N.ptr(x) for x in k],
                                         ctypes.byref(self._h)), "fa_batch_upload")
+        self._tlen = k[5]
         self._keep = None
 
     def set_cost(self, cost):

@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void extend16_plain_kernel(ExtArgs a, uint32_t
     extern __shared__ uint8_t lds[];
     constexpr int G = 64 / L;
     const int wave = (int)(threadIdx.x >> 6);
-    const uint32_t slot0 = (blockIdx.x * 4u + (uint32_t)wave) * (2u * G);
+    const uint32_t slot0 = (blockIdx.x * (blockDim.x >> 6) + (uint32_t)wave) * (2u * G);
     uint8_t *lds_wave = lds + wave * 2 * G * a.tpad;
     if (slot0 >= count) return;
     extend16_wave<L, S>(a, first + slot0, first + count, lds_wave);
@@ -523,8 +523,12 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
     const uint32_t per_block = 4u * 2u * G;
     const uint32_t need = (count + per_block - 1) / per_block;
     if (!persistent) {
-        hipLaunchKernelGGL((extend16_plain_kernel<L, S>), dim3(need), dim3(256), lds, st, a,
-                           first, count);
+        // one wavefront per workgroup: a workgroup's slot on a CU is reused
+        // only when all of its wavefronts are done, so single-wave groups let
+        // short wavefronts hand their slot on at once
+        const uint32_t pw = 2u * G;
+        hipLaunchKernelGGL((extend16_plain_kernel<L, S>), dim3((count + pw - 1) / pw), dim3(64),
+                           (size_t)2 * G * a.tpad, st, a, first, count);
         return hipGetLastError();
     }
     int occ = 0;

@@ -19,6 +19,7 @@
 #include "predictor.h"
 #include "mate_kernel.h"
 #include "valu_probe.h"
+#include "ref_kernel.h"
 
 struct fa_ctx {
     int device = 0;
@@ -47,12 +48,21 @@ struct DevBuf {
 
 }  // namespace
 
+// Device-resident reference in the .pac layout (4 bases per byte, first base
+// in the two most significant bits).
+struct fa_ref {
+    int device = 0;
+    uint8_t *d_pac = nullptr;
+    int64_t l_pac = 0;
+};
+
 struct fa_batch {
     int64_t n = 0;
     // host copies kept for re-packing
     std::vector<int32_t> qlen, tlen, h0, w;
     std::vector<uint8_t> elig16;  // scores provably fit the 16-bit kernel
     std::vector<float> cost;
+    std::vector<uint32_t> qo_host, to_host;  // word offsets of each query / target in d_seq2
     // device
     uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
     int32_t *d_qlen = nullptr, *d_tlen = nullptr, *d_h0 = nullptr, *d_w = nullptr;
@@ -152,6 +162,7 @@ void parallel_for(int64_t n, F &&f) {
 
 inline uint32_t seq_words(int32_t len) { return (uint32_t)((len + 31) / 32) * 2u; }
 
+// t == nullptr: the target comes from a device reference (codes 0-3 by construction)
 int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q,
                  const uint8_t *t) {
     if (h0 <= 0) return FA_ERR_H0;
@@ -161,8 +172,9 @@ int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q
     if (h0 > (1 << 19)) return FA_ERR_SCORING;
     for (int32_t j = 0; j < ql; ++j)
         if (q[j] > 4) return FA_ERR_CODE;
-    for (int32_t j = 0; j < tl; ++j)
-        if (t[j] > 4) return FA_ERR_CODE;
+    if (t)
+        for (int32_t j = 0; j < tl; ++j)
+            if (t[j] > 4) return FA_ERR_CODE;
     return FA_OK;
 }
 
@@ -272,11 +284,23 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         }
     }
     if (pack == FA_PACK_COST) {
+        // A wavefront runs until the last of its pairs stops, so what must
+        // match inside a wavefront is the row count.  Cost classes (a factor
+        // e^(1/2) apart) separate the pairs the costs mark as short (early
+        // z-drop, non-homologous targets) from the full-length ones, and
+        // length order inside a class keeps equal row counts together;
+        // sorting on the raw costs would let their noise scatter pairs of
+        // equal length over many wavefronts.
         if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
-        const float *cost = b->cost.data();
+        std::vector<int> cls((size_t)n);
+        for (int64_t k = 0; k < n; ++k)
+            cls[(size_t)k] = (int)std::lrint(2.0 * std::log1p(std::max(0.0f, b->cost[(size_t)k])));
+        const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
             if (skey[x] != skey[y]) return skey[x] > skey[y];
-            return cost[x] > cost[y];
+            if (cls[x] != cls[y]) return cls[x] > cls[y];
+            if (ql[x] != ql[y]) return ql[x] > ql[y];
+            return tl[x] > tl[y];
         });
     } else if (pack != FA_PACK_NONE) {
         const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
@@ -446,6 +470,7 @@ const char *fa_strerror(int code) {
         case FA_ERR_DEVICE: return "HIP device error";
         case FA_ERR_NODEVICE: return "no usable gfx950 device";
         case FA_ERR_OVERFLOW: return "8-bit score overflow";
+        case FA_ERR_SEGMENT: return "target segment outside the reference or across the strand boundary";
         default: return "unknown error";
     }
 }
@@ -517,19 +542,35 @@ int fa_ctx_describe(fa_ctx *ctx, char *buf, size_t cap) {
     return FA_OK;
 }
 
-int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
-                    const uint8_t *query_seqs, const int64_t *query_offset,
-                    const int32_t *query_len, const uint8_t *ref_seqs, const int64_t *ref_offset,
-                    const int32_t *ref_len, const int32_t *h0, const int32_t *band_width,
-                    fa_batch **out) {
+}  // extern "C"
+
+namespace {
+
+// Where the targets of a batch come from: host codes (seqs + offset) or
+// segments of a device reference (ref, seg_beg, seg_rev).
+struct TargetSource {
+    const uint8_t *seqs = nullptr;
+    const int64_t *offset = nullptr;
+    const fa_ref *ref = nullptr;
+    const int64_t *seg_beg = nullptr;
+    const uint8_t *seg_rev = nullptr;
+};
+
+int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                 const uint8_t *query_seqs, const int64_t *query_offset, const int32_t *query_len,
+                 const TargetSource &ts, const int32_t *ref_len, const int32_t *h0,
+                 const int32_t *band_width, fa_batch **out) {
     if (!ctx || !out || njobs < 0) return FA_ERR_ARG;
     *out = nullptr;
     int rc = check_scoring(sw);
     if (rc) return rc;
-    if (njobs > 0 && (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset ||
-                      !ref_len || !h0 || !band_width))
+    const bool from_ref = ts.ref != nullptr;
+    if (njobs > 0 && (!query_seqs || !query_offset || !query_len || !ref_len || !h0 ||
+                      !band_width || (from_ref ? (!ts.seg_beg || !ts.seg_rev)
+                                               : (!ts.seqs || !ts.offset))))
         return FA_ERR_ARG;
     if (njobs > (int64_t)0x7FFFFFFF) return FA_ERR_ARG;
+    if (from_ref && ts.ref->device != ctx->device) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
 
     fa_batch *b = new (std::nothrow) fa_batch();
@@ -549,10 +590,18 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     int tmax = 0;
     std::vector<int8_t> vrc((size_t)n);
     parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
-        for (int64_t k = lo; k < hi; ++k)
-            vrc[(size_t)k] = (int8_t)validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
-                                                  query_seqs + query_offset[k],
-                                                  ref_seqs + ref_offset[k]);
+        for (int64_t k = lo; k < hi; ++k) {
+            int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                                 query_seqs + query_offset[k],
+                                 from_ref ? nullptr : ts.seqs + ts.offset[k]);
+            if (r == FA_OK && from_ref) {
+                // inside [0, 2 l_pac) and on one strand (get_reference_segment
+                // returns an empty segment for one that bridges, bntseq.rs:461-469)
+                const int64_t b = ts.seg_beg[k], e = b + ref_len[k], lp = ts.ref->l_pac;
+                if (b < 0 || e > 2 * lp || (b < lp && e > lp)) r = FA_ERR_SEGMENT;
+            }
+            vrc[(size_t)k] = (int8_t)r;
+        }
     });
     for (int64_t k = 0; k < n; ++k) {
         int r = vrc[(size_t)k];
@@ -588,8 +637,9 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         for (int64_t k = lo; k < hi; ++k) {
             pack_seq(query_seqs + query_offset[k], query_len[k], &seq2[qo[(size_t)k]],
                      &seqn[qo[(size_t)k] / 2]);
-            pack_seq(ref_seqs + ref_offset[k], ref_len[k], &seq2[to[(size_t)k]],
-                     &seqn[to[(size_t)k] / 2]);
+            if (!from_ref)  // device-reference targets are written by pac_gather below
+                pack_seq(ts.seqs + ts.offset[k], ref_len[k], &seq2[to[(size_t)k]],
+                         &seqn[to[(size_t)k] / 2]);
         }
     });
 
@@ -608,10 +658,28 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         batch_release(b);
         return rc;
     }
+    if (from_ref && n > 0) {
+        int64_t *d_beg = nullptr;
+        uint8_t *d_rev = nullptr;
+        rc = dev_upload(&d_beg, ts.seg_beg, (size_t)n, st);
+        if (!rc) rc = dev_upload(&d_rev, ts.seg_rev, (size_t)n, st);
+        if (!rc && fa::pac_gather(ts.ref->d_pac, ts.ref->l_pac, d_beg, b->d_tlen, d_rev, b->d_to,
+                                  b->d_seq2, (uint32_t)n, st) != hipSuccess)
+            rc = FA_ERR_DEVICE;
+        if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = FA_ERR_DEVICE;
+        if (d_beg) (void)hipFree(d_beg);
+        if (d_rev) (void)hipFree(d_rev);
+        if (rc) {
+            batch_release(b);
+            return rc;
+        }
+    }
     if (hipStreamSynchronize(st) != hipSuccess) {
         batch_release(b);
         return FA_ERR_DEVICE;
     }
+    b->qo_host.swap(qo);
+    b->to_host.swap(to);
 
     fa::ExtArgs &a = b->args;
     a.seq2 = b->d_seq2;
@@ -648,6 +716,86 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     a.tpad = ((tmax + 16) / 16) * 16;
     a.chunk = 0;
     *out = b;
+    return FA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                    const uint8_t *query_seqs, const int64_t *query_offset,
+                    const int32_t *query_len, const uint8_t *ref_seqs, const int64_t *ref_offset,
+                    const int32_t *ref_len, const int32_t *h0, const int32_t *band_width,
+                    fa_batch **out) {
+    TargetSource ts;
+    ts.seqs = ref_seqs;
+    ts.offset = ref_offset;
+    return batch_upload(ctx, sw, njobs, query_seqs, query_offset, query_len, ts, ref_len, h0,
+                        band_width, out);
+}
+
+int fa_ref_upload(fa_ctx *ctx, const uint8_t *pac, int64_t l_pac, fa_ref **out) {
+    if (!ctx || !out || !pac || l_pac <= 0 || l_pac > ((int64_t)1 << 40)) return FA_ERR_ARG;
+    *out = nullptr;
+    FA_HIP(hipSetDevice(ctx->device));
+    fa_ref *r = new (std::nothrow) fa_ref();
+    if (!r) return FA_ERR_ARG;
+    r->device = ctx->device;
+    r->l_pac = l_pac;
+    const size_t bytes = (size_t)((l_pac + 3) / 4);
+    int rc = dev_upload(&r->d_pac, pac, bytes, ctx->stream);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FA_ERR_DEVICE;
+    if (rc) {
+        if (r->d_pac) (void)hipFree(r->d_pac);
+        delete r;
+        return rc;
+    }
+    *out = r;
+    return FA_OK;
+}
+
+int fa_ref_free(fa_ctx *ctx, fa_ref *ref) {
+    if (!ctx || !ref) return FA_ERR_ARG;
+    (void)hipSetDevice(ref->device);
+    if (ref->d_pac) (void)hipFree(ref->d_pac);
+    delete ref;
+    return FA_OK;
+}
+
+int fa_batch_upload_ref(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                        const uint8_t *query_seqs, const int64_t *query_offset,
+                        const int32_t *query_len, const fa_ref *ref, const int64_t *seg_beg,
+                        const int32_t *seg_len, const uint8_t *seg_rev, const int32_t *h0,
+                        const int32_t *band_width, fa_batch **out) {
+    if (!ref) return FA_ERR_ARG;
+    TargetSource ts;
+    ts.ref = ref;
+    ts.seg_beg = seg_beg;
+    ts.seg_rev = seg_rev;
+    return batch_upload(ctx, sw, njobs, query_seqs, query_offset, query_len, ts, seg_len, h0,
+                        band_width, out);
+}
+
+int fa_batch_download_targets(fa_ctx *ctx, fa_batch *b, uint8_t *codes, const int64_t *offset) {
+    if (!ctx || !b || (b->n > 0 && (!codes || !offset))) return FA_ERR_ARG;
+    if (b->n == 0) return FA_OK;
+    FA_HIP(hipSetDevice(ctx->device));
+    const uint32_t last = b->to_host[(size_t)b->n - 1];
+    const size_t words = (size_t)last + seq_words(b->tlen[(size_t)b->n - 1]);
+    std::vector<uint32_t> seq2(words), seqn(words / 2);
+    FA_HIP(hipMemcpy(seq2.data(), b->d_seq2, words * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    FA_HIP(hipMemcpy(seqn.data(), b->d_seqn, words / 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    parallel_for(b->n, [&](int64_t lo, int64_t hi, unsigned) {
+        for (int64_t k = lo; k < hi; ++k) {
+            const uint32_t o = b->to_host[(size_t)k];
+            uint8_t *dst = codes + offset[k];
+            for (int32_t p = 0; p < b->tlen[(size_t)k]; ++p) {
+                const uint32_t c = (seq2[o + (uint32_t)(p >> 4)] >> ((p & 15) * 2)) & 3u;
+                dst[p] = ((seqn[o / 2 + (uint32_t)(p >> 5)] >> (p & 31)) & 1u) ? 4 : (uint8_t)c;
+            }
+        }
+    });
     return FA_OK;
 }
 

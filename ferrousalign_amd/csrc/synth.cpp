@@ -136,13 +136,14 @@ void fa_synth_reference(uint64_t seed, int64_t len, uint8_t *out) {
 //           rising 0.2% -> 2%, 0.1% indels; 5% of targets drawn from a
 //           uniform reference position (non-homologous); 10% use w2.
 // Each pair k writes qlen[k] codes at qseq + k*qstride and tlen[k] codes at
-// tseq + k*tstride (qstride >= 250, tstride >= 350).  Returns 0, or -1 on bad
-// arguments.
+// tseq + k*tstride (qstride >= 250, tstride >= 350); the target is the reference
+// window [tstart, tstart + tlen), tstart[k] written when tstart_out is not NULL.
+// Returns 0, or -1 on bad arguments.
 int fa_synth_extension_pairs(int kind, uint64_t seed, const uint8_t *ref, int64_t ref_len,
                              int64_t n, uint8_t *qseq, int64_t qstride, int32_t *qlen,
                              uint8_t *tseq, int64_t tstride, int32_t *tlen, int32_t *h0,
                              int32_t *w, int32_t match_score, int32_t w1, int32_t w2,
-                             int nthreads) {
+                             int nthreads, int64_t *tstart_out) {
     if (kind < 0 || kind > 2 || n < 0 || ref_len < 4096) return -1;
     if (qstride < 250 || tstride < 350) return -1;
     parallel_for(n, nthreads, [&](int64_t lo, int64_t hi) {
@@ -178,6 +179,7 @@ int fa_synth_extension_pairs(int kind, uint64_t seed, const uint8_t *ref, int64_
             if (random_target) tstart = (int64_t)(rng.uniform() * (double)(ref_len - tl - 1));
             std::memcpy(qseq + k * qstride, read + s + L, (size_t)ql);
             std::memcpy(tseq + k * tstride, ref + tstart, (size_t)tl);
+            if (tstart_out) tstart_out[k] = tstart;
             qlen[k] = ql;
             tlen[k] = tl;
             h0[k] = L * match_score;

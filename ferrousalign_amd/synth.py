@@ -61,6 +61,7 @@ class PairSet:
     tlen: np.ndarray
     h0: np.ndarray
     w: np.ndarray
+    tstart: np.ndarray = None  # target k = reference()[tstart[k] : tstart[k] + tlen[k]]
 
     @property
     def n(self):
@@ -74,7 +75,8 @@ class PairSet:
         return (self.q.reshape(-1), qoff, self.qlen, self.t.reshape(-1), toff, self.tlen, self.h0, self.w)
 
     def subset(self, idx):
-        return PairSet(self.q[idx], self.t[idx], self.qlen[idx], self.tlen[idx], self.h0[idx], self.w[idx])
+        return PairSet(self.q[idx], self.t[idx], self.qlen[idx], self.tlen[idx], self.h0[idx], self.w[idx],
+                       None if self.tstart is None else self.tstart[idx])
 
 
 def extension_pairs(kind, n, seed, w1=100, w2=200, threads=0):
@@ -85,12 +87,13 @@ def extension_pairs(kind, n, seed, w1=100, w2=200, threads=0):
     tl = np.zeros(n, np.int32)
     h0 = np.zeros(n, np.int32)
     w = np.zeros(n, np.int32)
+    ts = np.zeros(n, np.int64)
     rc = N.synth().fa_synth_extension_pairs(kind, ctypes.c_uint64(seed), N.ptr(ref), ref.size, n, N.ptr(q),
                                             QSTRIDE, N.ptr(ql), N.ptr(t), TSTRIDE, N.ptr(tl), N.ptr(h0), N.ptr(w),
-                                            1, w1, w2, threads)
+                                            1, w1, w2, threads, N.ptr(ts))
     if rc != 0:
         raise ValueError("fa_synth_extension_pairs failed")
-    return PairSet(q, t, ql, tl, h0, w)
+    return PairSet(q, t, ql, tl, h0, w, ts)
 
 
 def workload(name, n=None, threads=0):

@@ -35,6 +35,7 @@ extern "C" {
 #define FA_ERR_DEVICE (-8)   /* HIP runtime error */
 #define FA_ERR_NODEVICE (-9) /* no usable gfx950 device */
 #define FA_ERR_OVERFLOW (-10) /* 8-bit path only: a pair exceeded 8-bit range (FA_SCORE_U8 without rerun) */
+#define FA_ERR_SEGMENT (-11)  /* reference target segment outside [0, 2 l_pac) or across l_pac */
 
 #define FA_MAX_QLEN 255
 #define FA_MAX_TLEN 2048
@@ -52,7 +53,8 @@ extern "C" {
 /* packing of pairs into lane groups (output order never changes) */
 #define FA_PACK_LENGTH 0x00u    /* sort by (qlen, tlen): the plain kernel */
 #define FA_PACK_NONE 0x10u      /* input order */
-#define FA_PACK_COST 0x20u      /* sort by a caller-supplied per-pair cost (fa_batch_set_cost) */
+#define FA_PACK_COST 0x20u      /* group by a caller-supplied per-pair cost (fa_batch_set_cost): cost
+                                   classes e^(1/2) apart, length order inside a class */
 /* launch strategy */
 #define FA_LAUNCH_PLAIN 0x000u  /* one launch per query-length bucket */
 #define FA_LAUNCH_PERSISTENT 0x100u /* persistent waves pulling chunks from a device queue */
@@ -91,6 +93,7 @@ typedef struct {
 
 typedef struct fa_ctx fa_ctx;
 typedef struct fa_batch fa_batch;
+typedef struct fa_ref fa_ref;
 
 /* ---- context ------------------------------------------------------------ */
 /* Opens `device` (HIP ordinal).  Replaces the CPU-SIMD engine selection of
@@ -163,6 +166,27 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                     const int32_t *query_len, const uint8_t *ref_seqs,
                     const int64_t *ref_offset, const int32_t *ref_len, const int32_t *h0,
                     const int32_t *band_width, fa_batch **out);
+/* ---- targets from a device-resident reference ------------------------------
+ * The reference's 2-bit .pac (src/pipelines/linear/index/bntseq.rs:559-575:
+ * base p in byte p/4, first base in the two most significant bits, codes
+ * A=0 C=1 G=2 T=3), l_pac forward bases, kept in HBM; replaces the host-side
+ * segment fetch get_reference_segment (:451-522) on the extension path. */
+int fa_ref_upload(fa_ctx *ctx, const uint8_t *pac, int64_t l_pac, fa_ref **out);
+int fa_ref_free(fa_ctx *ctx, fa_ref *ref);
+/* As fa_batch_upload, but target j is read on the device from the reference:
+ * the segment [seg_beg[j], seg_beg[j] + seg_len[j]) of the bidirectional
+ * coordinate space [0, 2 l_pac) (coordinates >= l_pac are the reverse
+ * complement, as get_reference_segment), read backwards when seg_rev[j] != 0
+ * (a left-extension job, collect_soa.rs:156-176).  A segment outside
+ * [0, 2 l_pac) or across l_pac fails with FA_ERR_SEGMENT.  `ref` must stay
+ * alive only during this call. */
+int fa_batch_upload_ref(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                        const uint8_t *query_seqs, const int64_t *query_offset,
+                        const int32_t *query_len, const fa_ref *ref, const int64_t *seg_beg,
+                        const int32_t *seg_len, const uint8_t *seg_rev, const int32_t *h0,
+                        const int32_t *band_width, fa_batch **out);
+/* The batch's targets as base codes, target j at codes + offset[j] (checking aid). */
+int fa_batch_download_targets(fa_ctx *ctx, fa_batch *b, uint8_t *codes, const int64_t *offset);
 /* Per-pair packing cost (e.g. predicted DP cells) used by FA_PACK_COST. */
 int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost);
 /* Launch configuration: lanes per pair (4, 8 or 16, widened per query length

@@ -52,6 +52,13 @@ int64_t ora_extend_batch(int64_t n, const uint8_t *qseq, const int64_t *qoff,
                          int32_t zdrop, ora_ext_result *out, uint64_t *cells,
                          int32_t *rc, int nthreads);
 
+/* Target segment of a 2-bit .pac reference (pac_oracle.c): `len` bases from
+ * `start` in the bidirectional coordinates [0, 2 l_pac) (>= l_pac: reverse
+ * complement), written backwards when `reverse`.  Returns the number of bases,
+ * or -1 for a segment outside [0, 2 l_pac) or across l_pac. */
+int64_t ora_pac_segment(const uint8_t *pac, int64_t l_pac, int64_t start, int64_t len,
+                        int reverse, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -156,3 +156,17 @@ def ksw_align2_batch(q, qoff, qlen, t, toff, tlen, mat, o_del, e_del, o_ins, e_i
     _align_lib().ora_ksw_align2_batch(n, *[P(a) for a in arrs], 5, P(mat), o_del, e_del, o_ins, e_ins, xtra,
                                       P(out), P(cells), P(rc), nthreads)
     return out, cells, rc
+
+
+def pac_segment(pac, l_pac, start, length, reverse=False):
+    """Target segment of a .pac reference (oracle/pac_oracle.c), None when the
+    reference would return an empty segment (outside [0, 2 l_pac) or across
+    l_pac)."""
+    l = lib()
+    l.ora_pac_segment.restype = ctypes.c_int64
+    l.ora_pac_segment.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                  ctypes.c_void_p]
+    pac = np.ascontiguousarray(pac, np.uint8)
+    out = np.zeros(max(int(length), 1), np.uint8)
+    n = l.ora_pac_segment(P(pac), int(l_pac), int(start), int(length), int(bool(reverse)), P(out))
+    return None if n < 0 else out[:n]

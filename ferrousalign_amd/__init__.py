@@ -16,6 +16,8 @@ from .banded_swa import (  # noqa: F401
     execute_batch_simd_scoring,
     extend_arrays,
     pack_pac,
+    scalar_banded_swa,
+    scalar_banded_swa_batch,
     simd_banded_swa_batch8_int16_soa,
     simd_banded_swa_batch16_int16_soa,
     simd_banded_swa_batch16_soa,

@@ -245,6 +245,64 @@ def extend_arrays(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, h0, 
     return out, cells
 
 
+CIGAR_OPS = "MIDNSHP=X"
+
+
+def scalar_banded_swa_batch(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, w, h0=None,
+                            ctx: Optional[Context] = None, want_cells=False):
+    """Batched scalar_banded_swa (banded_swa/scalar/implementation.rs:5-440) on
+    the GPU: banded local SW with traceback, as region.rs:1058 runs it to
+    regenerate CIGARs.  Returns (out[n,6] in OutScore order, CIGARs as lists
+    of (op char, len), aligned reference bases, aligned query bases, cells)."""
+    ctx = ctx or default_context()
+    n = len(qlen)
+    qseq = np.ascontiguousarray(qseq, np.uint8)
+    tseq = np.ascontiguousarray(tseq, np.uint8)
+    qoff = np.ascontiguousarray(qoff, np.int64)
+    toff = np.ascontiguousarray(toff, np.int64)
+    qlen = np.ascontiguousarray(qlen, np.int32)
+    tlen = np.ascontiguousarray(tlen, np.int32)
+    w = np.ascontiguousarray(w, np.int32)
+    h0 = np.ascontiguousarray(np.zeros(n, np.int32) if h0 is None else h0, np.int32)
+    cap_c = qlen.astype(np.int64) + tlen + 4
+    cap_a = qlen.astype(np.int64) + tlen + 2
+    coff = np.zeros(n, np.int64)
+    aoff = np.zeros(n, np.int64)
+    if n:
+        coff[1:] = np.cumsum(cap_c)[:-1]
+        aoff[1:] = np.cumsum(cap_a)[:-1]
+    out = np.zeros((n, 6), np.int32)
+    cells = np.zeros(n, np.uint32)
+    cig = np.zeros(int(cap_c.sum()) if n else 1, np.uint32)
+    ra = np.zeros(int(cap_a.sum()) if n else 1, np.uint8)
+    qa = np.zeros_like(ra)
+    nc = np.zeros(n, np.int32)
+    nr = np.zeros(n, np.int32)
+    nq = np.zeros(n, np.int32)
+    c = sw.to_c()
+    N.check(N.lib().fa_scalar_banded_swa_batch(
+        ctx.handle, ctypes.byref(c), n, N.ptr(qseq), N.ptr(qoff), N.ptr(qlen), N.ptr(tseq), N.ptr(toff),
+        N.ptr(tlen), N.ptr(w), N.ptr(h0), N.ptr(out), N.ptr(cells), N.ptr(cig), N.ptr(coff), N.ptr(nc),
+        N.ptr(ra), N.ptr(qa), N.ptr(aoff), N.ptr(nr), N.ptr(nq)), "fa_scalar_banded_swa_batch")
+    cigars = [[(CIGAR_OPS[int(x) & 15], int(x) >> 4) for x in cig[coff[k]:coff[k] + nc[k]]] for k in range(n)]
+    ref_al = [ra[aoff[k]:aoff[k] + nr[k]].copy() for k in range(n)]
+    qry_al = [qa[aoff[k]:aoff[k] + nq[k]].copy() for k in range(n)]
+    return out, cigars, ref_al, qry_al, (cells if want_cells else None)
+
+
+def scalar_banded_swa(sw_params: BandedPairWiseSW, qlen, query, tlen, target, w, h0,
+                      ctx: Optional[Context] = None):
+    """implementation.rs:5-13 signature: (OutScore, CIGAR [(op char, len)],
+    ref_aligned, query_aligned) for one pair, computed on the GPU."""
+    q = np.ascontiguousarray(query[:qlen], np.uint8)
+    t = np.ascontiguousarray(target[:tlen], np.uint8)
+    out, cig, ra, qa, _ = scalar_banded_swa_batch(sw_params, q, [0], [len(q)], t, [0], [len(t)], [w], [h0],
+                                                  ctx=ctx)
+    o = out[0]
+    return (OutScore(int(o[0]), int(o[1]), int(o[2]), int(o[3]), int(o[4]), int(o[5])), cig[0],
+            list(ra[0]), list(qa[0]))
+
+
 def execute_batch_simd_scoring(sw_params: BandedPairWiseSW, batch: ExtensionJobBatch, engine=None,
                                ctx: Optional[Context] = None) -> List[BatchExtensionResult]:
     """dispatch.rs:31-86: score every job of the batch; one result per job, in job order."""

@@ -20,6 +20,7 @@
 #include "mate_kernel.h"
 #include "valu_probe.h"
 #include "ref_kernel.h"
+#include "swtb_kernel.h"
 
 struct fa_ctx {
     int device = 0;
@@ -796,6 +797,206 @@ int fa_batch_download_targets(fa_ctx *ctx, fa_batch *b, uint8_t *codes, const in
             }
         }
     });
+    return FA_OK;
+}
+
+int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                               const uint8_t *query_seqs, const int64_t *query_offset,
+                               const int32_t *query_len, const uint8_t *ref_seqs,
+                               const int64_t *ref_offset, const int32_t *ref_len,
+                               const int32_t *band_width, const int32_t *h0, fa_out_score *out,
+                               uint32_t *cells, uint32_t *cigar, const int64_t *cigar_offset,
+                               int32_t *n_cigar, uint8_t *ref_aligned, uint8_t *query_aligned,
+                               const int64_t *aligned_offset, int32_t *n_ref_aligned,
+                               int32_t *n_query_aligned) {
+    if (!ctx || njobs < 0) return FA_ERR_ARG;
+    int rc = check_scoring(sw);
+    if (rc) return rc;
+    if (njobs == 0) return FA_OK;
+    if (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset || !ref_len ||
+        !band_width || !h0 || !out || !cigar || !cigar_offset || !n_cigar || !ref_aligned ||
+        !query_aligned || !aligned_offset || !n_ref_aligned || !n_query_aligned)
+        return FA_ERR_ARG;
+    if (njobs > (int64_t)0x7FFFFFFF) return FA_ERR_ARG;
+    // validation (h0 = 0 is allowed here: region.rs:1058 passes 0)
+    for (int64_t k = 0; k < njobs; ++k) {
+        const int32_t ql = query_len[k], tl = ref_len[k];
+        if (ql < 0 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
+        if (tl < 0 || tl > FA_MAX_TLEN) return FA_ERR_TLEN;
+        if (band_width[k] < 0) return FA_ERR_BAND;
+        if (h0[k] < 0 || h0[k] > (1 << 19)) return FA_ERR_H0;
+        for (int32_t j = 0; j < ql; ++j)
+            if (query_seqs[query_offset[k] + j] > 4) return FA_ERR_CODE;
+        for (int32_t j = 0; j < tl; ++j)
+            if (ref_seqs[ref_offset[k] + j] > 4) return FA_ERR_CODE;
+    }
+    FA_HIP(hipSetDevice(ctx->device));
+    // non-empty jobs run on the device; empty ones get the reference's zero
+    // record and an empty CIGAR (implementation.rs:17-30)
+    std::vector<int64_t> job;
+    for (int64_t k = 0; k < njobs; ++k) {
+        if (query_len[k] > 0 && ref_len[k] > 0) {
+            job.push_back(k);
+        } else {
+            std::memset(&out[k], 0, sizeof(fa_out_score));
+            if (cells) cells[k] = 0;
+            n_cigar[k] = n_ref_aligned[k] = n_query_aligned[k] = 0;
+        }
+    }
+    const size_t m = job.size();
+    if (m == 0) return FA_OK;
+    std::vector<uint32_t> qo(m), to(m), lsl(m), order(m);
+    std::vector<int32_t> ql(m), tl(m), ww(m), hh(m);
+    std::vector<uint64_t> tbo(m), cgo(m), alo(m);
+    uint64_t words = 0, tbw = 0, cgw = 0, alw = 0;
+    int tmax = 0;
+    for (size_t p = 0; p < m; ++p) {
+        const int64_t k = job[p];
+        ql[p] = query_len[k];
+        tl[p] = ref_len[k];
+        ww[p] = band_width[k];
+        hh[p] = h0[k];
+        int L = 16;
+        for (int c : {4, 8, 16})
+            if (fa::swtb_slot_count(c, ql[p]) > 0) {
+                L = c;
+                break;
+            }
+        const int S = fa::swtb_slot_count(L, ql[p]);
+        lsl[p] = ((uint32_t)L << 8) | (uint32_t)S;
+        qo[p] = (uint32_t)words;
+        words += seq_words(ql[p]);
+        to[p] = (uint32_t)words;
+        words += seq_words(tl[p]);
+        tbo[p] = tbw;
+        tbw += (uint64_t)tl[p] * (uint64_t)L * (uint64_t)((S + 15) / 16);
+        cgo[p] = cgw;
+        cgw += (uint64_t)(ql[p] + tl[p] + 4);
+        alo[p] = alw;
+        alw += (uint64_t)(ql[p] + tl[p] + 2);
+        tmax = std::max(tmax, tl[p]);
+        order[p] = (uint32_t)p;
+    }
+    if (words > 0xFFFFFFF0ull) return FA_ERR_ARG;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        if (lsl[x] != lsl[y]) return lsl[x] > lsl[y];
+        if (ql[x] != ql[y]) return ql[x] > ql[y];
+        return tl[x] > tl[y];
+    });
+    std::vector<uint32_t> seq2(words), seqn(words / 2);
+    for (size_t p = 0; p < m; ++p) {
+        const int64_t k = job[p];
+        pack_seq(query_seqs + query_offset[k], ql[p], &seq2[qo[p]], &seqn[qo[p] / 2]);
+        pack_seq(ref_seqs + ref_offset[k], tl[p], &seq2[to[p]], &seqn[to[p] / 2]);
+    }
+    hipStream_t st = ctx->stream;
+    uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr,
+             *d_order = nullptr, *d_lsl = nullptr, *d_tb = nullptr, *d_cells = nullptr,
+             *d_cig = nullptr;
+    int32_t *d_ql = nullptr, *d_tl = nullptr, *d_w = nullptr, *d_h0 = nullptr, *d_out = nullptr,
+            *d_nc = nullptr, *d_nr = nullptr, *d_nq = nullptr;
+    uint64_t *d_tbo = nullptr, *d_cgo = nullptr, *d_alo = nullptr;
+    uint8_t *d_ra = nullptr, *d_qa = nullptr;
+    auto release = [&] {
+        void *ptrs[] = {d_seq2, d_seqn, d_qo, d_to, d_order, d_lsl, d_tb, d_cells, d_cig, d_ql,
+                        d_tl, d_w, d_h0, d_out, d_nc, d_nr, d_nq, d_tbo, d_cgo, d_alo, d_ra, d_qa};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+    };
+    if ((rc = dev_upload(&d_seq2, seq2.data(), seq2.size(), st)) ||
+        (rc = dev_upload(&d_seqn, seqn.data(), seqn.size(), st)) ||
+        (rc = dev_upload(&d_qo, qo.data(), m, st)) || (rc = dev_upload(&d_to, to.data(), m, st)) ||
+        (rc = dev_upload(&d_ql, ql.data(), m, st)) || (rc = dev_upload(&d_tl, tl.data(), m, st)) ||
+        (rc = dev_upload(&d_w, ww.data(), m, st)) || (rc = dev_upload(&d_h0, hh.data(), m, st)) ||
+        (rc = dev_upload(&d_order, order.data(), m, st)) ||
+        (rc = dev_upload(&d_lsl, lsl.data(), m, st)) ||
+        (rc = dev_upload(&d_tbo, tbo.data(), m, st)) ||
+        (rc = dev_upload(&d_cgo, cgo.data(), m, st)) ||
+        (rc = dev_upload(&d_alo, alo.data(), m, st)) || (rc = dev_alloc(&d_tb, (size_t)tbw)) ||
+        (rc = dev_alloc(&d_out, m * 6)) || (rc = dev_alloc(&d_cells, m)) ||
+        (rc = dev_alloc(&d_cig, (size_t)cgw)) || (rc = dev_alloc(&d_nc, m)) ||
+        (rc = dev_alloc(&d_nr, m)) || (rc = dev_alloc(&d_nq, m)) ||
+        (rc = dev_alloc(&d_ra, (size_t)alw)) || (rc = dev_alloc(&d_qa, (size_t)alw))) {
+        release();
+        return rc;
+    }
+    fa::SwtbArgs a{};
+    a.seq2 = d_seq2;
+    a.seqn = d_seqn;
+    a.qo = d_qo;
+    a.to = d_to;
+    a.qlen = d_ql;
+    a.tlen = d_tl;
+    a.w = d_w;
+    a.h0 = d_h0;
+    a.order = d_order;
+    a.tbo = d_tbo;
+    a.lsl = d_lsl;
+    a.tb = d_tb;
+    a.out = d_out;
+    a.cells = d_cells;
+    a.cig_off = d_cgo;
+    a.cigar = d_cig;
+    a.n_cigar = d_nc;
+    a.aln_off = d_alo;
+    a.ref_aln = d_ra;
+    a.qry_aln = d_qa;
+    a.n_ref = d_nr;
+    a.n_qry = d_nq;
+    for (int q = 0; q < 5; ++q) {
+        uint32_t v = 0;
+        for (int t = 0; t < 4; ++t) v |= (uint32_t)(uint8_t)(sw->mat[t * 5 + q] + 128) << (8 * t);
+        a.profw[q] = v;
+    }
+    a.nrow = (uint32_t)(uint8_t)(sw->mat[20] + 128);
+    a.o_del = sw->o_del;
+    a.e_del = sw->e_del;
+    a.o_ins = sw->o_ins;
+    a.e_ins = sw->e_ins;
+    a.zdrop = sw->zdrop;
+    a.pen_clip3 = sw->pen_clip3;
+    a.tpad = ((tmax + 16) / 16) * 16;
+    hipError_t e = hipSuccess;
+    for (size_t x = 0; x < m && e == hipSuccess;) {  // one launch per (L, S) bucket
+        size_t y = x;
+        while (y < m && lsl[order[y]] == lsl[order[x]]) ++y;
+        const uint32_t key = lsl[order[x]];
+        e = fa::swtb_dp_launch((int)(key >> 8), (int)(key & 0xFFu), a, (uint32_t)x,
+                               (uint32_t)(y - x), st);
+        x = y;
+    }
+    if (e == hipSuccess) e = fa::swtb_traceback_launch(a, (uint32_t)m, st);
+    std::vector<int32_t> hout(m * 6), hnc(m), hnr(m), hnq(m);
+    std::vector<uint32_t> hcells(m), hcig((size_t)cgw);
+    std::vector<uint8_t> hra((size_t)alw), hqa((size_t)alw);
+    if (e == hipSuccess) e = hipMemcpyAsync(hout.data(), d_out, m * 6 * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hcells.data(), d_cells, m * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hnc.data(), d_nc, m * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hnr.data(), d_nr, m * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hnq.data(), d_nq, m * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hcig.data(), d_cig, cgw * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hra.data(), d_ra, alw, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hqa.data(), d_qa, alw, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    release();
+    if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
+    for (size_t p = 0; p < m; ++p) {
+        const int64_t k = job[p];
+        const int32_t *o = &hout[p * 6];
+        out[k].score = o[0];
+        out[k].target_end_pos = o[1];
+        out[k].gtarget_end_pos = o[2];
+        out[k].query_end_pos = o[3];
+        out[k].global_score = o[4];
+        out[k].max_offset = o[5];
+        if (cells) cells[k] = hcells[p];
+        n_cigar[k] = hnc[p];
+        n_ref_aligned[k] = hnr[p];
+        n_query_aligned[k] = hnq[p];
+        std::memcpy(cigar + cigar_offset[k], &hcig[cgo[p]], (size_t)hnc[p] * 4);
+        std::memcpy(ref_aligned + aligned_offset[k], &hra[alo[p]], (size_t)hnr[p]);
+        std::memcpy(query_aligned + aligned_offset[k], &hqa[alo[p]], (size_t)hnq[p]);
+    }
     return FA_OK;
 }
 

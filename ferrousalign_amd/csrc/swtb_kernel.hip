@@ -1,0 +1,395 @@
+// Banded local Smith-Waterman with traceback for gfx950, exact to the
+// reference's scalar_banded_swa (src/core/alignment/banded_swa/scalar/
+// implementation.rs:5-440), the aligner behind CIGAR regeneration
+// (src/pipelines/linear/region.rs:1058).
+//
+// DP kernel: the row-parallel schedule of the 32-bit extension kernel (one
+// lane group of L lanes per pair, S contiguous columns per lane, F as an
+// exclusive max-plus prefix with one DPP scan, DPP all-reduces for the row
+// maximum and the next band), with this aligner's recurrences:
+//   * M = max(eh.h + score, 0) (no M = 0 rule for eh.h = 0);
+//   * H = max(M, E, F) and a traceback code per cell: MATCH when M >= E and
+//     M >= F, else DEL when E >= F, else INS (:147-158);
+//   * the band w is used as given (no end_bonus cap).
+// Every executed row writes the 2-bit codes of the lane's S columns to the
+// pair's traceback block in HBM, 0 (MATCH, the reference's initial value)
+// outside the row's band.  F at the band start is the prefix identity
+// instead of the reference's f = 0; both lose to M >= 0 and E >= 0, so H and
+// the codes agree.
+//
+// Traceback kernel: one lane per pair walks the codes from (max_i+1, max_j+1)
+// to (0, 0) exactly as the reference (runs of equal codes, row 0 = INS,
+// column 0 = DEL, 10,000-step cap, stop without progress), then adds the 5'
+// soft clip and the 3' clip-or-extend decision (:355-414) and writes the
+// aligned reference and query bases.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swtb_kernel.h"
+
+namespace fa {
+
+namespace {
+
+constexpr int kNeg = -(1 << 28);
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
+constexpr int kRowShr = 0x110;
+enum { TB_MATCH = 0, TB_DEL = 1, TB_INS = 2 };
+enum { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4 };
+
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int old, int src) {
+    return __builtin_amdgcn_update_dpp(old, src, CTRL, 0xF, 0xF, false);
+}
+template <int L>
+__device__ __forceinline__ int group_allmax(int x) {
+    x = imax(x, dpp<kQuadXor1>(x, x));
+    if constexpr (L >= 4) x = imax(x, dpp<kQuadXor2>(x, x));
+    if constexpr (L >= 8) x = imax(x, dpp<kRowHalfMirror>(x, x));
+    if constexpr (L >= 16) x = imax(x, dpp<kRowMirror>(x, x));
+    return x;
+}
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+    const uint32_t lo = (uint32_t)imax((int)(a & 0xFFFFu), (int)(b & 0xFFFFu));
+    const uint32_t hi = (uint32_t)imax((int)(a >> 16), (int)(b >> 16));
+    return lo | (hi << 16);
+}
+template <int L>
+__device__ __forceinline__ uint32_t group_allmax_pk(uint32_t x) {
+    x = pk_max_u16(x, (uint32_t)dpp<kQuadXor1>((int)x, (int)x));
+    if constexpr (L >= 4) x = pk_max_u16(x, (uint32_t)dpp<kQuadXor2>((int)x, (int)x));
+    if constexpr (L >= 8) x = pk_max_u16(x, (uint32_t)dpp<kRowHalfMirror>((int)x, (int)x));
+    if constexpr (L >= 16) x = pk_max_u16(x, (uint32_t)dpp<kRowMirror>((int)x, (int)x));
+    return x;
+}
+template <int L, int K>
+__device__ __forceinline__ int group_shr(int x, int l, int ident) {
+    int y = dpp<kRowShr + K>(ident, x);
+    if constexpr (L < 16) y = (l >= K) ? y : ident;
+    return y;
+}
+template <int L>
+__device__ __forceinline__ int group_excl_scan_max(int x, int l) {
+    int y = group_shr<L, 1>(x, l, kNeg);
+    if constexpr (L > 1) y = imax(y, group_shr<L, 1>(y, l, kNeg));
+    if constexpr (L > 2) y = imax(y, group_shr<L, 2>(y, l, kNeg));
+    if constexpr (L > 4) y = imax(y, group_shr<L, 4>(y, l, kNeg));
+    if constexpr (L > 8) y = imax(y, group_shr<L, 8>(y, l, kNeg));
+    return y;
+}
+__device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *seqn, uint32_t off,
+                                        int pos) {
+    const int code = (int)((seq2[off + (pos >> 4)] >> ((pos & 15) * 2)) & 3u);
+    return ((seqn[(off >> 1) + (pos >> 5)] >> (pos & 31)) & 1u) ? 4 : code;
+}
+
+}  // namespace
+
+template <int L, int S>
+__global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first, uint32_t count) {
+    extern __shared__ uint8_t lds[];
+    constexpr int G = 64 / L;
+    constexpr int NWL = (S + 15) / 16;
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane / L;
+    const int l = lane % L;
+    const uint32_t slot = blockIdx.x * (uint32_t)G + (uint32_t)g;
+    const bool valid = slot < count;
+    const uint32_t pid = valid ? a.order[first + slot] : 0u;
+    const int qlen = valid ? a.qlen[pid] : 1;
+    const int tlen = valid ? a.tlen[pid] : 0;
+    const int h0 = valid ? a.h0[pid] : 0;
+    const int w = valid ? a.w[pid] : 0;
+    const uint32_t qo = valid ? a.qo[pid] : 0u, to = valid ? a.to[pid] : 0u;
+    uint32_t *tbp = a.tb + (valid ? a.tbo[pid] : 0ull) + (uint32_t)(l * NWL);
+    const int o_del = a.o_del, e_del = a.e_del, o_ins = a.o_ins, e_ins = a.e_ins;
+    const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
+
+    uint8_t *tcodes = lds + g * a.tpad;
+    for (int k = l; k < tlen; k += L) tcodes[k] = (uint8_t)seq_code(a.seq2, a.seqn, to, k);
+
+    const int cb = l * S;
+    uint32_t prof[S];
+    int Hh[S], Ee[S];
+    const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int c = cb + s;
+        prof[s] = a.profw[c < qlen ? seq_code(a.seq2, a.seqn, qo, c) : 4];
+        const int hv = c == 0 ? h0 : imax(v1 - (c - 1) * e_ins, 0);  // :73-81
+        Hh[s] = c <= qlen ? hv : 0;
+        Ee[s] = 0;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+
+    int beg = 0, end = qlen;
+    int best = h0, max_i = -1, max_j = -1, max_ie = -1, gscore = -1, max_off = 0;
+    uint32_t ncell = 0;
+    bool alive = valid && tlen > 0;
+    for (int i = 0;; ++i) {
+        if (__builtin_amdgcn_ballot_w64(alive) == 0) break;
+        const int tbase = alive ? tcodes[i] : 0;
+        int rb = imax(beg, i - w);
+        int re = imin(imin(end, i + w + 1), qlen);
+        if (!alive) {
+            rb = 1 << 20;
+            re = -4;
+        }
+        const int h1i = rb == 0 ? imax(h0 - (o_del + e_del * (i + 1)), 0) : 0;
+        const uint32_t sel = 0x0C0C0C00u | (uint32_t)tbase;
+
+        int M[S], pre[S];
+        int run = kNeg;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = cb + s;
+            const int sb = (int)__builtin_amdgcn_perm(a.nrow, prof[s], sel);
+            M[s] = imax(Hh[s] + sb - 128, 0);  // :144-146
+            int gk = imax(M[s] - oe_ins, 0) + c * e_ins;
+            gk = c >= rb ? gk : kNeg;
+            pre[s] = run;
+            run = imax(run, gk);
+        }
+        const int carry = group_excl_scan_max<L>(run, l);
+
+        int H[S];
+        int kmax = 0;
+        uint32_t codes[NWL];
+#pragma unroll
+        for (int k = 0; k < NWL; ++k) codes[k] = 0u;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = cb + s;
+            const int F = imax(carry, pre[s]) - (c - 1) * e_ins;
+            const int m = M[s], e = Ee[s];
+            const int h = imax(imax(m, e), F);
+            const uint32_t code = (m >= e && m >= F) ? TB_MATCH : (e >= F ? TB_DEL : TB_INS);
+            H[s] = h;
+            const int en = imax(imax(e - e_del, m - oe_del), 0);
+            const bool inb = c >= rb && c < re;
+            kmax = imax(kmax, inb ? ((h << 10) | (1023 - c)) : 0);
+            Ee[s] = inb ? en : (c == re ? 0 : e);
+            codes[s / 16] |= (inb ? code : 0u) << (2 * (s % 16));
+        }
+        if (alive) {
+#pragma unroll
+            for (int k = 0; k < NWL; ++k) tbp[(size_t)i * (L * NWL) + k] = codes[k];
+        }
+        const int hlast = dpp<kRowShr + 1>(0, H[S - 1]);
+        int hend = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = cb + s;
+            const int hp = s > 0 ? H[s - 1] : hlast;
+            const int hv = c <= rb ? h1i : hp;
+            const bool wr = c >= rb && c <= re;
+            Hh[s] = wr ? hv : Hh[s];
+            hend = c == re ? Hh[s] : hend;
+        }
+        uint32_t fl = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = cb + s;
+            const bool nz = (Hh[s] | Ee[s]) != 0;
+            const uint32_t f = (nz && c >= rb && c < re) ? (uint32_t)(1023 - c) : 0u;
+            const uint32_t la = (nz && c >= rb && c <= re) ? (uint32_t)(c + 1) : 0u;
+            fl = pk_max_u16(fl, f | (la << 16));
+        }
+        kmax = group_allmax<L>(kmax);
+        hend = group_allmax<L>(hend);
+        fl = group_allmax_pk<L>(fl);
+
+        if (alive) {
+            ncell += (uint32_t)imax(re - rb, 0);
+            if (re == qlen && gscore < hend) {  // :191-194
+                gscore = hend;
+                max_ie = i;
+            }
+            const int m_row = kmax >> 10;
+            const int mj = 1023 - (kmax & 1023);
+            bool brk = m_row == 0;  // :196-198
+            if (!brk) {
+                if (m_row > best) {
+                    best = m_row;
+                    max_i = i;
+                    max_j = mj;
+                    max_off = imax(max_off, mj > i ? mj - i : i - mj);
+                } else if (a.zdrop > 0) {
+                    const int di = i - max_i, dj = mj - max_j;
+                    if (di > dj) {
+                        if (best - m_row - (di - dj) * e_del > a.zdrop) brk = true;
+                    } else if (best - m_row - (dj - di) * e_ins > a.zdrop) {
+                        brk = true;
+                    }
+                }
+            }
+            if (!brk) {  // :215-224
+                const int fenc = (int)(fl & 0xFFFFu);
+                const int nb = fenc ? 1023 - fenc : re;
+                const int ne = imax(nb, (int)(fl >> 16) - 1);
+                beg = nb;
+                end = imin(ne + 2, qlen);
+            }
+            if (brk || i + 1 >= tlen) alive = false;
+        }
+    }
+    if (valid && l == 0) {
+        int32_t *o = a.out + (size_t)pid * 6;
+        o[0] = best;
+        o[1] = max_i + 1;
+        o[2] = max_ie + 1;
+        o[3] = max_j + 1;
+        o[4] = gscore;
+        o[5] = max_off;
+        if (a.cells) a.cells[pid] = ncell;
+    }
+}
+
+namespace {
+
+struct TbView {
+    const uint32_t *tb;
+    int L, S, nwl;
+    __device__ int code(int ci, int cj) const {
+        if (ci == 0) return cj > 0 ? TB_INS : TB_MATCH;
+        if (cj == 0) return TB_DEL;
+        const int j = cj - 1, lane = j / S, s = j % S;
+        const uint32_t w = tb[(size_t)(ci - 1) * (size_t)(L * nwl) + (size_t)(lane * nwl + s / 16)];
+        return (int)((w >> (2 * (s % 16))) & 3u);
+    }
+};
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int qlen = a.qlen[p], tlen = a.tlen[p];
+    uint32_t *cig = a.cigar + a.cig_off[p];
+    uint8_t *ra = a.ref_aln + a.aln_off[p];
+    uint8_t *qa = a.qry_aln + a.aln_off[p];
+    if (qlen <= 0 || tlen <= 0) {  // :17-30
+        a.n_cigar[p] = 0;
+        a.n_ref[p] = 0;
+        a.n_qry[p] = 0;
+        return;
+    }
+    const int32_t *o = a.out + (size_t)p * 6;
+    const int score = o[0], max_i = o[1] - 1, max_j = o[3] - 1, gscore = o[4];
+    const uint32_t lsl = a.lsl[p];
+    TbView v{a.tb + a.tbo[p], (int)(lsl >> 8), (int)(lsl & 0xFFu), (int)((lsl & 0xFFu) + 15) / 16};
+    const uint32_t qo = a.qo[p], to = a.to[p];
+    // runs and bases are produced end to start: keep them from index 1
+    // (cig[0] is kept for the 5' clip), reverse afterwards
+    int nrc = 0, nra = 0, nqa = 0;
+    int ci = max_i + 1, cj = max_j + 1, iter = 0;
+    while (ci > 0 || cj > 0) {
+        if (iter >= 10000) break;
+        ++iter;
+        const int pi = ci, pj = cj;
+        const int code = v.code(ci, cj);
+        int cnt = 0;
+        if (code == TB_MATCH) {
+            while (ci > 0 && cj > 0 && v.code(ci, cj) == TB_MATCH) {
+                ++cnt;
+                ra[nra++] = (uint8_t)seq_code(a.seq2, a.seqn, to, ci - 1);
+                qa[nqa++] = (uint8_t)seq_code(a.seq2, a.seqn, qo, cj - 1);
+                --ci;
+                --cj;
+            }
+            if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_M;
+        } else if (code == TB_DEL) {
+            while (ci > 0 && v.code(ci, cj) == TB_DEL) {
+                ++cnt;
+                ra[nra++] = (uint8_t)seq_code(a.seq2, a.seqn, to, ci - 1);
+                --ci;
+            }
+            if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_D;
+        } else if (code == TB_INS) {
+            while (cj > 0 && v.code(ci, cj) == TB_INS) {
+                ++cnt;
+                qa[nqa++] = (uint8_t)seq_code(a.seq2, a.seqn, qo, cj - 1);
+                --cj;
+            }
+            if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_I;
+        } else {
+            break;
+        }
+        if (ci == pi && cj == pj) break;
+    }
+    for (int x = 0, y = nrc - 1; x < y; ++x, --y) {
+        const uint32_t t = cig[1 + x];
+        cig[1 + x] = cig[1 + y];
+        cig[1 + y] = t;
+    }
+    for (int x = 0, y = nra - 1; x < y; ++x, --y) {
+        const uint8_t t = ra[x];
+        ra[x] = ra[y];
+        ra[y] = t;
+    }
+    for (int x = 0, y = nqa - 1; x < y; ++x, --y) {
+        const uint8_t t = qa[x];
+        qa[x] = qa[y];
+        qa[y] = t;
+    }
+    int nc;
+    const int query_start = cj, query_end = max_j + 1;
+    if (query_start > 0) {  // 5' soft clip
+        cig[0] = ((uint32_t)query_start << 4) | OP_S;
+        nc = 1 + nrc;
+    } else {
+        for (int x = 0; x < nrc; ++x) cig[x] = cig[x + 1];
+        nc = nrc;
+    }
+    if (query_end < qlen) {  // 3' clip or extend (:367-414)
+        const bool clip3 = gscore <= 0 || gscore <= score - a.pen_clip3;
+        cig[nc++] = ((uint32_t)(qlen - query_end) << 4) | (clip3 ? OP_S : OP_M);
+    }
+    a.n_cigar[p] = nc;
+    a.n_ref[p] = nra;
+    a.n_qry[p] = nqa;
+}
+
+template <int L, int S>
+static hipError_t launch_dp(const SwtbArgs &a, uint32_t first, uint32_t count, hipStream_t st) {
+    constexpr int G = 64 / L;
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL((swtb_dp_kernel<L, S>), dim3((count + G - 1) / G), dim3(64),
+                       (size_t)G * a.tpad, st, a, first, count);
+    return hipGetLastError();
+}
+
+int swtb_slot_count(int L, int qlen) {
+    static const int sl[] = {1, 2, 4, 8, 12, 16, 24, 32};
+    if (L != 4 && L != 8 && L != 16) return -1;
+    for (int s : sl)
+        if (L * s >= qlen + 1) return s;
+    return -1;
+}
+
+hipError_t swtb_dp_launch(int L, int S, const SwtbArgs &a, uint32_t first, uint32_t count,
+                          hipStream_t st) {
+#define FA_SW_CASE(LL, SS) \
+    if (L == LL && S == SS) return launch_dp<LL, SS>(a, first, count, st);
+#define FA_SW_LIST(LL)                                                                  \
+    FA_SW_CASE(LL, 1) FA_SW_CASE(LL, 2) FA_SW_CASE(LL, 4) FA_SW_CASE(LL, 8) FA_SW_CASE(LL, 12) \
+    FA_SW_CASE(LL, 16) FA_SW_CASE(LL, 24) FA_SW_CASE(LL, 32)
+    FA_SW_LIST(4)
+    FA_SW_LIST(8)
+    FA_SW_LIST(16)
+#undef FA_SW_LIST
+#undef FA_SW_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t swtb_traceback_launch(const SwtbArgs &a, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(swtb_traceback_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    return hipGetLastError();
+}
+
+}  // namespace fa

@@ -166,6 +166,31 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                     const int32_t *query_len, const uint8_t *ref_seqs,
                     const int64_t *ref_offset, const int32_t *ref_len, const int32_t *h0,
                     const int32_t *band_width, fa_batch **out);
+/* ---- CIGAR regeneration: banded local SW with traceback --------------------
+ * Batched scalar_banded_swa (src/core/alignment/banded_swa/scalar/
+ * implementation.rs:5-440), the aligner that region.rs:1058 calls once per
+ * alignment region to rebuild its CIGAR.  Job j aligns query_len[j] codes at
+ * query_seqs + query_offset[j] against ref_len[j] codes at ref_seqs +
+ * ref_offset[j] with band band_width[j] (used as given) and start score
+ * h0[j] >= 0.  Outputs per job: the OutScore record, the CIGAR as
+ * (len << 4) | op (M 0, I 1, D 2, S 4; the reference's (op char, len) pairs)
+ * at cigar + cigar_offset[j] with room for query_len + ref_len + 4 entries,
+ * its length in n_cigar[j], and the aligned reference and query bases at
+ * ref_aligned / query_aligned + aligned_offset[j] with room for
+ * query_len + ref_len + 2 codes each (lengths n_ref_aligned[j],
+ * n_query_aligned[j]).  An empty query or reference gives the reference's
+ * zero record and an empty CIGAR.  cells may be NULL.  Limits as for the
+ * extension (query <= 255, reference <= 2048, uniform N row). */
+int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                               const uint8_t *query_seqs, const int64_t *query_offset,
+                               const int32_t *query_len, const uint8_t *ref_seqs,
+                               const int64_t *ref_offset, const int32_t *ref_len,
+                               const int32_t *band_width, const int32_t *h0, fa_out_score *out,
+                               uint32_t *cells, uint32_t *cigar, const int64_t *cigar_offset,
+                               int32_t *n_cigar, uint8_t *ref_aligned, uint8_t *query_aligned,
+                               const int64_t *aligned_offset, int32_t *n_ref_aligned,
+                               int32_t *n_query_aligned);
+
 /* ---- targets from a device-resident reference ------------------------------
  * The reference's 2-bit .pac (src/pipelines/linear/index/bntseq.rs:559-575:
  * base p in byte p/4, first base in the two most significant bits, codes

@@ -59,6 +59,17 @@ int64_t ora_extend_batch(int64_t n, const uint8_t *qseq, const int64_t *qoff,
 int64_t ora_pac_segment(const uint8_t *pac, int64_t l_pac, int64_t start, int64_t len,
                         int reverse, uint8_t *out);
 
+/* scalar_banded_swa (swtb_oracle.c): banded local SW with traceback, score
+ * record in `out`, CIGAR as (len << 4) | op (M 0, I 1, D 2, S 4) into
+ * cigar[0 .. *n_cigar), aligned reference and query bases (capacity
+ * qlen + tlen + 2 each, CIGAR capacity qlen + tlen + 4).  Returns ORA_OK. */
+int ora_scalar_banded_swa(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target,
+                          const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins,
+                          int32_t e_ins, int32_t zdrop, int32_t pen_clip3, int32_t w, int32_t h0,
+                          ora_ext_result *out, uint32_t *cigar, int32_t *n_cigar,
+                          uint8_t *ref_aligned, int32_t *n_ref, uint8_t *query_aligned,
+                          int32_t *n_query, uint64_t *cells);
+
 #ifdef __cplusplus
 }
 #endif

@@ -170,3 +170,33 @@ def pac_segment(pac, l_pac, start, length, reverse=False):
     out = np.zeros(max(int(length), 1), np.uint8)
     n = l.ora_pac_segment(P(pac), int(l_pac), int(start), int(length), int(bool(reverse)), P(out))
     return None if n < 0 else out[:n]
+
+
+CIGAR_OPS = "MIDNSHP=X"
+
+
+def scalar_banded_swa(query, target, sw, w, h0=0):
+    """Oracle scalar_banded_swa (oracle/swtb_oracle.c): (OutScore-order record,
+    CIGAR [(op char, len)], ref_aligned, query_aligned, cells)."""
+    l = lib()
+    f = l.ora_scalar_banded_swa
+    f.restype = ctypes.c_int
+    f.argtypes = [_I32, _P, _I32, _P, _P] + [_I32] * 8 + [_P, _P, _P, _P, _P, _P, _P, _P]
+    q = np.ascontiguousarray(query, np.uint8)
+    t = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(sw.mat, np.int8)
+    cap = len(q) + len(t) + 4
+    out = np.zeros(6, np.int32)
+    cig = np.zeros(cap, np.uint32)
+    ra = np.zeros(cap, np.uint8)
+    qa = np.zeros(cap, np.uint8)
+    nc, nr, nq = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    cells = ctypes.c_uint64()
+    rc = f(len(q), P(q), len(t), P(t), P(mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins, sw.zdrop, sw.pen_clip3,
+           int(w), int(h0), P(out), P(cig), ctypes.byref(nc), P(ra), ctypes.byref(nr), P(qa), ctypes.byref(nq),
+           ctypes.byref(cells))
+    assert rc == 0
+    score, qle, tle, gtle, gscore, max_off = out.tolist()
+    rec = np.array([score, tle, gtle, qle, gscore, max_off], np.int32)  # OutScore order
+    cigar = [(CIGAR_OPS[int(c) & 15], int(c) >> 4) for c in cig[:nc.value]]
+    return rec, cigar, ra[:nr.value].copy(), qa[:nq.value].copy(), int(cells.value)

@@ -92,6 +92,7 @@ def _declare(lib):
                                         _P, _P, ctypes.POINTER(_P)]
     lib.fa_batch_download_targets.argtypes = [_P, _P, _P, _P]
     lib.fa_scalar_banded_swa_batch.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW), _I64] + [_P] * 18
+    lib.fa_scalar_banded_swa_last_ms.argtypes = [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     seqargs = [_I64, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.fa_execute_batch_simd_scoring.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
     lib.fa_batch_upload.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [ctypes.POINTER(_P)]
@@ -115,7 +116,7 @@ def _declare(lib):
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
     for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_valu_peak", "fa_hbm_copy", "fa_ref_upload", "fa_ref_free",
                  "fa_batch_upload_ref", "fa_batch_download_targets", "fa_scalar_banded_swa_batch",
-                 "fa_execute_batch_simd_scoring",
+                 "fa_scalar_banded_swa_last_ms", "fa_execute_batch_simd_scoring",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
                  "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",
@@ -147,6 +148,9 @@ def synth():
                                                _P, _I64, _P, _P, _P, _I32, _I32, _I32, ctypes.c_int, _P]
         s.fa_synth_extension_pairs.restype = ctypes.c_int
         s.fa_synth_mate_pairs.argtypes = [ctypes.c_uint64, _P, _I64, _I64, _P, _P, _P, ctypes.c_int]
+        s.fa_synth_region_pairs.argtypes = [ctypes.c_uint64, _P, _I64, _I64, _P, _I64, _P, _P, _I64, _P, _P,
+                                            ctypes.c_int]
+        s.fa_synth_region_pairs.restype = ctypes.c_int
         s.fa_synth_mate_pairs.restype = ctypes.c_int
         _synth = s
     return _synth

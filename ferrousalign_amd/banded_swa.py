@@ -249,7 +249,7 @@ CIGAR_OPS = "MIDNSHP=X"
 
 
 def scalar_banded_swa_batch(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, w, h0=None,
-                            ctx: Optional[Context] = None, want_cells=False):
+                            ctx: Optional[Context] = None, want_cells=False, raw=False):
     """Batched scalar_banded_swa (banded_swa/scalar/implementation.rs:5-440) on
     the GPU: banded local SW with traceback, as region.rs:1058 runs it to
     regenerate CIGARs.  Returns (out[n,6] in OutScore order, CIGARs as lists
@@ -284,10 +284,21 @@ def scalar_banded_swa_batch(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, 
         ctx.handle, ctypes.byref(c), n, N.ptr(qseq), N.ptr(qoff), N.ptr(qlen), N.ptr(tseq), N.ptr(toff),
         N.ptr(tlen), N.ptr(w), N.ptr(h0), N.ptr(out), N.ptr(cells), N.ptr(cig), N.ptr(coff), N.ptr(nc),
         N.ptr(ra), N.ptr(qa), N.ptr(aoff), N.ptr(nr), N.ptr(nq)), "fa_scalar_banded_swa_batch")
+    if raw:  # flat buffers: records, cells, CIGAR words + offsets + counts, aligned bases + offsets + counts
+        return out, cells, cig, coff, nc, ra, qa, aoff, nr, nq
     cigars = [[(CIGAR_OPS[int(x) & 15], int(x) >> 4) for x in cig[coff[k]:coff[k] + nc[k]]] for k in range(n)]
     ref_al = [ra[aoff[k]:aoff[k] + nr[k]].copy() for k in range(n)]
     qry_al = [qa[aoff[k]:aoff[k] + nq[k]].copy() for k in range(n)]
     return out, cigars, ref_al, qry_al, (cells if want_cells else None)
+
+
+def scalar_banded_swa_last_ms(ctx: Optional[Context] = None):
+    """(DP ms, traceback ms) of the last scalar_banded_swa_batch call (HIP events)."""
+    ctx = ctx or default_context()
+    a, b = ctypes.c_double(), ctypes.c_double()
+    N.check(N.lib().fa_scalar_banded_swa_last_ms(ctx.handle, ctypes.byref(a), ctypes.byref(b)),
+            "fa_scalar_banded_swa_last_ms")
+    return a.value, b.value
 
 
 def scalar_banded_swa(sw_params: BandedPairWiseSW, qlen, query, tlen, target, w, h0,

@@ -27,9 +27,10 @@ struct fa_ctx {
     hipStream_t stream = nullptr;           // main stream of the context
     hipStream_t side[3] = {nullptr, nullptr, nullptr};  // fork/join streams (default priority)
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     std::string arch;
     int cus = 0;
+    double last_dp_ms = 0.0, last_tb_ms = 0.0;  // kernels of the last fa_scalar_banded_swa_batch
 };
 
 namespace {
@@ -510,6 +511,7 @@ int fa_ctx_create(int device, fa_ctx **out) {
     c->cus = prop.multiProcessorCount;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+              hipEventCreate(&c->ev2) == hipSuccess &&
               hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
     for (int j = 0; ok && j < 3; ++j)
         ok = hipStreamCreateWithFlags(&c->side[j], hipStreamNonBlocking) == hipSuccess &&
@@ -527,6 +529,7 @@ int fa_ctx_destroy(fa_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     for (int j = 0; j < 3; ++j) {
         if (ctx->side[j]) (void)hipStreamDestroy(ctx->side[j]);
@@ -956,7 +959,7 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     a.zdrop = sw->zdrop;
     a.pen_clip3 = sw->pen_clip3;
     a.tpad = ((tmax + 16) / 16) * 16;
-    hipError_t e = hipSuccess;
+    hipError_t e = hipEventRecord(ctx->ev0, st);
     for (size_t x = 0; x < m && e == hipSuccess;) {  // one launch per (L, S) bucket
         size_t y = x;
         while (y < m && lsl[order[y]] == lsl[order[x]]) ++y;
@@ -965,7 +968,9 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
                                (uint32_t)(y - x), st);
         x = y;
     }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
     if (e == hipSuccess) e = fa::swtb_traceback_launch(a, (uint32_t)m, st);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev2, st);
     std::vector<int32_t> hout(m * 6), hnc(m), hnr(m), hnq(m);
     std::vector<uint32_t> hcells(m), hcig((size_t)cgw);
     std::vector<uint8_t> hra((size_t)alw), hqa((size_t)alw);
@@ -978,8 +983,13 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     if (e == hipSuccess) e = hipMemcpyAsync(hra.data(), d_ra, alw, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hqa.data(), d_qa, alw, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float t_dp = 0.f, t_tb = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_dp, ctx->ev0, ctx->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_tb, ctx->ev1, ctx->ev2);
     release();
     if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
+    ctx->last_dp_ms = t_dp;
+    ctx->last_tb_ms = t_tb;
     for (size_t p = 0; p < m; ++p) {
         const int64_t k = job[p];
         const int32_t *o = &hout[p * 6];
@@ -997,6 +1007,13 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         std::memcpy(ref_aligned + aligned_offset[k], &hra[alo[p]], (size_t)hnr[p]);
         std::memcpy(query_aligned + aligned_offset[k], &hqa[alo[p]], (size_t)hnq[p]);
     }
+    return FA_OK;
+}
+
+int fa_scalar_banded_swa_last_ms(fa_ctx *ctx, double *dp_ms, double *traceback_ms) {
+    if (!ctx) return FA_ERR_ARG;
+    if (dp_ms) *dp_ms = ctx->last_dp_ms;
+    if (traceback_ms) *traceback_ms = ctx->last_tb_ms;
     return FA_OK;
 }
 

@@ -221,4 +221,35 @@ int fa_synth_mate_pairs(uint64_t seed, const uint8_t *ref, int64_t ref_len, int6
     return 0;
 }
 
+// CIGAR-regeneration workload (the pairs region.rs:1058 aligns): a 150 bp
+// read (substitutions rising 0.2% -> 2%, 0.1% indels) against the reference
+// span it maps to, [rb, re) of its alignment region (the aligner starts at the
+// region's first base); band w = |tlen - qlen| + 10.  Query k at
+// qseq + k*qstride (>= 150), window at tseq + k*tstride (>= 180).
+int fa_synth_region_pairs(uint64_t seed, const uint8_t *ref, int64_t ref_len, int64_t n,
+                          uint8_t *qseq, int64_t qstride, int32_t *qlen, uint8_t *tseq,
+                          int64_t tstride, int32_t *tlen, int32_t *w, int nthreads) {
+    const int R = 150;
+    if (n < 0 || ref_len < 8192 || qstride < R || tstride < R + 30) return -1;
+    parallel_for(n, nthreads, [&](int64_t lo, int64_t hi) {
+        uint8_t read[256];
+        int64_t map[256];
+        ReadModel mdl = {0.002, 0.02, 0.001};
+        for (int64_t k = lo; k < hi; ++k) {
+            Xoshiro rng(pair_seed(seed, (uint64_t)k));
+            const int64_t p = 16 + (int64_t)(rng.uniform() * (double)(ref_len - R - 64));
+            simulate_read(rng, ref, p, R, mdl, read, map);
+            const int64_t a = map[0];
+            int64_t b = map[R - 1] + 1;
+            if (b - a > tstride) b = a + tstride;
+            std::memcpy(qseq + k * qstride, read, R);
+            std::memcpy(tseq + k * tstride, ref + a, (size_t)(b - a));
+            qlen[k] = R;
+            tlen[k] = (int32_t)(b - a);
+            w[k] = (tlen[k] > R ? tlen[k] - R : R - tlen[k]) + 10;
+        }
+    });
+    return 0;
+}
+
 }  // extern "C"

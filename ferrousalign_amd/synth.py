@@ -131,3 +131,27 @@ def mate_pairs(n=MATE_PAIRS, seed=MATE_SEED, threads=0):
     if rc != 0:
         raise ValueError("fa_synth_mate_pairs failed")
     return MatePairSet(q, t, hm)
+
+
+REGION_SEED = 0xFA05
+REGION_PAIRS = 1048576
+REGION_QSTRIDE = 150
+REGION_TSTRIDE = 180
+
+
+def region_pairs(n=REGION_PAIRS, seed=REGION_SEED, threads=0):
+    """CIGAR-regeneration workload: 150 bp reads against the reference span of
+    their alignment region, band |tlen - qlen| + 10.  Returns the flat arrays
+    (qseq, qoff, qlen, tseq, toff, tlen, w)."""
+    ref = reference()
+    q = np.zeros((n, REGION_QSTRIDE), np.uint8)
+    t = np.zeros((n, REGION_TSTRIDE), np.uint8)
+    ql = np.zeros(n, np.int32)
+    tl = np.zeros(n, np.int32)
+    w = np.zeros(n, np.int32)
+    rc = N.synth().fa_synth_region_pairs(ctypes.c_uint64(seed), N.ptr(ref), ref.size, n, N.ptr(q), REGION_QSTRIDE,
+                                         N.ptr(ql), N.ptr(t), REGION_TSTRIDE, N.ptr(tl), N.ptr(w), threads)
+    if rc != 0:
+        raise ValueError("fa_synth_region_pairs failed")
+    return (q.reshape(-1), np.arange(n, dtype=np.int64) * REGION_QSTRIDE, ql,
+            t.reshape(-1), np.arange(n, dtype=np.int64) * REGION_TSTRIDE, tl, w)

@@ -191,6 +191,11 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
                                const int64_t *aligned_offset, int32_t *n_ref_aligned,
                                int32_t *n_query_aligned);
 
+/* Kernel time (HIP events on the context's stream) of the last
+ * fa_scalar_banded_swa_batch call: the DP launches and the traceback launch,
+ * with the inputs already in HBM (measurement aid). */
+int fa_scalar_banded_swa_last_ms(fa_ctx *ctx, double *dp_ms, double *traceback_ms);
+
 /* ---- targets from a device-resident reference ------------------------------
  * The reference's 2-bit .pac (src/pipelines/linear/index/bntseq.rs:559-575:
  * base p in byte p/4, first base in the two most significant bits, codes

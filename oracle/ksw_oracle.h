@@ -69,6 +69,15 @@ int ora_scalar_banded_swa(int32_t qlen, const uint8_t *query, int32_t tlen, cons
                           ora_ext_result *out, uint32_t *cigar, int32_t *n_cigar,
                           uint8_t *ref_aligned, int32_t *n_ref, uint8_t *query_aligned,
                           int32_t *n_query, uint64_t *cells);
+/* The same over a batch on `nthreads` host threads (0 = all); records and
+ * cell counts only (CPU baseline). */
+int64_t ora_scalar_banded_swa_batch(int64_t n, const uint8_t *q, const int64_t *qoff,
+                                    const int32_t *qlen, const uint8_t *t, const int64_t *toff,
+                                    const int32_t *tlen, const int32_t *w, const int32_t *h0,
+                                    const int8_t *mat, int32_t o_del, int32_t e_del,
+                                    int32_t o_ins, int32_t e_ins, int32_t zdrop,
+                                    int32_t pen_clip3, ora_ext_result *out, uint64_t *cells,
+                                    int32_t *n_cigar, int nthreads);
 
 #ifdef __cplusplus
 }

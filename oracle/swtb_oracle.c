@@ -199,3 +199,65 @@ int ora_scalar_banded_swa(int32_t qlen, const uint8_t *query, int32_t tlen, cons
     free(qp), free(eh_h), free(eh_e), free(tb);
     return ORA_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* batch driver (threads over pairs), for the CPU baseline            */
+/* ------------------------------------------------------------------ */
+#include <pthread.h>
+#include <unistd.h>
+
+typedef struct {
+    int64_t lo, hi;
+    const uint8_t *q, *t;
+    const int64_t *qoff, *toff;
+    const int32_t *qlen, *tlen, *w, *h0;
+    const int8_t *mat;
+    int32_t o_del, e_del, o_ins, e_ins, zdrop, pen_clip3;
+    ora_ext_result *out;
+    uint64_t *cells;
+    int32_t *n_cigar;
+} swtb_job;
+
+static void *swtb_worker(void *arg) {
+    swtb_job *j = (swtb_job *)arg;
+    uint32_t *cig = (uint32_t *)malloc(sizeof(uint32_t) * (256 + 2048 + 8));
+    uint8_t *ra = (uint8_t *)malloc(256 + 2048 + 8), *qa = (uint8_t *)malloc(256 + 2048 + 8);
+    for (int64_t k = j->lo; k < j->hi; ++k) {
+        int32_t nc, nr, nq;
+        uint64_t c = 0;
+        ora_scalar_banded_swa(j->qlen[k], j->q + j->qoff[k], j->tlen[k], j->t + j->toff[k], j->mat,
+                              j->o_del, j->e_del, j->o_ins, j->e_ins, j->zdrop, j->pen_clip3, j->w[k],
+                              j->h0[k], &j->out[k], cig, &nc, ra, &nr, qa, &nq, &c);
+        if (j->cells) j->cells[k] = c;
+        if (j->n_cigar) j->n_cigar[k] = nc;
+    }
+    free(cig), free(ra), free(qa);
+    return NULL;
+}
+
+int64_t ora_scalar_banded_swa_batch(int64_t n, const uint8_t *q, const int64_t *qoff,
+                                    const int32_t *qlen, const uint8_t *t, const int64_t *toff,
+                                    const int32_t *tlen, const int32_t *w, const int32_t *h0,
+                                    const int8_t *mat, int32_t o_del, int32_t e_del,
+                                    int32_t o_ins, int32_t e_ins, int32_t zdrop,
+                                    int32_t pen_clip3, ora_ext_result *out, uint64_t *cells,
+                                    int32_t *n_cigar, int nthreads) {
+    if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    pthread_t th[64];
+    swtb_job jobs[64];
+    const int64_t per = (n + nthreads - 1) / nthreads;
+    int nt = 0;
+    for (int i = 0; i < nthreads; ++i) {
+        const int64_t lo = per * i, hi = lo + per < n ? lo + per : n;
+        if (lo >= hi) break;
+        swtb_job x = {lo, hi, q, t, qoff, toff, qlen, tlen, w, h0, mat, o_del, e_del, o_ins, e_ins,
+                      zdrop, pen_clip3, out, cells, n_cigar};
+        jobs[nt] = x;
+        pthread_create(&th[nt], NULL, swtb_worker, &jobs[nt]);
+        ++nt;
+    }
+    for (int i = 0; i < nt; ++i) pthread_join(th[i], NULL);
+    return 0;
+}

@@ -200,3 +200,22 @@ def scalar_banded_swa(query, target, sw, w, h0=0):
     rec = np.array([score, tle, gtle, qle, gscore, max_off], np.int32)  # OutScore order
     cigar = [(CIGAR_OPS[int(c) & 15], int(c) >> 4) for c in cig[:nc.value]]
     return rec, cigar, ra[:nr.value].copy(), qa[:nq.value].copy(), int(cells.value)
+
+
+def scalar_banded_swa_batch(sw, qseq, qoff, qlen, tseq, toff, tlen, w, h0, nthreads=0):
+    """Oracle scalar_banded_swa over a batch (records in OutScore order, cells)."""
+    l = lib()
+    f = l.ora_scalar_banded_swa_batch
+    f.restype = _I64
+    f.argtypes = [_I64] + [_P] * 9 + [_I32] * 6 + [_P, _P, _P, ctypes.c_int]
+    arrs = [np.ascontiguousarray(a, dt) for a, dt in ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (tseq, np.uint8), (toff, np.int64), (tlen, np.int32),
+                                                      (w, np.int32), (h0, np.int32))]
+    n = len(arrs[2])
+    mat = np.ascontiguousarray(sw.mat, np.int8)
+    out = np.zeros((n, 6), np.int32)
+    cells = np.zeros(n, np.uint64)
+    nc = np.zeros(n, np.int32)
+    f(n, *[P(a) for a in arrs], P(mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins, sw.zdrop, sw.pen_clip3,
+      P(out), P(cells), P(nc), int(nthreads))
+    return to_outscore_order(out), cells, nc

@@ -107,7 +107,7 @@ def roofline(db, flags, cells, ctx=None, reps=3):
     order = db.order(flags)
     dom_cells = float(cells[order[dom["first"]:dom["first"] + dom["count"]]].astype(np.float64).sum())
     tops = dom_cells * OPS_PER_CELL / (ms[k] / 1e3) / 1e12
-    packed = dom["kernel"].startswith("extend16")
+    packed = dom["kernel"].startswith(("extend16", "mate16"))
     peak = VALU_INT16X2_TOPS if packed else VALU_INT32_TOPS
     res = {"bound": "valu", "achieved": round(tops, 3), "peak": round(peak, 1), "unit": "TOPS",
            "frac": round(tops / peak, 4), "traffic": traffic_of(dom["kernel"], int(dom_cells)),
@@ -150,7 +150,7 @@ def mate_bench(args, ctx, golden):
     res = {"metric": "mate-rescue pairs/s (ksw_align2 16-bit, bit-exact)", "value": round(n / (ms / 1e3), 1),
            "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
            "ms_per_step_hip_events": round(ms_ev, 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "int16 (saturating, in int32 lanes)", "data": "synthetic",
+           "vs_baseline": None, "dtype": "int16x2 (saturating)", "data": "synthetic",
            "config": {"workload": "mate_rescue_2m", "pairs": n, "qlen": 151, "tlen": 1500, "xtra": hex(synth.MATE_XTRA),
                       "launches_per_step": nl},
            "gcups": round(total / (ms / 1e3) / 1e9, 2), "bit_exact_vs_golden": exact,

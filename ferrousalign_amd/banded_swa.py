@@ -535,7 +535,7 @@ class DeviceBatch:
         N.check(N.lib().fa_batch_profile_launches(self.ctx.handle, self._h, flags, cap, ctypes.byref(nl),
                                                   N.ptr(slots), N.ptr(kern), N.ptr(first), N.ptr(count), N.ptr(ms)),
                 "fa_batch_profile_launches")
-        names = {0: "extend_plain_kernel<%d,%d>", 1: "extend16_plain_kernel<%d,%d>", 2: "mate_kernel<%d>"}
+        names = {0: "extend_plain_kernel<%d,%d>", 1: "extend16_plain_kernel<%d,%d>", 2: "mate16_kernel<%d>"}
         out = []
         for i in range(min(nl.value, cap)):
             kind, lanes = int(kern[i]) >> 8, int(kern[i]) & 0xFF

@@ -41,6 +41,7 @@ struct Bucket {
     double work;   // estimated work: sum over pairs of qlen * tlen (or of the cost)
     int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group)
     int L = 16;    // lanes per pair of this bucket
+    int slen = 0;  // mate rescue: segments per lane ceil(qlen / 8) of every pair
 };
 
 struct DevBuf {
@@ -419,7 +420,7 @@ int run_align2(fa_ctx *ctx, fa_batch *b, int *nlaunch) {
     for (const Bucket &bk : b->mbuckets) {
         for (uint32_t off = 0; off < bk.count; off += kMateChunk) {
             const uint32_t cnt = std::min<uint32_t>(kMateChunk, bk.count - off);
-            FA_HIP(fa::mate_launch(bk.S, b->margs, bk.first + off, cnt, ctx->stream));
+            FA_HIP(fa::mate16_launch(b->margs, bk.first + off, cnt, bk.slen, ctx->stream));
             ++launches;
         }
     }
@@ -1100,7 +1101,7 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
             for (uint32_t off = 0; off < bk.count; off += kMateChunk) {
                 const uint32_t cnt = std::min<uint32_t>(kMateChunk, bk.count - off);
                 FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-                FA_HIP(fa::mate_launch(bk.S, b->margs, bk.first + off, cnt, ctx->stream));
+                FA_HIP(fa::mate16_launch(b->margs, bk.first + off, cnt, bk.slen, ctx->stream));
                 FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
                 FA_HIP(hipEventSynchronize(ctx->ev1));
                 float t = 0.f;
@@ -1403,11 +1404,13 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     if (b->tmax > 2032) return FA_ERR_TLEN;
     FA_HIP(hipSetDevice(ctx->device));
     const int64_t n = b->n;
-    // bucket by segments per lane; longest first
+    // bucket by segments per lane slen = ceil(qlen / 8), longest first: the packed
+    // kernel runs two pairs per lane group in the same stripe geometry
     std::vector<int> sk((size_t)n);
     for (int64_t k = 0; k < n; ++k) {
-        sk[(size_t)k] = fa::mate_slen_bucket(b->qlen[(size_t)k]);
-        if (sk[(size_t)k] < 0) return FA_ERR_QLEN;
+        const int ql = b->qlen[(size_t)k];
+        if (ql < 1 || ql > 8 * fa::kMateMaxSlen) return FA_ERR_QLEN;
+        sk[(size_t)k] = (ql + 7) / 8;
     }
     std::vector<uint32_t> order((size_t)n);
     for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
@@ -1415,10 +1418,12 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
                      [&](uint32_t x, uint32_t y) { return sk[x] > sk[y]; });
     std::vector<Bucket> bk;
     for (int64_t k = 0; k < n;) {
-        int S = sk[order[(size_t)k]];
+        const int sl = sk[order[(size_t)k]];
         int64_t e = k;
-        while (e < n && sk[order[(size_t)e]] == S) ++e;
-        bk.push_back({S, (uint32_t)k, (uint32_t)(e - k), (double)(e - k)});
+        while (e < n && sk[order[(size_t)e]] == sl) ++e;
+        Bucket mb{sl, (uint32_t)k, (uint32_t)(e - k), (double)(e - k)};
+        mb.slen = sl;
+        bk.push_back(mb);
         k = e;
     }
     int rc;

@@ -27,14 +27,17 @@ struct MateArgs {
     int32_t o_del, e_del, o_ins, e_ins;
     uint32_t xtra;
     int32_t qmax;         // largest matrix entry (>= 1)
-    int32_t tpad;         // LDS bytes of staged target per pair (multiple of 16)
+    int32_t tpad;         // LDS bytes of staged targets per lane group: one byte per
+                          // position for its two pairs (multiple of 16, > tlen)
 };
 
-// Slot bucket (segments per lane) for a query length, or -1.
-int mate_slen_bucket(int qlen);
-// Launch over packed positions [first, first+count): 32 pairs per block;
-// imax_log must hold count * log_stride entries.
-hipError_t mate_launch(int SLEN, const MateArgs &a, uint32_t first, uint32_t count,
-                       hipStream_t st);
+// Longest stripe (segments per lane) a launch takes: qlen <= 256.
+constexpr int kMateMaxSlen = 32;
+// Launch over packed positions [first, first+count), two pairs per 8-lane
+// group, 16 pairs per 64-thread block:
+// every pair of [first, first+count) must have slen = ceil(qlen/8) = `slen`
+// (1..32); imax_log must hold count * log_stride entries.
+hipError_t mate16_launch(const MateArgs &a, uint32_t first, uint32_t count, int slen,
+                         hipStream_t st);
 
 }  // namespace fa

@@ -1,474 +1,17 @@
-// Mate-rescue local Smith-Waterman, packed: two pairs per 8-lane group.
-//
-// Same execution model as the reference's 8-lane SSE ksw_i16
-// (src/core/alignment/ksw_affine_gap.rs:503-735, :773-861; SimdEngine128,
-// simd_abstraction/engine128.rs:247-326): GPU lane p of a group plays SSE
-// lane p, query position k lives in lane k / slen, segment k % slen, and the
-// lane walks its segments in order for every target row.  The reference's
-// arithmetic is signed 16-bit saturating (adds/subs_epi16) with signed max,
-// which is exactly v_pk_add_i16 / v_pk_sub_i16 with clamp and v_pk_max_i16,
-// so every register holds two pairs, pair A in the low and pair B in the high
-// half.  Per pair state (row maximum, best row, lazy-F flag, stopping) stays
-// per half.  Pairs of one launch share slen (the host buckets by it); the
-// reverse (XSTART) pass runs each half with its own query length, so its
-// segments past a half's slen are masked: they neither feed the row maximum,
-// nor the F chain, nor the lazy-F pass.
-#include <hip/hip_runtime.h>
-#include <stdint.h>
-
-#include <type_traits>
-
-#include "mate_kernel.h"
+// Packed mate-rescue kernels for general gap penalties, every stripe length,
+// and the launcher that picks the symmetric set when it applies.
+#include "mate16_impl.h"
 
 namespace fa {
 
-namespace {
+hipError_t mate16_launch_sym(const MateArgs &a, uint32_t first, uint32_t count, int slen,
+                             hipStream_t st);
 
-typedef short s2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t bits(s2 v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ __forceinline__ s2 vs(uint32_t v) { return __builtin_bit_cast(s2, v); }
-__device__ __forceinline__ uint32_t pk(int lo, int hi) {
-    return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16);
-}
-__device__ __forceinline__ int lo16(uint32_t x) { return (int)(short)(x & 0xFFFFu); }
-__device__ __forceinline__ int hi16(uint32_t x) { return (int)(short)(x >> 16); }
-__device__ __forceinline__ uint32_t smax(uint32_t a, uint32_t b) {
-    return bits(__builtin_elementwise_max(vs(a), vs(b)));
-}
-__device__ __forceinline__ uint32_t addsat(uint32_t a, uint32_t b) {
-    return bits(__builtin_elementwise_add_sat(vs(a), vs(b)));
-}
-__device__ __forceinline__ uint32_t subsat(uint32_t a, uint32_t b) {
-    return bits(__builtin_elementwise_sub_sat(vs(a), vs(b)));
-}
-__device__ __forceinline__ uint32_t sub16(uint32_t a, uint32_t b) {
-    return bits(vs(a) - vs(b));
-}
-// 0xFFFF in each half that is negative
-__device__ __forceinline__ uint32_t sgnmask(uint32_t d) {
-    uint32_t r;
-    asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(r) : "v"(d));
-    return r;
-}
-__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) {
-    return (m & a) | (~m & b);
-}
-// b = (m & a) | (~m & b) in b's own register, so a conditional update leaves
-// the loop-carried value where it was
-__device__ __forceinline__ void bfi_into(uint32_t &b, uint32_t m, uint32_t a) {
-    asm("v_bfi_b32 %0, %1, %2, %0" : "+v"(b) : "v"(m), "v"(a));
-}
-
-constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowShr1 = 0x111;
-
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xF, 0xF, false);
-}
-// per-half signed max over the 8 lanes of a group
-__device__ __forceinline__ uint32_t grp8_smax2(uint32_t x) {
-    x = smax(x, dpp<kQuadXor1>(x, x));
-    x = smax(x, dpp<kQuadXor2>(x, x));
-    x = smax(x, dpp<kRowHalfMirror>(x, x));
-    return x;
-}
-__device__ __forceinline__ uint32_t grp8_or(uint32_t x) {
-    x |= dpp<kQuadXor1>(x, x);
-    x |= dpp<kQuadXor2>(x, x);
-    x |= dpp<kRowHalfMirror>(x, x);
-    return x;
-}
-__device__ __forceinline__ int grp8_imax(int x) {
-    x = max(x, (int)dpp<kQuadXor1>((uint32_t)x, (uint32_t)x));
-    x = max(x, (int)dpp<kQuadXor2>((uint32_t)x, (uint32_t)x));
-    x = max(x, (int)dpp<kRowHalfMirror>((uint32_t)x, (uint32_t)x));
-    return x;
-}
-// value of lane p-1 of the group (lane 0: 0): the SSE shift by one lane
-__device__ __forceinline__ uint32_t shift_up(uint32_t x, int p) {
-    const uint32_t y = dpp<kRowShr1>(0u, x);
-    return p == 0 ? 0u : y;
-}
-
-__device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *seqn, uint32_t off,
-                                        int pos) {
-    const uint32_t c = (seq2[off + (pos >> 4)] >> ((pos & 15) * 2)) & 3u;
-    return ((seqn[(off >> 1) + (pos >> 5)] >> (pos & 31)) & 1u) ? 4 : (int)c;
-}
-
-struct Half {
-    bool active;
-    uint32_t qo;
-    int qlen, tlen, trev;
-    uint32_t endsc;
-};
-struct Res {
-    int score, te, qe, rows;
-};
-
-// wave-uniform largest j in [1, SLEN] with pred(j) true for some lane
-template <int SLEN, class P>
-__device__ __forceinline__ int wave_last(P pred) {
-    int r = 0;
-#pragma unroll
-    for (int j = 1; j <= SLEN; ++j)
-        if (__builtin_amdgcn_ballot_w64(pred(j)) != 0) r = j;
-    return r;
-}
-
-// One ksw_i16 pass for the group's two pairs (`qrev`: the XSTART reverse
-// pass, query position k = original qlen-1-k, target row i = original
-// trev - i while i <= trev).
-// UNIFORM: every pair of the wave has slen = SLEN (the host buckets the
-// forward pass by slen and instantiates the kernel per slen), so no slot is
-// conditional; otherwise pairs may have any slen <= SLEN and the slots past a
-// pair's slen are masked.
-// tc2: the group's staged targets, byte k = code of pair A at position k in
-// bits 0-3 and of pair B in bits 4-7.
-// logp: lane 0 logs pair A's row maxima, lane 1 pair B's (nullptr: no log).
-// hm: the wave's copy of H at the best row so far (Hmax), word j * 64 + lane.
-// SYM: o_del == o_ins and e_del == e_ins, so E and F share the gap-open term.
-template <int SLEN, bool UNIFORM, bool SYM>
-__device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (&hf)[2],
-                                          const uint8_t *tc2, uint16_t *logp, uint32_t *hm,
-                                          bool qrev, Res (&res)[2]) {
-    const int lane = (int)(threadIdx.x & 63);
-    int slen[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) slen[h] = hf[h].active ? (hf[h].qlen + 7) >> 3 : 0;
-    int smax_w = SLEN, sfull = SLEN;
-    if constexpr (!UNIFORM) {
-        smax_w = wave_last<SLEN>([&](int j) { return j <= slen[0] || j <= slen[1]; });
-        // the smallest slen among the wave's active pairs: slots below it exist
-        // for every pair
-        sfull = SLEN;
-#pragma unroll
-        for (int j = SLEN; j >= 1; --j)
-            if (__builtin_amdgcn_ballot_w64((hf[0].active && j > slen[0]) ||
-                                            (hf[1].active && j > slen[1])) != 0)
-                sfull = j - 1;
-    }
-    const uint32_t slenp = pk(slen[0], slen[1]);
-    const uint32_t slm1p = pk(slen[0] - 1, slen[1] - 1);
-
-    uint32_t profA[SLEN], profB[SLEN], H[SLEN], E[SLEN];
-#pragma unroll
-    for (int j = 0; j < SLEN; ++j) {
-        uint32_t pw[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int k = j + p * slen[h];
-            pw[h] = 0x80808080u;  // past the query: score 0 for every target base
-            if (hf[h].active && j < slen[h] && k < hf[h].qlen) {
-                const int orig = qrev ? hf[h].qlen - 1 - k : k;
-                pw[h] = a.profw[seq_code(a.seq2, a.seqn, hf[h].qo, orig)];
-            }
-        }
-        profA[j] = pw[0];
-        profB[j] = pw[1];
-        H[j] = 0u;
-        E[j] = 0u;
-        hm[j * 64 + lane] = 0u;
-    }
-    const uint32_t edel = pk(a.e_del, a.e_del), eins = pk(a.e_ins, a.e_ins);
-    const uint32_t oedel = pk(a.o_del + a.e_del, a.o_del + a.e_del);
-    const uint32_t oeins = pk(a.o_ins + a.e_ins, a.o_ins + a.e_ins);
-    // per-pair state, packed like the cells: 0xFFFF per live half, best score,
-    // its row, rows run, and the stop score min(endsc, 0x7FFF) (a score of 0x7FFF
-    // is saturated and stops too)
-    uint32_t alivem = (hf[0].active && hf[0].tlen > 0 ? 0xFFFFu : 0u) |
-                      (hf[1].active && hf[1].tlen > 0 ? 0xFFFF0000u : 0u);
-    uint32_t gmaxp = 0u, tep = 0xFFFFFFFFu, rowsp = 0u, ip = 0u;
-    const uint32_t endscp = pk((int)min(hf[0].endsc, 0x7FFFu), (int)min(hf[1].endsc, 0x7FFFu));
-    const uint32_t tlenp = pk(hf[0].tlen, hf[1].tlen);
-    const int trev0 = hf[0].trev, trev1 = hf[1].trev;
-    uint32_t hlast = 0u;  // H(i-1, slen-1) per half, carried to the next row's shift
-    for (int i = 0;; ++i) {
-        if (__builtin_amdgcn_ballot_w64(alivem != 0u) == 0) break;
-        // target codes of this row (dead halves read in-bounds bytes that do not matter)
-        uint32_t tw;
-        if (!qrev) {
-            tw = tc2[i];
-        } else {
-            const int t0 = i <= trev0 ? trev0 - i : i, t1 = i <= trev1 ? trev1 - i : i;
-            tw = ((uint32_t)tc2[t0] & 0x0Fu) | ((uint32_t)tc2[t1] & 0xF0u);
-        }
-        // score byte mat[tb][q] + 128 of each pair's profile, bias 0x80; an N target
-        // base scores 0 (the N row is zero, checked on the host): selector 0x0C
-        // gives the byte 0x00 and its half's bias is 0
-        const uint32_t nb = ((tw & 4u) >> 2) | ((tw & 0x40u) << 10);
-        const uint32_t sel = (tw & 3u) | ((tw & 0x30u) << 12) | 0x0C040C00u | (nb * 0x0Cu);
-        const uint32_t biasp = (nb ^ 0x00010001u) << 7;
-        uint32_t f = 0u, mx = 0u;
-        {
-            uint32_t h = shift_up(hlast, p);
-#pragma unroll
-            for (int j = 0; j < SLEN; ++j) {
-                if (j < smax_w) {
-                    const uint32_t sb = sub16(__builtin_amdgcn_perm(profB[j], profA[j], sel), biasp);
-                    uint32_t hv = addsat(h, sb);
-                    const uint32_t e = E[j];
-                    hv = smax(smax(hv, e), f);
-                    uint32_t fn;
-                    if constexpr (SYM) {
-                        const uint32_t t = subsat(hv, oedel);
-                        fn = smax(subsat(f, eins), t);
-                        E[j] = smax(subsat(e, edel), t);
-                    } else {
-                        fn = smax(subsat(f, eins), subsat(hv, oeins));
-                        E[j] = smax(subsat(e, edel), subsat(hv, oedel));
-                    }
-                    h = H[j];  // H(i-1, j): the next segment's diagonal
-                    H[j] = hv;
-                    if (UNIFORM || j < sfull) {
-                        mx = smax(mx, hv);
-                        f = fn;
-                    } else {  // some pair has no segment j
-                        const uint32_t vm = sgnmask(sub16((uint32_t)j * 0x00010001u, slenp));
-                        mx = smax(mx, hv & vm);
-                        f = bfi(vm, fn, f);
-                    }
-                }
-            }
-        }
-        // the row maximum is taken before lazy F (:601, :635); materialize it here so
-        // the row's H values need not stay live through the lazy-F pass
-        asm volatile("" : "+v"(mx));
-        // lazy F (:607-633): shift F one lane, raise H, decay F; a pair stops after a
-        // pass without change.  Every lane's F entered its segment 0 at 0 and H >= F
-        // along the chain, so a shifted F <= 0 in every lane of a pair changes no H:
-        // the first pass is skipped when that holds for the whole wave.
-        uint32_t lazym = alivem;
-        const uint32_t fs = shift_up(f, p);
-        if (__builtin_amdgcn_ballot_w64((sgnmask(sub16(0u, fs)) & lazym) != 0u) != 0) {
-            f = fs;
-            for (int pass = 0;;) {
-                uint32_t chg = 0u;
-#pragma unroll
-                for (int j = 0; j < SLEN; ++j) {
-                    if (j < smax_w) {
-                        uint32_t hn = smax(H[j], f);
-                        const uint32_t fn = subsat(f, eins);
-                        if (UNIFORM || j < sfull) {
-                            f = fn;
-                        } else {  // a pair without segment j keeps H and F
-                            const uint32_t vm = sgnmask(sub16((uint32_t)j * 0x00010001u, slenp));
-                            hn = bfi(vm, hn, H[j]);
-                            f = bfi(vm, fn, f);
-                        }
-                        chg |= hn ^ H[j];
-                        bfi_into(H[j], lazym, hn);
-                    }
-                }
-                chg = grp8_or(chg);
-                if ((chg & 0xFFFFu) == 0u) lazym &= 0xFFFF0000u;
-                if ((chg >> 16) == 0u) lazym &= 0x0000FFFFu;
-                if (++pass == 8 || __builtin_amdgcn_ballot_w64(lazym != 0u) == 0) break;
-                f = shift_up(f, p);
-            }
-        }
-        // H(i, slen-1) per half for the next row's shift
-#pragma unroll
-        for (int j = 0; j < SLEN; ++j) {
-            if constexpr (UNIFORM) {
-                if (j == SLEN - 1) hlast = H[j];
-            } else if (j < smax_w && j + 1 >= sfull) {
-                const uint32_t d = sub16((uint32_t)j * 0x00010001u, slm1p);
-                const uint32_t eq = ~(sgnmask(d) | sgnmask(sub16(0u, d)));  // d == 0
-                hlast = bfi(eq, H[j], hlast);
-            }
-        }
-        const uint32_t rmax = grp8_smax2(mx);  // >= 0 in both halves
-        if (logp != nullptr && ((alivem >> (16 * (p & 1))) & 1u))
-            logp[i] = (uint16_t)(rmax >> (16 * (p & 1)));
-        rowsp += alivem & 0x00010001u;
-        const uint32_t upd = sgnmask(sub16(gmaxp, rmax)) & alivem;  // rmax > gmax
-        gmaxp = bfi(upd, rmax, gmaxp);
-        tep = bfi(upd, ip, tep);
-        ip += 0x00010001u;
-        const uint32_t stop = (upd & ~sgnmask(sub16(gmaxp, endscp))) | ~sgnmask(sub16(ip, tlenp));
-        alivem &= ~stop;
-        if (__builtin_amdgcn_ballot_w64(upd != 0u) != 0) {
-            uint16_t *const hm16 = (uint16_t *)hm;
-            if (upd & 0xFFFFu) {
-#pragma unroll
-                for (int j = 0; j < SLEN; ++j)
-                    if (j < smax_w) hm16[(j * 64 + lane) * 2] = (uint16_t)H[j];
-            }
-            if (upd >> 16) {
-#pragma unroll
-                for (int j = 0; j < SLEN; ++j)
-                    if (j < smax_w) hm16[(j * 64 + lane) * 2 + 1] = (uint16_t)(H[j] >> 16);
-            }
-        }
-    }
-    const int gmax[2] = {lo16(gmaxp), hi16(gmaxp)};
-    const int te[2] = {lo16(tep), hi16(tep)};
-    const int rows[2] = {lo16(rowsp), hi16(rowsp)};
-#pragma unroll
-    for (int hh = 0; hh < 2; ++hh) {
-        res[hh].score = gmax[hh];
-        res[hh].te = te[hh];
-        res[hh].rows = rows[hh];
-        res[hh].qe = -1;
-        // qe: largest Hmax over real positions, smallest position on ties
-        int key = 0;
-#pragma unroll
-        for (int j = 0; j < SLEN; ++j) {
-            const int k = j + p * slen[hh];
-            const uint32_t hmj = hm[j * 64 + lane];
-            const int hv = hh ? hi16(hmj) : lo16(hmj);
-            if (j < slen[hh] && k < hf[hh].qlen) key = max(key, ((hv + 32769) << 12) | (4095 - k));
-        }
-        key = grp8_imax(key);
-        if (gmax[hh] != 0x7FFF) res[hh].qe = key ? 4095 - (key & 4095) : -1;
-    }
-}
-
-}  // namespace
-
-template <int SLEN, bool SYM>
-__global__ __launch_bounds__(64) void mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
-    extern __shared__ uint8_t lds[];
-    const int lane = (int)(threadIdx.x & 63);
-    const int g = lane >> 3, p = lane & 7;
-    Half hf[2];
-    uint32_t pid[2], slot[2], to[2];
-    bool valid[2];
-    uint8_t *const tc2 = lds + g * a.tpad;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        slot[h] = blockIdx.x * 16u + 2u * (uint32_t)g + (uint32_t)h;
-        valid[h] = slot[h] < count;
-        pid[h] = valid[h] ? a.order[first + slot[h]] : 0u;
-        hf[h].qlen = valid[h] ? a.qlen[pid[h]] : 1;
-        hf[h].tlen = valid[h] ? a.tlen[pid[h]] : 0;
-        hf[h].qo = valid[h] ? a.qo[pid[h]] : 0u;
-        const uint32_t to_h = valid[h] ? a.to[pid[h]] : 0u;
-        hf[h].active = valid[h];
-        hf[h].trev = 0;
-        to[h] = to_h;
-    }
-    for (int k = p; k < max(hf[0].tlen, hf[1].tlen); k += 8) {
-        const uint32_t c0 = k < hf[0].tlen ? (uint32_t)seq_code(a.seq2, a.seqn, to[0], k) : 0u;
-        const uint32_t c1 = k < hf[1].tlen ? (uint32_t)seq_code(a.seq2, a.seqn, to[1], k) : 0u;
-        tc2[k] = (uint8_t)(c0 | (c1 << 4));
-    }
-    // row-maximum log of pair p (lanes 0 and 1 of the group)
-    uint16_t *const logp =
-        (p < 2 && valid[p & 1]) ? a.imax_log + (size_t)slot[p & 1] * (size_t)a.log_stride : nullptr;
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-
-    const uint32_t xtra = a.xtra;
-    const uint32_t minsc = (xtra & kKswXSubo) ? (xtra & 0xffffu) : 0x10000u;
-    const uint32_t endsc = (xtra & kKswXStop) ? (xtra & 0xffffu) : 0x10000u;
-    hf[0].endsc = hf[1].endsc = endsc;
-    Res r[2];
-    uint32_t *const hm = (uint32_t *)(lds + 8 * a.tpad);
-    ksw_pass2<SLEN, true, SYM>(a, p, hf, tc2, logp, hm, false, r);
-
-    // suboptimal hits (:639-655, :717-731) from the row log: lane 0 of the group for
-    // pair A, lane 1 for pair B (selects, not indexing, keep the state in registers)
-    int score2 = -1, te2 = -1;
-    const bool mineB = p == 1;
-    const bool mine_valid = mineB ? valid[1] : valid[0];
-    const Res mine = mineB ? r[1] : r[0];
-    if (p < 2 && mine_valid && mine.score != 0x7FFF && minsc <= 0xFFFFu) {
-        const Res rr = mine;
-        const uint16_t *log = logp;
-        const int iv = (rr.score + a.qmax - 1) / a.qmax;
-        const int low = rr.te - iv, high = rr.te + iv;
-        int cs = -1, ce = -1;
-        for (int i = 0; i < rr.rows; ++i) {
-            const int im = log[i];
-            if ((uint32_t)im < minsc) continue;
-            if (cs < 0 || ce + 1 != i) {
-                if (cs >= 0 && (ce < low || ce > high) && cs > score2) {
-                    score2 = cs;
-                    te2 = ce;
-                }
-                cs = im;
-                ce = i;
-            } else if (cs < im) {
-                cs = im;
-                ce = i;
-            }
-        }
-        if (cs >= 0 && (ce < low || ce > high) && cs > score2) {
-            score2 = cs;
-            te2 = ce;
-        }
-    }
-
-    // XSTART reverse pass (:804-855), per pair
-    Half hr[2];
-    bool rev[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const bool find_start =
-            (xtra & kKswXStart) && !((xtra & kKswXSubo) && r[h].score < (int)(xtra & 0xffffu));
-        rev[h] = valid[h] && find_start && r[h].qe >= 0 && r[h].te >= 0;
-        hr[h] = hf[h];
-        hr[h].active = rev[h];
-        hr[h].qlen = rev[h] ? r[h].qe + 1 : 1;
-        hr[h].trev = r[h].te;
-        hr[h].endsc = (uint32_t)r[h].score & 0xffffu;
-    }
-    Res rr[2];
-    ksw_pass2<SLEN, false, SYM>(a, p, hr, tc2, nullptr, hm, true, rr);
-
-    if (p < 2 && mine_valid) {
-        const Res f = mine, b = mineB ? rr[1] : rr[0];
-        const bool rv = mineB ? rev[1] : rev[0];
-        const int ql = mineB ? hf[1].qlen : hf[0].qlen, ql2 = mineB ? hr[1].qlen : hr[0].qlen;
-        const uint32_t id = mineB ? pid[1] : pid[0];
-        int tb = -1, qb = -1;
-        uint32_t cells = (uint32_t)f.rows * (uint32_t)ql;
-        if (rv) {
-            cells += (uint32_t)b.rows * (uint32_t)ql2;
-            if (b.score >= f.score) {
-                tb = f.te - b.te;
-                qb = f.qe - b.qe;
-            }
-        }
-        int32_t *o = a.out + (size_t)id * 7;
-        o[0] = f.score;
-        o[1] = f.te;
-        o[2] = f.qe;
-        o[3] = score2;
-        o[4] = te2;
-        o[5] = tb;
-        o[6] = qb;
-        if (a.cells) a.cells[id] = cells;
-    }
-}
-
-template <int SLEN>
-static hipError_t launch16(const MateArgs &a, uint32_t first, uint32_t count, hipStream_t st) {
-    if (count == 0) return hipSuccess;
-    const dim3 grid((count + 15) / 16);
-    const size_t lds = (size_t)8 * a.tpad + (size_t)256 * SLEN;  // targets, then Hmax
-    if (a.o_del == a.o_ins && a.e_del == a.e_ins)
-        hipLaunchKernelGGL((mate16_kernel<SLEN, true>), grid, dim3(64), lds, st, a, first, count);
-    else
-        hipLaunchKernelGGL((mate16_kernel<SLEN, false>), grid, dim3(64), lds, st, a, first, count);
-    return hipGetLastError();
-}
-
-// every pair of the launch has slen = ceil(qlen / 8) = `slen` (1..32)
 hipError_t mate16_launch(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                          hipStream_t st) {
-    switch (slen) {
-#define FA_M16(N) \
-    case N: return launch16<N>(a, first, count, st);
-        FA_M16(1) FA_M16(2) FA_M16(3) FA_M16(4) FA_M16(5) FA_M16(6) FA_M16(7) FA_M16(8)
-        FA_M16(9) FA_M16(10) FA_M16(11) FA_M16(12) FA_M16(13) FA_M16(14) FA_M16(15) FA_M16(16)
-        FA_M16(17) FA_M16(18) FA_M16(19) FA_M16(20) FA_M16(21) FA_M16(22) FA_M16(23) FA_M16(24)
-        FA_M16(25) FA_M16(26) FA_M16(27) FA_M16(28) FA_M16(29) FA_M16(30) FA_M16(31) FA_M16(32)
-#undef FA_M16
-        default: return hipErrorInvalidValue;
-    }
+    if (slen < 1 || slen > kMateMaxSlen) return hipErrorInvalidValue;
+    if (a.o_del == a.o_ins && a.e_del == a.e_ins) return mate16_launch_sym(a, first, count, slen, st);
+    return mate16_launch_set<false>(a, first, count, slen, st);
 }
 
 }  // namespace fa

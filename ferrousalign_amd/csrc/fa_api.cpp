@@ -1464,7 +1464,6 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     a.e_ins = e_ins;
     a.xtra = xtra;
     a.qmax = qmax;
-    a.tpad = ((b->tmax + 16) / 16) * 16;
     b->mbuckets.swap(bk);
     b->align2_set = true;
     return FA_OK;

@@ -98,13 +98,18 @@ __device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *se
 
 struct Half {
     bool active;
-    uint32_t qo;
+    uint32_t qo, to;
     int qlen, tlen, trev;
     uint32_t endsc;
 };
 struct Res {
     int score, te, qe, rows;
 };
+
+// Target rows staged per lane group at a time: byte r of a chunk holds row
+// i0 + r of pair A in bits 0-3 and of pair B in bits 4-7.
+constexpr int kTgtChunk = 512;  // the forward staging decodes 4 + 2 words per lane
+static_assert(kTgtChunk == 512, "forward staging reads 64 rows per lane");
 
 // wave-uniform largest j in [1, SLEN] with pred(j) true for some lane
 template <int SLEN, class P>
@@ -123,14 +128,13 @@ __device__ __forceinline__ int wave_last(P pred) {
 // forward pass by slen and instantiates the kernel per slen), so no slot is
 // conditional; otherwise pairs may have any slen <= SLEN and the slots past a
 // pair's slen are masked.
-// tc2: the group's staged targets, byte k = code of pair A at position k in
-// bits 0-3 and of pair B in bits 4-7.
+// tc2: the group's kTgtChunk-byte target staging area.
 // logp: lane 0 logs pair A's row maxima, lane 1 pair B's (nullptr: no log).
 // hm: the wave's copy of H at the best row so far (Hmax), word j * 64 + lane.
 // SYM: o_del == o_ins and e_del == e_ins, so E and F share the gap-open term.
 template <int SLEN, bool UNIFORM, bool SYM>
 __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (&hf)[2],
-                                          const uint8_t *tc2, uint16_t *logp, uint32_t *hm,
+                                          uint8_t *tc2, uint16_t *logp, uint32_t *hm,
                                           bool qrev, Res (&res)[2]) {
     const int lane = (int)(threadIdx.x & 63);
     int slen[2];
@@ -182,19 +186,73 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
     uint32_t gmaxp = 0u, tep = 0xFFFFFFFFu, rowsp = 0u, ip = 0u;
     const uint32_t endscp = pk((int)min(hf[0].endsc, 0x7FFFu), (int)min(hf[1].endsc, 0x7FFFu));
     const uint32_t tlenp = pk(hf[0].tlen, hf[1].tlen);
-    const int trev0 = hf[0].trev, trev1 = hf[1].trev;
+    // stage target rows [i0, i0 + kTgtChunk) of both pairs; the reverse pass reads
+    // the target reversed up to trev and forward past it (the reference reverses
+    // the first te + 1 bases in place, :812-813)
+    auto stage = [&](int i0) {
+        if (!qrev) {
+            // lane p decodes rows [i0 + 64p, i0 + 64p + 64) of both pairs from four
+            // 2-bit words and two N-bit words per pair (targets start on a 32-base
+            // boundary: `to` counts 16-base words and is even)
+            const int r0 = (kTgtChunk / 8) * p, i = i0 + r0;
+            uint32_t w[2][4], nw[2][2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int nwords = hf[h].active ? (hf[h].tlen + 15) >> 4 : 0;
+                const int nnw = hf[h].active ? (hf[h].tlen + 31) >> 5 : 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int x = (i >> 4) + q;
+                    w[h][q] = x < nwords ? a.seq2[hf[h].to + x] : 0u;
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int x = (i >> 5) + q;
+                    nw[h][q] = x < nnw ? a.seqn[(hf[h].to >> 1) + x] : 0u;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < kTgtChunk / 32; ++d) {
+                uint32_t v = 0u;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int r = 4 * d + b;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        uint32_t c = (w[h][r >> 4] >> ((r & 15) * 2)) & 3u;
+                        if ((nw[h][r >> 5] >> (r & 31)) & 1u) c = 4u;
+                        v |= c << (8 * b + 4 * h);
+                    }
+                }
+                *(uint32_t *)(tc2 + r0 + 4 * d) = v;
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            return;
+        }
+#pragma unroll 8
+        for (int m = 0; m < kTgtChunk / 8; ++m) {
+            const int r = p + 8 * m, i = i0 + r;
+            uint32_t c[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                c[h] = 0u;
+                if (hf[h].active && i < hf[h].tlen) {
+                    const int pos = (qrev && i <= hf[h].trev) ? hf[h].trev - i : i;
+                    c[h] = (uint32_t)seq_code(a.seq2, a.seqn, hf[h].to, pos);
+                }
+            }
+            tc2[r] = (uint8_t)(c[0] | (c[1] << 4));
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    };
     uint32_t hlast = 0u;  // H(i-1, slen-1) per half, carried to the next row's shift
     // one target row: H(i-1) is read from Hd, H(i) written to Hn; the row loop
     // alternates two register sets so that no row copies H back
     auto row = [&](const int i, const uint32_t (&Hd)[SLEN], uint32_t (&Hn)[SLEN]) {
-        // target codes of this row (dead halves read in-bounds bytes that do not matter)
-        uint32_t tw;
-        if (!qrev) {
-            tw = tc2[i];
-        } else {
-            const int t0 = i <= trev0 ? trev0 - i : i, t1 = i <= trev1 ? trev1 - i : i;
-            tw = ((uint32_t)tc2[t0] & 0x0Fu) | ((uint32_t)tc2[t1] & 0xF0u);
-        }
+        // target codes of this row
+        const uint32_t tw = tc2[i & (kTgtChunk - 1)];
         // score byte mat[tb][q] + 128 of each pair's profile, bias 0x80; an N target
         // base scores 0 (the N row is zero, checked on the host): selector 0x0C
         // gives the byte 0x00 and its half's bias is 0
@@ -306,6 +364,10 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         };
     for (int i = 0;; i += 2) {
         if (__builtin_amdgcn_ballot_w64(alivem != 0u) == 0) break;
+        if ((i & (kTgtChunk - 1)) == 0) {
+            __builtin_amdgcn_wave_barrier();  // the previous chunk's reads are issued
+            stage(i);
+        }
         row(i, H, H2);
         if (__builtin_amdgcn_ballot_w64(alivem != 0u) == 0) break;
         row(i + 1, H2, H);
@@ -335,15 +397,21 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
 
 }  // namespace
 
+// Stripe lengths 14..20 fit three waves per SIMD in 168 VGPRs with spills only
+// outside the row loop; the shorter ones fit three or more by themselves, the
+// longer ones would spill inside it.
+constexpr int mate16_waves(int slen) { return slen >= 14 && slen <= 20 ? 3 : 1; }
+
 template <int SLEN, bool SYM>
-__global__ __launch_bounds__(64) void mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(mate16_waves(SLEN)))) void
+mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
     extern __shared__ uint8_t lds[];
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane >> 3, p = lane & 7;
     Half hf[2];
-    uint32_t pid[2], slot[2], to[2];
+    uint32_t pid[2], slot[2];
     bool valid[2];
-    uint8_t *const tc2 = lds + g * a.tpad;
+    uint8_t *const tc2 = lds + g * kTgtChunk;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         slot[h] = blockIdx.x * 16u + 2u * (uint32_t)g + (uint32_t)h;
@@ -352,15 +420,9 @@ __global__ __launch_bounds__(64) void mate16_kernel(MateArgs a, uint32_t first, 
         hf[h].qlen = valid[h] ? a.qlen[pid[h]] : 1;
         hf[h].tlen = valid[h] ? a.tlen[pid[h]] : 0;
         hf[h].qo = valid[h] ? a.qo[pid[h]] : 0u;
-        const uint32_t to_h = valid[h] ? a.to[pid[h]] : 0u;
         hf[h].active = valid[h];
         hf[h].trev = 0;
-        to[h] = to_h;
-    }
-    for (int k = p; k < max(hf[0].tlen, hf[1].tlen); k += 8) {
-        const uint32_t c0 = k < hf[0].tlen ? (uint32_t)seq_code(a.seq2, a.seqn, to[0], k) : 0u;
-        const uint32_t c1 = k < hf[1].tlen ? (uint32_t)seq_code(a.seq2, a.seqn, to[1], k) : 0u;
-        tc2[k] = (uint8_t)(c0 | (c1 << 4));
+        hf[h].to = valid[h] ? a.to[pid[h]] : 0u;
     }
     // row-maximum log of pair p (lanes 0 and 1 of the group)
     uint16_t *const logp =
@@ -373,7 +435,7 @@ __global__ __launch_bounds__(64) void mate16_kernel(MateArgs a, uint32_t first, 
     const uint32_t endsc = (xtra & kKswXStop) ? (xtra & 0xffffu) : 0x10000u;
     hf[0].endsc = hf[1].endsc = endsc;
     Res r[2];
-    uint32_t *const hm = (uint32_t *)(lds + 8 * a.tpad);
+    uint32_t *const hm = (uint32_t *)(lds + 8 * kTgtChunk);
     ksw_pass2<SLEN, true, SYM>(a, p, hf, tc2, logp, hm, false, r);
 
     // suboptimal hits (:639-655, :717-731) from the row log: lane 0 of the group for
@@ -463,7 +525,7 @@ hipError_t mate16_launch_set(const MateArgs &a, uint32_t first, uint32_t count, 
 #define FA_M16(N)                                                                            \
     case N:                                                                                  \
         hipLaunchKernelGGL((mate16_kernel<N, SYM>), grid, dim3(64),                          \
-                           (size_t)8 * a.tpad + (size_t)256 * N, st, a, first, count); \
+                           (size_t)8 * kTgtChunk + (size_t)256 * N, st, a, first, count); \
         return hipGetLastError();
         FA_M16(1) FA_M16(2) FA_M16(3) FA_M16(4) FA_M16(5) FA_M16(6) FA_M16(7) FA_M16(8)
         FA_M16(9) FA_M16(10) FA_M16(11) FA_M16(12) FA_M16(13) FA_M16(14) FA_M16(15) FA_M16(16)

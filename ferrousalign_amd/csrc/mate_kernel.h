@@ -27,8 +27,6 @@ struct MateArgs {
     int32_t o_del, e_del, o_ins, e_ins;
     uint32_t xtra;
     int32_t qmax;         // largest matrix entry (>= 1)
-    int32_t tpad;         // LDS bytes of staged targets per lane group: one byte per
-                          // position for its two pairs (multiple of 16, > tlen)
 };
 
 // Longest stripe (segments per lane) a launch takes: qlen <= 256.

@@ -18,13 +18,12 @@
 //    when H == 0, which every later use (max with E >= 0, max with 0) treats
 //    like 0;
 //  * the F prefix identity is -16384 (F(beg) stays below E >= 0 without
-//    wrapping);
+//    wrapping); the cross-lane scan runs on values biased by +16384 as
+//    unsigned, so its identity is 0;
 //  * E' = max(E -sat e_del, M - oe_del): E >= 0, so the unsigned saturating
 //    subtract equals max(E - e_del, 0).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
-#include <type_traits>
 
 #include "extend_kernel.h"
 
@@ -106,48 +105,49 @@ __device__ __forceinline__ uint32_t bfi1(uint32_t m, uint32_t a, uint32_t b) {
     return r;
 }
 
-constexpr uint32_t kNeg2 = 0xC000C000u;  // (-16384, -16384)
-
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr = 0x110;
 
+// DPP move with bound_ctrl: a lane without a source lane (row_shr at the
+// start of a 16-lane row) reads 0, and no old value is needed, so the
+// compiler neither copies the source first nor keeps the move apart from a
+// 32-bit consumer (v_max_i32_dpp)
 template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xF, 0xF, false);
+__device__ __forceinline__ uint32_t dmov(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 template <int L>
 __device__ __forceinline__ int allmax_i(int x) {
-    x = max(x, (int)dpp<kQuadXor1>((uint32_t)x, (uint32_t)x));
-    if constexpr (L >= 4) x = max(x, (int)dpp<kQuadXor2>((uint32_t)x, (uint32_t)x));
-    if constexpr (L >= 8) x = max(x, (int)dpp<kRowHalfMirror>((uint32_t)x, (uint32_t)x));
-    if constexpr (L >= 16) x = max(x, (int)dpp<kRowMirror>((uint32_t)x, (uint32_t)x));
+    x = max(x, (int)dmov<kQuadXor1>((uint32_t)x));
+    if constexpr (L >= 4) x = max(x, (int)dmov<kQuadXor2>((uint32_t)x));
+    if constexpr (L >= 8) x = max(x, (int)dmov<kRowHalfMirror>((uint32_t)x));
+    if constexpr (L >= 16) x = max(x, (int)dmov<kRowMirror>((uint32_t)x));
     return x;
 }
 template <int L>
 __device__ __forceinline__ uint32_t allmax_u2(uint32_t x) {
-    x = umax(x, dpp<kQuadXor1>(x, x));
-    if constexpr (L >= 4) x = umax(x, dpp<kQuadXor2>(x, x));
-    if constexpr (L >= 8) x = umax(x, dpp<kRowHalfMirror>(x, x));
-    if constexpr (L >= 16) x = umax(x, dpp<kRowMirror>(x, x));
+    x = umax(x, dmov<kQuadXor1>(x));
+    if constexpr (L >= 4) x = umax(x, dmov<kQuadXor2>(x));
+    if constexpr (L >= 8) x = umax(x, dmov<kRowHalfMirror>(x));
+    if constexpr (L >= 16) x = umax(x, dmov<kRowMirror>(x));
     return x;
 }
-// y from lane l-K of the group for a max-scan step: a lane without such a
-// source gets its own value back, which leaves the maximum unchanged
+// y from lane l-K of the group for a max-scan step, 0 (the identity of the
+// unsigned scan) where the group has no such lane
 template <int L, int K>
-__device__ __forceinline__ uint32_t shr_self(uint32_t y, int l) {
-    uint32_t z = dpp<kRowShr + K>(y, y);
-    if constexpr (L < 16) z = (l >= K) ? z : y;
+__device__ __forceinline__ uint32_t shr0(uint32_t y, int l) {
+    uint32_t z = dmov<kRowShr + K>(y);
+    if constexpr (L < 16) z = (l >= K) ? z : 0u;
     return z;
 }
-// exclusive packed signed max-scan over the L lanes of a group
+// exclusive packed unsigned max-scan over the L lanes of a group (lane 0: 0)
 template <int L>
-__device__ __forceinline__ uint32_t excl_scan_s2(uint32_t x, int l) {
-    uint32_t y = dpp<kRowShr + 1>(kNeg2, x);
-    y = (l >= 1) ? y : kNeg2;
-    if constexpr (L > 1) y = smax(y, shr_self<L, 1>(y, l));
-    if constexpr (L > 2) y = smax(y, shr_self<L, 2>(y, l));
-    if constexpr (L > 4) y = smax(y, shr_self<L, 4>(y, l));
-    if constexpr (L > 8) y = smax(y, shr_self<L, 8>(y, l));
+__device__ __forceinline__ uint32_t excl_scan_u2(uint32_t x, int l) {
+    uint32_t y = shr0<L, 1>(x, l);
+    if constexpr (L > 1) y = umax(y, shr0<L, 1>(y, l));
+    if constexpr (L > 2) y = umax(y, shr0<L, 2>(y, l));
+    if constexpr (L > 4) y = umax(y, shr0<L, 4>(y, l));
+    if constexpr (L > 8) y = umax(y, shr0<L, 8>(y, l));
     return y;
 }
 
@@ -159,12 +159,21 @@ __device__ __forceinline__ int decode_base(const uint32_t *seq2, const uint32_t 
     return ((wn >> (pos & 31)) & 1u) ? 4 : code;
 }
 
-__device__ __forceinline__ void stage_target(const ExtArgs &a, uint8_t *tcodes, int tlen,
-                                             uint32_t to, int l, int L) {
-    const int nw = (tlen + 15) >> 4;
+// Stage the target codes of a lane group's two pairs: byte i = code of row i
+// of pair A in bits 0-3 and of pair B in bits 4-7 (0 past a pair's target).
+// Lane l of the group decodes 16-base words l, l+L, ...
+__device__ __forceinline__ void stage_targets2(const ExtArgs &a, uint8_t *tc2, const int (&tlen)[2],
+                                               const uint32_t (&to)[2], int l, int L) {
+    const int nw0 = (tlen[0] + 15) >> 4, nw1 = (tlen[1] + 15) >> 4;
+    const int nw = max(nw0, nw1);
     for (int k = l; k < nw; k += L) {
-        const uint32_t w2 = a.seq2[to + k];
-        const uint32_t wn = (a.seqn[(to >> 1) + (k >> 1)] >> ((k & 1) * 16)) & 0xFFFFu;
+        uint32_t w2[2], wn[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool in = k < (h ? nw1 : nw0);
+            w2[h] = in ? a.seq2[to[h] + k] : 0u;
+            wn[h] = in ? (a.seqn[(to[h] >> 1) + (k >> 1)] >> ((k & 1) * 16)) & 0xFFFFu : 0u;
+        }
         uint32_t v[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -172,13 +181,16 @@ __device__ __forceinline__ void stage_target(const ExtArgs &a, uint8_t *tcodes, 
 #pragma unroll
             for (int y = 0; y < 4; ++y) {
                 const int p = b * 4 + y;
-                uint32_t c = (w2 >> (p * 2)) & 3u;
-                c = ((wn >> p) & 1u) ? 4u : c;
-                x |= c << (y * 8);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    uint32_t c = (w2[h] >> (p * 2)) & 3u;
+                    c = ((wn[h] >> p) & 1u) ? 4u : c;
+                    x |= c << (y * 8 + h * 4);
+                }
             }
             v[b] = x;
         }
-        uint32_t *dst = reinterpret_cast<uint32_t *>(tcodes + k * 16);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(tc2 + k * 16);
         dst[0] = v[0];
         dst[1] = v[1];
         dst[2] = v[2];
@@ -244,14 +256,16 @@ __device__ __forceinline__ void pair_row_end(const ExtArgs &a, PairState &P, int
 template <int L, int S>
 __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, uint32_t nslots,
                                               uint8_t *lds_wave) {
+    // lds_wave: tpad bytes of staged target codes per lane group
     constexpr int KB = S <= 8 ? 3 : 4;  // row-argmax key per half: (H << KB) | (KM - s)
     constexpr uint32_t KM = (1u << KB) - 1u;
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane / L;
     const int l = lane % L;
     PairState P[2];
-    uint32_t qo[2];
-    uint8_t *tc[2];
+    uint32_t qo[2], to[2];
+    int tl[2];
+    uint8_t *const tc2 = lds_wave + g * a.tpad;
     int cbh[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -269,9 +283,8 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         P[h].h0 = (int)m1.y;
         P[h].w = (int)m1.z;
         qo[h] = m0.y;
-        const uint32_t to = m0.z;
-        tc[h] = lds_wave + (2 * g + h) * a.tpad;
-        stage_target(a, tc[h], P[h].tlen, to, l, L);
+        to[h] = m0.z;
+        tl[h] = P[h].tlen;
         P[h].beg = 0;
         P[h].end = P[h].qlen;
         P[h].best = P[h].h0;
@@ -282,11 +295,11 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         P[h].alive = P[h].valid && P[h].tlen > 0;
         cbh[h] = l * S - (L * S - 1 - P[h].qlen);  // column of slot 0 in this lane
     }
+    stage_targets2(a, tc2, tl, to, l, L);
 
     const int oe_ins = a.o_ins + a.e_ins, oe_del = a.o_del + a.e_del;
     const uint32_t einsp = pk(a.e_ins, a.e_ins), edelp = pk(a.e_del, a.e_del);
     const uint32_t oedelp = pk(oe_del, oe_del);
-    const uint32_t nvalp = a.nrow | (a.nrow << 16);  // biased N-row score in both halves
     const uint32_t kp = pk(a.mguard, a.mguard);
     const uint32_t cbp = pk(cbh[0], cbh[1]);
 
@@ -317,6 +330,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     // and a lane-local part max(M + (s*e_ins - oe_ins), s*e_ins) whose
     // constants are wave-uniform.
     const uint32_t cbe = mul16(cbp, einsp);  // cb * e_ins per half
+    const uint32_t cbeb = add16(cbe, 0x40004000u);
     uint32_t se[S], seo[S];                  // s*e_ins, s*e_ins - oe_ins (wave-uniform)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -332,13 +346,9 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
 
     for (int i = 0;; ++i) {
         if (__builtin_amdgcn_ballot_w64(P[0].alive || P[1].alive) == 0) break;
-        int rb[2], re[2], h1i[2], tb[2];
+        int rb[2], re[2], h1i[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            // rows past the end are never used; the read stays inside the pair's region
-            const int ti = i < P[h].tlen ? i : 0;
-            const int t = tc[h][ti];
-            tb[h] = P[h].alive ? t : 0;
             rb[h] = max(P[h].beg, i - P[h].w);  // :1038-1046
             re[h] = min(min(P[h].end, i + P[h].w + 1), P[h].qlen);
             if (!P[h].alive) {
@@ -347,40 +357,40 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             }
             h1i[h] = rb[h] == 0 ? max(P[h].h0 - (a.o_del + a.e_del * (i + 1)), 0) : 0;  // :1049-1054
         }
+        // both pairs' target codes of this row (a row past a pair's target reads
+        // its code 0, and a finished pair's half is never used)
+        const uint32_t tw = tc2[i];
         const uint32_t rbp = pk(rb[0], rb[1]), rep = pk(re[0], re[1]);
         const uint32_t h1ip = pk(h1i[0], h1i[1]);
-        const uint32_t sel = (uint32_t)(tb[0] & 3) | 0x0C00u | ((uint32_t)(4 + (tb[1] & 3)) << 16) |
-                             0x0C000000u;
-        const uint32_t nmask = (tb[0] == 4 ? 0xFFFFu : 0u) | (tb[1] == 4 ? 0xFFFF0000u : 0u);
+        // score byte mat[t][q] + 128 from each pair's profile word; an N target
+        // (uniform N row) selects the byte 0x00 (selector 0x0C) and its half's
+        // bias becomes 128 - (mat[20] + 128)
+        const uint32_t nb = ((tw & 4u) >> 2) | ((tw & 0x40u) << 10);
+        const uint32_t sel = (tw & 3u) | ((tw & 0x30u) << 12) | 0x0C040C00u | (nb * 0x0Cu);
+        const uint32_t biasp = 0x00800080u - nb * a.nrow;
 
         // Band tests are lane-relative: with t = bound - cb, column cb+s is
         // below the bound iff s - t < 0, and c <= bound at slot s is c < bound
         // at slot s-1.
         const uint32_t trb = sub16(rbp, cbp), tre = sub16(rep, cbp);
-        const bool anyN = __builtin_amdgcn_ballot_w64(tb[0] == 4 || tb[1] == 4) != 0;
 
         // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
         uint32_t Mv[S], pre[S], mrb[S];
         uint32_t run = negs;
-        auto pass1 = [&](auto hasN) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
-                if constexpr (decltype(hasN)::value) sb = bfi(nmask, nvalp, sb);
-                const uint32_t M = smin(sub16(add16(Hh[s], sb), 0x00800080u), mul16(Hh[s], kp));
-                Mv[s] = M;
-                mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
-                const uint32_t G = bfi(mrb[s], negs, smax(add16(M, seo[s]), se[s]));
-                pre[s] = run;
-                run = smax(run, G);
-            }
-        };
-        if (anyN)
-            pass1(std::true_type{});
-        else
-            pass1(std::false_type{});
-        // carry relative to this lane's cb*e_ins
-        const uint32_t carry = sub16(excl_scan_s2<L>(add16(run, cbe), l), cbe);
+        for (int s = 0; s < S; ++s) {
+            const uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
+            const uint32_t M = smin(sub16(add16(Hh[s], sb), biasp), mul16(Hh[s], kp));
+            Mv[s] = M;
+            mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
+            const uint32_t G = bfi(mrb[s], negs, smax(add16(M, seo[s]), se[s]));
+            pre[s] = run;
+            run = smax(run, G);
+        }
+        // carry relative to this lane's cb*e_ins; the scanned values run + cb*e_ins
+        // lie in (-16384, 16384), so with 0x4000 added they scan as unsigned with
+        // identity 0 (the empty prefix of lane 0 comes back as -16384 - cb*e_ins)
+        const uint32_t carry = sub16(excl_scan_u2<L>(add16(run, cbeb), l), cbeb);
 
         // pass 2: H, E, row-max key, write-back, non-zero columns
         uint32_t pkey = 0, acc = 0, Hprev = 0;
@@ -414,7 +424,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             Hprev = H;
         }
         {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
-            const uint32_t hl = dpp<kRowShr + 1>(0u, Hprev);
+            const uint32_t hl = dmov<kRowShr + 1>(Hprev);
             const uint32_t hv = bfi1(mle_rb0, h1ip, hl);
             Hh[0] = bfi1(mle_re0, hv, Hh[0]);
             const uint32_t x = (Hh[0] | Ee[0]) & wr0;
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(256) void extend16_plain_kernel(ExtArgs a, uint32_t
     constexpr int G = 64 / L;
     const int wave = (int)(threadIdx.x >> 6);
     const uint32_t slot0 = (blockIdx.x * (blockDim.x >> 6) + (uint32_t)wave) * (2u * G);
-    uint8_t *lds_wave = lds + wave * 2 * G * a.tpad;
+    uint8_t *lds_wave = lds + wave * G * a.tpad;
     if (slot0 >= count) return;
     extend16_wave<L, S>(a, first + slot0, first + count, lds_wave);
 }
@@ -498,7 +508,7 @@ __global__ __launch_bounds__(256) void extend16_persistent_kernel(ExtArgs a, uin
     extern __shared__ uint8_t lds[];
     constexpr int G = 64 / L;
     const int wave = (int)(threadIdx.x >> 6);
-    uint8_t *lds_wave = lds + wave * 2 * G * a.tpad;
+    uint8_t *lds_wave = lds + wave * G * a.tpad;
     const uint32_t ntasks = (count + 2 * G - 1) / (2 * G);
     const uint32_t chunk = (uint32_t)a.chunk;
     uint32_t *counter = a.queue + qidx;
@@ -518,7 +528,7 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
                            uint32_t qidx, int cus, int blocks_per_cu, hipStream_t st) {
     constexpr int G = 64 / L;
     if (count == 0) return hipSuccess;
-    const size_t lds = (size_t)4 * 2 * G * a.tpad;
+    const size_t lds = (size_t)4 * G * a.tpad;
     if (lds > 65536) return hipErrorInvalidValue;
     const uint32_t per_block = 4u * 2u * G;
     const uint32_t need = (count + per_block - 1) / per_block;
@@ -528,7 +538,7 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
         // short wavefronts hand their slot on at once
         const uint32_t pw = 2u * G;
         hipLaunchKernelGGL((extend16_plain_kernel<L, S>), dim3((count + pw - 1) / pw), dim3(64),
-                           (size_t)2 * G * a.tpad, st, a, first, count);
+                           (size_t)G * a.tpad, st, a, first, count);
         return hipGetLastError();
     }
     int occ = 0;

@@ -35,7 +35,8 @@ struct ExtArgs {
     uint32_t nrow;       // mat[20]+128 (the N target row is uniform)
     int32_t o_del, e_del, o_ins, e_ins, zdrop;
     int32_t mguard;      // 16-bit kernel: K = max(2, 1 + max(mat)), M = min(H + s, H * K)
-    int32_t tpad;        // LDS bytes of staged target per pair (multiple of 16)
+    int32_t tpad;        // LDS bytes of staged target codes (one per row, multiple of 16, > tlen):
+                         // per pair, or per lane group of two pairs in the packed kernel
     int32_t chunk;       // persistent launch: pairs per queue pull (multiple of 64/L)
 };
 

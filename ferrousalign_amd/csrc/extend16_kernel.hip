@@ -198,46 +198,65 @@ __device__ __forceinline__ void stage_targets2(const ExtArgs &a, uint8_t *tc2, c
     }
 }
 
-// Per-pair scalar state of the reference's row loop (replicated in every
-// lane of the group).
-struct PairState {
-    bool valid, alive;
-    uint32_t pid;
-    int qlen, tlen, h0, w;
-    int beg, end, best, max_i, max_j, max_ie, gscore, max_off;
-    uint32_t ncell;
+// Per-pair state of the reference's row loop, both pairs of a lane group
+// packed like the cells (pair A in the low, pair B in the high 16 bits) and
+// replicated in every lane of the group.  Every field fits a signed 16-bit
+// half: scores < 4096, rows < 2048, columns <= 256, w clamped to 4095 (any
+// w >= 2304 gives the same band, since i < 2048 and qlen <= 256).
+struct PairPack {
+    uint32_t alive;                     // 0xFFFF per pair still running
+    uint32_t qlen, tlen, w, w1, h0;     // w1 = w + 1
+    uint32_t beg, end, best, max_i, max_j, max_ie, gscore, max_off;
+    uint32_t ncell[2];                  // cells per pair (32-bit)
 };
 
-// End of row i for one pair (:1110-1158), written with selects only so that
-// the lane groups of a wavefront never diverge.
-__device__ __forceinline__ void pair_row_end(const ExtArgs &a, PairState &P, int i, int rb, int re,
-                                             int key, int hend, int fenc, int lastp1) {
-    const bool alive = P.alive;
-    P.ncell += alive ? (uint32_t)max(re - rb, 0) : 0u;
-    const bool g = alive && re == P.qlen && P.gscore < hend;  // :1110-1120
-    P.gscore = g ? hend : P.gscore;
-    P.max_ie = g ? i : P.max_ie;
-    const int m_row = key >> 10;
-    const int mj = 1023 - (key & 1023);
-    const bool zero = m_row == 0;                          // :1122
-    const bool better = !zero && m_row > P.best;           // :1126-1133
-    const int di = i - P.max_i, dj = mj - P.max_j;         // :1134-1145
-    const int drop = di > dj ? P.best - m_row - (di - dj) * a.e_del
-                             : P.best - m_row - (dj - di) * a.e_ins;
-    const bool zbrk = !zero && !better && a.zdrop > 0 && drop > a.zdrop;
-    const bool brk = zero || zbrk;
-    const bool upd = alive && better;
-    P.best = upd ? m_row : P.best;
-    P.max_i = upd ? i : P.max_i;
-    P.max_j = upd ? mj : P.max_j;
-    const int off = mj > i ? mj - i : i - mj;
-    P.max_off = upd ? max(P.max_off, off) : P.max_off;
-    const int nb = fenc ? 1023 - fenc : re;                 // :1148-1158
-    const int ne = max(nb, lastp1 - 1);
-    const bool band = alive && !brk;
-    P.beg = band ? nb : P.beg;
-    P.end = band ? min(ne + 2, P.qlen) : P.end;
-    P.alive = alive && !brk && i + 1 < P.tlen;
+// 0xFFFF per half where a < b (signed, |a - b| < 32768)
+__device__ __forceinline__ uint32_t lt16(uint32_t a, uint32_t b) { return sgnmask(sub16(a, b)); }
+
+// End of row i for both pairs (:1110-1158), branch-free on packed halves.
+// ip = i in both halves; rbp/rep the row's band; kA/kB the pairs' row-argmax
+// keys (m_row << 10) | (1023 - mj); hend H(i, qlen-1) per half; fenc/lastp1
+// the first (as 1023 - c) and last (as c + 1) non-zero column, 0 when none.
+__device__ __forceinline__ void pair_row_end2(const ExtArgs &a, PairPack &P, uint32_t ip, uint32_t rbp,
+                                              uint32_t rep, int kA, int kB, uint32_t hend,
+                                              uint32_t fenc, uint32_t lastp1, uint32_t capdelp,
+                                              uint32_t capinsp) {
+    const uint32_t alive = P.alive;
+    const uint32_t d = smax(sub16(rep, rbp), 0u) & alive;
+    P.ncell[0] += d & 0xFFFFu;
+    P.ncell[1] += d >> 16;
+    // :1110-1120 (re <= qlen, so re == qlen is !(re < qlen))
+    const uint32_t g = alive & ~lt16(rep, P.qlen) & lt16(P.gscore, hend);
+    P.gscore = bfi1(g, hend, P.gscore);
+    P.max_ie = bfi1(g, ip, P.max_ie);
+    const uint32_t mrow = (uint32_t)(kA >> 10) | ((uint32_t)(kB >> 10) << 16);
+    const uint32_t mj = sub16(0x03FF03FFu, ((uint32_t)kA & 1023u) | (((uint32_t)kB & 1023u) << 16));
+    const uint32_t nz = lt16(0u, mrow);        // :1122 m_row != 0
+    const uint32_t better = lt16(P.best, mrow);  // :1126 (best >= h0 > 0, so m_row != 0)
+    // :1134-1145 z-drop: gap = (di - dj) * e_del if di > dj else (dj - di) * e_ins,
+    // with |di - dj| clamped to cap = 32767 / e (a clamped gap exceeds any
+    // best - m_row - zdrop, so the test is unchanged and the product fits)
+    const uint32_t dd = sub16(sub16(ip, P.max_i), sub16(mj, P.max_j));
+    const uint32_t pos = lt16(0u, dd);
+    const uint32_t ad = smax(dd, sub16(0u, dd));
+    const uint32_t gap = mul16(umin(ad, bfi1(pos, capdelp, capinsp)),
+                               bfi1(pos, pk(a.e_del, a.e_del), pk(a.e_ins, a.e_ins)));
+    // zdrop >= 4096 never breaks (best < 4096); clamped at 8191 it stays in range
+    const int zd = min(a.zdrop, 8191);
+    const uint32_t zbrk = zd > 0 ? nz & ~better & lt16(gap, sub16(sub16(P.best, mrow), pk(zd, zd))) : 0u;
+    const uint32_t upd = alive & better;
+    P.best = bfi1(upd, mrow, P.best);
+    P.max_i = bfi1(upd, ip, P.max_i);
+    P.max_j = bfi1(upd, mj, P.max_j);
+    const uint32_t od = sub16(mj, ip);
+    P.max_off = bfi1(upd, smax(P.max_off, smax(od, sub16(0u, od))), P.max_off);
+    // :1148-1158
+    const uint32_t nb = bfi1(lt16(0u, fenc), sub16(0x03FF03FFu, fenc), rep);
+    const uint32_t ne = smax(nb, sub16(lastp1, 0x00010001u));
+    const uint32_t band = alive & nz & ~zbrk;
+    P.beg = bfi1(band, nb, P.beg);
+    P.end = bfi1(band, smin(add16(ne, 0x00020002u), P.qlen), P.end);
+    P.alive = band & lt16(add16(ip, 0x00010001u), P.tlen);
 }
 
 }  // namespace
@@ -262,39 +281,44 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane / L;
     const int l = lane % L;
-    PairState P[2];
-    uint32_t qo[2], to[2];
-    int tl[2];
+    PairPack P;
+    bool valid[2];
+    uint32_t pid[2], qo[2], to[2];
+    int tl[2], ql[2], h0v[2];
     uint8_t *const tc2 = lds_wave + g * a.tpad;
     int cbh[2];
+    int wv[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
-        P[h].valid = slot < nslots;
+        valid[h] = slot < nslots;
         uint4 m0 = {0u, 0u, 0u, 1u}, m1 = {0u, 1u, 0u, 0u};  // empty slot: qlen 1, tlen 0, h0 1
-        if (P[h].valid) {
+        if (valid[h]) {
             const uint4 *mp = reinterpret_cast<const uint4 *>(a.meta) + 2 * (size_t)slot;
             m0 = mp[0];
             m1 = mp[1];
         }
-        P[h].pid = m0.x;
-        P[h].qlen = (int)m0.w;
-        P[h].tlen = (int)m1.x;
-        P[h].h0 = (int)m1.y;
-        P[h].w = (int)m1.z;
+        pid[h] = m0.x;
+        ql[h] = (int)m0.w;
+        tl[h] = (int)m1.x;
+        h0v[h] = (int)m1.y;
+        wv[h] = min((int)m1.z, 4095);
         qo[h] = m0.y;
         to[h] = m0.z;
-        tl[h] = P[h].tlen;
-        P[h].beg = 0;
-        P[h].end = P[h].qlen;
-        P[h].best = P[h].h0;
-        P[h].max_i = P[h].max_j = P[h].max_ie = -1;
-        P[h].gscore = -1;
-        P[h].max_off = 0;
-        P[h].ncell = 0;
-        P[h].alive = P[h].valid && P[h].tlen > 0;
-        cbh[h] = l * S - (L * S - 1 - P[h].qlen);  // column of slot 0 in this lane
+        cbh[h] = l * S - (L * S - 1 - ql[h]);  // column of slot 0 in this lane
     }
+    P.alive = (valid[0] && tl[0] > 0 ? 0xFFFFu : 0u) | (valid[1] && tl[1] > 0 ? 0xFFFF0000u : 0u);
+    P.qlen = pk(ql[0], ql[1]);
+    P.tlen = pk(tl[0], tl[1]);
+    P.w = pk(wv[0], wv[1]);
+    P.w1 = add16(P.w, 0x00010001u);
+    P.h0 = pk(h0v[0], h0v[1]);
+    P.beg = 0u;
+    P.end = P.qlen;
+    P.best = P.h0;
+    P.max_i = P.max_j = P.max_ie = P.gscore = 0xFFFFFFFFu;  // -1
+    P.max_off = 0u;
+    P.ncell[0] = P.ncell[1] = 0u;
     stage_targets2(a, tc2, tl, to, l, L);
 
     const int oe_ins = a.o_ins + a.e_ins, oe_del = a.o_del + a.e_del;
@@ -312,9 +336,9 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         for (int h = 0; h < 2; ++h) {
             const int c = cbh[h] + s;
             int code = 4;
-            if (P[h].valid && c >= 0 && c < P[h].qlen) code = decode_base(a.seq2, a.seqn, qo[h], c);
+            if (valid[h] && c >= 0 && c < ql[h]) code = decode_base(a.seq2, a.seqn, qo[h], c);
             pw[h] = a.profw[code];
-            const int h0 = P[h].h0;
+            const int h0 = h0v[h];
             const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
             const int x = c == 0 ? h0 : max(v1 - (c - 1) * a.e_ins, 0);  // :992-998
             hv[h] = c >= 0 ? x : 0;
@@ -344,24 +368,24 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     uint32_t negs;
     asm("s_mov_b32 %0, 0xf000f000" : "=s"(negs));
 
-    for (int i = 0;; ++i) {
-        if (__builtin_amdgcn_ballot_w64(P[0].alive || P[1].alive) == 0) break;
-        int rb[2], re[2], h1i[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            rb[h] = max(P[h].beg, i - P[h].w);  // :1038-1046
-            re[h] = min(min(P[h].end, i + P[h].w + 1), P[h].qlen);
-            if (!P[h].alive) {
-                rb[h] = 4000;
-                re[h] = -3;
-            }
-            h1i[h] = rb[h] == 0 ? max(P[h].h0 - (a.o_del + a.e_del * (i + 1)), 0) : 0;  // :1049-1054
-        }
+    // z-drop gap clamps 32767 / e per direction (e = 0: the gap is 0 anyway)
+    const int capdel = a.e_del > 0 ? 32767 / a.e_del : 32767;
+    const int capins = a.e_ins > 0 ? 32767 / a.e_ins : 32767;
+    const uint32_t capdelp = pk(capdel, capdel), capinsp = pk(capins, capins);
+
+    uint32_t ip = 0u;  // i in both halves
+    for (int i = 0;; ++i, ip += 0x00010001u) {
+        if (__builtin_amdgcn_ballot_w64(P.alive != 0u) == 0) break;
+        // :1038-1046; a finished pair gets the empty band [4000, -3)
+        const uint32_t rbp = bfi1(P.alive, smax(P.beg, sub16(ip, P.w)), 0x0FA00FA0u);
+        const uint32_t rep = bfi1(P.alive, smin(smin(P.end, add16(ip, P.w1)), P.qlen), 0xFFFDFFFDu);
+        // :1049-1054 h1 = max(h0 - (o_del + e_del*(i+1)), 0) when rb == 0 (h0 < 4096,
+        // so clamping the subtrahend at 32767 keeps the difference in range)
+        const int dsub = min(a.o_del + a.e_del * (i + 1), 32767);
+        const uint32_t h1ip = smax(sub16(P.h0, pk(dsub, dsub)), 0u) & lt16(rbp, 0x00010001u);
         // both pairs' target codes of this row (a row past a pair's target reads
         // its code 0, and a finished pair's half is never used)
         const uint32_t tw = tc2[i];
-        const uint32_t rbp = pk(rb[0], rb[1]), rep = pk(re[0], re[1]);
-        const uint32_t h1ip = pk(h1i[0], h1i[1]);
         // score byte mat[t][q] + 128 from each pair's profile word; an N target
         // (uniform N row) selects the byte 0x00 (selector 0x0C) and its half's
         // bias becomes 128 - (mat[20] + 128)
@@ -454,23 +478,21 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         hend = allmax_u2<L>(hend);
         ffirst = allmax_u2<L>(ffirst);
         flast = allmax_u2<L>(flast);
-        pair_row_end(a, P[0], i, rb[0], re[0], kA, lo16(hend), (int)(ffirst & 0xFFFFu),
-                     (int)(flast & 0xFFFFu));
-        pair_row_end(a, P[1], i, rb[1], re[1], kB, hi16(hend), (int)(ffirst >> 16),
-                     (int)(flast >> 16));
+        pair_row_end2(a, P, ip, rbp, rep, kA, kB, hend, ffirst, flast, capdelp, capinsp);
     }
     if (l == 0) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (!P[h].valid) continue;
-            int32_t *o = a.out + (size_t)P[h].pid * 6;
-            o[0] = P[h].best;
-            o[1] = P[h].max_i + 1;
-            o[2] = P[h].max_ie + 1;
-            o[3] = P[h].max_j + 1;
-            o[4] = P[h].gscore;
-            o[5] = P[h].max_off;
-            if (a.cells) a.cells[P[h].pid] = P[h].ncell;
+            if (!valid[h]) continue;
+            const int sh = 16 * h;
+            int32_t *o = a.out + (size_t)pid[h] * 6;
+            o[0] = (int)(short)(P.best >> sh);
+            o[1] = (int)(short)(P.max_i >> sh) + 1;
+            o[2] = (int)(short)(P.max_ie >> sh) + 1;
+            o[3] = (int)(short)(P.max_j >> sh) + 1;
+            o[4] = (int)(short)(P.gscore >> sh);
+            o[5] = (int)(short)(P.max_off >> sh);
+            if (a.cells) a.cells[pid[h]] = P.ncell[h];
         }
     }
 }

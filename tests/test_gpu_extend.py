@@ -117,3 +117,20 @@ def test_parity_random_edge_cases(seed):
             print("random", seed, ci, flags, flush=True)
             ora, oc, gpu, gc = _run_both(ps, sw, flags=flags)
             _assert_same(ora, oc, gpu, gc, "random seed=%d cfg=%d flags=%d" % (seed, ci, flags))
+
+
+def test_parity_large_band_and_zdrop():
+    """Bands and z-drops beyond what 16-bit state can hold directly: a huge
+    end_bonus keeps the w cap (ksw_affine_gap.rs:1001-1018) above 4096, and
+    z-drop values of 5,000 and 100,000 exceed every reachable best - m_row."""
+    from ferrousalign_amd.banded_swa import BandedPairWiseSW, bwa_fill_scmat
+    from ferrousalign_amd import _native as N
+    rng = np.random.default_rng(11)
+    ps = _random_case_batch(rng, 2000)
+    ps.w[:] = 50000
+    for zdrop in (100, 5000, 100000):
+        sw = BandedPairWiseSW(6, 1, 6, 1, zdrop, 100000, 5, 5, bwa_fill_scmat(1, 4, -1))
+        for flags in (0, N.FA_SCORE_I32):
+            print("large band", zdrop, flags, flush=True)
+            ora, oc, gpu, gc = _run_both(ps, sw, flags=flags)
+            _assert_same(ora, oc, gpu, gc, "large band zdrop=%d flags=%d" % (zdrop, flags))

@@ -107,6 +107,7 @@ __device__ __forceinline__ uint32_t bfi1(uint32_t m, uint32_t a, uint32_t b) {
 
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr = 0x110;
+constexpr int kRowNewBcast15 = 0x15F;  // every lane of a 16-lane row reads its lane 15
 
 // DPP move with bound_ctrl: a lane without a source lane (row_shr at the
 // start of a 16-lane row) reads 0, and no old value is needed, so the
@@ -391,7 +392,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         // bias becomes 128 - (mat[20] + 128)
         const uint32_t nb = ((tw & 4u) >> 2) | ((tw & 0x40u) << 10);
         const uint32_t sel = (tw & 3u) | ((tw & 0x30u) << 12) | 0x0C040C00u | (nb * 0x0Cu);
-        const uint32_t biasp = 0x00800080u - nb * a.nrow;
+        const uint32_t biasp = 0x00800080u - nb * (a.nrow & 0xFFu);  // (24-bit multiply)
 
         // Band tests are lane-relative: with t = bound - cb, column cb+s is
         // below the bound iff s - t < 0, and c <= bound at slot s is c < bound
@@ -472,10 +473,14 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         }
         uint32_t ffirst = fenc[0] | (fenc[1] << 16);
         uint32_t flast = lenc[0] | (lenc[1] << 16);
-        uint32_t hend = l == L - 1 ? Hh[S - 1] : 0u;  // column qlen: H(i, qlen-1) when re == qlen
+        // column qlen (the group's last slot): H(i, qlen-1) when re == qlen
+        uint32_t hend;
+        if constexpr (L == 16)
+            hend = dmov<kRowNewBcast15>(Hh[S - 1]);  // lane 15 of the 16-lane row
+        else
+            hend = allmax_u2<L>(l == L - 1 ? Hh[S - 1] : 0u);
         const int kA = allmax_i<L>(kk[0]);
         const int kB = allmax_i<L>(kk[1]);
-        hend = allmax_u2<L>(hend);
         ffirst = allmax_u2<L>(ffirst);
         flast = allmax_u2<L>(flast);
         pair_row_end2(a, P, ip, rbp, rep, kA, kB, hend, ffirst, flast, capdelp, capinsp);

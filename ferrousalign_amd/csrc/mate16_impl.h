@@ -61,32 +61,34 @@ __device__ __forceinline__ void bfi_into(uint32_t &b, uint32_t m, uint32_t a) {
 
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowShr1 = 0x111;
 
+// DPP move with bound_ctrl (a lane without a source lane reads 0): no old
+// value, so no copy of the source, and 32-bit consumers fuse with it
 template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xF, 0xF, false);
+__device__ __forceinline__ uint32_t dmov(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 // per-half signed max over the 8 lanes of a group
 __device__ __forceinline__ uint32_t grp8_smax2(uint32_t x) {
-    x = smax(x, dpp<kQuadXor1>(x, x));
-    x = smax(x, dpp<kQuadXor2>(x, x));
-    x = smax(x, dpp<kRowHalfMirror>(x, x));
+    x = smax(x, dmov<kQuadXor1>(x));
+    x = smax(x, dmov<kQuadXor2>(x));
+    x = smax(x, dmov<kRowHalfMirror>(x));
     return x;
 }
 __device__ __forceinline__ uint32_t grp8_or(uint32_t x) {
-    x |= dpp<kQuadXor1>(x, x);
-    x |= dpp<kQuadXor2>(x, x);
-    x |= dpp<kRowHalfMirror>(x, x);
+    x |= dmov<kQuadXor1>(x);
+    x |= dmov<kQuadXor2>(x);
+    x |= dmov<kRowHalfMirror>(x);
     return x;
 }
 __device__ __forceinline__ int grp8_imax(int x) {
-    x = max(x, (int)dpp<kQuadXor1>((uint32_t)x, (uint32_t)x));
-    x = max(x, (int)dpp<kQuadXor2>((uint32_t)x, (uint32_t)x));
-    x = max(x, (int)dpp<kRowHalfMirror>((uint32_t)x, (uint32_t)x));
+    x = max(x, (int)dmov<kQuadXor1>((uint32_t)x));
+    x = max(x, (int)dmov<kQuadXor2>((uint32_t)x));
+    x = max(x, (int)dmov<kRowHalfMirror>((uint32_t)x));
     return x;
 }
 // value of lane p-1 of the group (lane 0: 0): the SSE shift by one lane
 __device__ __forceinline__ uint32_t shift_up(uint32_t x, int p) {
-    const uint32_t y = dpp<kRowShr1>(0u, x);
+    const uint32_t y = dmov<kRowShr1>(x);
     return p == 0 ? 0u : y;
 }
 

@@ -853,7 +853,7 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     std::vector<int32_t> ql(m), tl(m), ww(m), hh(m);
     std::vector<uint64_t> tbo(m), cgo(m), alo(m);
     uint64_t words = 0, tbw = 0, cgw = 0, alw = 0;
-    int tmax = 0;
+    int tmax = 0, qmax = 0;
     for (size_t p = 0; p < m; ++p) {
         const int64_t k = job[p];
         ql[p] = query_len[k];
@@ -866,8 +866,15 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
                 L = c;
                 break;
             }
-        const int S = fa::swtb_slot_count(L, ql[p]);
+        int S = fa::swtb_slot_count(L, ql[p]);
         lsl[p] = ((uint32_t)L << 8) | (uint32_t)S;
+        int bl, bs;
+        if (fa::swtb_band_config(ql[p], ww[p], L * S, &bl, &bs)) {  // band-window kernel
+            L = bl;
+            S = bs;
+            lsl[p] = fa::kSwtbBand | ((uint32_t)L << 8) | (uint32_t)S;
+            qmax = std::max(qmax, ql[p]);
+        }
         qo[p] = (uint32_t)words;
         words += seq_words(ql[p]);
         to[p] = (uint32_t)words;
@@ -960,13 +967,14 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     a.zdrop = sw->zdrop;
     a.pen_clip3 = sw->pen_clip3;
     a.tpad = ((tmax + 16) / 16) * 16;
+    a.qpad = ((qmax + 16) / 16) * 16;
     hipError_t e = hipEventRecord(ctx->ev0, st);
-    for (size_t x = 0; x < m && e == hipSuccess;) {  // one launch per (L, S) bucket
+    for (size_t x = 0; x < m && e == hipSuccess;) {  // one launch per (kernel, L, S) bucket
         size_t y = x;
         while (y < m && lsl[order[y]] == lsl[order[x]]) ++y;
         const uint32_t key = lsl[order[x]];
-        e = fa::swtb_dp_launch((int)(key >> 8), (int)(key & 0xFFu), a, (uint32_t)x,
-                               (uint32_t)(y - x), st);
+        e = fa::swtb_dp_launch((int)((key >> 8) & 0xFFu), (int)(key & 0xFFu),
+                               (key & fa::kSwtbBand) != 0, a, (uint32_t)x, (uint32_t)(y - x), st);
         x = y;
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);

@@ -11,7 +11,9 @@ namespace fa {
 // block: tlen rows of L * ((S + 15) / 16) words from tbo[p], lane l's S
 // columns (2 bits each, column l*S + s at bits 2*(s % 16) of word
 // l * ((S+15)/16) + s/16) of DP row i at word i * L * ((S+15)/16).
-// lsl[p] = (L << 8) | S of the kernel that ran the pair.
+// lsl[p] = (L << 8) | S of the kernel that ran the pair, | kSwtbBand for the
+// band-window kernel, whose rows store the L*S columns of their window
+// (starting at column (i / S) * S - w[p]) instead of columns 0 .. L*S-1.
 struct SwtbArgs {
     const uint32_t *seq2;
     const uint32_t *seqn;
@@ -40,12 +42,19 @@ struct SwtbArgs {
     uint32_t nrow;           // mat[20]+128 (uniform N target row)
     int32_t o_del, e_del, o_ins, e_ins, zdrop, pen_clip3;
     int32_t tpad;            // LDS bytes of staged target per pair
+    int32_t qpad;            // LDS bytes of staged query per pair (band-window kernel)
 };
+
+constexpr uint32_t kSwtbBand = 1u << 16;
 
 // Slot count S for L lanes with L*S >= qlen+1, or -1 (S <= 32).
 int swtb_slot_count(int L, int qlen);
-hipError_t swtb_dp_launch(int L, int S, const SwtbArgs &a, uint32_t first, uint32_t count,
-                          hipStream_t st);
+// Band-window configuration (L lanes x S slots) for a pair whose band
+// [i - w, i + w + 1] fits 2w + S + 1 <= L*S columns and L*S < full_cols (the
+// full-width layout's columns); false when the full-width kernel is the better fit.
+bool swtb_band_config(int qlen, int w, int full_cols, int *L, int *S);
+hipError_t swtb_dp_launch(int L, int S, bool band, const SwtbArgs &a, uint32_t first,
+                          uint32_t count, hipStream_t st);
 hipError_t swtb_traceback_launch(const SwtbArgs &a, uint32_t n, hipStream_t st);
 
 }  // namespace fa

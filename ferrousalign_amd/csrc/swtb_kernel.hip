@@ -17,6 +17,16 @@
 // instead of the reference's f = 0; both lose to M >= 0 and E >= 0, so H and
 // the codes agree.
 //
+// Band-window variant (BAND): the band of row i lies inside [i - w, i + w + 1]
+// (:108-113), so when 2w + S + 1 <= L*S the group holds only a window of
+// L*S columns, starting at (i / S) * S - w for the rows of block i / S; every
+// S rows each lane takes over the registers of the next lane (one DPP per
+// array and slot) and the last lane loads the S columns entering the window.
+// A column beyond the window has never been inside a band, so it enters with
+// its first-row values (:73-81) and E = 0, exactly what the reference's eh[]
+// holds there; a column leaving on the left is never read again (the band
+// start never decreases).  Traceback codes are stored per window slot.
+//
 // Traceback kernel: one lane per pair walks the codes from (max_i+1, max_j+1)
 // to (0, 0) exactly as the reference (runs of equal codes, row 0 = INS,
 // column 0 = DEL, 10,000-step cap, stop without progress), then adds the 5'
@@ -34,6 +44,7 @@ namespace {
 constexpr int kNeg = -(1 << 28);
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr = 0x110;
+constexpr int kRowShl1 = 0x101;  // lane i reads lane i+1 of its 16-lane row
 enum { TB_MATCH = 0, TB_DEL = 1, TB_INS = 2 };
 enum { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4 };
 
@@ -88,7 +99,7 @@ __device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *se
 
 }  // namespace
 
-template <int L, int S>
+template <int L, int S, bool BAND>
 __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first, uint32_t count) {
     extern __shared__ uint8_t lds[];
     constexpr int G = 64 / L;
@@ -108,19 +119,33 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
     const int o_del = a.o_del, e_del = a.e_del, o_ins = a.o_ins, e_ins = a.e_ins;
     const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
 
-    uint8_t *tcodes = lds + g * a.tpad;
+    uint8_t *tcodes = lds + g * (a.tpad + (BAND ? a.qpad : 0));
+    uint8_t *qcodes = tcodes + a.tpad;  // BAND: the query's codes
     for (int k = l; k < tlen; k += L) tcodes[k] = (uint8_t)seq_code(a.seq2, a.seqn, to, k);
+    if constexpr (BAND) {
+        for (int k = l; k < qlen; k += L) qcodes[k] = (uint8_t)seq_code(a.seq2, a.seqn, qo, k);
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    }
 
-    const int cb = l * S;
+    // column of this lane's slot 0 (BAND: the window of rows [0, S) starts at -w)
+    int cb = (BAND ? -w : 0) + l * S;
+    const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
+    auto col_prof = [&](int c) -> uint32_t {
+        int code = 4;
+        if (c >= 0 && c < qlen) code = BAND ? (int)qcodes[c] : seq_code(a.seq2, a.seqn, qo, c);
+        return a.profw[code];
+    };
+    auto col_init = [&](int c) -> int {  // first-row eh.h (:73-81); 0 outside [0, qlen]
+        if (c < 0 || c > qlen) return 0;
+        return c == 0 ? h0 : imax(v1 - (c - 1) * e_ins, 0);
+    };
     uint32_t prof[S];
     int Hh[S], Ee[S];
-    const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const int c = cb + s;
-        prof[s] = a.profw[c < qlen ? seq_code(a.seq2, a.seqn, qo, c) : 4];
-        const int hv = c == 0 ? h0 : imax(v1 - (c - 1) * e_ins, 0);  // :73-81
-        Hh[s] = c <= qlen ? hv : 0;
+        prof[s] = col_prof(cb + s);
+        Hh[s] = col_init(cb + s);
         Ee[s] = 0;
     }
     __builtin_amdgcn_s_waitcnt(0);
@@ -132,6 +157,21 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
     bool alive = valid && tlen > 0;
     for (int i = 0;; ++i) {
         if (__builtin_amdgcn_ballot_w64(alive) == 0) break;
+        if constexpr (BAND) {
+            if (i > 0 && i % S == 0) {  // the window moves S columns right
+                cb += S;
+                const bool last = l == L - 1;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const uint32_t p2 = (uint32_t)dpp<kRowShl1>(0, (int)prof[s]);
+                    const int h2 = dpp<kRowShl1>(0, Hh[s]);
+                    const int e2 = dpp<kRowShl1>(0, Ee[s]);
+                    prof[s] = last ? col_prof(cb + s) : p2;
+                    Hh[s] = last ? col_init(cb + s) : h2;
+                    Ee[s] = last ? 0 : e2;
+                }
+            }
+        }
         const int tbase = alive ? tcodes[i] : 0;
         int rb = imax(beg, i - w);
         int re = imin(imin(end, i + w + 1), qlen);
@@ -253,12 +293,19 @@ namespace {
 
 struct TbView {
     const uint32_t *tb;
-    int L, S, nwl;
+    int L, S, nwl, band_w;
+    bool band;
     __device__ int code(int ci, int cj) const {
         if (ci == 0) return cj > 0 ? TB_INS : TB_MATCH;
         if (cj == 0) return TB_DEL;
-        const int j = cj - 1, lane = j / S, s = j % S;
-        const uint32_t w = tb[(size_t)(ci - 1) * (size_t)(L * nwl) + (size_t)(lane * nwl + s / 16)];
+        const int i = ci - 1;
+        int j = cj - 1;
+        if (band) {  // slot in the window of row i; outside it: never in a band
+            j -= (i / S) * S - band_w;
+            if (j < 0 || j >= L * S) return TB_MATCH;
+        }
+        const int lane = j / S, s = j % S;
+        const uint32_t w = tb[(size_t)i * (size_t)(L * nwl) + (size_t)(lane * nwl + s / 16)];
         return (int)((w >> (2 * (s % 16))) & 3u);
     }
 };
@@ -281,7 +328,8 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
     const int32_t *o = a.out + (size_t)p * 6;
     const int score = o[0], max_i = o[1] - 1, max_j = o[3] - 1, gscore = o[4];
     const uint32_t lsl = a.lsl[p];
-    TbView v{a.tb + a.tbo[p], (int)(lsl >> 8), (int)(lsl & 0xFFu), (int)((lsl & 0xFFu) + 15) / 16};
+    TbView v{a.tb + a.tbo[p], (int)((lsl >> 8) & 0xFFu), (int)(lsl & 0xFFu),
+             (int)((lsl & 0xFFu) + 15) / 16, a.w[p], (lsl & kSwtbBand) != 0};
     const uint32_t qo = a.qo[p], to = a.to[p];
     // runs and bases are produced end to start: keep them from index 1
     // (cig[0] is kept for the 5' clip), reverse afterwards
@@ -354,12 +402,13 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
     a.n_qry[p] = nqa;
 }
 
-template <int L, int S>
+template <int L, int S, bool BAND>
 static hipError_t launch_dp(const SwtbArgs &a, uint32_t first, uint32_t count, hipStream_t st) {
     constexpr int G = 64 / L;
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL((swtb_dp_kernel<L, S>), dim3((count + G - 1) / G), dim3(64),
-                       (size_t)G * a.tpad, st, a, first, count);
+    const size_t lds = (size_t)G * (size_t)(a.tpad + (BAND ? a.qpad : 0));
+    hipLaunchKernelGGL((swtb_dp_kernel<L, S, BAND>), dim3((count + G - 1) / G), dim3(64), lds, st, a,
+                       first, count);
     return hipGetLastError();
 }
 
@@ -371,10 +420,28 @@ int swtb_slot_count(int L, int qlen) {
     return -1;
 }
 
-hipError_t swtb_dp_launch(int L, int S, const SwtbArgs &a, uint32_t first, uint32_t count,
-                          hipStream_t st) {
+bool swtb_band_config(int qlen, int w, int full_cols, int *L, int *S) {
+    static const int cfg[][2] = {{4, 8}, {4, 16}, {8, 16}};
+    if (w < 0 || w > 1024) return false;
+    for (const auto &c : cfg)
+        if (2 * w + c[1] + 1 <= c[0] * c[1] && c[0] * c[1] < full_cols && qlen > 0) {
+            *L = c[0];
+            *S = c[1];
+            return true;
+        }
+    return false;
+}
+
+hipError_t swtb_dp_launch(int L, int S, bool band, const SwtbArgs &a, uint32_t first,
+                          uint32_t count, hipStream_t st) {
+    if (band) {
+        if (L == 4 && S == 8) return launch_dp<4, 8, true>(a, first, count, st);
+        if (L == 4 && S == 16) return launch_dp<4, 16, true>(a, first, count, st);
+        if (L == 8 && S == 16) return launch_dp<8, 16, true>(a, first, count, st);
+        return hipErrorInvalidValue;
+    }
 #define FA_SW_CASE(LL, SS) \
-    if (L == LL && S == SS) return launch_dp<LL, SS>(a, first, count, st);
+    if (L == LL && S == SS) return launch_dp<LL, SS, false>(a, first, count, st);
 #define FA_SW_LIST(LL)                                                                  \
     FA_SW_CASE(LL, 1) FA_SW_CASE(LL, 2) FA_SW_CASE(LL, 4) FA_SW_CASE(LL, 8) FA_SW_CASE(LL, 12) \
     FA_SW_CASE(LL, 16) FA_SW_CASE(LL, 24) FA_SW_CASE(LL, 32)

@@ -109,3 +109,29 @@ def test_region_like_pairs():
         hs.append(0)
     print("region-like", flush=True)
     _check(qs, ts, ws, hs, "region-like")
+
+
+def test_band_window_boundaries():
+    """Bands at and just past each band-window configuration's limit
+    2w + S + 1 <= L*S (w = 11, 23, 55 fit 4x8, 4x16, 8x16; one more goes to
+    the next configuration or the full-width kernel), on 150-255 bp queries
+    against long targets, so the window moves many times and the traceback
+    reads codes from windows far from column 0."""
+    rng = np.random.default_rng(21)
+    qs, ts, ws, hs = [], [], [], []
+    for w in (0, 1, 11, 12, 23, 24, 55, 56):
+        for _ in range(40):
+            t = rng.integers(0, 4, int(rng.integers(300, 900))).astype(np.uint8)
+            ql = int(rng.integers(150, 256))
+            a = int(rng.integers(0, len(t) - ql))
+            q = t[a:a + ql].copy()
+            q[rng.random(ql) < 0.03] = rng.integers(0, 4)
+            if rng.random() < 0.5:  # an indel so the path leaves the diagonal
+                k = int(rng.integers(10, ql - 10))
+                q = np.insert(q, k, rng.integers(0, 4, int(rng.integers(1, 4))).astype(np.uint8))[:255]
+            qs.append(q)
+            ts.append(t)
+            ws.append(w)
+            hs.append(0 if rng.random() < 0.5 else int(rng.integers(1, 60)))
+    print("band windows", flush=True)
+    _check(qs, ts, ws, hs, "band windows")

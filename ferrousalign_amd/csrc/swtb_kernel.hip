@@ -30,8 +30,17 @@
 // Traceback kernel: one lane per pair walks the codes from (max_i+1, max_j+1)
 // to (0, 0) exactly as the reference (runs of equal codes, row 0 = INS,
 // column 0 = DEL, 10,000-step cap, stop without progress), then adds the 5'
-// soft clip and the 3' clip-or-extend decision (:355-414) and writes the
-// aligned reference and query bases.
+// soft clip and the 3' clip-or-extend decision (:355-414).
+//
+// Aligned bases: each M or D step of the walk emits target base ci-1 and
+// decrements ci, each M or I step emits query base cj-1 and decrements cj, so
+// a walk from (max_i+1, max_j+1) that ends at (ci, cj) emits target
+// max_i .. ci and query max_j .. cj, and after the reversal (:333-337) the
+// aligned reference is target[ci .. max_i] and the aligned query
+// query[cj .. max_j].  The walk only counts them; the bases kernel writes them
+// with one wavefront per pair, coalesced, instead of one byte per step.  (The
+// walk always ends at (0, 0): every step makes progress, the edges are DEL and
+// INS, and qlen + tlen < 10,000.)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -295,7 +304,9 @@ struct TbView {
     const uint32_t *tb;
     int L, S, nwl, band_w;
     bool band;
-    __device__ int code(int ci, int cj) const {
+    size_t last = ~(size_t)0;  // index of the word last read (runs of I reuse it)
+    uint32_t word = 0;
+    __device__ int code(int ci, int cj) {
         if (ci == 0) return cj > 0 ? TB_INS : TB_MATCH;
         if (cj == 0) return TB_DEL;
         const int i = ci - 1;
@@ -305,8 +316,12 @@ struct TbView {
             if (j < 0 || j >= L * S) return TB_MATCH;
         }
         const int lane = j / S, s = j % S;
-        const uint32_t w = tb[(size_t)i * (size_t)(L * nwl) + (size_t)(lane * nwl + s / 16)];
-        return (int)((w >> (2 * (s % 16))) & 3u);
+        const size_t idx = (size_t)i * (size_t)(L * nwl) + (size_t)(lane * nwl + s / 16);
+        if (idx != last) {
+            word = tb[idx];
+            last = idx;
+        }
+        return (int)((word >> (2 * (s % 16))) & 3u);
     }
 };
 
@@ -317,8 +332,6 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
     if (p >= n) return;
     const int qlen = a.qlen[p], tlen = a.tlen[p];
     uint32_t *cig = a.cigar + a.cig_off[p];
-    uint8_t *ra = a.ref_aln + a.aln_off[p];
-    uint8_t *qa = a.qry_aln + a.aln_off[p];
     if (qlen <= 0 || tlen <= 0) {  // :17-30
         a.n_cigar[p] = 0;
         a.n_ref[p] = 0;
@@ -330,10 +343,9 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
     const uint32_t lsl = a.lsl[p];
     TbView v{a.tb + a.tbo[p], (int)((lsl >> 8) & 0xFFu), (int)(lsl & 0xFFu),
              (int)((lsl & 0xFFu) + 15) / 16, a.w[p], (lsl & kSwtbBand) != 0};
-    const uint32_t qo = a.qo[p], to = a.to[p];
-    // runs and bases are produced end to start: keep them from index 1
-    // (cig[0] is kept for the 5' clip), reverse afterwards
-    int nrc = 0, nra = 0, nqa = 0;
+    // runs are produced end to start: keep them from index 1 (cig[0] is kept
+    // for the 5' clip), reverse afterwards
+    int nrc = 0;
     int ci = max_i + 1, cj = max_j + 1, iter = 0;
     while (ci > 0 || cj > 0) {
         if (iter >= 10000) break;
@@ -344,8 +356,6 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
         if (code == TB_MATCH) {
             while (ci > 0 && cj > 0 && v.code(ci, cj) == TB_MATCH) {
                 ++cnt;
-                ra[nra++] = (uint8_t)seq_code(a.seq2, a.seqn, to, ci - 1);
-                qa[nqa++] = (uint8_t)seq_code(a.seq2, a.seqn, qo, cj - 1);
                 --ci;
                 --cj;
             }
@@ -353,14 +363,12 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
         } else if (code == TB_DEL) {
             while (ci > 0 && v.code(ci, cj) == TB_DEL) {
                 ++cnt;
-                ra[nra++] = (uint8_t)seq_code(a.seq2, a.seqn, to, ci - 1);
                 --ci;
             }
             if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_D;
         } else if (code == TB_INS) {
             while (cj > 0 && v.code(ci, cj) == TB_INS) {
                 ++cnt;
-                qa[nqa++] = (uint8_t)seq_code(a.seq2, a.seqn, qo, cj - 1);
                 --cj;
             }
             if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_I;
@@ -373,16 +381,6 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
         const uint32_t t = cig[1 + x];
         cig[1 + x] = cig[1 + y];
         cig[1 + y] = t;
-    }
-    for (int x = 0, y = nra - 1; x < y; ++x, --y) {
-        const uint8_t t = ra[x];
-        ra[x] = ra[y];
-        ra[y] = t;
-    }
-    for (int x = 0, y = nqa - 1; x < y; ++x, --y) {
-        const uint8_t t = qa[x];
-        qa[x] = qa[y];
-        qa[y] = t;
     }
     int nc;
     const int query_start = cj, query_end = max_j + 1;
@@ -398,8 +396,27 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
         cig[nc++] = ((uint32_t)(qlen - query_end) << 4) | (clip3 ? OP_S : OP_M);
     }
     a.n_cigar[p] = nc;
-    a.n_ref[p] = nra;
-    a.n_qry[p] = nqa;
+    a.n_ref[p] = max_i + 1 - ci;  // target [ci, max_i]
+    a.n_qry[p] = max_j + 1 - cj;  // query [cj, max_j]
+}
+
+// Aligned bases (see the traceback kernel, which ran before on the same
+// stream): one wavefront per pair writes target[ci .. max_i] and
+// query[cj .. max_j] as codes 0..4.
+__global__ __launch_bounds__(256) void swtb_bases_kernel(SwtbArgs a, uint32_t n) {
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    if (p >= n) return;
+    const int qlen = a.qlen[p], tlen = a.tlen[p];
+    if (qlen <= 0 || tlen <= 0) return;
+    const int32_t *o = a.out + (size_t)p * 6;
+    const int nr = a.n_ref[p], nq = a.n_qry[p];
+    const int r0 = o[1] - nr, q0 = o[3] - nq;  // ci, cj where the walk ended
+    uint8_t *ra = a.ref_aln + a.aln_off[p];
+    uint8_t *qa = a.qry_aln + a.aln_off[p];
+    const uint32_t to = a.to[p], qo = a.qo[p];
+    for (int k = lane; k < nr; k += 64) ra[k] = (uint8_t)seq_code(a.seq2, a.seqn, to, r0 + k);
+    for (int k = lane; k < nq; k += 64) qa[k] = (uint8_t)seq_code(a.seq2, a.seqn, qo, q0 + k);
 }
 
 template <int L, int S, bool BAND>
@@ -456,6 +473,9 @@ hipError_t swtb_dp_launch(int L, int S, bool band, const SwtbArgs &a, uint32_t f
 hipError_t swtb_traceback_launch(const SwtbArgs &a, uint32_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(swtb_traceback_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(swtb_bases_kernel, dim3((n + 3) / 4), dim3(256), 0, st, a, n);
     return hipGetLastError();
 }
 

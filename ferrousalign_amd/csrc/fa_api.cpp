@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
@@ -822,17 +823,34 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         !query_aligned || !aligned_offset || !n_ref_aligned || !n_query_aligned)
         return FA_ERR_ARG;
     if (njobs > (int64_t)0x7FFFFFFF) return FA_ERR_ARG;
-    // validation (h0 = 0 is allowed here: region.rs:1058 passes 0)
-    for (int64_t k = 0; k < njobs; ++k) {
-        const int32_t ql = query_len[k], tl = ref_len[k];
-        if (ql < 0 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
-        if (tl < 0 || tl > FA_MAX_TLEN) return FA_ERR_TLEN;
-        if (band_width[k] < 0) return FA_ERR_BAND;
-        if (h0[k] < 0 || h0[k] > (1 << 19)) return FA_ERR_H0;
-        for (int32_t j = 0; j < ql; ++j)
-            if (query_seqs[query_offset[k] + j] > 4) return FA_ERR_CODE;
-        for (int32_t j = 0; j < tl; ++j)
-            if (ref_seqs[ref_offset[k] + j] > 4) return FA_ERR_CODE;
+    // validation (h0 = 0 is allowed here: region.rs:1058 passes 0), in parallel;
+    // the code returned is the one of the first invalid job
+    {
+        std::vector<std::pair<int64_t, int>> first_bad(16, {njobs, FA_OK});
+        parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned t) {
+            for (int64_t k = lo; k < hi; ++k) {
+                const int32_t ql = query_len[k], tl = ref_len[k];
+                int c = FA_OK;
+                if (ql < 0 || ql > FA_MAX_QLEN) c = FA_ERR_QLEN;
+                else if (tl < 0 || tl > FA_MAX_TLEN) c = FA_ERR_TLEN;
+                else if (band_width[k] < 0) c = FA_ERR_BAND;
+                else if (h0[k] < 0 || h0[k] > (1 << 19)) c = FA_ERR_H0;
+                else {
+                    for (int32_t j = 0; j < ql && c == FA_OK; ++j)
+                        if (query_seqs[query_offset[k] + j] > 4) c = FA_ERR_CODE;
+                    for (int32_t j = 0; j < tl && c == FA_OK; ++j)
+                        if (ref_seqs[ref_offset[k] + j] > 4) c = FA_ERR_CODE;
+                }
+                if (c != FA_OK) {
+                    first_bad[t] = {k, c};
+                    return;
+                }
+            }
+        });
+        std::pair<int64_t, int> bad{njobs, FA_OK};
+        for (const auto &b : first_bad)
+            if (b.first < bad.first) bad = b;
+        if (bad.second != FA_OK) return bad.second;
     }
     FA_HIP(hipSetDevice(ctx->device));
     // non-empty jobs run on the device; empty ones get the reference's zero
@@ -895,11 +913,13 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         return tl[x] > tl[y];
     });
     std::vector<uint32_t> seq2(words), seqn(words / 2);
-    for (size_t p = 0; p < m; ++p) {
-        const int64_t k = job[p];
-        pack_seq(query_seqs + query_offset[k], ql[p], &seq2[qo[p]], &seqn[qo[p] / 2]);
-        pack_seq(ref_seqs + ref_offset[k], tl[p], &seq2[to[p]], &seqn[to[p] / 2]);
-    }
+    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
+        for (int64_t p = lo; p < hi; ++p) {
+            const int64_t k = job[(size_t)p];
+            pack_seq(query_seqs + query_offset[k], ql[p], &seq2[qo[p]], &seqn[qo[p] / 2]);
+            pack_seq(ref_seqs + ref_offset[k], tl[p], &seq2[to[p]], &seqn[to[p] / 2]);
+        }
+    });
     hipStream_t st = ctx->stream;
     uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr,
              *d_order = nullptr, *d_lsl = nullptr, *d_tb = nullptr, *d_cells = nullptr,
@@ -980,42 +1000,77 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
     if (e == hipSuccess) e = fa::swtb_traceback_launch(a, (uint32_t)m, st);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev2, st);
+    // records and counts first; then the CIGARs and aligned bases are gathered
+    // on the device into dense buffers (prefix sums of the counts) so only the
+    // used entries cross PCIe
     std::vector<int32_t> hout(m * 6), hnc(m), hnr(m), hnq(m);
-    std::vector<uint32_t> hcells(m), hcig((size_t)cgw);
-    std::vector<uint8_t> hra((size_t)alw), hqa((size_t)alw);
+    std::vector<uint32_t> hcells(m);
     if (e == hipSuccess) e = hipMemcpyAsync(hout.data(), d_out, m * 6 * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hcells.data(), d_cells, m * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hnc.data(), d_nc, m * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hnr.data(), d_nr, m * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(hnq.data(), d_nq, m * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hcig.data(), d_cig, cgw * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hra.data(), d_ra, alw, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hqa.data(), d_qa, alw, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     float t_dp = 0.f, t_tb = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&t_dp, ctx->ev0, ctx->ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(&t_tb, ctx->ev1, ctx->ev2);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(e, "fa_scalar_banded_swa_batch");
+    }
+    std::vector<uint64_t> dense(3 * m);  // per pair: CIGAR, reference-base, query-base offsets
+    uint64_t nc_all = 0, nr_all = 0, nq_all = 0;
+    for (size_t p = 0; p < m; ++p) {
+        dense[3 * p] = nc_all;
+        dense[3 * p + 1] = nr_all;
+        dense[3 * p + 2] = nq_all;
+        nc_all += (uint64_t)hnc[p];
+        nr_all += (uint64_t)hnr[p];
+        nq_all += (uint64_t)hnq[p];
+    }
+    uint64_t *d_dense = nullptr;
+    uint32_t *d_ocig = nullptr;
+    uint8_t *d_ora = nullptr, *d_oqa = nullptr;
+    if ((rc = dev_upload(&d_dense, dense.data(), dense.size(), st)) ||
+        (rc = dev_alloc(&d_ocig, (size_t)nc_all + 1)) || (rc = dev_alloc(&d_ora, (size_t)nr_all + 1)) ||
+        (rc = dev_alloc(&d_oqa, (size_t)nq_all + 1))) {
+        for (void *q : {(void *)d_dense, (void *)d_ocig, (void *)d_ora, (void *)d_oqa})
+            if (q) (void)hipFree(q);
+        release();
+        return rc;
+    }
+    std::unique_ptr<uint32_t[]> hcig(new uint32_t[nc_all + 1]);
+    std::unique_ptr<uint8_t[]> hra(new uint8_t[nr_all + 1]), hqa(new uint8_t[nq_all + 1]);
+    e = fa::swtb_compact_launch(a, (uint32_t)m, d_dense, d_ocig, d_ora, d_oqa, st);
+    if (e == hipSuccess && nc_all)
+        e = hipMemcpyAsync(hcig.get(), d_ocig, nc_all * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nr_all) e = hipMemcpyAsync(hra.get(), d_ora, nr_all, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nq_all) e = hipMemcpyAsync(hqa.get(), d_oqa, nq_all, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    for (void *q : {(void *)d_dense, (void *)d_ocig, (void *)d_ora, (void *)d_oqa}) (void)hipFree(q);
     release();
     if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
     ctx->last_dp_ms = t_dp;
     ctx->last_tb_ms = t_tb;
-    for (size_t p = 0; p < m; ++p) {
-        const int64_t k = job[p];
-        const int32_t *o = &hout[p * 6];
-        out[k].score = o[0];
-        out[k].target_end_pos = o[1];
-        out[k].gtarget_end_pos = o[2];
-        out[k].query_end_pos = o[3];
-        out[k].global_score = o[4];
-        out[k].max_offset = o[5];
-        if (cells) cells[k] = hcells[p];
-        n_cigar[k] = hnc[p];
-        n_ref_aligned[k] = hnr[p];
-        n_query_aligned[k] = hnq[p];
-        std::memcpy(cigar + cigar_offset[k], &hcig[cgo[p]], (size_t)hnc[p] * 4);
-        std::memcpy(ref_aligned + aligned_offset[k], &hra[alo[p]], (size_t)hnr[p]);
-        std::memcpy(query_aligned + aligned_offset[k], &hqa[alo[p]], (size_t)hnq[p]);
-    }
+    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
+        for (int64_t p = lo; p < hi; ++p) {
+            const int64_t k = job[(size_t)p];
+            const int32_t *o = &hout[(size_t)p * 6];
+            out[k].score = o[0];
+            out[k].target_end_pos = o[1];
+            out[k].gtarget_end_pos = o[2];
+            out[k].query_end_pos = o[3];
+            out[k].global_score = o[4];
+            out[k].max_offset = o[5];
+            if (cells) cells[k] = hcells[p];
+            n_cigar[k] = hnc[p];
+            n_ref_aligned[k] = hnr[p];
+            n_query_aligned[k] = hnq[p];
+            std::memcpy(cigar + cigar_offset[k], &hcig[dense[3 * p]], (size_t)hnc[p] * 4);
+            std::memcpy(ref_aligned + aligned_offset[k], &hra[dense[3 * p + 1]], (size_t)hnr[p]);
+            std::memcpy(query_aligned + aligned_offset[k], &hqa[dense[3 * p + 2]], (size_t)hnq[p]);
+        }
+    });
     return FA_OK;
 }
 

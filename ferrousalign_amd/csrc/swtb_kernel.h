@@ -56,5 +56,10 @@ bool swtb_band_config(int qlen, int w, int full_cols, int *L, int *S);
 hipError_t swtb_dp_launch(int L, int S, bool band, const SwtbArgs &a, uint32_t first,
                           uint32_t count, hipStream_t st);
 hipError_t swtb_traceback_launch(const SwtbArgs &a, uint32_t n, hipStream_t st);
+// Gather each pair's CIGAR, aligned reference and aligned query into dense
+// buffers at dense[3p], dense[3p+1], dense[3p+2] (prefix sums of the counts).
+hipError_t swtb_compact_launch(const SwtbArgs &a, uint32_t n, const uint64_t *dense,
+                               uint32_t *cigar, uint8_t *ref_aln, uint8_t *qry_aln,
+                               hipStream_t st);
 
 }  // namespace fa

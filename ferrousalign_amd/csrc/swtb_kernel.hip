@@ -419,6 +419,24 @@ __global__ __launch_bounds__(256) void swtb_bases_kernel(SwtbArgs a, uint32_t n)
     for (int k = lane; k < nq; k += 64) qa[k] = (uint8_t)seq_code(a.seq2, a.seqn, qo, q0 + k);
 }
 
+// One wavefront per pair copies its CIGAR and aligned bases to the dense
+// buffers.
+__global__ __launch_bounds__(256) void swtb_compact_kernel(SwtbArgs a, uint32_t n,
+                                                           const uint64_t *dense, uint32_t *cigar,
+                                                           uint8_t *ref_aln, uint8_t *qry_aln) {
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    if (p >= n) return;
+    const int nc = a.n_cigar[p], nr = a.n_ref[p], nq = a.n_qry[p];
+    const uint32_t *cs = a.cigar + a.cig_off[p];
+    const uint8_t *rs = a.ref_aln + a.aln_off[p], *qs = a.qry_aln + a.aln_off[p];
+    uint32_t *cd = cigar + dense[3 * (size_t)p];
+    uint8_t *rd = ref_aln + dense[3 * (size_t)p + 1], *qd = qry_aln + dense[3 * (size_t)p + 2];
+    for (int k = lane; k < nc; k += 64) cd[k] = cs[k];
+    for (int k = lane; k < nr; k += 64) rd[k] = rs[k];
+    for (int k = lane; k < nq; k += 64) qd[k] = qs[k];
+}
+
 template <int L, int S, bool BAND>
 static hipError_t launch_dp(const SwtbArgs &a, uint32_t first, uint32_t count, hipStream_t st) {
     constexpr int G = 64 / L;
@@ -468,6 +486,15 @@ hipError_t swtb_dp_launch(int L, int S, bool band, const SwtbArgs &a, uint32_t f
 #undef FA_SW_LIST
 #undef FA_SW_CASE
     return hipErrorInvalidValue;
+}
+
+hipError_t swtb_compact_launch(const SwtbArgs &a, uint32_t n, const uint64_t *dense,
+                               uint32_t *cigar, uint8_t *ref_aln, uint8_t *qry_aln,
+                               hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(swtb_compact_kernel, dim3((n + 3) / 4), dim3(256), 0, st, a, n, dense, cigar,
+                       ref_aln, qry_aln);
+    return hipGetLastError();
 }
 
 hipError_t swtb_traceback_launch(const SwtbArgs &a, uint32_t n, hipStream_t st) {

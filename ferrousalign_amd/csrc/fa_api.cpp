@@ -164,6 +164,37 @@ void parallel_for(int64_t n, F &&f) {
     for (auto &x : th) x.join();
 }
 
+// Stable ascending order of keys[0..n) (LSD radix sort, 11-bit digits, the
+// keys carried along; a digit every key shares is skipped): equal keys keep
+// their index order, as std::stable_sort would, at a fraction of its cost on
+// millions of pairs.
+std::vector<uint32_t> radix_order(std::vector<uint64_t> keys) {
+    const size_t n = keys.size();
+    std::vector<uint32_t> a(n), b(n);
+    for (size_t k = 0; k < n; ++k) a[k] = (uint32_t)k;
+    uint64_t any = 0;
+    for (uint64_t x : keys) any |= x;
+    std::vector<uint64_t> kb(n);
+    std::vector<uint32_t> cnt(2049);
+    for (int sh = 0; sh < 64 && (any >> sh) != 0; sh += 11) {
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (size_t k = 0; k < n; ++k) ++cnt[((keys[k] >> sh) & 2047u) + 1];
+        bool one = false;
+        for (int d = 1; d <= 2048; ++d)
+            if (cnt[d] == n) one = true;
+        if (one) continue;
+        for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
+        for (size_t k = 0; k < n; ++k) {
+            const uint32_t pos = cnt[(keys[k] >> sh) & 2047u]++;
+            b[pos] = a[k];
+            kb[pos] = keys[k];
+        }
+        a.swap(b);
+        keys.swap(kb);
+    }
+    return a;
+}
+
 inline uint32_t seq_words(int32_t len) { return (uint32_t)((len + 31) / 32) * 2u; }
 
 // t == nullptr: the target comes from a device reference (codes 0-3 by construction)
@@ -296,26 +327,26 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         // sorting on the raw costs would let their noise scatter pairs of
         // equal length over many wavefronts.
         if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
-        std::vector<int> cls((size_t)n);
-        for (int64_t k = 0; k < n; ++k)
-            cls[(size_t)k] = (int)std::lrint(2.0 * std::log1p(std::max(0.0f, b->cost[(size_t)k])));
+    }
+    // one sortable key per pair, every field descending (inverted): kernel
+    // key (17 bits), cost class (8), qlen (8), tlen (12)
+    {
+        std::vector<uint64_t> k64((size_t)n);
         const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            if (skey[x] != skey[y]) return skey[x] > skey[y];
-            if (cls[x] != cls[y]) return cls[x] > cls[y];
-            if (ql[x] != ql[y]) return ql[x] > ql[y];
-            return tl[x] > tl[y];
+        parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
+            for (int64_t k = lo; k < hi; ++k) {
+                uint64_t x = (uint64_t)(0x1FFFF - skey[(size_t)k]) << 28;
+                if (pack == FA_PACK_COST) {
+                    const int cls =
+                        (int)std::lrint(2.0 * std::log1p(std::max(0.0f, b->cost[(size_t)k])));
+                    x |= (uint64_t)(255 - std::min(cls, 255)) << 20;
+                }
+                if (pack != FA_PACK_NONE)
+                    x |= ((uint64_t)(255 - ql[k]) << 12) | (uint64_t)(4095 - tl[k]);
+                k64[(size_t)k] = x;
+            }
         });
-    } else if (pack != FA_PACK_NONE) {
-        const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            if (skey[x] != skey[y]) return skey[x] > skey[y];
-            if (ql[x] != ql[y]) return ql[x] > ql[y];
-            return tl[x] > tl[y];
-        });
-    } else {
-        std::stable_sort(order.begin(), order.end(),
-                         [&](uint32_t x, uint32_t y) { return skey[x] > skey[y]; });
+        order = radix_order(std::move(k64));
     }
     std::vector<Bucket> buckets;
     for (int64_t k = 0; k < n;) {
@@ -560,6 +591,7 @@ struct TargetSource {
     const fa_ref *ref = nullptr;
     const int64_t *seg_beg = nullptr;
     const uint8_t *seg_rev = nullptr;
+    bool validated = false;  // the caller already ran validate_job on every job
 };
 
 int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
@@ -597,9 +629,10 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     std::vector<int8_t> vrc((size_t)n);
     parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
         for (int64_t k = lo; k < hi; ++k) {
-            int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
-                                 query_seqs + query_offset[k],
-                                 from_ref ? nullptr : ts.seqs + ts.offset[k]);
+            int r = ts.validated ? FA_OK
+                                 : validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                                                query_seqs + query_offset[k],
+                                                from_ref ? nullptr : ts.seqs + ts.offset[k]);
             if (r == FA_OK && from_ref) {
                 // inside [0, 2 l_pac) and on one strand (get_reference_segment
                 // returns an empty segment for one that bridges, bntseq.rs:461-469)
@@ -609,6 +642,8 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             vrc[(size_t)k] = (int8_t)r;
         }
     });
+    int matmax = 0;
+    for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw->mat[x]);
     for (int64_t k = 0; k < n; ++k) {
         int r = vrc[(size_t)k];
         if (r != FA_OK) {
@@ -625,10 +660,8 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         }
         b->w[(size_t)k] = capped_w(*sw, query_len[k], band_width[k]);
         {
-            int mx = 0;
-            for (int x = 0; x < 25; ++x) mx = std::max(mx, (int)sw->mat[x]);
-            const int64_t bound = (int64_t)h0[k] + (int64_t)query_len[k] * mx;
-            const int64_t K = std::max(2, 1 + mx);
+            const int64_t bound = (int64_t)h0[k] + (int64_t)query_len[k] * matmax;
+            const int64_t K = std::max(2, 1 + matmax);
             b->elig16[(size_t)k] = bound < 4096 && bound * K < 32768 && sw->e_ins < 32 &&
                                    sw->o_ins + sw->e_ins < 8192 && sw->o_del + sw->e_del < 8192;
         }
@@ -907,11 +940,13 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         order[p] = (uint32_t)p;
     }
     if (words > 0xFFFFFFF0ull) return FA_ERR_ARG;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        if (lsl[x] != lsl[y]) return lsl[x] > lsl[y];
-        if (ql[x] != ql[y]) return ql[x] > ql[y];
-        return tl[x] > tl[y];
-    });
+    {  // stable order by (kernel key, qlen, tlen), all descending
+        std::vector<uint64_t> k64(m);
+        for (size_t p = 0; p < m; ++p)
+            k64[p] = ((uint64_t)(0x1FFFFu - lsl[p]) << 20) | ((uint64_t)(255 - ql[p]) << 12) |
+                     (uint64_t)(4095 - tl[p]);
+        order = radix_order(std::move(k64));
+    }
     std::vector<uint32_t> seq2(words), seqn(words / 2);
     parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
         for (int64_t p = lo; p < hi; ++p) {
@@ -1308,13 +1343,20 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, 
     if (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset || !ref_len ||
         !h0 || !band_width)
         return FA_ERR_ARG;
-    // per-job validation: invalid jobs are reported and left out of the launch
+    // per-job validation (on the host threads): invalid jobs are reported and
+    // left out of the launch
+    std::vector<int8_t> vrc((size_t)njobs);
+    parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned) {
+        for (int64_t k = lo; k < hi; ++k)
+            vrc[(size_t)k] = (int8_t)validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                                                  query_seqs + query_offset[k],
+                                                  ref_seqs + ref_offset[k]);
+    });
     std::vector<int64_t> keep;
     keep.reserve((size_t)njobs);
     int first_err = FA_OK;
     for (int64_t k = 0; k < njobs; ++k) {
-        int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
-                             query_seqs + query_offset[k], ref_seqs + ref_offset[k]);
+        const int r = vrc[(size_t)k];
         if (per_job_rc) per_job_rc[k] = r;
         if (r == FA_OK)
             keep.push_back(k);
@@ -1348,8 +1390,11 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, 
             ww = w2.data();
         }
         fa_batch *b = nullptr;
-        rc = fa_batch_upload(ctx, sw, (int64_t)keep.size(), query_seqs, qo, ql, ref_seqs, to, tl,
-                             hh, ww, &b);
+        TargetSource ts;
+        ts.seqs = ref_seqs;
+        ts.offset = to;
+        ts.validated = true;
+        rc = batch_upload(ctx, sw, (int64_t)keep.size(), query_seqs, qo, ql, ts, tl, hh, ww, &b);
         if (rc) return rc;
         rc = fa_batch_run(ctx, b, flags);
         std::vector<fa_out_score> o2;
@@ -1475,10 +1520,15 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
         if (ql < 1 || ql > 8 * fa::kMateMaxSlen) return FA_ERR_QLEN;
         sk[(size_t)k] = (ql + 7) / 8;
     }
+    // stable counting sort by slen, longest first
     std::vector<uint32_t> order((size_t)n);
-    for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t x, uint32_t y) { return sk[x] > sk[y]; });
+    {
+        std::vector<int64_t> start(fa::kMateMaxSlen + 2, 0);
+        for (int64_t k = 0; k < n; ++k) ++start[(size_t)(fa::kMateMaxSlen - sk[(size_t)k] + 1)];
+        for (int x = 0; x <= fa::kMateMaxSlen; ++x) start[(size_t)x + 1] += start[(size_t)x];
+        for (int64_t k = 0; k < n; ++k)
+            order[(size_t)(start[(size_t)(fa::kMateMaxSlen - sk[(size_t)k])]++)] = (uint32_t)k;
+    }
     std::vector<Bucket> bk;
     for (int64_t k = 0; k < n;) {
         const int sl = sk[order[(size_t)k]];

@@ -18,7 +18,10 @@ scoring A=1 B=4 O=6 E=1 w=100 zdrop=100 end_bonus=5.  Measured:
     predictor replaced by length sorting (what the network gains).
 The predictor is trained first, on the GPU with the library's kernels, on
 cells that the plain kernel records for a separate 16,777,216-pair training
-mix drawn the same way (other seed).
+mix drawn the same way (other seed).  Also reported, never as the value:
+the same pairs through the host-buffer entry point
+(fa_execute_batch_simd_scoring: validation, packing, PCIe both ways, run,
+download).
 --workload illumina_150bp_1m times the plain kernel on 1,048,576 pairs from
 150 bp reads instead.
 """
@@ -364,6 +367,18 @@ def main():
     ms_len, ms_len_ev, _ = timed(db, lflags, max(2, args.steps // 2), 1)
     out_len, cells_len = db.download()
 
+    # the same pairs through the host-buffer entry point (validation, 2-bit
+    # packing, PCIe both ways, packing plan, run, download): the rate a caller
+    # of fa_execute_batch_simd_scoring sees; never the headline value
+    from ferrousalign_amd.banded_swa import extend_arrays
+    flat = ps.flat()
+    extend_arrays(sw, *flat, want_cells=True, ctx=ctx)  # first call pays one-time setup
+    t0 = time.perf_counter()
+    out_h, cells_h = extend_arrays(sw, *flat, want_cells=True, ctx=ctx)
+    ms_host = (time.perf_counter() - t0) * 1e3
+    host_same = bool(np.array_equal(out_h, out_plain) and np.array_equal(cells_h, cells))
+    del flat
+
     dg = digest_of(out, cells2)
     exact = (dg == golden.get(args.workload)) if golden.get(args.workload) else None
     same = bool(np.array_equal(out, out_plain) and np.array_equal(cells2, cells)
@@ -397,6 +412,9 @@ def main():
         "learned_gain_vs_length_sorted": round(ms_len / ms, 3),
         "bit_exact_vs_golden": exact,
         "identical_across_packings": same,
+        "host_buffer_ms": round(ms_host, 1),
+        "host_buffer_pairs_per_s": round(ps.n / (ms_host / 1e3), 1),
+        "host_buffer_identical": host_same,
         "digest": dg,
         "predictor": {"train_pairs": ntrain, "epochs": args.epochs, "train_s": round(train_s, 2),
                       "final_train_loss": round(losses[-1], 4), "heldout_mae_cells": round(pred_mae, 1),

@@ -65,6 +65,19 @@ class BandedPairwiseSW(ctypes.Structure):
     ]
 
 
+# Record layouts of include/ferrousalign_amd.h (numpy structured dtypes)
+MEM_OPT_DTYPE = np.dtype([(f, np.int32) for f in ("a", "b", "o_del", "e_del", "o_ins", "e_ins", "w", "zdrop",
+                                                    "pen_clip5", "pen_clip3")])
+SEED_DTYPE = np.dtype([("ref_pos", np.int64), ("query_pos", np.int32), ("len", np.int32)])
+JOB_DTYPE = np.dtype([("read_idx", np.int32), ("chain_idx", np.int32), ("seed_idx", np.int64), ("qbeg", np.int32),
+                      ("qlen", np.int32), ("h0", np.int32), ("w", np.int32), ("seg_beg", np.int64),
+                      ("seg_len", np.int32), ("seg_rev", np.int32)])
+REGION_DTYPE = np.dtype([("rb", np.int64), ("re", np.int64), ("qb", np.int32), ("qe", np.int32), ("score", np.int32),
+                         ("truesc", np.int32), ("seedcov", np.int32), ("seedlen0", np.int32), ("chain_idx", np.int32),
+                         ("w", np.int32), ("seed_idx", np.int64), ("frac_rep", np.float32), ("pad", np.int32)])
+assert MEM_OPT_DTYPE.itemsize == 40 and SEED_DTYPE.itemsize == 16
+assert JOB_DTYPE.itemsize == 48 and REGION_DTYPE.itemsize == 64
+
 # OutScore field order (banded_swa/types.rs:77-84)
 OUT_FIELDS = ("score", "target_end_pos", "gtarget_end_pos", "query_end_pos",
               "global_score", "max_offset")
@@ -88,6 +101,12 @@ def _declare(lib):
     lib.fa_hbm_copy.argtypes = [_P, _I64, ctypes.POINTER(ctypes.c_double)]
     lib.fa_ref_upload.argtypes = [_P, _P, _I64, ctypes.POINTER(_P)]
     lib.fa_ref_free.argtypes = [_P, _P]
+    lib.fa_ref_length.argtypes = [_P, ctypes.POINTER(_I64)]
+    lib.fa_collect_extension_jobs.argtypes = [_P, _I64, _I64] + [_P] * 8 + [_I64, _P, ctypes.POINTER(_I64), _P,
+                                                                         ctypes.POINTER(_I64)] + [_P] * 6
+    lib.fa_merge_scores_to_regions.argtypes = [_P, _I64] + [_P] * 16 + [_I32, _P]
+    lib.fa_extend_chains.argtypes = [_P, _P, ctypes.POINTER(BandedPairwiseSW), _P, _I64, _P, _P, _P, _P, _P, _I64,
+                                     _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]
     lib.fa_batch_upload_ref.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW), _I64, _P, _P, _P, _P, _P, _P, _P,
                                         _P, _P, ctypes.POINTER(_P)]
     lib.fa_batch_download_targets.argtypes = [_P, _P, _P, _P]
@@ -115,6 +134,7 @@ def _declare(lib):
     lib.fa_simd_banded_swa_batch_soa.argtypes = soa
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
     for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_valu_peak", "fa_hbm_copy", "fa_ref_upload", "fa_ref_free",
+                 "fa_ref_length", "fa_collect_extension_jobs", "fa_merge_scores_to_regions", "fa_extend_chains",
                  "fa_batch_upload_ref", "fa_batch_download_targets", "fa_scalar_banded_swa_batch",
                  "fa_scalar_banded_swa_last_ms", "fa_execute_batch_simd_scoring",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",

@@ -22,7 +22,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HIP_SOURCES = ["extend_kernel.hip", "extend16_kernel.hip", "predictor.hip", "mate16_kernel.hip", "mate16_sym.hip", "valu_probe.hip",
-               "ref_kernel.hip", "swtb_kernel.hip", "fa_api.cpp"]
+               "ref_kernel.hip", "swtb_kernel.hip", "fa_api.cpp",
+               "fa_pipeline.cpp"]
 
 
 def _run(cmd):

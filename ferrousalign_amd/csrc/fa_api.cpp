@@ -774,6 +774,12 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                         band_width, out);
 }
 
+int fa_ref_length(const fa_ref *ref, int64_t *l_pac) {
+    if (!ref || !l_pac) return FA_ERR_ARG;
+    *l_pac = ref->l_pac;
+    return FA_OK;
+}
+
 int fa_ref_upload(fa_ctx *ctx, const uint8_t *pac, int64_t l_pac, fa_ref **out) {
     if (!ctx || !out || !pac || l_pac <= 0 || l_pac > ((int64_t)1 << 40)) return FA_ERR_ARG;
     *out = nullptr;

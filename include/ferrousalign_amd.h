@@ -203,6 +203,8 @@ int fa_scalar_banded_swa_last_ms(fa_ctx *ctx, double *dp_ms, double *traceback_m
  * segment fetch get_reference_segment (:451-522) on the extension path. */
 int fa_ref_upload(fa_ctx *ctx, const uint8_t *pac, int64_t l_pac, fa_ref **out);
 int fa_ref_free(fa_ctx *ctx, fa_ref *ref);
+/* Number of forward bases (l_pac) of an uploaded reference. */
+int fa_ref_length(const fa_ref *ref, int64_t *l_pac);
 /* As fa_batch_upload, but target j is read on the device from the reference:
  * the segment [seg_beg[j], seg_beg[j] + seg_len[j]) of the bidirectional
  * coordinate space [0, 2 l_pac) (coordinates >= l_pac are the reverse
@@ -316,6 +318,96 @@ int fa_mlp_gradient(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *pairs, 
  * recorded by the batch's last run); *mean_loss = mean training loss. */
 int fa_mlp_train(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *order, int64_t count,
                  int32_t minibatch, float lr, float *mean_loss);
+
+/* ---- extension jobs from chains, and regions from their scores ----------
+ * The callers either side of the extension batch: the job producer
+ * collect_extension_jobs_batch_soa (src/pipelines/linear/batch_extension/
+ * collect_soa.rs:28-228) and the consumer merge_scores_to_regions
+ * (src/pipelines/linear/region.rs:737-890).  Layout (SoA, like the
+ * reference's SoAChainBatch / SoASeedBatch): read r owns chains
+ * [read_chain_start[r], +read_chain_count[r]); chain c owns the seed indices
+ * chain_seeds[chain_seed_start[c] .. +chain_seed_count[c]) into seeds[]. */
+typedef struct fa_mem_opt {   /* the MemOpt fields the path reads (mem_opt.rs) */
+    int32_t a, b, o_del, e_del, o_ins, e_ins, w, zdrop, pen_clip5, pen_clip3;
+} fa_mem_opt;
+typedef struct fa_seed {      /* Seed: query_pos, ref_pos (FM coordinates), len */
+    int64_t ref_pos;
+    int32_t query_pos, len;
+} fa_seed;
+typedef struct fa_ext_job {   /* one extension job (batch_extension/types.rs:20-98) */
+    int32_t read_idx, chain_idx;  /* chain index local to its read, as the reference */
+    int64_t seed_idx;             /* into seeds[] */
+    int32_t qbeg, qlen;           /* query segment of the read; a left job reads it backwards */
+    int32_t h0, w;
+    int64_t seg_beg;              /* target: .pac segment [seg_beg, +seg_len) in [0, 2 l_pac), */
+    int32_t seg_len;              /* read backwards for a left job (seg_rev = 1) */
+    int32_t seg_rev;
+} fa_ext_job;
+typedef struct fa_region {    /* AlignmentRegion boundaries and scores (region.rs:47-120) */
+    int64_t rb, re;               /* FM coordinates, [rb, re) */
+    int32_t qb, qe;
+    int32_t score, truesc, seedcov, seedlen0;
+    int32_t chain_idx;            /* local to its read */
+    int32_t w;
+    int64_t seed_idx;             /* best seed of the chain; -1: the chain gave no region */
+    float frac_rep;
+    int32_t pad;
+} fa_region;
+
+/* Host-side producer (collect_soa.rs:28-228): for every kept chain the
+ * reference window [rmax0, rmax1) (cal_max_gap margins, chaining.rs:1108-1116,
+ * kept on one strand), then per seed in reverse order a left job (query
+ * prefix reversed, window prefix backwards, h0 = seed_len * a) and a right
+ * job (query suffix, window suffix, h0 = 0 as the reference: fa_extend_chains
+ * replaces it, see there).  Capacity: the total seed count of all chains
+ * (jobs per direction and seed mappings).  Per chain c: mappings
+ * [map_start[c], +map_count[c]) give the seed and its left/right job index
+ * (-1 = none); rmax[2c], rmax[2c+1] the window (-1 when the chain made no
+ * jobs).  Runs on the host only. */
+int fa_collect_extension_jobs(const fa_mem_opt *opt, int64_t l_pac, int64_t nreads,
+                              const int32_t *query_len, const int64_t *read_chain_start,
+                              const int32_t *read_chain_count, const uint8_t *chain_kept,
+                              const int64_t *chain_seed_start, const int32_t *chain_seed_count,
+                              const int64_t *chain_seeds, const fa_seed *seeds, int64_t capacity,
+                              fa_ext_job *left, int64_t *n_left, fa_ext_job *right,
+                              int64_t *n_right, int64_t *map_start, int32_t *map_count,
+                              int64_t *map_seed, int64_t *map_left, int64_t *map_right,
+                              int64_t *rmax);
+/* Host-side consumer (region.rs:737-890): per chain with a window, one region
+ * per seed mapping from the seed (qb, qe, rb, re, seed score) extended by its
+ * left score (pen_clip5 decision: local qle/tle or global to the query start
+ * with gtle) and right score (pen_clip3 decision), seedcov over the chain's
+ * seeds inside the region, and the chain keeps the first highest-scoring
+ * region.  right_from_left = 1: the right scores started from the left score
+ * (fa_extend_chains), so the region score is the right score; 0: they started
+ * from 0 and are added to it (the reference).  regions[c].seed_idx = -1 for
+ * a chain without a region.  FM-to-chromosome coordinates are not filled. */
+int fa_merge_scores_to_regions(const fa_mem_opt *opt, int64_t nreads, const int32_t *query_len,
+                               const int64_t *read_chain_start, const int32_t *read_chain_count,
+                               const float *chain_frac_rep, const int64_t *chain_seed_start,
+                               const int32_t *chain_seed_count, const int64_t *chain_seeds,
+                               const fa_seed *seeds, const int64_t *map_start,
+                               const int32_t *map_count, const int64_t *map_seed,
+                               const int64_t *map_left, const int64_t *map_right,
+                               const int64_t *rmax, const fa_out_score *left_scores,
+                               const fa_out_score *right_scores, int32_t right_from_left,
+                               fa_region *regions);
+/* The producer, both extension batches on the device with targets cut from
+ * the resident .pac (fa_ref_upload), and the consumer: the left batch, then
+ * the right batch with h0 = the seed's left-extended score (or seed_len * a
+ * without a left job), as BWA-MEM2's mem_chain2aln chains them (the
+ * reference's h0 = 0 right jobs are outside ksw_extend2's h0 > 0 domain,
+ * ksw_affine_gap.rs:968).  sw supplies the scoring and end_bonus of both
+ * batches.  regions has one entry per chain; the job counts go to
+ * n_left / n_right when non-NULL. */
+int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw *sw,
+                     const fa_mem_opt *opt, int64_t nreads, const uint8_t *query_seqs,
+                     const int64_t *query_offset, const int32_t *query_len,
+                     const int64_t *read_chain_start, const int32_t *read_chain_count,
+                     int64_t nchains, const uint8_t *chain_kept, const float *chain_frac_rep,
+                     const int64_t *chain_seed_start, const int32_t *chain_seed_count,
+                     const int64_t *chain_seeds, const fa_seed *seeds, fa_region *regions,
+                     int64_t *n_left, int64_t *n_right);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,143 @@
+"""Pure-Python restatement of the reference's extension-job producer and
+region merge.  TEST INFRASTRUCTURE ONLY (the library never calls it):
+
+  cal_max_gap              src/pipelines/linear/chaining.rs:1108-1116
+  collect_extension_jobs   src/pipelines/linear/batch_extension/collect_soa.rs:28-228
+  merge_scores_to_regions  src/pipelines/linear/region.rs:737-890
+
+Integers follow the Rust types: ref_pos, rmax and the window arithmetic are
+u64 (saturating_sub where the reference uses it, wrapping otherwise), so a
+seed left of the window gives no job; `as i32` of an f64 truncates toward zero.
+A job is a dict with the fields of the library's fa_ext_job record.
+"""
+U64 = (1 << 64) - 1
+
+
+def _as_i32(x):
+    if x != x:
+        return 0
+    if x >= 2147483647.0:
+        return 2147483647
+    if x <= -2147483648.0:
+        return -2147483648
+    return int(x)  # truncation toward zero
+
+
+def cal_max_gap(opt, qlen):
+    l_del = _as_i32((qlen * opt.a - opt.o_del) / float(opt.e_del) + 1.0)
+    l_ins = _as_i32((qlen * opt.a - opt.o_ins) / float(opt.e_ins) + 1.0)
+    l = l_del if l_del > l_ins else l_ins
+    l = l if l > 1 else 1
+    return l if l < (opt.w << 1) else (opt.w << 1)
+
+
+def collect_extension_jobs(opt, l_pac, cb):
+    """(left jobs, right jobs, per chain: list of (seed, left job, right job),
+    per chain: (rmax0, rmax1) or None)."""
+    left, right = [], []
+    maps = [[] for _ in range(cb.nchains)]
+    rmaxes = [None] * cb.nchains
+    for r in range(cb.nreads):
+        qlen = int(cb.query_len[r])
+        for lc in range(int(cb.read_chain_count[r])):
+            c = int(cb.read_chain_start[r]) + lc
+            ns = int(cb.chain_seed_count[c])
+            if not cb.chain_kept[c] or ns == 0:
+                continue
+            idx = [int(x) for x in cb.chain_seeds[int(cb.chain_seed_start[c]):int(cb.chain_seed_start[c]) + ns]]
+            rmax0, rmax1 = l_pac << 1, 0
+            for si in idx:
+                qp, rp, ln = int(cb.seeds[si]["query_pos"]), int(cb.seeds[si]["ref_pos"]), int(cb.seeds[si]["len"])
+                left_margin = qp + cal_max_gap(opt, qp)
+                b = rp - left_margin if rp >= left_margin else 0  # saturating_sub
+                remaining = qlen - qp - ln
+                right_margin = remaining + cal_max_gap(opt, remaining)
+                e = (rp + ln + right_margin) & U64
+                rmax0 = min(rmax0, b)
+                rmax1 = max(rmax1, e)
+            rmax1 = min(rmax1, l_pac << 1)
+            if rmax0 < l_pac < rmax1:
+                if int(cb.seeds[idx[0]]["ref_pos"]) < l_pac:
+                    rmax1 = l_pac
+                else:
+                    rmax0 = l_pac
+            # get_reference_segment: bridging gives an empty segment
+            if rmax0 < l_pac < rmax1:
+                rseq_len = 0
+            else:
+                rseq_len = max(rmax1 - rmax0, 0)
+            rmaxes[c] = (rmax0, rmax1)
+            for si in reversed(idx):
+                qp, rp, ln = int(cb.seeds[si]["query_pos"]), int(cb.seeds[si]["ref_pos"]), int(cb.seeds[si]["len"])
+                lj = rj = -1
+                if qp > 0:
+                    tmp = (rp - rmax0) & U64
+                    if 0 < tmp <= rseq_len:
+                        lj = len(left)
+                        left.append(dict(read_idx=r, chain_idx=lc, seed_idx=si, qbeg=0, qlen=qp, h0=ln * opt.a,
+                                         w=opt.w, seg_beg=rmax0, seg_len=tmp, seg_rev=1))
+                qend = qp + ln
+                if qend < qlen:
+                    tmp = (rp + ln - rmax0) & U64
+                    if tmp < rseq_len:
+                        rj = len(right)
+                        right.append(dict(read_idx=r, chain_idx=lc, seed_idx=si, qbeg=qend, qlen=qlen - qend, h0=0,
+                                          w=opt.w, seg_beg=rmax0 + tmp, seg_len=rseq_len - tmp, seg_rev=0))
+                maps[c].append((si, lj, rj))
+    return left, right, maps, rmaxes
+
+
+def merge_scores_to_regions(opt, cb, maps, rmaxes, left_scores, right_scores, right_from_left):
+    """Per chain: the region dict or None.  Scores: rows in OutScore order
+    (score, target_end_pos, gtarget_end_pos, query_end_pos, global_score,
+    max_offset)."""
+    out = [None] * cb.nchains
+    for r in range(cb.nreads):
+        qlen = int(cb.query_len[r])
+        for lc in range(int(cb.read_chain_count[r])):
+            c = int(cb.read_chain_start[r]) + lc
+            if int(cb.chain_seed_count[c]) == 0 or rmaxes[c] is None:
+                continue
+            chain_seeds = [int(x) for x in
+                           cb.chain_seeds[int(cb.chain_seed_start[c]):int(cb.chain_seed_start[c]) + int(cb.chain_seed_count[c])]]
+            best_score, best = -(1 << 31), None
+            for si, lj, rj in maps[c]:
+                qp, rp, ln = int(cb.seeds[si]["query_pos"]), int(cb.seeds[si]["ref_pos"]), int(cb.seeds[si]["len"])
+                g = dict(chain_idx=lc, seed_idx=si, qb=qp, qe=qp + ln, rb=rp, re=rp + ln, seedlen0=ln,
+                         frac_rep=float(cb.chain_frac_rep[c]), w=opt.w, truesc=0)
+                total = ln * opt.a
+                if lj >= 0:
+                    sc, tle, gtle, qle, gsc = [int(x) for x in left_scores[lj][:5]]
+                    total = sc
+                    if gsc <= 0 or gsc <= sc - opt.pen_clip5:
+                        g["qb"] = qp - qle
+                        g["rb"] = rp - tle
+                        g["truesc"] = sc
+                    else:
+                        g["qb"] = 0
+                        g["rb"] = rp - gtle
+                        g["truesc"] = gsc
+                elif qp == 0:
+                    g["truesc"] = ln * opt.a
+                if rj >= 0:
+                    sc, tle, gtle, qle, gsc = [int(x) for x in right_scores[rj][:5]]
+                    total = sc if right_from_left else total + sc
+                    if gsc <= 0 or gsc <= sc - opt.pen_clip3:
+                        g["qe"] = qp + ln + qle
+                        g["re"] = rp + ln + tle
+                    else:
+                        g["qe"] = qlen
+                        g["re"] = rp + ln + gtle
+                g["score"] = total
+                if g["truesc"] == 0:
+                    g["truesc"] = total
+                cov = 0
+                for oi in chain_seeds:
+                    oq, orp, ol = int(cb.seeds[oi]["query_pos"]), int(cb.seeds[oi]["ref_pos"]), int(cb.seeds[oi]["len"])
+                    if oq >= g["qb"] and oq + ol <= g["qe"] and orp >= g["rb"] and orp + ol <= g["re"]:
+                        cov += ol
+                g["seedcov"] = cov
+                if total > best_score:
+                    best_score, best = total, g
+            out[c] = best
+    return out

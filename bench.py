@@ -23,7 +23,8 @@ the same pairs through the host-buffer entry point
 (fa_execute_batch_simd_scoring: validation, packing, PCIe both ways, run,
 download).
 --workload illumina_150bp_1m times the plain kernel on 1,048,576 pairs from
-150 bp reads instead.
+150 bp reads instead; mate_rescue_2m, cigar_regen_1m and chains_150bp_1m time
+the mate-rescue kernel, CIGAR regeneration and the chain-extension path.
 """
 import argparse
 import hashlib
@@ -241,13 +242,108 @@ def cigar_bench(args, ctx):
     print(json.dumps(res), flush=True)
 
 
+def chains_bench(args, ctx):
+    """Chain extension end to end (fa_extend_chains) on 1,048,576 reads of
+    150 bp: the job producer, the left and the right extension batch on the
+    GPU with targets cut from the resident .pac, right h0 chained from the
+    left score, and the merge into one region per chain.  Host buffers in
+    and out, so the value includes the host stages and PCIe."""
+    from ferrousalign_amd import synth, pack_pac
+    from ferrousalign_amd.banded_swa import DeviceBatch, DeviceReference
+    from ferrousalign_amd.pipeline import ChainBatch, MemOpt, collect_extension_jobs, extend_chains, CHAIN_STAGES
+    from tests import pipeline_oracle as PO
+    cb, qseq, qoff, _ = synth.chain_reads()
+    ref_codes = synth.reference()
+    l_pac = ref_codes.size
+    pac = pack_pac(ref_codes)
+    ref = DeviceReference(pac, l_pac, ctx=ctx)
+    sw = synth.default_scoring()
+    opt = MemOpt()
+    steps = max(1, min(args.steps, 5))
+    for _ in range(max(1, min(args.warmup, 2))):
+        regions, nl, nr = extend_chains(ref, sw, opt, qseq, qoff, cb, ctx=ctx)
+    wall, stages = [], []
+    for _ in range(steps):
+        st = {}
+        t0 = time.perf_counter()
+        regions, nl, nr = extend_chains(ref, sw, opt, qseq, qoff, cb, ctx=ctx, stage_ms=st)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        stages.append(st)
+    ms = float(np.mean(wall))
+    stage_ms = {k: round(float(np.mean([x[k] for x in stages])), 3) for k in CHAIN_STAGES}
+
+    # dominant kernel: the two batches rebuilt from the same jobs and replayed launch by launch
+    left, right, m = collect_extension_jobs(opt, l_pac, cb)
+
+    def batch(jobs, reverse):
+        ql = jobs["qlen"].astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(ql)[:-1]])
+        local = np.arange(int(ql.sum())) - np.repeat(offs, ql)
+        base = np.repeat(qoff[jobs["read_idx"]] + jobs["qbeg"], ql)
+        idx = base + (np.repeat(ql, ql) - 1 - local if reverse else local)
+        return DeviceBatch.from_reference(sw, qseq[idx], offs, jobs["qlen"], ref, jobs["seg_beg"], jobs["seg_len"],
+                                          jobs["seg_rev"], jobs["h0"], jobs["w"], ctx=ctx)
+    lb = batch(left, True)
+    lb.run(0)
+    lout, lcells = lb.download()
+    rj = right.copy()
+    cnt = m.count[:cb.nchains].astype(np.int64)
+    idx = np.repeat(m.start[:cb.nchains] - (np.cumsum(cnt) - cnt), cnt) + np.arange(int(cnt.sum()))
+    idx = idx[m.right[idx] >= 0]
+    li = m.left[idx]
+    rj["h0"][m.right[idx]] = np.where(li >= 0, lout[np.maximum(li, 0), 0],
+                                      cb.seeds["len"][m.seed[idx]].astype(np.int64) * opt.a)
+    rb = batch(rj, False)
+    rb.run(0)
+    _, rcells = rb.download()
+    roofs = [roofline(lb, 0, lcells, ctx), roofline(rb, 0, rcells, ctx)]
+    roof = max(roofs, key=lambda r: r["launch_ms"])
+    roof["batch"] = "left" if roof is roofs[0] else "right"
+    kernel_ms = [lb.time(0, 3)[0], rb.time(0, 3)[0]]
+    lb.free()
+    rb.free()
+    total_cells = float(lcells.astype(np.float64).sum() + rcells.astype(np.float64).sum())
+
+    # bit-exactness: the regions of the first reads against the CPU pipeline
+    msub = min(cb.nreads, 131072)
+    nch = int(cb.read_chain_start[msub]) if msub < cb.nreads else cb.nchains
+    nsd = int(cb.chain_seed_start[nch]) if nch < cb.nchains else len(cb.seeds)
+    sub = ChainBatch(cb.query_len[:msub], cb.read_chain_start[:msub], cb.read_chain_count[:msub],
+                     cb.chain_kept[:nch], cb.chain_frac_rep[:nch], cb.chain_seed_start[:nch],
+                     cb.chain_seed_count[:nch], cb.chain_seeds[:nsd], cb.seeds[:nsd])
+    threads = os.cpu_count() or 1
+    tcpu = {}
+    want, _, _ = PO.extend_chains_cpu(sub, pac, l_pac, qseq, qoff[:msub], sw, opt, nthreads=threads, timing=tcpu)
+    exact = bool(np.array_equal(want.view(np.uint8), regions[:nch].view(np.uint8)))
+    res = {"metric": "chain-extension reads/s (producer + left/right ksw_extend2 batches + region merge, bit-exact)",
+           "value": round(cb.nreads / (ms / 1e3), 1), "unit": "reads/s", "n_gpus": 1, "steps": steps,
+           "warmup": max(1, min(args.warmup, 2)), "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int16x2", "data": "synthetic",
+           "config": {"workload": "chains_150bp_1m", "reads": cb.nreads, "chains": cb.nchains,
+                      "seeds": int(len(cb.seeds)), "left_jobs": nl, "right_jobs": nr, "read_len": 150,
+                      "w": opt.w, "zdrop": opt.zdrop, "inputs": "host buffers"},
+           "stage_ms": stage_ms, "batch_kernel_ms": [round(x, 4) for x in kernel_ms],
+           "gcups_end_to_end": round(total_cells / (ms / 1e3) / 1e9, 2),
+           "bit_exact_vs_oracle": exact, "bit_exact_sample_reads": msub, "roofline": roof}
+    if not args.no_cpu:
+        dt = tcpu["producer"] + tcpu["extend"] + tcpu["merge"]
+        res["cpu_baseline"] = {"value": round(msub / dt, 1), "unit": "reads/s", "cores": threads, "kind": "port",
+                               "sample": "first %d reads: the library's producer and merge with both extension "
+                                         "batches through the C restatement of ksw_extend2 on %d host threads "
+                                         "(query gathering and target cutting not timed): %.2f s"
+                                         % (msub, threads, dt)}
+    ref.free()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uneven_mix_4m",
-                    choices=["uneven_mix_4m", "illumina_150bp_1m", "mate_rescue_2m", "cigar_regen_1m"])
+                    choices=["uneven_mix_4m", "illumina_150bp_1m", "mate_rescue_2m", "cigar_regen_1m",
+                             "chains_150bp_1m"])
     ap.add_argument("--train-pairs", type=int, default=16777216)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -278,6 +374,9 @@ def main():
         return
     if args.workload == "cigar_regen_1m":
         cigar_bench(args, ctx)
+        return
+    if args.workload == "chains_150bp_1m":
+        chains_bench(args, ctx)
         return
 
     if args.workload == "illumina_150bp_1m":

@@ -106,7 +106,7 @@ def _declare(lib):
                                                                          ctypes.POINTER(_I64)] + [_P] * 6
     lib.fa_merge_scores_to_regions.argtypes = [_P, _I64] + [_P] * 16 + [_I32, _P]
     lib.fa_extend_chains.argtypes = [_P, _P, ctypes.POINTER(BandedPairwiseSW), _P, _I64, _P, _P, _P, _P, _P, _I64,
-                                     _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]
+                                     _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _P]
     lib.fa_batch_upload_ref.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW), _I64, _P, _P, _P, _P, _P, _P, _P,
                                         _P, _P, ctypes.POINTER(_P)]
     lib.fa_batch_download_targets.argtypes = [_P, _P, _P, _P]
@@ -172,6 +172,9 @@ def synth():
                                             ctypes.c_int]
         s.fa_synth_region_pairs.restype = ctypes.c_int
         s.fa_synth_mate_pairs.restype = ctypes.c_int
+        s.fa_synth_chain_reads.argtypes = [ctypes.c_uint64, _P, _I64, _I64, _I32, _I32, _I32, _P, _P, _P, _P, _P,
+                                           _P, _P, _P, _P, ctypes.c_int]
+        s.fa_synth_chain_reads.restype = ctypes.c_int
         _synth = s
     return _synth
 

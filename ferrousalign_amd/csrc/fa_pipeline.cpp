@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdint>
 #include <cstring>
@@ -234,12 +235,19 @@ int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw
                      int64_t nchains, const uint8_t *chain_kept, const float *chain_frac_rep,
                      const int64_t *chain_seed_start, const int32_t *chain_seed_count,
                      const int64_t *chain_seeds, const fa_seed *seeds, fa_region *regions,
-                     int64_t *n_left, int64_t *n_right) {
+                     int64_t *n_left, int64_t *n_right, double *stage_ms) {
     if (!ctx || !ref || !sw || !opt || nreads < 0 || nchains < 0) return FA_ERR_ARG;
     if (nreads > 0 && (!query_seqs || !query_offset || !regions)) return FA_ERR_ARG;
     int64_t l_pac = 0;
     int rc = fa_ref_length(ref, &l_pac);
     if (rc) return rc;
+    double st[FA_CHAIN_STAGES] = {};
+    auto clk = std::chrono::steady_clock::now();
+    auto lap = [&](int k) {
+        const auto now = std::chrono::steady_clock::now();
+        st[k] += std::chrono::duration<double, std::milli>(now - clk).count();
+        clk = now;
+    };
     int64_t cap = 0;
     for (int64_t c = 0; c < nchains; ++c) cap += chain_seed_count[c];
     std::vector<fa_ext_job> lj((size_t)cap + 1), rj((size_t)cap + 1);
@@ -252,10 +260,11 @@ int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw
                                    cap, lj.data(), &nl, rj.data(), &nr, mstart.data(), mcount.data(),
                                    mseed.data(), mleft.data(), mright.data(), rmax.data());
     if (rc) return rc;
+    lap(0);
     // one extension batch: queries gathered per job (left ones reversed),
     // targets cut on the device from the resident reference
     auto run_batch = [&](const std::vector<fa_ext_job> &jobs, int64_t n, bool reverse_query,
-                         std::vector<fa_out_score> &out) -> int {
+                         std::vector<fa_out_score> &out, int stage) -> int {
         out.assign((size_t)n, fa_out_score{});
         if (n == 0) return FA_OK;
         std::vector<int64_t> qo((size_t)n), sb((size_t)n);
@@ -286,13 +295,16 @@ int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw
         int r = fa_batch_upload_ref(ctx, sw, n, q.data(), qo.data(), ql.data(), ref, sb.data(),
                                     sl.data(), sr.data(), h0.data(), w.data(), &b);
         if (r) return r;
+        lap(stage);
         r = fa_batch_run(ctx, b, 0);
+        lap(stage + 1);
         if (!r) r = fa_batch_download(ctx, b, out.data(), nullptr);
         fa_batch_free(ctx, b);
+        lap(stage + 2);
         return r;
     };
     std::vector<fa_out_score> ls, rs;
-    if ((rc = run_batch(lj, nl, true, ls))) return rc;
+    if ((rc = run_batch(lj, nl, true, ls, 1))) return rc;
     // right jobs start from the seed's left-extended score (mem_chain2aln)
     for (int64_t c = 0; c < nchains; ++c)
         for (int32_t m = 0; m < mcount[(size_t)c]; ++m) {
@@ -302,12 +314,14 @@ int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw
             const int64_t l = mleft[(size_t)mi];
             rj[(size_t)mright[(size_t)mi]].h0 = l >= 0 ? ls[(size_t)l].score : s.len * opt->a;
         }
-    if ((rc = run_batch(rj, nr, false, rs))) return rc;
+    if ((rc = run_batch(rj, nr, false, rs, 4))) return rc;
     rc = fa_merge_scores_to_regions(opt, nreads, query_len, read_chain_start, read_chain_count,
                                     chain_frac_rep, chain_seed_start, chain_seed_count, chain_seeds,
                                     seeds, mstart.data(), mcount.data(), mseed.data(), mleft.data(),
                                     mright.data(), rmax.data(), ls.data(), rs.data(), 1, regions);
     if (rc) return rc;
+    lap(7);
+    if (stage_ms) std::memcpy(stage_ms, st, sizeof st);
     if (n_left) *n_left = nl;
     if (n_right) *n_right = nr;
     return FA_OK;

@@ -252,4 +252,75 @@ int fa_synth_region_pairs(uint64_t seed, const uint8_t *ref, int64_t ref_len, in
     return 0;
 }
 
+// Chain-extension workload (seeds and chains in, regions out): a read of R
+// bases (substitutions rising 0.2% -> 2%, 0.1% indels) from a uniform
+// position on a uniform strand of the bidirectional reference [0, 2 l_pac)
+// (p >= l_pac: reverse complement of forward base 2 l_pac - 1 - p).  Its
+// seeds are the maximal exact runs of at least kmin bases on the true
+// alignment (at most max_seeds, in query order, which is also reference
+// order), in bidirectional coordinates.  Half the reads carry a decoy
+// seed of U{kmin..kmin+11} bases at a uniform position (decoy_len 0: none).
+// Read k: query at qseq + k*R, seeds at [k*max_seeds, +nseeds[k]).
+int fa_synth_chain_reads(uint64_t seed, const uint8_t *ref, int64_t l_pac, int64_t n, int32_t R,
+                         int32_t kmin, int32_t max_seeds, uint8_t *qseq, int64_t *seed_rp,
+                         int32_t *seed_qp, int32_t *seed_len, int32_t *nseeds, int64_t *decoy_rp,
+                         int32_t *decoy_qp, int32_t *decoy_len, int64_t *true_pos, int nthreads) {
+    if (n < 0 || R < 2 * kmin || R > 250 || kmin < 1 || max_seeds < 1 || l_pac < 8192) return -1;
+    parallel_for(n, nthreads, [&](int64_t lo, int64_t hi) {
+        uint8_t read[256];
+        int64_t map[256];
+        int32_t rs[256], re[256];
+        ReadModel mdl = {0.002, 0.02, 0.001};
+        for (int64_t k = lo; k < hi; ++k) {
+            Xoshiro rng(pair_seed(seed, (uint64_t)k));
+            const int64_t p = 64 + (int64_t)(rng.uniform() * (double)(l_pac - R - 256));
+            const bool rev = rng.below(2) == 1;
+            simulate_read(rng, ref, p, R, mdl, read, map);
+            // maximal exact runs on the true alignment, forward read coordinates
+            int nr = 0, s = 0;
+            for (int j = 0; j <= R; ++j) {
+                const bool ext = j < R && read[j] == ref[map[j]] && (j == s || map[j] == map[j - 1] + 1);
+                if (ext) continue;
+                if (j - s >= kmin) {
+                    rs[nr] = s;
+                    re[nr] = j;
+                    ++nr;
+                }
+                s = (j < R && read[j] == ref[map[j]]) ? j : j + 1;
+            }
+            uint8_t *q = qseq + k * R;
+            int32_t ns = 0;
+            if (!rev) {
+                std::memcpy(q, read, (size_t)R);
+                for (int x = 0; x < nr && ns < max_seeds; ++x, ++ns) {
+                    seed_rp[k * max_seeds + ns] = map[rs[x]];
+                    seed_qp[k * max_seeds + ns] = rs[x];
+                    seed_len[k * max_seeds + ns] = re[x] - rs[x];
+                }
+                true_pos[k] = map[0];
+            } else {
+                for (int j = 0; j < R; ++j) q[j] = (uint8_t)(3 - read[R - 1 - j]);
+                for (int x = nr - 1; x >= 0 && ns < max_seeds; --x, ++ns) {
+                    seed_rp[k * max_seeds + ns] = 2 * l_pac - 1 - map[re[x] - 1];
+                    seed_qp[k * max_seeds + ns] = R - re[x];
+                    seed_len[k * max_seeds + ns] = re[x] - rs[x];
+                }
+                true_pos[k] = 2 * l_pac - 1 - map[R - 1];
+            }
+            nseeds[k] = ns;
+            if (rng.below(2) == 0) {
+                const int32_t len = kmin + (int32_t)rng.below(12);
+                decoy_len[k] = len;
+                decoy_qp[k] = (int32_t)rng.below((uint32_t)(R - len + 1));
+                decoy_rp[k] = (int64_t)(rng.uniform() * (double)(2 * l_pac - len));
+            } else {
+                decoy_len[k] = 0;
+                decoy_qp[k] = 0;
+                decoy_rp[k] = 0;
+            }
+        }
+    });
+    return 0;
+}
+
 }  // extern "C"

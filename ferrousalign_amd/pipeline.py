@@ -135,10 +135,15 @@ def merge_scores_to_regions(opt: MemOpt, cb: ChainBatch, m: Mappings, left_score
     return regions[:cb.nchains]
 
 
+CHAIN_STAGES = ("producer", "left_upload", "left_run", "left_download", "right_upload", "right_run",
+                "right_download", "merge")
+
+
 def extend_chains(ref: DeviceReference, sw: BandedPairWiseSW, opt: MemOpt, query_seqs, query_offset,
-                  cb: ChainBatch, ctx=None):
+                  cb: ChainBatch, ctx=None, stage_ms=None):
     """Producer, both extension batches on the GPU, consumer: (regions,
-    number of left jobs, number of right jobs)."""
+    number of left jobs, number of right jobs).  stage_ms: an optional dict
+    that receives the host wall time of each stage (CHAIN_STAGES) in ms."""
     ctx = ctx or ref.ctx or default_context()
     cb = cb.normalised()
     q = np.ascontiguousarray(query_seqs, np.uint8)
@@ -147,9 +152,12 @@ def extend_chains(ref: DeviceReference, sw: BandedPairWiseSW, opt: MemOpt, query
     nl, nr = ctypes.c_int64(), ctypes.c_int64()
     o = opt.to_c()
     c = sw.to_c()
+    st = np.zeros(len(CHAIN_STAGES), np.float64)
     N.check(N.lib().fa_extend_chains(
         ctx.handle, ref.handle, ctypes.byref(c), N.ptr(o), cb.nreads, N.ptr(q), N.ptr(qo), N.ptr(cb.query_len),
         N.ptr(cb.read_chain_start), N.ptr(cb.read_chain_count), cb.nchains, N.ptr(cb.chain_kept),
         N.ptr(cb.chain_frac_rep), N.ptr(cb.chain_seed_start), N.ptr(cb.chain_seed_count), N.ptr(cb.chain_seeds),
-        N.ptr(cb.seeds), N.ptr(regions), ctypes.byref(nl), ctypes.byref(nr)), "fa_extend_chains")
+        N.ptr(cb.seeds), N.ptr(regions), ctypes.byref(nl), ctypes.byref(nr), N.ptr(st)), "fa_extend_chains")
+    if stage_ms is not None:
+        stage_ms.update(zip(CHAIN_STAGES, map(float, st)))
     return regions[:cb.nchains], nl.value, nr.value

@@ -155,3 +155,66 @@ def region_pairs(n=REGION_PAIRS, seed=REGION_SEED, threads=0):
         raise ValueError("fa_synth_region_pairs failed")
     return (q.reshape(-1), np.arange(n, dtype=np.int64) * REGION_QSTRIDE, ql,
             t.reshape(-1), np.arange(n, dtype=np.int64) * REGION_TSTRIDE, tl, w)
+
+
+CHAIN_SEED = 0xFA06
+CHAIN_READS = 1048576
+CHAIN_READ_LEN = 150
+CHAIN_KMIN = 19
+CHAIN_MAX_SEEDS = 8
+
+
+def chain_reads(n=CHAIN_READS, seed=CHAIN_SEED, threads=0):
+    """Chain-extension workload: n reads of 150 bases drawn from both strands
+    of the synthetic reference (substitutions rising 0.2% -> 2%, 0.1% indels).
+    Chain 0 of a read holds its seeds (maximal exact runs of >= 19 bases on
+    the true alignment, at most 8); half the reads carry a second chain with
+    one decoy seed at a uniform position, kept with probability 0.7.
+    Returns (ChainBatch, qseq, qoff, true bidirectional position)."""
+    from .pipeline import ChainBatch
+    ref = reference()
+    l_pac = ref.size
+    R, S = CHAIN_READ_LEN, CHAIN_MAX_SEEDS
+    q = np.zeros(n * R, np.uint8)
+    srp = np.zeros(n * S, np.int64)
+    sqp = np.zeros(n * S, np.int32)
+    sln = np.zeros(n * S, np.int32)
+    ns = np.zeros(n, np.int32)
+    drp = np.zeros(n, np.int64)
+    dqp = np.zeros(n, np.int32)
+    dln = np.zeros(n, np.int32)
+    tp = np.zeros(n, np.int64)
+    rc = N.synth().fa_synth_chain_reads(ctypes.c_uint64(seed), N.ptr(ref), l_pac, n, R, CHAIN_KMIN, S, N.ptr(q),
+                                        N.ptr(srp), N.ptr(sqp), N.ptr(sln), N.ptr(ns), N.ptr(drp), N.ptr(dqp),
+                                        N.ptr(dln), N.ptr(tp), threads)
+    if rc != 0:
+        raise ValueError("fa_synth_chain_reads failed")
+    rng = np.random.default_rng(seed)
+    has0 = ns > 0
+    has1 = dln > 0
+    nch = has0.astype(np.int32) + has1
+    rcs = np.zeros(n, np.int64)
+    rcs[1:] = np.cumsum(nch)[:-1]
+    nchains = int(nch.sum())
+    ccount = np.zeros(nchains, np.int32)
+    ccount[rcs[has0]] = ns[has0]
+    ccount[rcs[has1] + has0[has1]] = 1
+    cstart = np.zeros(nchains, np.int64)
+    cstart[1:] = np.cumsum(ccount)[:-1]
+    kept = np.ones(nchains, np.uint8)
+    kept[rcs[has1] + has0[has1]] = (rng.random(int(has1.sum())) < 0.7).astype(np.uint8)
+    seeds = np.zeros(int(ccount.sum()), N.SEED_DTYPE)
+    # chain 0 seeds: per read, slots [0, ns) of its fixed-stride block
+    slot = np.arange(n * S) % S
+    m0 = slot < np.repeat(ns, S)
+    first = np.repeat(cstart[rcs], S) + slot
+    seeds["ref_pos"][first[m0]] = srp[m0]
+    seeds["query_pos"][first[m0]] = sqp[m0]
+    seeds["len"][first[m0]] = sln[m0]
+    dc = cstart[rcs[has1] + has0[has1]]
+    seeds["ref_pos"][dc] = drp[has1]
+    seeds["query_pos"][dc] = dqp[has1]
+    seeds["len"][dc] = dln[has1]
+    cb = ChainBatch(np.full(n, R, np.int32), rcs, nch, kept, rng.random(nchains).astype(np.float32), cstart,
+                    ccount, np.arange(len(seeds), dtype=np.int64), seeds)
+    return cb, q, np.arange(n, dtype=np.int64) * R, tp

@@ -399,7 +399,11 @@ int fa_merge_scores_to_regions(const fa_mem_opt *opt, int64_t nreads, const int3
  * reference's h0 = 0 right jobs are outside ksw_extend2's h0 > 0 domain,
  * ksw_affine_gap.rs:968).  sw supplies the scoring and end_bonus of both
  * batches.  regions has one entry per chain; the job counts go to
- * n_left / n_right when non-NULL. */
+ * n_left / n_right when non-NULL.  stage_ms (NULL or FA_CHAIN_STAGES
+ * entries) receives host wall times in ms: producer, left batch
+ * (query gather + upload + target cut), left run, left download, right
+ * batch, right run, right download, merge. */
+#define FA_CHAIN_STAGES 8
 int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw *sw,
                      const fa_mem_opt *opt, int64_t nreads, const uint8_t *query_seqs,
                      const int64_t *query_offset, const int32_t *query_len,
@@ -407,7 +411,7 @@ int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw
                      int64_t nchains, const uint8_t *chain_kept, const float *chain_frac_rep,
                      const int64_t *chain_seed_start, const int32_t *chain_seed_count,
                      const int64_t *chain_seeds, const fa_seed *seeds, fa_region *regions,
-                     int64_t *n_left, int64_t *n_right);
+                     int64_t *n_left, int64_t *n_right, double *stage_ms);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,15 @@ Integers follow the Rust types: ref_pos, rmax and the window arithmetic are
 u64 (saturating_sub where the reference uses it, wrapping otherwise), so a
 seed left of the window gives no job; `as i32` of an f64 truncates toward zero.
 A job is a dict with the fields of the library's fa_ext_job record.
+
+extend_chains_cpu is the whole chain-extension path on the CPU: the
+library's producer and merge (host code, checked against the functions above)
+around extensions run by the ksw_extend2 C restatement on targets cut by the
+.pac restatement, with the right jobs' h0 chained from the left score.
 """
+import time
+
+import numpy as np
 U64 = (1 << 64) - 1
 
 
@@ -141,3 +149,50 @@ def merge_scores_to_regions(opt, cb, maps, rmaxes, left_scores, right_scores, ri
                     best_score, best = total, g
             out[c] = best
     return out
+
+
+def extend_chains_cpu(cb, pac, l_pac, qseq, qoff, sw, opt, nthreads=0, timing=None):
+    """(regions, number of left jobs, number of right jobs); timing, when a
+    dict, receives the seconds spent in the producer, the two extension
+    batches and the merge (query gathering and target cutting excluded)."""
+    from ferrousalign_amd.pipeline import collect_extension_jobs, merge_scores_to_regions
+    from tests import _oracle as O
+    t = {"producer": 0.0, "extend": 0.0, "merge": 0.0}
+    t0 = time.perf_counter()
+    left, right, m = collect_extension_jobs(opt, l_pac, cb)
+    t["producer"] += time.perf_counter() - t0
+
+    def run(jobs, rev):
+        if len(jobs) == 0:
+            return np.zeros((0, 6), np.int32)
+        qs, ts = [], []
+        for j in jobs:
+            b = int(qoff[j["read_idx"]]) + int(j["qbeg"])
+            q = qseq[b:b + int(j["qlen"])]
+            qs.append(q[::-1] if rev else q)
+            ts.append(O.pac_segment(pac, l_pac, int(j["seg_beg"]), int(j["seg_len"]), reverse=bool(j["seg_rev"])))
+        ql = np.array([len(x) for x in qs], np.int32)
+        tl = np.array([len(x) for x in ts], np.int32)
+        qo = np.concatenate([[0], np.cumsum(ql)[:-1]]).astype(np.int64)
+        to = np.concatenate([[0], np.cumsum(tl)[:-1]]).astype(np.int64)
+        qcat, tcat = np.concatenate(qs), np.concatenate(ts)
+        t1 = time.perf_counter()
+        out, _, rc = O.extend_batch(sw, qcat, qo, ql, tcat, to, tl, jobs["h0"], jobs["w"], nthreads=nthreads)
+        t["extend"] += time.perf_counter() - t1
+        assert (rc == 0).all()
+        return O.to_outscore_order(out).astype(np.int32)
+    ls = run(left, True)
+    right = right.copy()
+    cnt = m.count[:cb.nchains].astype(np.int64)
+    idx = np.repeat(m.start[:cb.nchains] - (np.cumsum(cnt) - cnt), cnt) + np.arange(int(cnt.sum()))
+    idx = idx[m.right[idx] >= 0]
+    li = m.left[idx]
+    seed_len = cb.seeds["len"][m.seed[idx]].astype(np.int64)
+    right["h0"][m.right[idx]] = np.where(li >= 0, ls[np.maximum(li, 0), 0] if len(ls) else 0, seed_len * opt.a)
+    rs = run(right, False)
+    t0 = time.perf_counter()
+    regions = merge_scores_to_regions(opt, cb, m, ls, rs, right_from_left=True)
+    t["merge"] += time.perf_counter() - t0
+    if timing is not None:
+        timing.update(t)
+    return regions, len(left), len(right)

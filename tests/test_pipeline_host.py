@@ -113,35 +113,7 @@ def test_merge_matches_oracle(seed):
 
 
 def _cpu_pipeline(d, sw, opt):
-    """Producer (library), extensions by the ksw_extend2 oracle on targets cut
-    by the .pac oracle, right h0 chained from the left score, merge (library)."""
-    cb, l_pac, pac = d["cb"], d["l_pac"], d["pac"]
-    left, right, m = collect_extension_jobs(opt, l_pac, cb)
-
-    def run(jobs, rev):
-        if len(jobs) == 0:
-            return np.zeros((0, 6), np.int32)
-        qs, ts = [], []
-        for j in jobs:
-            q = d["qseq"][d["qoff"][j["read_idx"]] + j["qbeg"]:d["qoff"][j["read_idx"]] + j["qbeg"] + j["qlen"]]
-            qs.append(q[::-1] if rev else q)
-            ts.append(O.pac_segment(pac, l_pac, int(j["seg_beg"]), int(j["seg_len"]), reverse=bool(j["seg_rev"])))
-        ql = np.array([len(x) for x in qs], np.int32)
-        tl = np.array([len(x) for x in ts], np.int32)
-        qo = np.concatenate([[0], np.cumsum(ql)[:-1]]).astype(np.int64)
-        to = np.concatenate([[0], np.cumsum(tl)[:-1]]).astype(np.int64)
-        out, _, rc = O.extend_batch(sw, np.concatenate(qs), qo, ql, np.concatenate(ts), to, tl,
-                                    jobs["h0"], jobs["w"])
-        assert (rc == 0).all()
-        return O.to_outscore_order(out).astype(np.int32)
-    ls = run(left, True)
-    for c in range(cb.nchains):
-        for i in range(int(m.start[c]), int(m.start[c]) + int(m.count[c])):
-            if m.right[i] >= 0:
-                s = cb.seeds[m.seed[i]]
-                right["h0"][m.right[i]] = ls[m.left[i], 0] if m.left[i] >= 0 else int(s["len"]) * opt.a
-    rs = run(right, False)
-    return merge_scores_to_regions(opt, cb, m, ls, rs, right_from_left=True), len(left), len(right)
+    return PO.extend_chains_cpu(d["cb"], d["pac"], d["l_pac"], d["qseq"], d["qoff"], sw, opt)
 
 
 def test_exact_reads_give_full_length_regions():

@@ -105,8 +105,14 @@ def _declare(lib):
     lib.fa_collect_extension_jobs.argtypes = [_P, _I64, _I64] + [_P] * 8 + [_I64, _P, ctypes.POINTER(_I64), _P,
                                                                          ctypes.POINTER(_I64)] + [_P] * 6
     lib.fa_merge_scores_to_regions.argtypes = [_P, _I64] + [_P] * 16 + [_I32, _P]
-    lib.fa_extend_chains.argtypes = [_P, _P, ctypes.POINTER(BandedPairwiseSW), _P, _I64, _P, _P, _P, _P, _P, _I64,
-                                     _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _P]
+    chain_in = [_I64, _P, _I64, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _P]
+    lib.fa_extend_chains.argtypes = ([_P, _P, ctypes.POINTER(BandedPairwiseSW), _P] + chain_in +
+                                     [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _P])
+    lib.fa_chains_upload.argtypes = [_P] + chain_in + [ctypes.POINTER(_P)]
+    lib.fa_chains_extend.argtypes = [_P, _P, _P, ctypes.POINTER(BandedPairwiseSW), _P, ctypes.POINTER(_I64),
+                                     ctypes.POINTER(_I64), _P]
+    lib.fa_chains_download.argtypes = [_P, _P, _P]
+    lib.fa_chains_free.argtypes = [_P, _P]
     lib.fa_batch_upload_ref.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW), _I64, _P, _P, _P, _P, _P, _P, _P,
                                         _P, _P, ctypes.POINTER(_P)]
     lib.fa_batch_download_targets.argtypes = [_P, _P, _P, _P]
@@ -135,6 +141,7 @@ def _declare(lib):
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
     for name in ("fa_ctx_create", "fa_ctx_destroy", "fa_ctx_describe", "fa_valu_peak", "fa_hbm_copy", "fa_ref_upload", "fa_ref_free",
                  "fa_ref_length", "fa_collect_extension_jobs", "fa_merge_scores_to_regions", "fa_extend_chains",
+                 "fa_chains_upload", "fa_chains_extend", "fa_chains_download", "fa_chains_free",
                  "fa_batch_upload_ref", "fa_batch_download_targets", "fa_scalar_banded_swa_batch",
                  "fa_scalar_banded_swa_last_ms", "fa_execute_batch_simd_scoring",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",

@@ -22,8 +22,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HIP_SOURCES = ["extend_kernel.hip", "extend16_kernel.hip", "predictor.hip", "mate16_kernel.hip", "mate16_sym.hip", "valu_probe.hip",
-               "ref_kernel.hip", "swtb_kernel.hip", "fa_api.cpp",
-               "fa_pipeline.cpp"]
+               "ref_kernel.hip", "swtb_kernel.hip", "chain_kernel.hip", "fa_api.cpp",
+               "fa_pipeline.cpp", "chain_api.cpp"]
 
 
 def _run(cmd):

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/ferrousalign_amd.h"
+#include "fa_internal.h"
 #include "extend_kernel.h"
 #include "predictor.h"
 #include "mate_kernel.h"
@@ -23,93 +24,21 @@
 #include "ref_kernel.h"
 #include "swtb_kernel.h"
 
-struct fa_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;           // main stream of the context
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};  // fork/join streams (default priority)
-    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    std::string arch;
-    int cus = 0;
-    double last_dp_ms = 0.0, last_tb_ms = 0.0;  // kernels of the last fa_scalar_banded_swa_batch
-};
+using fa_host::Bucket;
 
-namespace {
-
-struct Bucket {
-    int S;
-    uint32_t first, count;
-    double work;   // estimated work: sum over pairs of qlen * tlen (or of the cost)
-    int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group)
-    int L = 16;    // lanes per pair of this bucket
-    int slen = 0;  // mate rescue: segments per lane ceil(qlen / 8) of every pair
-};
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-}  // namespace
-
-// Device-resident reference in the .pac layout (4 bases per byte, first base
-// in the two most significant bits).
-struct fa_ref {
-    int device = 0;
-    uint8_t *d_pac = nullptr;
-    int64_t l_pac = 0;
-};
-
-struct fa_batch {
-    int64_t n = 0;
-    // host copies kept for re-packing
-    std::vector<int32_t> qlen, tlen, h0, w;
-    std::vector<uint8_t> elig16;  // scores provably fit the 16-bit kernel
-    std::vector<float> cost;
-    std::vector<uint32_t> qo_host, to_host;  // word offsets of each query / target in d_seq2
-    // device
-    uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
-    int32_t *d_qlen = nullptr, *d_tlen = nullptr, *d_h0 = nullptr, *d_w = nullptr;
-    uint32_t *d_order = nullptr;
-    uint32_t *d_meta = nullptr;  // per packed slot: pid, qo, to, qlen, tlen, h0, w, 0
-    int32_t *d_out = nullptr;
-    uint32_t *d_cells = nullptr;
-    uint32_t *d_queue = nullptr;
-    fa::ExtArgs args{};
-    // launch plan for the current packing
-    uint32_t packed_flags = 0xFFFFFFFFu;
-    std::vector<Bucket> buckets;
-    int lanes = 0;  // 0: narrowest lane group per query length
-    int chunk = 0;
-    int waves_per_cu = 0;
-    int streams = 4;
-    int tmax = 0;
-    // mate-rescue local SW (FA_RUN_ALIGN2)
-    bool align2_set = false;
-    fa::MateArgs margs{};
-    int32_t *d_mout = nullptr;     // 7 int32 per pair
-    uint16_t *d_imax_log = nullptr;
-    uint32_t log_slots = 0;
-    std::vector<Bucket> mbuckets;
-    float *d_feat = nullptr;       // predictor features [n][6]
-    float *d_pred = nullptr;       // predicted cells [n]
-    hipGraphExec_t graph_exec = nullptr;
-    uint32_t graph_key = 0;
-    int graph_wpc = 0;
-};
-
-namespace {
+namespace fa_host {
 
 int hip_fail(hipError_t e, const char *what) {
     if (e == hipSuccess) return FA_OK;
     std::fprintf(stderr, "ferrousalign_amd: %s failed: %s\n", what, hipGetErrorString(e));
     return FA_ERR_DEVICE;
 }
-#define FA_HIP(call)                                   \
-    do {                                               \
-        int _rc = hip_fail((call), #call);             \
-        if (_rc != FA_OK) return _rc;                  \
-    } while (0)
+
+}  // namespace fa_host
+
+namespace {
+
+using fa_host::hip_fail;
 
 // Rust `f64 as i32`: truncation toward zero, saturating, NaN -> 0.
 int32_t f64_as_i32(double x) {
@@ -134,6 +63,10 @@ int32_t capped_w(const fa_banded_pairwise_sw &sw, int32_t qlen, int32_t w) {
     return std::min(w, max_del);
 }
 
+}  // namespace
+
+namespace fa_host {
+
 int check_scoring(const fa_banded_pairwise_sw *sw) {
     if (!sw) return FA_ERR_ARG;
     if (sw->m != 5) return FA_ERR_SCORING;
@@ -144,6 +77,70 @@ int check_scoring(const fa_banded_pairwise_sw *sw) {
         if (sw->mat[20 + q] != sw->mat[20]) return FA_ERR_SCORING;
     return FA_OK;
 }
+
+bool pair_kernel(int L, bool wide, bool elig16, int q, int *kind, int *lanes, int *S) {
+    // lanes per pair of a query length: the configured L when its slot
+    // counts reach, else the next wider group; L == 0 picks the narrowest
+    // group that holds the query (the fewest lanes per pair amortise the
+    // per-row reductions over the most pairs per wavefront)
+    if (!wide && elig16) {
+        int l16 = 16;
+        if (L == 0) {
+            for (int c : {4, 8, 16})
+                if (fa::extend16_slot_count(c, q) > 0) {
+                    l16 = c;
+                    break;
+                }
+        } else {
+            l16 = fa::extend16_slot_count(L, q) > 0 ? L : 16;
+        }
+        const int s16 = fa::extend16_slot_count(l16, q);
+        if (s16 > 0) {
+            *kind = 1;
+            *lanes = l16;
+            *S = s16;
+            return true;
+        }
+    }
+    const int c = L ? L : 8;
+    const int l32 = fa::extend_slot_count(c, q) > 0 ? c : 8;
+    *kind = 0;
+    *lanes = l32;
+    *S = fa::extend_slot_count(l32, q);
+    return *S > 0;
+}
+
+bool fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen) {
+    int matmax = 0;
+    for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw.mat[x]);
+    const int64_t bound = h0 + qlen * matmax;
+    const int64_t K = std::max(2, 1 + matmax);
+    return bound < 4096 && bound * K < 32768 && sw.e_ins < 32 && sw.o_ins + sw.e_ins < 8192 &&
+           sw.o_del + sw.e_del < 8192;
+}
+
+void set_scoring(fa::ExtArgs &a, const fa_banded_pairwise_sw &sw) {
+    for (int q = 0; q < 5; ++q) {
+        uint32_t v = 0;
+        for (int t = 0; t < 4; ++t) v |= (uint32_t)(uint8_t)(sw.mat[t * 5 + q] + 128) << (8 * t);
+        a.profw[q] = v;
+    }
+    a.nrow = (uint32_t)(uint8_t)(sw.mat[20] + 128);
+    int mx = 0;
+    for (int x = 0; x < 25; ++x) mx = std::max(mx, (int)sw.mat[x]);
+    a.mguard = std::max(2, 1 + mx);
+    a.o_del = sw.o_del;
+    a.e_del = sw.e_del;
+    a.o_ins = sw.o_ins;
+    a.e_ins = sw.e_ins;
+    a.zdrop = sw.zdrop;
+}
+
+}  // namespace fa_host
+
+namespace {
+
+using fa_host::check_scoring;
 
 template <class F>
 void parallel_for(int64_t n, F &&f) {
@@ -275,36 +272,19 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
     // kernel of each pair: (kind, lanes, slots) packed in one sortable key
     std::vector<int> skey((size_t)n);
-    // lanes per pair of a query length: the configured L when its slot
-    // counts reach, else the next wider group; L == 0 picks the narrowest
-    // group that holds the query (the fewest lanes per pair amortise the
-    // per-row reductions over the most pairs per wavefront)
     auto lanes16 = [&](int q) {
-        if (L == 0) {
-            for (int c : {4, 8, 16})
-                if (fa::extend16_slot_count(c, q) > 0) return c;
-            return 16;
-        }
-        return fa::extend16_slot_count(L, q) > 0 ? L : 16;
+        int kind, lanes, S;
+        fa_host::pair_kernel(L, false, true, q, &kind, &lanes, &S);
+        return lanes;
     };
     auto lanes32 = [&](int q) {
-        const int c = L ? L : 8;
-        return fa::extend_slot_count(c, q) > 0 ? c : 8;
+        int kind, lanes, S;
+        fa_host::pair_kernel(L, true, false, q, &kind, &lanes, &S);
+        return lanes;
     };
     auto pair_key = [&](int64_t k, int &kind, int &lanes, int &S) -> bool {
-        const int q = b->qlen[(size_t)k];
-        if (!wide && b->elig16[(size_t)k]) {
-            lanes = lanes16(q);
-            S = fa::extend16_slot_count(lanes, q);
-            if (S > 0) {
-                kind = 1;
-                return true;
-            }
-        }
-        kind = 0;
-        lanes = lanes32(q);
-        S = fa::extend_slot_count(lanes, q);
-        return S > 0;
+        return fa_host::pair_kernel(L, wide != 0, b->elig16[(size_t)k] != 0, b->qlen[(size_t)k], &kind,
+                                    &lanes, &S);
     };
     for (int64_t k = 0; k < n; ++k) {
         int kind, lanes, S;
@@ -394,6 +374,10 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     return FA_OK;
 }
 
+}  // namespace
+
+namespace fa_host {
+
 hipError_t launch_bucket(fa_ctx *ctx, fa_batch *b, uint32_t flags, size_t k, hipStream_t st) {
     const Bucket &bk = b->buckets[k];
     fa::ExtArgs a = b->args;
@@ -443,6 +427,13 @@ int enqueue_run(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     }
     return FA_OK;
 }
+
+}  // namespace fa_host
+
+namespace {
+
+using fa_host::enqueue_run;
+using fa_host::launch_bucket;
 
 constexpr uint32_t kMateChunk = 262144;  // pairs per mate launch (row-log scratch reuse)
 
@@ -560,6 +551,7 @@ int fa_ctx_create(int device, fa_ctx **out) {
 int fa_ctx_destroy(fa_ctx *ctx) {
     if (!ctx) return FA_ERR_ARG;
     (void)hipSetDevice(ctx->device);
+    fa_host::chain_workspace_free(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -642,8 +634,6 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             vrc[(size_t)k] = (int8_t)r;
         }
     });
-    int matmax = 0;
-    for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw->mat[x]);
     for (int64_t k = 0; k < n; ++k) {
         int r = vrc[(size_t)k];
         if (r != FA_OK) {
@@ -659,12 +649,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             break;
         }
         b->w[(size_t)k] = capped_w(*sw, query_len[k], band_width[k]);
-        {
-            const int64_t bound = (int64_t)h0[k] + (int64_t)query_len[k] * matmax;
-            const int64_t K = std::max(2, 1 + matmax);
-            b->elig16[(size_t)k] = bound < 4096 && bound * K < 32768 && sw->e_ins < 32 &&
-                                   sw->o_ins + sw->e_ins < 8192 && sw->o_del + sw->e_del < 8192;
-        }
+        b->elig16[(size_t)k] = fa_host::fits16(*sw, h0[k], query_len[k]);
         tmax = std::max(tmax, ref_len[k]);
     }
     if (first_err != FA_OK) {
@@ -735,22 +720,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     a.cells = b->d_cells;
     a.queue = b->d_queue;
     a.ovf = nullptr;
-    for (int q = 0; q < 5; ++q) {
-        uint32_t v = 0;
-        for (int t = 0; t < 4; ++t) v |= (uint32_t)(uint8_t)(sw->mat[t * 5 + q] + 128) << (8 * t);
-        a.profw[q] = v;
-    }
-    a.nrow = (uint32_t)(uint8_t)(sw->mat[20] + 128);
-    {
-        int mx = 0;
-        for (int x = 0; x < 25; ++x) mx = std::max(mx, (int)sw->mat[x]);
-        a.mguard = std::max(2, 1 + mx);
-    }
-    a.o_del = sw->o_del;
-    a.e_del = sw->e_del;
-    a.o_ins = sw->o_ins;
-    a.e_ins = sw->e_ins;
-    a.zdrop = sw->zdrop;
+    fa_host::set_scoring(a, *sw);
     b->tmax = tmax;
     a.tpad = ((tmax + 16) / 16) * 16;
     a.chunk = 0;

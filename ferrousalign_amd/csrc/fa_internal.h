@@ -1,0 +1,118 @@
+// Library-internal types shared by the host translation units (fa_api.cpp,
+// fa_pipeline.cpp): the context, the device reference, a batch of extension
+// pairs with its launch plan, and the host helpers both sides use.  Not part
+// of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ferrousalign_amd.h"
+#include "extend_kernel.h"
+#include "mate_kernel.h"
+
+struct fa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;           // main stream of the context
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};  // fork/join streams (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::string arch;
+    int cus = 0;
+    double last_dp_ms = 0.0, last_tb_ms = 0.0;  // kernels of the last fa_scalar_banded_swa_batch
+    void *chain_ws = nullptr;  // device workspace of the chain-extension path (fa_pipeline.cpp)
+};
+
+namespace fa_host {
+
+struct Bucket {
+    int S;
+    uint32_t first, count;
+    double work;   // estimated work: sum over pairs of qlen * tlen (or of the cost)
+    int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group)
+    int L = 16;    // lanes per pair of this bucket
+    int slen = 0;  // mate rescue: segments per lane ceil(qlen / 8) of every pair
+};
+
+}  // namespace fa_host
+
+// Device-resident reference in the .pac layout (4 bases per byte, first base
+// in the two most significant bits).
+struct fa_ref {
+    int device = 0;
+    uint8_t *d_pac = nullptr;
+    int64_t l_pac = 0;
+};
+
+struct fa_batch {
+    int64_t n = 0;
+    // host copies kept for re-packing
+    std::vector<int32_t> qlen, tlen, h0, w;
+    std::vector<uint8_t> elig16;  // scores provably fit the 16-bit kernel
+    std::vector<float> cost;
+    std::vector<uint32_t> qo_host, to_host;  // word offsets of each query / target in d_seq2
+    // device
+    uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
+    int32_t *d_qlen = nullptr, *d_tlen = nullptr, *d_h0 = nullptr, *d_w = nullptr;
+    uint32_t *d_order = nullptr;
+    uint32_t *d_meta = nullptr;  // per packed slot: pid, qo, to, qlen, tlen, h0, w, 0
+    int32_t *d_out = nullptr;
+    uint32_t *d_cells = nullptr;
+    uint32_t *d_queue = nullptr;
+    fa::ExtArgs args{};
+    // launch plan for the current packing
+    uint32_t packed_flags = 0xFFFFFFFFu;
+    std::vector<fa_host::Bucket> buckets;
+    int lanes = 0;  // 0: narrowest lane group per query length
+    int chunk = 0;
+    int waves_per_cu = 0;
+    int streams = 4;
+    int tmax = 0;
+    // mate-rescue local SW (FA_RUN_ALIGN2)
+    bool align2_set = false;
+    fa::MateArgs margs{};
+    int32_t *d_mout = nullptr;     // 7 int32 per pair
+    uint16_t *d_imax_log = nullptr;
+    uint32_t log_slots = 0;
+    std::vector<fa_host::Bucket> mbuckets;
+    float *d_feat = nullptr;       // predictor features [n][6]
+    float *d_pred = nullptr;       // predicted cells [n]
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_key = 0;
+    int graph_wpc = 0;
+};
+
+namespace fa_host {
+
+// FA_OK, or FA_ERR_DEVICE after printing what failed.
+int hip_fail(hipError_t e, const char *what);
+#define FA_HIP(call)                                        \
+    do {                                                    \
+        int _rc = fa_host::hip_fail((call), #call);         \
+        if (_rc != FA_OK) return _rc;                       \
+    } while (0)
+
+int check_scoring(const fa_banded_pairwise_sw *sw);
+
+// Kernel choice of one pair: kind (1: packed 16-bit, 0: 32-bit), lanes per
+// pair and slot count, for the configured lane count L (0: narrowest group
+// that holds the query) and the FA_SCORE_I32 request `wide`; false when no
+// instantiated kernel holds qlen.
+bool pair_kernel(int L, bool wide, bool elig16, int qlen, int *kind, int *lanes, int *S);
+
+// Whether a pair's scores provably fit the packed 16-bit kernel.
+bool fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen);
+
+// Scoring fields of the kernel argument block (profile bytes, gap costs,
+// z-drop, 16-bit guard).
+void set_scoring(fa::ExtArgs &a, const fa_banded_pairwise_sw &sw);
+
+// Enqueue one run of a planned batch (buckets over up to 4 streams).
+int enqueue_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
+
+// Releases the chain-extension workspace of a context (fa_pipeline.cpp).
+void chain_workspace_free(fa_ctx *ctx);
+
+}  // namespace fa_host

@@ -1,13 +1,9 @@
-// The callers either side of the extension batch: the job producer
-// (collect_extension_jobs_batch_soa, src/pipelines/linear/batch_extension/
-// collect_soa.rs:28-228), the consumer (merge_scores_to_regions,
-// src/pipelines/linear/region.rs:737-890) and a driver that runs both
-// extension batches on the device between them.  Host code; the DP runs
-// through the batch entry points of fa_api.cpp.
-#include <hip/hip_runtime.h>
-
+// The callers either side of the extension batch on the host: the job
+// producer (collect_extension_jobs_batch_soa, src/pipelines/linear/
+// batch_extension/collect_soa.rs:28-228) and the consumer
+// (merge_scores_to_regions, src/pipelines/linear/region.rs:737-890).  The
+// device path that runs both around the GPU batches is chain_api.cpp.
 #include <algorithm>
-#include <chrono>
 #include <climits>
 #include <cstdint>
 #include <cstring>
@@ -69,6 +65,8 @@ int fa_collect_extension_jobs(const fa_mem_opt *opt, int64_t l_pac, int64_t nrea
             uint64_t r0 = lp << 1, r1 = 0;
             for (int32_t k = 0; k < ns; ++k) {
                 const fa_seed &s = seeds[si[k]];
+                // a seed lies inside its read (the reference's seeds always do)
+                if (s.query_pos < 0 || s.len < 0 || s.query_pos > qlen - s.len) return FA_ERR_ARG;
                 const int32_t lm = s.query_pos + cal_max_gap(*opt, s.query_pos);
                 const uint64_t b = (uint64_t)s.ref_pos > (uint64_t)lm ? (uint64_t)s.ref_pos - (uint64_t)lm : 0;
                 const int32_t rem = qlen - s.query_pos - s.len;
@@ -225,105 +223,6 @@ int fa_merge_scores_to_regions(const fa_mem_opt *opt, int64_t nreads, const int3
             regions[c] = best;
         }
     }
-    return FA_OK;
-}
-
-int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw *sw,
-                     const fa_mem_opt *opt, int64_t nreads, const uint8_t *query_seqs,
-                     const int64_t *query_offset, const int32_t *query_len,
-                     const int64_t *read_chain_start, const int32_t *read_chain_count,
-                     int64_t nchains, const uint8_t *chain_kept, const float *chain_frac_rep,
-                     const int64_t *chain_seed_start, const int32_t *chain_seed_count,
-                     const int64_t *chain_seeds, const fa_seed *seeds, fa_region *regions,
-                     int64_t *n_left, int64_t *n_right, double *stage_ms) {
-    if (!ctx || !ref || !sw || !opt || nreads < 0 || nchains < 0) return FA_ERR_ARG;
-    if (nreads > 0 && (!query_seqs || !query_offset || !regions)) return FA_ERR_ARG;
-    int64_t l_pac = 0;
-    int rc = fa_ref_length(ref, &l_pac);
-    if (rc) return rc;
-    double st[FA_CHAIN_STAGES] = {};
-    auto clk = std::chrono::steady_clock::now();
-    auto lap = [&](int k) {
-        const auto now = std::chrono::steady_clock::now();
-        st[k] += std::chrono::duration<double, std::milli>(now - clk).count();
-        clk = now;
-    };
-    int64_t cap = 0;
-    for (int64_t c = 0; c < nchains; ++c) cap += chain_seed_count[c];
-    std::vector<fa_ext_job> lj((size_t)cap + 1), rj((size_t)cap + 1);
-    std::vector<int64_t> mstart((size_t)nchains + 1), mseed((size_t)cap + 1), mleft((size_t)cap + 1),
-        mright((size_t)cap + 1), rmax(2 * (size_t)nchains + 2);
-    std::vector<int32_t> mcount((size_t)nchains + 1);
-    int64_t nl = 0, nr = 0;
-    rc = fa_collect_extension_jobs(opt, l_pac, nreads, query_len, read_chain_start, read_chain_count,
-                                   chain_kept, chain_seed_start, chain_seed_count, chain_seeds, seeds,
-                                   cap, lj.data(), &nl, rj.data(), &nr, mstart.data(), mcount.data(),
-                                   mseed.data(), mleft.data(), mright.data(), rmax.data());
-    if (rc) return rc;
-    lap(0);
-    // one extension batch: queries gathered per job (left ones reversed),
-    // targets cut on the device from the resident reference
-    auto run_batch = [&](const std::vector<fa_ext_job> &jobs, int64_t n, bool reverse_query,
-                         std::vector<fa_out_score> &out, int stage) -> int {
-        out.assign((size_t)n, fa_out_score{});
-        if (n == 0) return FA_OK;
-        std::vector<int64_t> qo((size_t)n), sb((size_t)n);
-        std::vector<int32_t> ql((size_t)n), sl((size_t)n), h0((size_t)n), w((size_t)n);
-        std::vector<uint8_t> sr((size_t)n);
-        int64_t total = 0;
-        for (int64_t k = 0; k < n; ++k) {
-            qo[(size_t)k] = total;
-            total += jobs[(size_t)k].qlen;
-        }
-        std::vector<uint8_t> q((size_t)total + 1);
-        for (int64_t k = 0; k < n; ++k) {
-            const fa_ext_job &j = jobs[(size_t)k];
-            const uint8_t *src = query_seqs + query_offset[j.read_idx] + j.qbeg;
-            uint8_t *dst = q.data() + qo[(size_t)k];
-            if (reverse_query)
-                for (int32_t x = 0; x < j.qlen; ++x) dst[x] = src[j.qlen - 1 - x];
-            else
-                std::memcpy(dst, src, (size_t)j.qlen);
-            ql[(size_t)k] = j.qlen;
-            sb[(size_t)k] = j.seg_beg;
-            sl[(size_t)k] = j.seg_len;
-            sr[(size_t)k] = (uint8_t)j.seg_rev;
-            h0[(size_t)k] = j.h0;
-            w[(size_t)k] = j.w;
-        }
-        fa_batch *b = nullptr;
-        int r = fa_batch_upload_ref(ctx, sw, n, q.data(), qo.data(), ql.data(), ref, sb.data(),
-                                    sl.data(), sr.data(), h0.data(), w.data(), &b);
-        if (r) return r;
-        lap(stage);
-        r = fa_batch_run(ctx, b, 0);
-        lap(stage + 1);
-        if (!r) r = fa_batch_download(ctx, b, out.data(), nullptr);
-        fa_batch_free(ctx, b);
-        lap(stage + 2);
-        return r;
-    };
-    std::vector<fa_out_score> ls, rs;
-    if ((rc = run_batch(lj, nl, true, ls, 1))) return rc;
-    // right jobs start from the seed's left-extended score (mem_chain2aln)
-    for (int64_t c = 0; c < nchains; ++c)
-        for (int32_t m = 0; m < mcount[(size_t)c]; ++m) {
-            const int64_t mi = mstart[(size_t)c] + m;
-            if (mright[(size_t)mi] < 0) continue;
-            const fa_seed &s = seeds[mseed[(size_t)mi]];
-            const int64_t l = mleft[(size_t)mi];
-            rj[(size_t)mright[(size_t)mi]].h0 = l >= 0 ? ls[(size_t)l].score : s.len * opt->a;
-        }
-    if ((rc = run_batch(rj, nr, false, rs, 4))) return rc;
-    rc = fa_merge_scores_to_regions(opt, nreads, query_len, read_chain_start, read_chain_count,
-                                    chain_frac_rep, chain_seed_start, chain_seed_count, chain_seeds,
-                                    seeds, mstart.data(), mcount.data(), mseed.data(), mleft.data(),
-                                    mright.data(), rmax.data(), ls.data(), rs.data(), 1, regions);
-    if (rc) return rc;
-    lap(7);
-    if (stage_ms) std::memcpy(stage_ms, st, sizeof st);
-    if (n_left) *n_left = nl;
-    if (n_right) *n_right = nr;
     return FA_OK;
 }
 

@@ -9,9 +9,11 @@
 * ``merge_scores_to_regions`` -- the consumer (region.rs:737-890): one region
   per seed from its left/right scores and the pen_clip5 / pen_clip3 clipping
   decisions, seed coverage, and the chain's first highest-scoring region.
-* ``extend_chains`` -- both, with the two extension batches on the GPU and the
-  targets cut on the device from the resident .pac.  The right jobs start from
-  the seed's left-extended score, as BWA-MEM2's mem_chain2aln chains them: the
+* ``extend_chains`` / ``DeviceChains`` -- the whole path on the GPU: the
+  producer, the packing of queries and of targets cut from the resident .pac,
+  the left and the right extension batch and the merge (host buffers in and
+  out, or inputs and regions resident in HBM).  The right jobs start from the
+  seed's left-extended score, as BWA-MEM2's mem_chain2aln chains them: the
   reference gives them h0 = 0 (collect_soa.rs:206), outside ksw_extend2's
   h0 > 0 domain (ksw_affine_gap.rs:968), and adds the two scores instead.
 
@@ -135,29 +137,76 @@ def merge_scores_to_regions(opt: MemOpt, cb: ChainBatch, m: Mappings, left_score
     return regions[:cb.nchains]
 
 
-CHAIN_STAGES = ("producer", "left_upload", "left_run", "left_download", "right_upload", "right_run",
-                "right_download", "merge")
+CHAIN_STAGES = ("upload", "producer", "left_prep", "left_run", "right_prep", "right_run", "merge", "download")
+
+
+def _chain_args(query_seqs, query_offset, cb: ChainBatch):
+    q = np.ascontiguousarray(query_seqs, np.uint8)
+    qo = np.ascontiguousarray(query_offset, np.int64)
+    keep = (q, qo, cb)
+    args = [cb.nreads, N.ptr(q), q.size, N.ptr(qo), N.ptr(cb.query_len), N.ptr(cb.read_chain_start),
+            N.ptr(cb.read_chain_count), cb.nchains, N.ptr(cb.chain_kept), N.ptr(cb.chain_frac_rep),
+            N.ptr(cb.chain_seed_start), N.ptr(cb.chain_seed_count), len(cb.chain_seeds), N.ptr(cb.chain_seeds),
+            len(cb.seeds), N.ptr(cb.seeds)]
+    return args, keep
 
 
 def extend_chains(ref: DeviceReference, sw: BandedPairWiseSW, opt: MemOpt, query_seqs, query_offset,
                   cb: ChainBatch, ctx=None, stage_ms=None):
-    """Producer, both extension batches on the GPU, consumer: (regions,
-    number of left jobs, number of right jobs).  stage_ms: an optional dict
-    that receives the host wall time of each stage (CHAIN_STAGES) in ms."""
+    """Producer, both extension batches and the merge on the GPU over host
+    buffers (fa_extend_chains): (regions, number of left jobs, number of
+    right jobs).  stage_ms: an optional dict that receives the time of each
+    stage (CHAIN_STAGES) in ms."""
     ctx = ctx or ref.ctx or default_context()
     cb = cb.normalised()
-    q = np.ascontiguousarray(query_seqs, np.uint8)
-    qo = np.ascontiguousarray(query_offset, np.int64)
+    args, keep = _chain_args(query_seqs, query_offset, cb)
     regions = np.zeros(cb.nchains + 1, N.REGION_DTYPE)
     nl, nr = ctypes.c_int64(), ctypes.c_int64()
     o = opt.to_c()
     c = sw.to_c()
     st = np.zeros(len(CHAIN_STAGES), np.float64)
-    N.check(N.lib().fa_extend_chains(
-        ctx.handle, ref.handle, ctypes.byref(c), N.ptr(o), cb.nreads, N.ptr(q), N.ptr(qo), N.ptr(cb.query_len),
-        N.ptr(cb.read_chain_start), N.ptr(cb.read_chain_count), cb.nchains, N.ptr(cb.chain_kept),
-        N.ptr(cb.chain_frac_rep), N.ptr(cb.chain_seed_start), N.ptr(cb.chain_seed_count), N.ptr(cb.chain_seeds),
-        N.ptr(cb.seeds), N.ptr(regions), ctypes.byref(nl), ctypes.byref(nr), N.ptr(st)), "fa_extend_chains")
+    N.check(N.lib().fa_extend_chains(ctx.handle, ref.handle, ctypes.byref(c), N.ptr(o), *args, N.ptr(regions),
+                                     ctypes.byref(nl), ctypes.byref(nr), N.ptr(st)), "fa_extend_chains")
     if stage_ms is not None:
         stage_ms.update(zip(CHAIN_STAGES, map(float, st)))
     return regions[:cb.nchains], nl.value, nr.value
+
+
+class DeviceChains:
+    """Reads, chains and seeds resident in HBM (fa_chains_upload); extend()
+    runs the whole path on the device, download() returns the regions."""
+
+    def __init__(self, query_seqs, query_offset, cb: ChainBatch, ctx=None):
+        self.ctx = ctx or default_context()
+        self.cb = cb.normalised()
+        args, keep = _chain_args(query_seqs, query_offset, self.cb)
+        self._h = ctypes.c_void_p()
+        N.check(N.lib().fa_chains_upload(self.ctx.handle, *args, ctypes.byref(self._h)), "fa_chains_upload")
+
+    def extend(self, ref: DeviceReference, sw: BandedPairWiseSW, opt: MemOpt, stage_ms=None):
+        """(left jobs, right jobs); stage_ms as in extend_chains (device stages)."""
+        nl, nr = ctypes.c_int64(), ctypes.c_int64()
+        st = np.zeros(len(CHAIN_STAGES), np.float64)
+        c = sw.to_c()
+        o = opt.to_c()
+        N.check(N.lib().fa_chains_extend(self.ctx.handle, self._h, ref.handle, ctypes.byref(c), N.ptr(o),
+                                         ctypes.byref(nl), ctypes.byref(nr), N.ptr(st)), "fa_chains_extend")
+        if stage_ms is not None:
+            stage_ms.update(zip(CHAIN_STAGES, map(float, st)))
+        return nl.value, nr.value
+
+    def download(self):
+        regions = np.zeros(self.cb.nchains + 1, N.REGION_DTYPE)
+        N.check(N.lib().fa_chains_download(self.ctx.handle, self._h, N.ptr(regions)), "fa_chains_download")
+        return regions[:self.cb.nchains]
+
+    def free(self):
+        if self._h:
+            N.lib().fa_chains_free(self.ctx.handle, self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

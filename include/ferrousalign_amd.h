@@ -363,7 +363,8 @@ typedef struct fa_region {    /* AlignmentRegion boundaries and scores (region.r
  * (jobs per direction and seed mappings).  Per chain c: mappings
  * [map_start[c], +map_count[c]) give the seed and its left/right job index
  * (-1 = none); rmax[2c], rmax[2c+1] the window (-1 when the chain made no
- * jobs).  Runs on the host only. */
+ * jobs).  FA_ERR_ARG for a seed outside its read (query_pos < 0, len < 0 or
+ * query_pos + len > query_len).  Runs on the host only. */
 int fa_collect_extension_jobs(const fa_mem_opt *opt, int64_t l_pac, int64_t nreads,
                               const int32_t *query_len, const int64_t *read_chain_start,
                               const int32_t *read_chain_count, const uint8_t *chain_kept,
@@ -392,26 +393,57 @@ int fa_merge_scores_to_regions(const fa_mem_opt *opt, int64_t nreads, const int3
                                const int64_t *rmax, const fa_out_score *left_scores,
                                const fa_out_score *right_scores, int32_t right_from_left,
                                fa_region *regions);
-/* The producer, both extension batches on the device with targets cut from
- * the resident .pac (fa_ref_upload), and the consumer: the left batch, then
- * the right batch with h0 = the seed's left-extended score (or seed_len * a
- * without a left job), as BWA-MEM2's mem_chain2aln chains them (the
- * reference's h0 = 0 right jobs are outside ksw_extend2's h0 > 0 domain,
- * ksw_affine_gap.rs:968).  sw supplies the scoring and end_bonus of both
- * batches.  regions has one entry per chain; the job counts go to
- * n_left / n_right when non-NULL.  stage_ms (NULL or FA_CHAIN_STAGES
- * entries) receives host wall times in ms: producer, left batch
- * (query gather + upload + target cut), left run, left download, right
- * batch, right run, right download, merge. */
-#define FA_CHAIN_STAGES 8
-int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw *sw,
-                     const fa_mem_opt *opt, int64_t nreads, const uint8_t *query_seqs,
+/* ---- chain extension on the device --------------------------------------
+ * The producer (as fa_collect_extension_jobs), both extension batches and the
+ * merge (as fa_merge_scores_to_regions) on the GPU, with targets cut from the
+ * resident .pac (fa_ref_upload).  The right jobs start from the seed's
+ * left-extended score (or seed_len * a without a left job), as BWA-MEM2's
+ * mem_chain2aln chains them: the reference's h0 = 0 right jobs are outside
+ * ksw_extend2's h0 > 0 domain (ksw_affine_gap.rs:968), and region score and
+ * clipping then use the right score alone (right_from_left = 1).  sw supplies
+ * the scoring and end_bonus of both batches.  Replaces the producer / batch /
+ * consumer sequence of collect_soa.rs:28-228, dispatch.rs:31-268 and
+ * region.rs:737-890 for one chain batch.
+ *
+ * Every chain owned by a read gets one region (seed_idx -1: none); chains in
+ * no read's range get the empty region.  Errors: FA_ERR_ARG for a chain in
+ * two reads' ranges, a range or index outside its array, or a seed outside
+ * its read; FA_ERR_SCORING for opt or (when there is a job) sw; otherwise the
+ * error of the first invalid job in the reference's job order (left batch
+ * first), as fa_batch_upload_ref would report it. */
+typedef struct fa_chain_batch fa_chain_batch;
+/* Device copy of reads (query_bytes codes; read r = [query_offset[r],
+ * +query_len[r])), chains and seeds (n_chain_seeds indices into nseeds
+ * seeds); chain_frac_rep may be NULL (frac_rep 0). */
+int fa_chains_upload(fa_ctx *ctx, int64_t nreads, const uint8_t *query_seqs, int64_t query_bytes,
                      const int64_t *query_offset, const int32_t *query_len,
                      const int64_t *read_chain_start, const int32_t *read_chain_count,
                      int64_t nchains, const uint8_t *chain_kept, const float *chain_frac_rep,
                      const int64_t *chain_seed_start, const int32_t *chain_seed_count,
-                     const int64_t *chain_seeds, const fa_seed *seeds, fa_region *regions,
-                     int64_t *n_left, int64_t *n_right, double *stage_ms);
+                     int64_t n_chain_seeds, const int64_t *chain_seeds, int64_t nseeds,
+                     const fa_seed *seeds, fa_chain_batch **out);
+/* Runs the path on an uploaded batch; the regions stay on the device until
+ * fa_chains_download.  stage_ms: NULL, or FA_CHAIN_STAGES entries of which
+ * [1..6] receive the device time in ms of producer, left preparation, left
+ * batch, right preparation, right batch, merge. */
+#define FA_CHAIN_STAGES 8
+int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
+                     const fa_banded_pairwise_sw *sw, const fa_mem_opt *opt, int64_t *n_left,
+                     int64_t *n_right, double *stage_ms);
+/* One fa_region per chain. */
+int fa_chains_download(fa_ctx *ctx, fa_chain_batch *cb, fa_region *regions);
+int fa_chains_free(fa_ctx *ctx, fa_chain_batch *cb);
+/* Upload, extend, download over host buffers; stage_ms[0] and [7] receive the
+ * host wall time of the upload and the download. */
+int fa_extend_chains(fa_ctx *ctx, const fa_ref *ref, const fa_banded_pairwise_sw *sw,
+                     const fa_mem_opt *opt, int64_t nreads, const uint8_t *query_seqs,
+                     int64_t query_bytes, const int64_t *query_offset, const int32_t *query_len,
+                     const int64_t *read_chain_start, const int32_t *read_chain_count,
+                     int64_t nchains, const uint8_t *chain_kept, const float *chain_frac_rep,
+                     const int64_t *chain_seed_start, const int32_t *chain_seed_count,
+                     int64_t n_chain_seeds, const int64_t *chain_seeds, int64_t nseeds,
+                     const fa_seed *seeds, fa_region *regions, int64_t *n_left, int64_t *n_right,
+                     double *stage_ms);
 
 #ifdef __cplusplus
 }

@@ -71,10 +71,10 @@ struct fa_chain_batch {
 namespace {
 
 struct SideBufs {
-    DBuf qbeg, qlen, tlen, h0, w, read, seg, okey, err, words, wend, qo, to, seq2, seqn, key, slot,
+    DBuf qlen, tlen, h0, w, qsrc, seg, okey, err, words, wend, qo, to, seq2, seqn, key, slot,
         key_out, order, out;
     void release_all() {
-        for (DBuf *b : {&qbeg, &qlen, &tlen, &h0, &w, &read, &seg, &okey, &err, &words, &wend, &qo,
+        for (DBuf *b : {&qlen, &tlen, &h0, &w, &qsrc, &seg, &okey, &err, &words, &wend, &qo,
                         &to, &seq2, &seqn, &key, &slot, &key_out, &order, &out})
             release(*b);
     }
@@ -105,12 +105,11 @@ int workspace(fa_ctx *ctx, ChainWork **out) {
 
 fa::ChainSide side_view(SideBufs &b) {
     fa::ChainSide s;
-    s.qbeg = P<int32_t>(b.qbeg);
     s.qlen = P<int32_t>(b.qlen);
     s.tlen = P<int32_t>(b.tlen);
     s.h0 = P<int32_t>(b.h0);
     s.w = P<int32_t>(b.w);
-    s.read = P<int32_t>(b.read);
+    s.qsrc = P<int64_t>(b.qsrc);
     s.seg = P<int64_t>(b.seg);
     s.okey = P<uint64_t>(b.okey);
     s.err = P<uint8_t>(b.err);
@@ -128,8 +127,8 @@ fa::ChainSide side_view(SideBufs &b) {
 
 int grow_side(SideBufs &b, size_t n) {
     int rc;
-    if ((rc = grow(b.qbeg, 4 * n)) || (rc = grow(b.qlen, 4 * n)) || (rc = grow(b.tlen, 4 * n)) ||
-        (rc = grow(b.h0, 4 * n)) || (rc = grow(b.w, 4 * n)) || (rc = grow(b.read, 4 * n)) ||
+    if ((rc = grow(b.qlen, 4 * n)) || (rc = grow(b.tlen, 4 * n)) ||
+        (rc = grow(b.h0, 4 * n)) || (rc = grow(b.w, 4 * n)) || (rc = grow(b.qsrc, 8 * n)) ||
         (rc = grow(b.seg, 8 * n)) || (rc = grow(b.okey, 8 * n)) || (rc = grow(b.err, n)) ||
         (rc = grow(b.words, 8 * n)) || (rc = grow(b.wend, 8 * n)) || (rc = grow(b.qo, 4 * n)) ||
         (rc = grow(b.to, 4 * n)) || (rc = grow(b.key, 4 * n)) || (rc = grow(b.slot, 4 * n)) ||
@@ -174,8 +173,12 @@ int upload_into(fa_ctx *ctx, fa_chain_batch *cb, int64_t nreads, const uint8_t *
     FA_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     int rc;
-    if ((rc = put(cb->qseq, query_seqs, query_bytes, st)) ||
-        (rc = put(cb->qoff, query_offset, nreads, st)) || (rc = put(cb->qlen, query_len, nreads, st)) ||
+    // reads with 64 bytes of padding on both sides (word-wide loads of the job windows)
+    if ((rc = grow(cb->qseq, (size_t)query_bytes + 128))) return rc;
+    if (query_bytes > 0)
+        FA_HIP(hipMemcpyAsync((uint8_t *)cb->qseq.p + 64, query_seqs, (size_t)query_bytes,
+                              hipMemcpyHostToDevice, st));
+    if ((rc = put(cb->qoff, query_offset, nreads, st)) || (rc = put(cb->qlen, query_len, nreads, st)) ||
         (rc = put(cb->rcs, read_chain_start, nreads, st)) ||
         (rc = put(cb->rcc, read_chain_count, nreads, st)) ||
         (rc = put(cb->kept, chain_kept, nchains, st)) ||
@@ -357,7 +360,7 @@ int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
     }
 
     fa::ChainIn in;
-    in.qseq = P<uint8_t>(cb->qseq);
+    in.qseq = P<uint8_t>(cb->qseq) + 64;
     in.qbytes = cb->qbytes;
     in.qoff = P<int64_t>(cb->qoff);
     in.qlen = P<int32_t>(cb->qlen);
@@ -449,8 +452,7 @@ int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
 
     // left batch: sequences, keys, sort, extension
     FA_HIP(hipEventRecord(ev[1], st));
-    FA_HIP(fa::chain_fill(in, ref->d_pac, ref->l_pac, L, 0, N, dst, st));
-    FA_HIP(fa::chain_fill(in, ref->d_pac, ref->l_pac, R, 1, N, dst, st));
+    FA_HIP(fa::chain_fill(in, ref->d_pac, ref->l_pac, L, R, N, dst, st));
     FA_HIP(fa::chain_keys(o, rk, L, L, 0, N, dst, st));
     if (N > 0) {
         size_t tb = ws->temp.bytes;

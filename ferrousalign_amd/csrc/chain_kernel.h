@@ -17,7 +17,7 @@ constexpr int kChainRanks = 32;  // distinct kernel choices a sort key can name
 // exclusive prefix of chain_seed_count over chains (slot_base, nchains + 1
 // entries): chain c's seed k owns job slot slot_base[c] + k on each side.
 struct ChainIn {
-    const uint8_t *qseq;
+    const uint8_t *qseq;  // readable from qseq - 64 to qseq + qbytes + 64 (padding)
     int64_t qbytes;
     const int64_t *qoff;
     const int32_t *qlen;
@@ -53,7 +53,8 @@ struct ChainRanks {
 
 // One side (left or right) of the job slots.  qlen[s] == 0: no job in slot s.
 struct ChainSide {
-    int32_t *qbeg, *qlen, *tlen, *h0, *w, *read;
+    int32_t *qlen, *tlen, *h0, *w;
+    int64_t *qsrc;        // offset in qseq of the job's query (a left job reads it backwards)
     int64_t *seg;
     uint64_t *okey;       // (read << 33) | (position of the job in the read's job order << 4)
     uint8_t *err;         // right side: first validation failure known before h0 (negated code)
@@ -80,9 +81,10 @@ hipError_t chain_owner(const ChainIn &in, int32_t *chain_read, ChainStatus *st, 
 hipError_t chain_jobs(const ChainIn &in, const ChainOpt &o, const int32_t *chain_read,
                       uint8_t *chain_state, ChainSide L, ChainSide R, ChainStatus *st,
                       hipStream_t s);
-// qbeg/qlen/... of side `sd` (0 left: queries read backwards from qbeg + qlen - 1,
-// targets backwards; 1 right: forwards) into seq2/seqn at the scanned offsets.
-hipError_t chain_fill(const ChainIn &in, const uint8_t *pac, int64_t l_pac, ChainSide S, int sd,
+// Queries and targets of both sides (left: query and target read backwards;
+// right: forwards) into seq2/seqn at the scanned offsets.  pac must be
+// readable 8 bytes past its last byte (fa_ref_upload pads it).
+hipError_t chain_fill(const ChainIn &in, const uint8_t *pac, int64_t l_pac, ChainSide L, ChainSide R,
                       int64_t nslots, ChainStatus *st, hipStream_t s);
 // Sort keys and rank counts of side `sd`; for the right side (sd == 1) first
 // sets h0 from the left scores (or seed_len * a) and finishes validation.

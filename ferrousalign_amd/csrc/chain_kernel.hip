@@ -9,10 +9,11 @@
 //   chain_jobs   one thread per chain: the reference window over the chain's
 //                seeds and, per seed, the left and the right job
 //                (collect_soa.rs:28-228; cal_max_gap chaining.rs:1108-1116)
-//   chain_fill   16 lanes per job: the query (read backwards for a left job)
-//                and the target (.pac segment, backwards for a left job,
-//                bntseq.rs:451-522) packed 16 bases per word, as the extension
-//                kernels read them; query base codes checked
+//   chain_fill   16 lanes per job, both sides in one launch: the query
+//                (read backwards for a left job) and the target (.pac
+//                segment, backwards for a left job, bntseq.rs:451-522) packed
+//                16 bases per word, as the extension kernels read them; query
+//                base codes checked
 //   chain_keys   one thread per job: right jobs' h0 from the left score, the
 //                rest of the validation, the length-packing sort key and the
 //                per-kernel job counts
@@ -194,12 +195,11 @@ __global__ __launch_bounds__(256) void chain_jobs_kernel(ChainIn in, ChainOpt o,
                     atomicMin((unsigned long long *)&st->left_key, okey | (uint64_t)(-e));
                 } else {
                     lq = s.query_pos;
-                    L.qbeg[slot] = 0;
+                    L.qsrc[slot] = in.qoff[r];
                     L.tlen[slot] = tl;
                     L.seg[slot] = (int64_t)r0;
                     L.h0[slot] = h0;
                     L.w[slot] = capped_w(o, lq, o.w);
-                    L.read[slot] = r;
                 }
             }
         }
@@ -220,12 +220,11 @@ __global__ __launch_bounds__(256) void chain_jobs_kernel(ChainIn in, ChainOpt o,
                 else if (o.w < 0) e = FA_ERR_BAND;
                 rerr = (uint8_t)(-e);
                 R.okey[slot] = okey;
-                R.qbeg[slot] = qend;
+                R.qsrc[slot] = in.qoff[r] + qend;
                 R.tlen[slot] = tl;
                 R.seg[slot] = (int64_t)r0 + tmp;
                 R.h0[slot] = s.len * o.a;
                 R.w[slot] = capped_w(o, rq, o.w);
-                R.read[slot] = r;
             }
         }
         R.qlen[slot] = rq;
@@ -237,15 +236,36 @@ __global__ __launch_bounds__(256) void chain_jobs_kernel(ChainIn in, ChainOpt o,
     if (had_job && (int)__lane_id() == __ffsll((unsigned long long)m) - 1) atomicOr(&st->any_job, 1u);
 }
 
+// 16 lanes per job, lane w builds 2-bit words w, w + 16, ... of the query,
+// then of the target.  A query word takes its 16 codes from five aligned
+// 32-bit loads funnelled to the word's first byte (the codes of a left job
+// sit backwards in the 16 bytes that end at its first base); a target word
+// takes its 16 bases from two aligned 32-bit loads of the .pac (16 bases span
+// at most 5 bytes).
+__device__ __forceinline__ void window16(const uint8_t *base, int64_t at, uint32_t b[4]) {
+    const int64_t w0 = at >> 2;
+    const uint32_t sh = (uint32_t)(at & 3);
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(base) + w0;
+    const uint32_t x0 = p[0], x1 = p[1], x2 = p[2], x3 = p[3], x4 = p[4];
+    b[0] = __builtin_amdgcn_alignbyte(x1, x0, sh);
+    b[1] = __builtin_amdgcn_alignbyte(x2, x1, sh);
+    b[2] = __builtin_amdgcn_alignbyte(x3, x2, sh);
+    b[3] = __builtin_amdgcn_alignbyte(x4, x3, sh);
+}
+
 __global__ __launch_bounds__(256) void chain_fill_kernel(ChainIn in, const uint8_t *__restrict__ pac,
-                                                         int64_t l_pac, ChainSide S, int sd,
+                                                         int64_t l_pac, ChainSide L, ChainSide R,
                                                          int64_t nslots, ChainStatus *st) {
-    const int64_t slot = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int64_t gs = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int lane = (int)(threadIdx.x & 15);
-    if (slot >= nslots) return;
+    if (gs >= 2 * nslots) return;
+    const int sd = gs >= nslots;  // uniform over the slot's 16 lanes
+    const int64_t slot = sd ? gs - nslots : gs;
+    const ChainSide &S = sd ? R : L;
     const uint64_t nw = S.words[slot];
-    if (nw == 0) return;  // uniform over the slot's 16 lanes
+    if (nw == 0) return;
     const int32_t ql = S.qlen[slot], tl = S.tlen[slot];
+    const int64_t qsrc = S.qsrc[slot], seg = S.seg[slot];
     const uint32_t qo = (uint32_t)(S.wend[slot] - nw);
     const uint32_t nqw = seq_words(ql), ntw = seq_words(tl);
     const uint32_t to = qo + nqw;
@@ -253,39 +273,65 @@ __global__ __launch_bounds__(256) void chain_fill_kernel(ChainIn in, const uint8
         S.qo[slot] = qo;
         S.to[slot] = to;
     }
-    const uint8_t *q = in.qseq + in.qoff[S.read[slot]] + S.qbeg[slot];
+    // the 64-byte padding before qseq keeps a left job's first window in the allocation
+    const uint8_t *qbase = in.qseq - 64;
     bool bad = false;
     for (uint32_t b0 = 0; b0 < nqw; b0 += 16) {
         const uint32_t w = b0 + (uint32_t)lane;
         uint32_t word = 0, nb = 0;
         if (w < nqw) {
-            for (int j = 0; j < 16; ++j) {
-                const int32_t p = (int32_t)w * 16 + j;
-                if (p >= ql) break;
-                uint32_t c = sd == 0 ? q[ql - 1 - p] : q[p];
-                if (c > 4) bad = true;
-                if (c >= 4) {
-                    nb |= 1u << j;
-                    c = 0;
+            const int32_t p0 = (int32_t)w * 16;
+            const int n = ql - p0 < 16 ? ql - p0 : 16;  // <= 0 past the query (padding words)
+            if (n > 0) {
+                uint32_t b[4];
+                window16(qbase, 64 + (sd == 0 ? qsrc + ql - p0 - 16 : qsrc + p0), b);
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const int x = sd == 0 ? 15 - t : t;
+                    uint32_t c = (b[x >> 2] >> (8 * (x & 3))) & 0xFFu;
+                    if (t < n) {
+                        if (c > 4) bad = true;
+                        if (c >= 4) {
+                            nb |= 1u << t;
+                            c = 0;
+                        }
+                        word |= (c & 3u) << (2 * t);
+                    }
                 }
-                word |= (c & 3u) << (2 * j);
             }
             S.seq2[qo + w] = word;
         }
         const uint32_t nb_hi = __shfl_down(nb, 1, 16);
         if (w < nqw && (w & 1u) == 0) S.seqn[qo / 2 + w / 2] = nb | (nb_hi << 16);
     }
-    const int64_t seg = S.seg[slot], two = l_pac << 1;
+    // target base k is bidirectional position seg + k (right) or seg + tl - 1 - k
+    // (left); on the reverse strand it is the complement of forward base
+    // 2 l_pac - 1 - p.  Forward positions f of one word are consecutive.
+    const bool rstrand = seg >= l_pac;
+    const int fdir = (sd == 0) != rstrand ? -1 : 1;
+    const uint32_t *pac32 = reinterpret_cast<const uint32_t *>(pac);
     for (uint32_t b0 = 0; b0 < ntw; b0 += 16) {
         const uint32_t w = b0 + (uint32_t)lane;
         if (w >= ntw) break;
         uint32_t word = 0;
-        for (int j = 0; j < 16; ++j) {
-            const int32_t k = (int32_t)w * 16 + j;
-            if (k >= tl) break;
-            const int64_t p = sd == 0 ? seg + tl - 1 - k : seg + k;
-            const uint32_t c = p < l_pac ? pac_base(pac, p) : 3u - pac_base(pac, two - 1 - p);
-            word |= c << (2 * j);
+        const int32_t k0 = (int32_t)w * 16;
+        const int n = tl - k0 < 16 ? tl - k0 : 16;
+        if (n > 0) {
+            const int64_t p0 = sd == 0 ? seg + tl - 1 - k0 : seg + k0;
+            const int64_t f0 = rstrand ? 2 * l_pac - 1 - p0 : p0;
+            const int64_t fmin = fdir > 0 ? f0 : f0 - (n - 1);
+            const int64_t wi = fmin >> 4;  // 32-bit word holding the byte of fmin
+            const uint64_t v = (uint64_t)pac32[wi] | ((uint64_t)pac32[wi + 1] << 32);
+            const int32_t g0 = (int32_t)(f0 - (wi << 4));  // base offset inside the 8 bytes
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                if (t < n) {
+                    const int32_t g = g0 + fdir * t;
+                    uint32_t c = (uint32_t)(v >> (8 * (g >> 2) + 6 - 2 * (g & 3))) & 3u;
+                    if (rstrand) c = 3u - c;
+                    word |= c << (2 * t);
+                }
+            }
         }
         S.seq2[to + w] = word;
         if ((w & 1u) == 0) S.seqn[to / 2 + w / 2] = 0;
@@ -458,11 +504,11 @@ hipError_t chain_jobs(const ChainIn &in, const ChainOpt &o, const int32_t *chain
     return hipGetLastError();
 }
 
-hipError_t chain_fill(const ChainIn &in, const uint8_t *pac, int64_t l_pac, ChainSide S, int sd,
+hipError_t chain_fill(const ChainIn &in, const uint8_t *pac, int64_t l_pac, ChainSide L, ChainSide R,
                       int64_t nslots, ChainStatus *st, hipStream_t s) {
     if (nslots == 0) return hipSuccess;
-    hipLaunchKernelGGL(chain_fill_kernel, dim3(grid(nslots, 16)), dim3(256), 0, s, in, pac, l_pac,
-                       S, sd, nslots, st);
+    hipLaunchKernelGGL(chain_fill_kernel, dim3(grid(2 * nslots, 16)), dim3(256), 0, s, in, pac, l_pac,
+                       L, R, nslots, st);
     return hipGetLastError();
 }
 

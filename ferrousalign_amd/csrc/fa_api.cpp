@@ -759,7 +759,11 @@ int fa_ref_upload(fa_ctx *ctx, const uint8_t *pac, int64_t l_pac, fa_ref **out) 
     r->device = ctx->device;
     r->l_pac = l_pac;
     const size_t bytes = (size_t)((l_pac + 3) / 4);
-    int rc = dev_upload(&r->d_pac, pac, bytes, ctx->stream);
+    // 16 zero bytes after the last base: kernels read the .pac in aligned words
+    int rc = dev_alloc(&r->d_pac, bytes + 16);
+    if (!rc && (hipMemcpyAsync(r->d_pac, pac, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+                hipMemsetAsync(r->d_pac + bytes, 0, 16, ctx->stream) != hipSuccess))
+        rc = FA_ERR_DEVICE;
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FA_ERR_DEVICE;
     if (rc) {
         if (r->d_pac) (void)hipFree(r->d_pac);

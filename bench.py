@@ -50,6 +50,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_reference_simd(ps, sw, cells):
+    """The reference's own CPU engine restated (oracle/refsimd_baseline.c: its
+    SIMD batch path, dispatch.rs:95-268 with kernel.rs / kernel_i16.rs, every
+    chunk scored, all host threads, AVX-512BW when present else AVX2) on the
+    whole workload.  Its results are not ksw_extend2's (0-based ends, no
+    gscore, rows and columns capped at 128), so GCUPS uses the oracle's cell
+    count of the same pairs, as the GPU's does."""
+    from tests import _oracle as O
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    r = O.refsimd_batch(sw, *ps.flat(), isa="auto", nthreads=threads)
+    dt = time.perf_counter() - t0
+    if r is None:
+        return None
+    return {"value": round(ps.n / dt, 1), "unit": "pairs/s", "gcups": round(float(cells.sum()) / dt / 1e9, 3),
+            "cores": threads, "kind": "port", "isa": r[1], "kernel": "i16" if r[2] else "i8",
+            "sample": "all %d pairs through the restated SIMD batch path of the reference (%s, %s kernel, "
+                      "%d host threads): %.2f s" % (ps.n, r[1], "i16" if r[2] else "i8", threads, dt)}
+
+
 def cpu_baseline(ps, sw, budget_s=15.0):
     """The CPU oracle (C restatement of ksw_extend2, all host threads) on a
     bounded sample of the same workload."""
@@ -414,6 +434,7 @@ def main():
                "roofline": roofline(db, flags, cells, ctx)}
         if not args.no_cpu:
             res["cpu_baseline"] = cpu_baseline(ps, sw)
+            res["cpu_baseline_reference_simd"] = cpu_reference_simd(ps, sw, cells)
         print(json.dumps(res), flush=True)
         return
 
@@ -539,6 +560,7 @@ def main():
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(ps, sw)
+        res["cpu_baseline_reference_simd"] = cpu_reference_simd(ps, sw, cells)
     print(json.dumps(res), flush=True)
 
 

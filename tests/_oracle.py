@@ -219,3 +219,30 @@ def scalar_banded_swa_batch(sw, qseq, qoff, qlen, tseq, toff, tlen, w, h0, nthre
     f(n, *[P(a) for a in arrs], P(mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins, sw.zdrop, sw.pen_clip3,
       P(out), P(cells), P(nc), int(nthreads))
     return to_outscore_order(out), cells, nc
+
+
+REFSIMD_ISA = {"auto": 0, "avx2": 1, "avx512": 2, "scalar512": 3, "scalar2": 4}
+
+
+def refsimd_batch(sw, qseq, qoff, qlen, tseq, toff, tlen, h0, w, isa="auto", nthreads=1):
+    """The restatement of the reference's SIMD batch path (oracle/refsimd_baseline.c):
+    (out[n,6] in OutScore order, isa run, whether the i16 kernel ran); None when
+    the requested instruction set is missing."""
+    l = lib()
+    if not hasattr(l, "_rs"):
+        l.ora_refsimd_batch.argtypes = [_I64] + [_P] * 8 + [_I32] * 5 + [_P, _I32, _I32, _P, _P]
+        l.ora_refsimd_batch.restype = ctypes.c_int
+        l._rs = True
+    n = len(qlen)
+    out = np.zeros((max(n, 1), 6), np.int32)
+    info = np.zeros(2, np.int32)
+    arrs = [np.ascontiguousarray(x, dt) for x, dt in ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (tseq, np.uint8), (toff, np.int64), (tlen, np.int32),
+                                                      (h0, np.int32), (w, np.int32))]
+    mat = np.ascontiguousarray(sw.mat, np.int8)
+    rc = l.ora_refsimd_batch(n, *[P(a) for a in arrs], sw.o_del, sw.e_del, sw.o_ins, sw.e_ins, sw.zdrop, P(mat),
+                             REFSIMD_ISA[isa], int(nthreads), P(out), P(info))
+    if rc != 0:
+        return None
+    names = {v: k for k, v in REFSIMD_ISA.items()}
+    return out[:n], names[int(info[0])], bool(info[1])

@@ -355,7 +355,7 @@ int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
                                                 (int)std::max<int64_t>(N, 1), st));
         FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (uint32_t *)nullptr, (uint32_t *)nullptr,
                                                   (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                                  (int)std::max<int64_t>(N, 1), 0, 25, st));
+                                                  (int)std::max<int64_t>(N, 1), 0, 26, st));
         if ((rc = grow(ws->temp, std::max(a, b)))) return rc;
     }
 
@@ -403,18 +403,18 @@ int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
     fa::ChainRanks rk{};
     std::vector<int> rank_skey;
     {
-        int sk[2][256] = {};
-        for (int f = 0; f < 2; ++f)
+        int sk[3][256] = {};
+        for (int f = 0; f < 3; ++f)
             for (int q = 1; q < 256; ++q) {
                 int kind = 0, lanes = 0, S = 0;
-                if (!fa_host::pair_kernel(0, false, f != 0, q, &kind, &lanes, &S)) return FA_ERR_QLEN;
+                if (!fa_host::pair_kernel(0, false, f, q, &kind, &lanes, &S)) return FA_ERR_QLEN;
                 sk[f][q] = (kind << 16) | (lanes << 8) | S;
                 rank_skey.push_back(sk[f][q]);
             }
         std::sort(rank_skey.begin(), rank_skey.end(), [](int x, int y) { return x > y; });
         rank_skey.erase(std::unique(rank_skey.begin(), rank_skey.end()), rank_skey.end());
         if ((int)rank_skey.size() > fa::kChainRanks) return FA_ERR_ARG;
-        for (int f = 0; f < 2; ++f)
+        for (int f = 0; f < 3; ++f)
             for (int q = 1; q < 256; ++q)
                 rk.r[f][q] = (uint8_t)(std::find(rank_skey.begin(), rank_skey.end(), sk[f][q]) -
                                        rank_skey.begin());
@@ -457,7 +457,7 @@ int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
     if (N > 0) {
         size_t tb = ws->temp.bytes;
         FA_HIP(hipcub::DeviceRadixSort::SortPairs(ws->temp.p, tb, L.key, P<uint32_t>(ws->side[0].key_out),
-                                                  L.slot, P<uint32_t>(ws->side[0].order), (int)N, 0, 25, st));
+                                                  L.slot, P<uint32_t>(ws->side[0].order), (int)N, 0, 26, st));
     }
     if ((rc = read_status(ctx, ws, false, N))) return rc;
     if (hs.left_key != ~0ull) return -(int)(hs.left_key & 15u);
@@ -472,7 +472,7 @@ int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,
     if (N > 0) {
         size_t tb = ws->temp.bytes;
         FA_HIP(hipcub::DeviceRadixSort::SortPairs(ws->temp.p, tb, R.key, P<uint32_t>(ws->side[1].key_out),
-                                                  R.slot, P<uint32_t>(ws->side[1].order), (int)N, 0, 25, st));
+                                                  R.slot, P<uint32_t>(ws->side[1].order), (int)N, 0, 26, st));
     }
     if ((rc = read_status(ctx, ws, false, N))) return rc;
     if (hs.right_key != ~0ull) return -(int)(hs.right_key & 15u);

@@ -11,7 +11,7 @@
 
 namespace fa {
 
-constexpr int kChainRanks = 32;  // distinct kernel choices a sort key can name
+constexpr int kChainRanks = 64;  // distinct kernel choices a sort key can name (6 bits)
 
 // Device inputs: reads, chains, seeds (the fa_extend_chains arrays) and the
 // exclusive prefix of chain_seed_count over chains (slot_base, nchains + 1
@@ -46,9 +46,10 @@ struct ChainOpt {
     int32_t sw16ok;   // gap costs allow the 16-bit kernel
 };
 
-// Kernel rank of a job: rank[fits16][qlen] (ranks order the kernels widest first).
+// Kernel rank of a job: rank[fits16 level][qlen] (levels as fa_host::fits16;
+// ranks order the kernels widest first).
 struct ChainRanks {
-    uint8_t r[2][256];
+    uint8_t r[3][256];
 };
 
 // One side (left or right) of the job slots.  qlen[s] == 0: no job in slot s.

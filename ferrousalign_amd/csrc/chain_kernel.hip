@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void chain_keys_kernel(ChainOpt o, ChainRanks 
         if (valid) {
             const int32_t tl = S.tlen[slot];
             const int64_t bound = (int64_t)h0 + (int64_t)ql * o.max_score;
-            const int fits = o.sw16ok && bound < 4096 && bound * o.k16 < 32768;
+            const int fits = (o.sw16ok && bound < 4096 && bound * o.k16 < 32768) ? (bound < 2048 ? 2 : 1) : 0;
             const uint32_t rank = rk.r[fits][ql];
             key = (rank << 20) | ((uint32_t)(255 - ql) << 12) | (uint32_t)(4095 - tl);
             atomicAdd(&cnt[rank], 1ull);

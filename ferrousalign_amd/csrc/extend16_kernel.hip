@@ -11,7 +11,8 @@
 // fits: with Hmax = h0 + qlen*max(mat), Hmax < 4096 and Hmax*K < 32768
 // (K = max(2, 1 + max(mat))), e_ins < 32 and o+e < 8192, so H, E, F, G stay
 // in (-32768, 16384) and the row-argmax key (H << 4) | slot fits 16 bits;
-// other pairs run the 32-bit kernel.
+// the wide slot counts (S = 20, 24) take only pairs with Hmax < 2048, whose
+// key (H << 5) | slot fits; other pairs run the 32-bit kernel.
 //
 // Differences from the 32-bit kernel that keep results identical:
 //  * M = H ? H + s : 0 becomes min(H + s, H*K): equal when H > 0, and <= 0
@@ -82,11 +83,12 @@ __device__ __forceinline__ uint32_t dbl_add(uint32_t a, uint32_t b) {
     asm("v_pk_mad_u16 %0, %1, 2, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
-// a * MUL + b per half, MUL an inline constant (0..64), b wave-uniform
+// a * MUL + b per half, MUL an inline constant (0..64), b wave-uniform (the
+// "n" constraint: the host pass checks "I" against x86, whose range is 0..31)
 template <int MUL>
 __device__ __forceinline__ uint32_t mad_imm(uint32_t a, uint32_t b) {
     uint32_t r;
-    asm("v_pk_mad_u16 %0, %1, %2, %3 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "I"(MUL), "s"(b));
+    asm("v_pk_mad_u16 %0, %1, %2, %3 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "n"(MUL), "s"(b));
     return r;
 }
 // a * m + b per half, m wave-uniform
@@ -277,7 +279,9 @@ template <int L, int S>
 __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, uint32_t nslots,
                                               uint8_t *lds_wave) {
     // lds_wave: tpad bytes of staged target codes per lane group
-    constexpr int KB = S <= 8 ? 3 : 4;  // row-argmax key per half: (H << KB) | (KM - s)
+    // row-argmax key per half: (H << KB) | (KM - s); with S > 16 the host
+    // routes only pairs whose scores stay below 2048 here, so the key fits
+    constexpr int KB = S <= 8 ? 3 : (S <= 16 ? 4 : 5);
     constexpr uint32_t KM = (1u << KB) - 1u;
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane / L;
@@ -418,7 +422,9 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         const uint32_t carry = sub16(excl_scan_u2<L>(add16(run, cbeb), l), cbeb);
 
         // pass 2: H, E, row-max key, write-back, non-zero columns
-        uint32_t pkey = 0, acc = 0, Hprev = 0;
+        // acc: one bit per slot and half; with S > 16 the first S - 16 slots
+        // go to accH (the high bits of a 32-bit mask per half)
+        uint32_t pkey = 0, acc = 0, accH = 0, Hprev = 0;
         uint32_t mle_rb0 = 0, mle_re0 = 0, wr0 = 0;
         uint32_t mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
         uint32_t mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
@@ -444,7 +450,10 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
                 const uint32_t hv = bfi1(mle_rb, h1ip, Hprev);  // eh[c].h <- H(i, c-1) or h1
                 Hh[s] = bfi1(mle_re, hv, Hh[s]);
                 const uint32_t x = (Hh[s] | Ee[s]) & wr;
-                acc = dbl_add(acc, nz01(x));
+                if (S > 16 && s < S - 16)
+                    accH = dbl_add(accH, nz01(x));
+                else
+                    acc = dbl_add(acc, nz01(x));
             }
             Hprev = H;
         }
@@ -453,7 +462,10 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
             const uint32_t hv = bfi1(mle_rb0, h1ip, hl);
             Hh[0] = bfi1(mle_re0, hv, Hh[0]);
             const uint32_t x = (Hh[0] | Ee[0]) & wr0;
-            acc = mad_s(nz01(x), (1u << (S - 1)) * 0x00010001u, acc);
+            if constexpr (S > 16)
+                accH = mad_s(nz01(x), (1u << (S - 17)) * 0x00010001u, accH);
+            else
+                acc = mad_s(nz01(x), (1u << (S - 1)) * 0x00010001u, acc);
         }
         // acc: bit S-1-s of each half set where column cb+s in [rb, re] has
         // eh.h or eh.e non-zero after the row.  First such column as 1023 - c,
@@ -462,9 +474,10 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         int kk[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const uint32_t bitsh = h ? (acc >> 16) : (acc & 0xFFFFu);
+            uint32_t bitsh = h ? (acc >> 16) : (acc & 0xFFFFu);
+            if constexpr (S > 16) bitsh |= (h ? (accH >> 16) : (accH & 0xFFFFu)) << 16;
             const int clz = __builtin_clz(bitsh | 1u);
-            const int ctz = __builtin_ctz(bitsh | 0x10000u);
+            const int ctz = __builtin_ctz(bitsh | (S > 16 ? 0x80000000u : 0x10000u));
             fenc[h] = bitsh ? (uint32_t)(1023 - (cbh[h] + S - 32 + clz)) : 0u;
             lenc[h] = bitsh ? (uint32_t)(cbh[h] + S - ctz) : 0u;
             const int key = (int)(h ? (pkey >> 16) : (pkey & 0xFFFFu));
@@ -583,12 +596,16 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
 
 #define FA16_LIST(X, L) \
     X(L, 1) X(L, 2) X(L, 3) X(L, 4) X(L, 6) X(L, 8) X(L, 10) X(L, 12) X(L, 14) X(L, 16)
+// wide slot counts (4 and 8 lanes): twice the pairs per wavefront for the same
+// query length, so the per-row bookkeeping is shared by twice as many pairs
+#define FA16_WIDE(X, L) X(L, 20) X(L, 24)
 
-int extend16_slot_count(int L, int qlen) {
-    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16};
+int extend16_slot_count(int L, int qlen, bool wide) {
+    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24};
     if (L != 4 && L != 8 && L != 16) return -1;
-    for (int s : sl)
-        if (L * s >= qlen + 1) return s;
+    const int n = wide && L <= 8 ? 12 : 10;
+    for (int k = 0; k < n; ++k)
+        if (L * sl[k] >= qlen + 1) return sl[k];
     return -1;
 }
 
@@ -601,6 +618,8 @@ hipError_t extend16_launch(int L, int S, const ExtArgs &a, uint32_t first, uint3
     FA16_LIST(FA_CASE, 16)
     FA16_LIST(FA_CASE, 8)
     FA16_LIST(FA_CASE, 4)
+    FA16_WIDE(FA_CASE, 8)
+    FA16_WIDE(FA_CASE, 4)
 #undef FA_CASE
     return hipErrorInvalidValue;
 }

@@ -58,8 +58,9 @@ hipError_t queue_reset(uint32_t *q, uint32_t n, hipStream_t st);
 hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st);
 
 // 16-bit packed kernel (two pairs per lane group): slot count for L lanes,
-// or -1, and its launcher (plain or persistent).
-int extend16_slot_count(int L, int qlen);
+// or -1 (wide: the pair may take the slot counts above 16, which need
+// Hmax < 2048), and its launcher (plain or persistent).
+int extend16_slot_count(int L, int qlen, bool wide);
 hipError_t extend16_launch(int L, int S, const ExtArgs &a, uint32_t first, uint32_t count,
                            bool persistent, uint32_t qidx, int cus, int blocks_per_cu,
                            hipStream_t st);

@@ -78,23 +78,24 @@ int check_scoring(const fa_banded_pairwise_sw *sw) {
     return FA_OK;
 }
 
-bool pair_kernel(int L, bool wide, bool elig16, int q, int *kind, int *lanes, int *S) {
+bool pair_kernel(int L, bool wide, int elig16, int q, int *kind, int *lanes, int *S) {
     // lanes per pair of a query length: the configured L when its slot
     // counts reach, else the next wider group; L == 0 picks the narrowest
     // group that holds the query (the fewest lanes per pair amortise the
     // per-row reductions over the most pairs per wavefront)
-    if (!wide && elig16) {
+    if (!wide && elig16 > 0) {
+        const bool wok = elig16 > 1;
         int l16 = 16;
         if (L == 0) {
             for (int c : {4, 8, 16})
-                if (fa::extend16_slot_count(c, q) > 0) {
+                if (fa::extend16_slot_count(c, q, wok) > 0) {
                     l16 = c;
                     break;
                 }
         } else {
-            l16 = fa::extend16_slot_count(L, q) > 0 ? L : 16;
+            l16 = fa::extend16_slot_count(L, q, wok) > 0 ? L : 16;
         }
-        const int s16 = fa::extend16_slot_count(l16, q);
+        const int s16 = fa::extend16_slot_count(l16, q, wok);
         if (s16 > 0) {
             *kind = 1;
             *lanes = l16;
@@ -110,13 +111,15 @@ bool pair_kernel(int L, bool wide, bool elig16, int q, int *kind, int *lanes, in
     return *S > 0;
 }
 
-bool fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen) {
+int fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen) {
     int matmax = 0;
     for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw.mat[x]);
     const int64_t bound = h0 + qlen * matmax;
     const int64_t K = std::max(2, 1 + matmax);
-    return bound < 4096 && bound * K < 32768 && sw.e_ins < 32 && sw.o_ins + sw.e_ins < 8192 &&
-           sw.o_del + sw.e_del < 8192;
+    if (!(bound < 4096 && bound * K < 32768 && sw.e_ins < 32 && sw.o_ins + sw.e_ins < 8192 &&
+          sw.o_del + sw.e_del < 8192))
+        return 0;
+    return bound < 2048 ? 2 : 1;
 }
 
 void set_scoring(fa::ExtArgs &a, const fa_banded_pairwise_sw &sw) {
@@ -272,19 +275,21 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
     // kernel of each pair: (kind, lanes, slots) packed in one sortable key
     std::vector<int> skey((size_t)n);
+    // merged buckets (one per kernel kind) hold pairs of any score range, so
+    // they take the narrow slot counts
     auto lanes16 = [&](int q) {
         int kind, lanes, S;
-        fa_host::pair_kernel(L, false, true, q, &kind, &lanes, &S);
+        fa_host::pair_kernel(L, false, 1, q, &kind, &lanes, &S);
         return lanes;
     };
     auto lanes32 = [&](int q) {
         int kind, lanes, S;
-        fa_host::pair_kernel(L, true, false, q, &kind, &lanes, &S);
+        fa_host::pair_kernel(L, true, 0, q, &kind, &lanes, &S);
         return lanes;
     };
     auto pair_key = [&](int64_t k, int &kind, int &lanes, int &S) -> bool {
-        return fa_host::pair_kernel(L, wide != 0, b->elig16[(size_t)k] != 0, b->qlen[(size_t)k], &kind,
-                                    &lanes, &S);
+        return fa_host::pair_kernel(L, wide != 0, b->elig16[(size_t)k], b->qlen[(size_t)k], &kind, &lanes,
+                                    &S);
     };
     for (int64_t k = 0; k < n; ++k) {
         int kind, lanes, S;
@@ -350,7 +355,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             int kind = bk.kind;
             if (kind == 1) {
                 bk.L = lanes16(maxq);
-                bk.S = fa::extend16_slot_count(bk.L, maxq);
+                bk.S = fa::extend16_slot_count(bk.L, maxq, false);
             } else {
                 bk.L = lanes32(maxq);
                 bk.S = fa::extend_slot_count(bk.L, maxq);

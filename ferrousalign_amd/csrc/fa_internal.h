@@ -50,7 +50,7 @@ struct fa_batch {
     int64_t n = 0;
     // host copies kept for re-packing
     std::vector<int32_t> qlen, tlen, h0, w;
-    std::vector<uint8_t> elig16;  // scores provably fit the 16-bit kernel
+    std::vector<uint8_t> elig16;  // fits16 of each pair
     std::vector<float> cost;
     std::vector<uint32_t> qo_host, to_host;  // word offsets of each query / target in d_seq2
     // device
@@ -98,12 +98,14 @@ int check_scoring(const fa_banded_pairwise_sw *sw);
 
 // Kernel choice of one pair: kind (1: packed 16-bit, 0: 32-bit), lanes per
 // pair and slot count, for the configured lane count L (0: narrowest group
-// that holds the query) and the FA_SCORE_I32 request `wide`; false when no
-// instantiated kernel holds qlen.
-bool pair_kernel(int L, bool wide, bool elig16, int qlen, int *kind, int *lanes, int *S);
+// that holds the query), the FA_SCORE_I32 request `wide` and the pair's
+// fits16 level; false when no instantiated kernel holds qlen.
+bool pair_kernel(int L, bool wide, int elig16, int qlen, int *kind, int *lanes, int *S);
 
-// Whether a pair's scores provably fit the packed 16-bit kernel.
-bool fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen);
+// 0: the pair needs the 32-bit kernel; 1: its scores provably fit the packed
+// 16-bit kernel (Hmax = h0 + qlen * max(mat) < 4096, Hmax * K < 32768, gap
+// costs in range); 2: also Hmax < 2048, which the wide slot counts need.
+int fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen);
 
 // Scoring fields of the kernel argument block (profile bytes, gap costs,
 // z-drop, 16-bit guard).

@@ -303,28 +303,49 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             skey[(size_t)k] = kind << 16;
         }
     }
+    // A wavefront runs until the last of its pairs stops, so what must match
+    // inside a wavefront is the row count, which length order already
+    // groups.  What the costs add is the pairs that stop early (z-drop,
+    // non-homologous targets): a pair whose cost is below half the mean cost
+    // of its (kernel, qlen) group goes to a group of its own, ordered by cost
+    // class (a factor e^(1/2) apart) and length; the rest keep length order.
+    // Sorting on the raw costs would let their noise scatter pairs of equal
+    // length over many wavefronts.
+    std::vector<uint8_t> early;
     if (pack == FA_PACK_COST) {
-        // A wavefront runs until the last of its pairs stops, so what must
-        // match inside a wavefront is the row count.  Cost classes (a factor
-        // e^(1/2) apart) separate the pairs the costs mark as short (early
-        // z-drop, non-homologous targets) from the full-length ones, and
-        // length order inside a class keeps equal row counts together;
-        // sorting on the raw costs would let their noise scatter pairs of
-        // equal length over many wavefronts.
         if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
+        std::vector<int> keys(skey);
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        std::vector<double> sum(keys.size() * 256, 0.0);
+        std::vector<uint32_t> cnt(keys.size() * 256, 0u);
+        std::vector<uint32_t> gid((size_t)n);
+        for (int64_t k = 0; k < n; ++k) {
+            const size_t g = (size_t)(std::lower_bound(keys.begin(), keys.end(), skey[(size_t)k]) - keys.begin()) * 256 +
+                             (size_t)std::min(b->qlen[(size_t)k], 255);
+            gid[(size_t)k] = (uint32_t)g;
+            sum[g] += std::max(0.0f, b->cost[(size_t)k]);
+            ++cnt[g];
+        }
+        early.resize((size_t)n);
+        for (int64_t k = 0; k < n; ++k) {
+            const size_t g = gid[(size_t)k];
+            early[(size_t)k] = (double)std::max(0.0f, b->cost[(size_t)k]) * 2.0 * cnt[g] < sum[g];
+        }
     }
-    // one sortable key per pair, every field descending (inverted): kernel
-    // key (17 bits), cost class (8), qlen (8), tlen (12)
+    // one sortable key per pair, every field descending (inverted) except the
+    // early-stop flag: kernel key (17 bits), early stop (1), cost class of an
+    // early-stopping pair (7), qlen (8), tlen (12)
     {
         std::vector<uint64_t> k64((size_t)n);
         const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
         parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
             for (int64_t k = lo; k < hi; ++k) {
                 uint64_t x = (uint64_t)(0x1FFFF - skey[(size_t)k]) << 28;
-                if (pack == FA_PACK_COST) {
+                if (pack == FA_PACK_COST && early[(size_t)k]) {
                     const int cls =
                         (int)std::lrint(2.0 * std::log1p(std::max(0.0f, b->cost[(size_t)k])));
-                    x |= (uint64_t)(255 - std::min(cls, 255)) << 20;
+                    x |= ((uint64_t)1 << 27) | (uint64_t)(127 - std::min(cls, 127)) << 20;
                 }
                 if (pack != FA_PACK_NONE)
                     x |= ((uint64_t)(255 - ql[k]) << 12) | (uint64_t)(4095 - tl[k]);
@@ -342,8 +363,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         while (e < n && skey[order[(size_t)e]] == key0) {
             const uint32_t p = order[(size_t)e];
             maxq = std::max(maxq, b->qlen[p]);
-            work += pack == FA_PACK_COST ? (double)b->cost[p]
-                                         : (double)b->qlen[p] * (double)std::max(b->tlen[p], 1);
+            work += (double)std::max(b->tlen[p], 1);
             ++e;
         }
         Bucket bk;
@@ -365,6 +385,10 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             bk.L = (key0 >> 8) & 0xFF;
             bk.S = key0 & 0xFF;
         }
+        // work: rows x VALU instructions per wavefront-row / pairs per
+        // wavefront (31 per slot for the packed kernel's two pairs, ~40 for
+        // the 32-bit kernel's one, ~175 per row of bookkeeping)
+        bk.work *= bk.kind == 1 ? (31.0 * bk.S + 175.0) * bk.L / 128.0 : (40.0 * bk.S + 175.0) * bk.L / 64.0;
         buckets.push_back(bk);
         k = e;
     }

@@ -30,7 +30,7 @@ namespace fa_host {
 struct Bucket {
     int S;
     uint32_t first, count;
-    double work;   // estimated work: sum over pairs of qlen * tlen (or of the cost)
+    double work;   // estimated work: target rows x VALU instructions per row / pairs per wavefront
     int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group)
     int L = 16;    // lanes per pair of this bucket
     int slen = 0;  // mate rescue: segments per lane ceil(qlen / 8) of every pair

@@ -94,3 +94,45 @@ def test_autotuner_picks_a_valid_config(mix, tmp_path):
     best2, flags2, table2 = autotune.tune(db, ps.qlen, ps.tlen, ps.w, cache_path=str(tmp_path / "c.json"))
     assert not table2 and best2["lanes"] == best["lanes"]
     db.free()
+
+
+def test_concurrent_contexts_from_threads():
+    """The reference calls the batch path from every rayon worker at once
+    (single_end.rs:92-110): one context per host thread, four threads calling
+    at the same time (ctypes releases the GIL during each call), each batch
+    checked against the oracle."""
+    import threading
+    from ferrousalign_amd import synth
+    from ferrousalign_amd.banded_swa import Context, extend_arrays
+    from tests import _oracle as O
+    sw = synth.default_scoring()
+    batches = [synth.workload("uneven_mix_4m", n=4096 + 512 * k) for k in range(4)]
+    want = []
+    for ps in batches:
+        ora, oc, rc = O.extend_batch(sw, *ps.flat())
+        assert (rc == 0).all()
+        want.append((O.to_outscore_order(ora), oc.astype(np.uint32)))
+    got = [None] * 4
+    errors = []
+
+    def work(k):
+        try:
+            ctx = Context(0)
+            res = []
+            for _ in range(3):
+                out, cells = extend_arrays(sw, *batches[k].flat(), ctx=ctx, want_cells=True)[:2]
+                res.append((out.copy(), cells.copy()))
+            got[k] = res
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(120)
+    assert not errors, errors
+    for k in range(4):
+        assert got[k] is not None
+        for out, cells in got[k]:
+            assert np.array_equal(out, want[k][0]) and np.array_equal(cells.astype(np.uint32), want[k][1])

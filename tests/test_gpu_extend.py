@@ -134,3 +134,22 @@ def test_parity_large_band_and_zdrop():
             print("large band", zdrop, flags, flush=True)
             ora, oc, gpu, gc = _run_both(ps, sw, flags=flags)
             _assert_same(ora, oc, gpu, gc, "large band zdrop=%d flags=%d" % (zdrop, flags))
+
+
+def test_parity_wide_slot_boundary():
+    """Pairs on both sides of the score bound the wide slot counts need
+    (Hmax = h0 + qlen * max(mat) < 2048 for S = 20, 24 on 4- and 8-lane
+    groups) in one batch, for each lane configuration: the narrow and the wide
+    kernels give the oracle's results."""
+    from ferrousalign_amd import synth
+    rng = np.random.default_rng(23)
+    ps = synth.workload("uneven_mix_4m", n=16384)
+    ps = ps.subset(np.nonzero((ps.qlen >= 64) & (ps.qlen <= 191))[0])
+    ps.h0[:] = rng.integers(1850, 2150, ps.n).astype(np.int32)
+    hmax = ps.h0 + ps.qlen
+    assert (hmax < 2048).any() and (hmax >= 2048).any()
+    sw = synth.default_scoring()
+    for lanes in (0, 4, 8):
+        print("wide slot boundary", lanes, ps.n, flush=True)
+        ora, oc, gpu, gc = _run_both(ps, sw, lanes=lanes)
+        _assert_same(ora, oc, gpu, gc, "wide slot boundary lanes=%d" % lanes)

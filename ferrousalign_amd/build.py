@@ -23,7 +23,7 @@ HIP_FLAGS = [
 ]
 HIP_SOURCES = ["extend_kernel.hip", "extend16_kernel.hip", "predictor.hip", "mate16_kernel.hip", "mate16_sym.hip", "valu_probe.hip",
                "ref_kernel.hip", "swtb_kernel.hip", "chain_kernel.hip", "fa_api.cpp",
-               "fa_pipeline.cpp", "chain_api.cpp"]
+               "fa_pipeline.cpp", "chain_api.cpp", "pack_host.cc"]
 
 
 def _run(cmd):
@@ -42,6 +42,9 @@ def build_hip_library():
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
         objs.append(obj)
+        if src.endswith(".cc"):  # plain host C++ (x86 intrinsics), no device pass
+            cmds.append(["g++", "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
+            continue
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:

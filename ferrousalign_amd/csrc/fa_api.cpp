@@ -4,6 +4,7 @@
 // (src/pipelines/linear/batch_extension/dispatch.rs:31-268 and
 // src/core/alignment/banded_swa/shared.rs:161-389) for one GPU.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -48,11 +49,9 @@ int32_t f64_as_i32(double x) {
     return (int32_t)x;
 }
 
-// Band width after end-bonus capping, ksw_affine_gap.rs:1001-1018.
-int32_t capped_w(const fa_banded_pairwise_sw &sw, int32_t qlen, int32_t w) {
-    int32_t max_score = 0;
-    for (int i = 0; i < 25; ++i)
-        if (sw.mat[i] > (int8_t)max_score) max_score = sw.mat[i];
+// Band width after end-bonus capping, ksw_affine_gap.rs:1001-1018
+// (max_score = max(0, max(mat))).
+int32_t capped_w(const fa_banded_pairwise_sw &sw, int32_t max_score, int32_t qlen, int32_t w) {
     int32_t max_ins = f64_as_i32(((double)qlen * max_score + sw.end_bonus - sw.o_ins) /
                                      (double)sw.e_ins + 1.0);
     max_ins = std::max(max_ins, 1);
@@ -111,9 +110,7 @@ bool pair_kernel(int L, bool wide, int elig16, int q, int *kind, int *lanes, int
     return *S > 0;
 }
 
-int fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen) {
-    int matmax = 0;
-    for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw.mat[x]);
+int fits16(const fa_banded_pairwise_sw &sw, int matmax, int64_t h0, int64_t qlen) {
     const int64_t bound = h0 + qlen * matmax;
     const int64_t K = std::max(2, 1 + matmax);
     if (!(bound < 4096 && bound * K < 32768 && sw.e_ins < 32 && sw.o_ins + sw.e_ins < 8192 &&
@@ -195,7 +192,7 @@ std::vector<uint32_t> radix_order(std::vector<uint64_t> keys) {
     return a;
 }
 
-inline uint32_t seq_words(int32_t len) { return (uint32_t)((len + 31) / 32) * 2u; }
+using fa_host::seq_words;
 
 // t == nullptr: the target comes from a device reference (codes 0-3 by construction)
 int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q,
@@ -205,26 +202,13 @@ int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q
     if (tl < 0 || tl > FA_MAX_TLEN) return FA_ERR_TLEN;
     if (w < 0) return FA_ERR_BAND;
     if (h0 > (1 << 19)) return FA_ERR_SCORING;
-    for (int32_t j = 0; j < ql; ++j)
-        if (q[j] > 4) return FA_ERR_CODE;
-    if (t)
-        for (int32_t j = 0; j < tl; ++j)
-            if (t[j] > 4) return FA_ERR_CODE;
+    if (!fa_host::codes_ok(q, ql)) return FA_ERR_CODE;
+    if (t && !fa_host::codes_ok(t, tl)) return FA_ERR_CODE;
     return FA_OK;
 }
 
-void pack_seq(const uint8_t *s, int32_t len, uint32_t *w2, uint32_t *wn) {
-    const uint32_t nw = seq_words(len);
-    for (uint32_t k = 0; k < nw; ++k) w2[k] = 0;
-    for (uint32_t k = 0; k < nw / 2; ++k) wn[k] = 0;
-    for (int32_t p = 0; p < len; ++p) {
-        uint32_t c = s[p];
-        if (c == 4) {
-            wn[p >> 5] |= 1u << (p & 31);
-            c = 0;
-        }
-        w2[p >> 4] |= (c & 3u) << ((p & 15) * 2);
-    }
+inline void pack_seq(const uint8_t *s, int32_t len, uint32_t *w2, uint32_t *wn) {
+    fa_host::pack_codes(s, len, w2, wn);
 }
 
 template <class T>
@@ -253,7 +237,7 @@ void batch_release(fa_batch *b) {
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
-                    b->d_mout, b->d_imax_log};
+                    b->d_mout, b->d_imax_log, b->d_sort};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -263,6 +247,11 @@ void batch_release(fa_batch *b) {
 // run widest first; inside a bucket pairs run from the most to the least
 // expensive (by (qlen, tlen), or by the caller's cost), so that lane groups
 // of one wavefront hold pairs of similar cost and the long pairs start first.
+__global__ __launch_bounds__(256) void iota_kernel(uint32_t *v, uint32_t n) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < n) v[k] = k;
+}
+
 int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t pack = flags & 0xF0u;
     const uint32_t single = flags & FA_LAUNCH_SINGLE;
@@ -271,116 +260,152 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t key = pack | single | wide | (uint32_t)L << 16;
     if (b->packed_flags == key) return FA_OK;
     const int64_t n = b->n;
-    std::vector<uint32_t> order((size_t)n);
-    for (int64_t k = 0; k < n; ++k) order[(size_t)k] = (uint32_t)k;
-    // kernel of each pair: (kind, lanes, slots) packed in one sortable key
-    std::vector<int> skey((size_t)n);
-    // merged buckets (one per kernel kind) hold pairs of any score range, so
-    // they take the narrow slot counts
-    auto lanes16 = [&](int q) {
-        int kind, lanes, S;
-        fa_host::pair_kernel(L, false, 1, q, &kind, &lanes, &S);
-        return lanes;
-    };
-    auto lanes32 = [&](int q) {
-        int kind, lanes, S;
-        fa_host::pair_kernel(L, true, 0, q, &kind, &lanes, &S);
-        return lanes;
-    };
-    auto pair_key = [&](int64_t k, int &kind, int &lanes, int &S) -> bool {
-        return fa_host::pair_kernel(L, wide != 0, b->elig16[(size_t)k], b->qlen[(size_t)k], &kind, &lanes,
-                                    &S);
-    };
-    for (int64_t k = 0; k < n; ++k) {
-        int kind, lanes, S;
-        if (!pair_key(k, kind, lanes, S)) return FA_ERR_QLEN;
-        skey[(size_t)k] = (kind << 16) | (lanes << 8) | S;
-    }
-    if (pack == FA_PACK_NONE || single) {
-        // one bucket per kernel kind at its widest slot count, input or sorted order
-        for (int64_t k = 0; k < n; ++k) {
-            const int kind = skey[(size_t)k] >> 16;
-            skey[(size_t)k] = kind << 16;
+    if (pack == FA_PACK_COST && (int64_t)b->cost.size() != n) return FA_ERR_ARG;
+    // kernel of a pair, (kind << 16) | (lanes << 8) | slots, as a function of
+    // its fits16 level and qlen (-1: no kernel holds it); merged buckets (one
+    // per kernel kind) take the kind alone
+    const bool merged = pack == FA_PACK_NONE || single;
+    int tab[3][256];
+    std::vector<int> kinds;  // distinct keys, descending: the bucket order
+    for (int f = 0; f < 3; ++f)
+        for (int q = 0; q < 256; ++q) {
+            int kind, lanes, S;
+            const bool ok = q > 0 && fa_host::pair_kernel(L, wide != 0, f, q, &kind, &lanes, &S);
+            tab[f][q] = !ok ? -1 : (merged ? kind << 16 : (kind << 16) | (lanes << 8) | S);
+            if (tab[f][q] >= 0) kinds.push_back(tab[f][q]);
         }
-    }
-    // A wavefront runs until the last of its pairs stops, so what must match
-    // inside a wavefront is the row count, which length order already
-    // groups.  What the costs add is the pairs that stop early (z-drop,
-    // non-homologous targets): a pair whose cost is below half the mean cost
-    // of its (kernel, qlen) group goes to a group of its own, ordered by cost
-    // class (a factor e^(1/2) apart) and length; the rest keep length order.
-    // Sorting on the raw costs would let their noise scatter pairs of equal
-    // length over many wavefronts.
-    std::vector<uint8_t> early;
-    if (pack == FA_PACK_COST) {
-        if ((int64_t)b->cost.size() != n) return FA_ERR_ARG;
-        std::vector<int> keys(skey);
-        std::sort(keys.begin(), keys.end());
-        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-        std::vector<double> sum(keys.size() * 256, 0.0);
-        std::vector<uint32_t> cnt(keys.size() * 256, 0u);
-        std::vector<uint32_t> gid((size_t)n);
-        for (int64_t k = 0; k < n; ++k) {
-            const size_t g = (size_t)(std::lower_bound(keys.begin(), keys.end(), skey[(size_t)k]) - keys.begin()) * 256 +
-                             (size_t)std::min(b->qlen[(size_t)k], 255);
-            gid[(size_t)k] = (uint32_t)g;
-            sum[g] += std::max(0.0f, b->cost[(size_t)k]);
-            ++cnt[g];
+    std::sort(kinds.begin(), kinds.end(), [](int x, int y) { return x > y; });
+    kinds.erase(std::unique(kinds.begin(), kinds.end()), kinds.end());
+    const size_t nk = kinds.size();
+    int kidx[3][256];
+    for (int f = 0; f < 3; ++f)
+        for (int q = 0; q < 256; ++q)
+            kidx[f][q] = tab[f][q] < 0 ? -1
+                                       : (int)(std::find(kinds.begin(), kinds.end(), tab[f][q]) - kinds.begin());
+    // per-thread statistics per kernel key: pairs, sum of tlen, max qlen, and
+    // (cost packing) the cost sum and count per (key, qlen)
+    const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
+    const uint8_t *el = b->elig16.data();
+    const bool cost = pack == FA_PACK_COST;
+    std::vector<uint64_t> tcnt(16 * nk, 0), ttl(16 * nk, 0);
+    std::vector<int> tmaxq(16 * nk, 1);
+    std::vector<double> csum(cost ? 16 * nk * 256 : 0, 0.0);
+    std::vector<uint32_t> ccnt(cost ? 16 * nk * 256 : 0, 0u);
+    std::vector<int32_t> pk((size_t)n);
+    std::vector<int> bad(16, 0);
+    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned t) {
+        for (int64_t k = lo; k < hi; ++k) {
+            const int q = ql[k];
+            const int f = el[k] > 2 ? 2 : el[k];
+            const int x = (q > 0 && q < 256) ? kidx[f][q] : -1;
+            if (x < 0) {
+                bad[t] = 1;
+                pk[(size_t)k] = 0;
+                continue;
+            }
+            pk[(size_t)k] = x;
+            const size_t g = (size_t)t * nk + (size_t)x;
+            ++tcnt[g];
+            ttl[g] += (uint64_t)std::max(tl[k], 1);
+            tmaxq[g] = std::max(tmaxq[g], q);
+            if (cost) {
+                csum[g * 256 + (size_t)q] += std::max(0.0f, b->cost[(size_t)k]);
+                ++ccnt[g * 256 + (size_t)q];
+            }
         }
-        early.resize((size_t)n);
-        for (int64_t k = 0; k < n; ++k) {
-            const size_t g = gid[(size_t)k];
-            early[(size_t)k] = (double)std::max(0.0f, b->cost[(size_t)k]) * 2.0 * cnt[g] < sum[g];
-        }
-    }
-    // one sortable key per pair, every field descending (inverted) except the
-    // early-stop flag: kernel key (17 bits), early stop (1), cost class of an
-    // early-stopping pair (7), qlen (8), tlen (12)
-    {
-        std::vector<uint64_t> k64((size_t)n);
-        const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
-        parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
-            for (int64_t k = lo; k < hi; ++k) {
-                uint64_t x = (uint64_t)(0x1FFFF - skey[(size_t)k]) << 28;
-                if (pack == FA_PACK_COST && early[(size_t)k]) {
-                    const int cls =
-                        (int)std::lrint(2.0 * std::log1p(std::max(0.0f, b->cost[(size_t)k])));
+    });
+    for (int v : bad)
+        if (v) return FA_ERR_QLEN;
+    std::vector<double> gsum(cost ? nk * 256 : 0, 0.0);
+    std::vector<uint64_t> gcnt(cost ? nk * 256 : 0, 0);
+    if (cost)
+        for (size_t t = 0; t < 16; ++t)
+            for (size_t g = 0; g < nk * 256; ++g) {
+                gsum[g] += csum[t * nk * 256 + g];
+                gcnt[g] += ccnt[t * nk * 256 + g];
+            }
+    // One sortable key per pair, ascending: kernel key rank (the widest kernel
+    // first), then -- cost packing -- the early-stop flag and the cost class
+    // of an early stopper, then qlen and tlen descending.  A wavefront runs
+    // until the last of its pairs stops, so what must match inside a
+    // wavefront is the row count, which length order groups.  What the costs
+    // add is the pairs that stop early (z-drop, non-homologous targets): a
+    // pair whose cost is below half the mean cost of its (kernel, qlen) group
+    // goes to a group of its own, ordered by cost class (a factor e^(1/2)
+    // apart) and length.  Sorting on the raw costs would let their noise
+    // scatter pairs of equal length over many wavefronts.
+    std::vector<uint64_t> k64((size_t)n);
+    std::vector<uint64_t> tor(16, 0);
+    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned t) {
+        uint64_t acc = 0;
+        for (int64_t k = lo; k < hi; ++k) {
+            uint64_t x = (uint64_t)pk[(size_t)k] << 28;
+            if (cost) {
+                const size_t g = (size_t)pk[(size_t)k] * 256 + (size_t)ql[k];
+                const double c = std::max(0.0f, b->cost[(size_t)k]);
+                if (c * 2.0 * (double)gcnt[g] < gsum[g]) {
+                    const int cls = (int)std::lrint(2.0 * std::log1p(c));
                     x |= ((uint64_t)1 << 27) | (uint64_t)(127 - std::min(cls, 127)) << 20;
                 }
-                if (pack != FA_PACK_NONE)
-                    x |= ((uint64_t)(255 - ql[k]) << 12) | (uint64_t)(4095 - tl[k]);
-                k64[(size_t)k] = x;
             }
-        });
-        order = radix_order(std::move(k64));
-    }
-    std::vector<Bucket> buckets;
-    for (int64_t k = 0; k < n;) {
-        const int key0 = skey[order[(size_t)k]];
-        int64_t e = k;
-        int maxq = 1;
-        double work = 0.0;
-        while (e < n && skey[order[(size_t)e]] == key0) {
-            const uint32_t p = order[(size_t)e];
-            maxq = std::max(maxq, b->qlen[p]);
-            work += (double)std::max(b->tlen[p], 1);
-            ++e;
+            if (pack != FA_PACK_NONE) x |= ((uint64_t)(255 - ql[k]) << 12) | (uint64_t)(4095 - tl[k]);
+            k64[(size_t)k] = x;
+            acc |= x;
         }
+        tor[t] |= acc;
+    });
+    uint64_t any = 0;
+    for (uint64_t v : tor) any |= v;
+    // stable device radix sort of (key, pair id): the pair order (ties keep
+    // the input order)
+    if (n > 0) {
+        const int end_bit = any ? 64 - __builtin_clzll(any) : 1;
+        size_t tb = 0;
+        FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                                  (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0,
+                                                  end_bit, ctx->stream));
+        const size_t need = 16 * (size_t)n + 4 * (size_t)n + tb + 256;
+        if (b->sort_bytes < need) {
+            if (b->d_sort) (void)hipFree(b->d_sort);
+            b->d_sort = nullptr;
+            b->sort_bytes = 0;
+            FA_HIP(hipMalloc(&b->d_sort, need));
+            b->sort_bytes = need;
+        }
+        uint8_t *base = (uint8_t *)b->d_sort;
+        uint64_t *kin = (uint64_t *)base, *kout = kin + n;
+        uint32_t *vin = (uint32_t *)(kout + n);
+        void *temp = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
+        FA_HIP(hipMemcpyAsync(kin, k64.data(), (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, vin,
+                           (uint32_t)n);
+        FA_HIP(hipGetLastError());
+        FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, b->d_order, (int)n, 0, end_bit,
+                                                  ctx->stream));
+    }
+    // buckets: the kernel keys in rank order with their pair counts
+    std::vector<Bucket> buckets;
+    uint32_t first = 0;
+    for (size_t x = 0; x < nk; ++x) {
+        uint64_t count = 0, rows = 0;
+        int maxq = 1;
+        for (size_t t = 0; t < 16; ++t) {
+            count += tcnt[t * nk + x];
+            rows += ttl[t * nk + x];
+            maxq = std::max(maxq, tmaxq[t * nk + x]);
+        }
+        if (!count) continue;
+        const int key0 = kinds[x];
         Bucket bk;
         bk.kind = key0 >> 16;
-        bk.first = (uint32_t)k;
-        bk.count = (uint32_t)(e - k);
-        bk.work = work;
+        bk.first = first;
+        bk.count = (uint32_t)count;
+        first += (uint32_t)count;
         if ((key0 & 0xFFFF) == 0) {  // merged bucket: widest slot count of its pairs
-            int kind = bk.kind;
-            if (kind == 1) {
-                bk.L = lanes16(maxq);
-                bk.S = fa::extend16_slot_count(bk.L, maxq, false);
-            } else {
-                bk.L = lanes32(maxq);
-                bk.S = fa::extend_slot_count(bk.L, maxq);
-            }
-            if (bk.S < 0) return FA_ERR_QLEN;
+            int kind, lanes, S;
+            if (!fa_host::pair_kernel(L, bk.kind == 0, bk.kind, maxq, &kind, &lanes, &S)) return FA_ERR_QLEN;
+            bk.L = lanes;
+            bk.S = S;
         } else {
             bk.L = (key0 >> 8) & 0xFF;
             bk.S = key0 & 0xFF;
@@ -388,13 +413,11 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         // work: rows x VALU instructions per wavefront-row / pairs per
         // wavefront (31 per slot for the packed kernel's two pairs, ~40 for
         // the 32-bit kernel's one, ~175 per row of bookkeeping)
-        bk.work *= bk.kind == 1 ? (31.0 * bk.S + 175.0) * bk.L / 128.0 : (40.0 * bk.S + 175.0) * bk.L / 64.0;
+        bk.work = (double)rows *
+                  (bk.kind == 1 ? (31.0 * bk.S + 175.0) * bk.L / 128.0 : (40.0 * bk.S + 175.0) * bk.L / 64.0);
         buckets.push_back(bk);
-        k = e;
     }
     if (buckets.size() > 64) return FA_ERR_ARG;
-    FA_HIP(hipMemcpyAsync(b->d_order, order.data(), order.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, ctx->stream));
     FA_HIP(fa::meta_gather(b->args, (uint32_t)n, ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
     b->buckets.swap(buckets);
@@ -532,8 +555,7 @@ const char *fa_strerror(int code) {
 
 int fa_pack_2bit(const uint8_t *codes, int32_t len, uint32_t *w2, uint32_t *wn) {
     if (len < 0 || (len > 0 && (!codes || !w2 || !wn))) return FA_ERR_ARG;
-    for (int32_t p = 0; p < len; ++p)
-        if (codes[p] > 4) return FA_ERR_CODE;
+    if (!fa_host::codes_ok(codes, len)) return FA_ERR_CODE;
     pack_seq(codes, len, w2, wn);
     return FA_OK;
 }
@@ -581,6 +603,7 @@ int fa_ctx_destroy(fa_ctx *ctx) {
     if (!ctx) return FA_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     fa_host::chain_workspace_free(ctx);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -615,6 +638,19 @@ struct TargetSource {
     bool validated = false;  // the caller already ran validate_job on every job
 };
 
+// Grow-only pinned host buffer of the context (staging for uploads; the
+// caller synchronises before the next use).
+int pinned_staging(fa_ctx *ctx, size_t bytes) {
+    if (ctx->pin_bytes >= bytes) return FA_OK;
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
+    ctx->pin = nullptr;
+    ctx->pin_bytes = 0;
+    const size_t want = bytes + bytes / 8 + 4096;
+    FA_HIP(hipHostMalloc(&ctx->pin, want));
+    ctx->pin_bytes = want;
+    return FA_OK;
+}
+
 int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                  const uint8_t *query_seqs, const int64_t *query_offset, const int32_t *query_len,
                  const TargetSource &ts, const int32_t *ref_len, const int32_t *h0,
@@ -648,7 +684,11 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     int first_err = FA_OK;
     int tmax = 0;
     std::vector<int8_t> vrc((size_t)n);
-    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
+    int matmax = 0;  // max(0, max(mat))
+    for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw->mat[x]);
+    std::vector<int> ttmax(16, 0);
+    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned t) {
+        int tm = 0;
         for (int64_t k = lo; k < hi; ++k) {
             int r = ts.validated ? FA_OK
                                  : validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
@@ -661,8 +701,15 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                 if (b < 0 || e > 2 * lp || (b < lp && e > lp)) r = FA_ERR_SEGMENT;
             }
             vrc[(size_t)k] = (int8_t)r;
+            if (r == FA_OK) {
+                b->w[(size_t)k] = capped_w(*sw, matmax, query_len[k], band_width[k]);
+                b->elig16[(size_t)k] = (uint8_t)fa_host::fits16(*sw, matmax, h0[k], query_len[k]);
+                tm = std::max(tm, ref_len[k]);
+            }
         }
+        ttmax[t] = std::max(ttmax[t], tm);
     });
+    for (int v : ttmax) tmax = std::max(tmax, v);
     for (int64_t k = 0; k < n; ++k) {
         int r = vrc[(size_t)k];
         if (r != FA_OK) {
@@ -677,28 +724,37 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             first_err = FA_ERR_ARG;
             break;
         }
-        b->w[(size_t)k] = capped_w(*sw, query_len[k], band_width[k]);
-        b->elig16[(size_t)k] = fa_host::fits16(*sw, h0[k], query_len[k]);
-        tmax = std::max(tmax, ref_len[k]);
     }
     if (first_err != FA_OK) {
         delete b;
         return first_err;
     }
-    std::vector<uint32_t> seq2((size_t)words), seqn((size_t)(words / 2));
+    // packed sequences staged in the context's pinned buffer (every word is
+    // written: sequences are packed back to back, and a device-reference
+    // target, which pac_gather writes on the device, is zeroed here)
+    if ((rc = pinned_staging(ctx, (size_t)words * 4 + (size_t)(words / 2) * 4))) {
+        delete b;
+        return rc;
+    }
+    uint32_t *seq2 = (uint32_t *)ctx->pin, *seqn = seq2 + words;
     parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
         for (int64_t k = lo; k < hi; ++k) {
             pack_seq(query_seqs + query_offset[k], query_len[k], &seq2[qo[(size_t)k]],
                      &seqn[qo[(size_t)k] / 2]);
-            if (!from_ref)  // device-reference targets are written by pac_gather below
+            if (!from_ref) {
                 pack_seq(ts.seqs + ts.offset[k], ref_len[k], &seq2[to[(size_t)k]],
                          &seqn[to[(size_t)k] / 2]);
+            } else {
+                const uint32_t nw = seq_words(ref_len[k]);
+                std::memset(&seq2[to[(size_t)k]], 0, nw * sizeof(uint32_t));
+                std::memset(&seqn[to[(size_t)k] / 2], 0, nw / 2 * sizeof(uint32_t));
+            }
         }
     });
 
     hipStream_t st = ctx->stream;
-    if ((rc = dev_upload(&b->d_seq2, seq2.data(), seq2.size(), st)) ||
-        (rc = dev_upload(&b->d_seqn, seqn.data(), seqn.size(), st)) ||
+    if ((rc = dev_upload(&b->d_seq2, seq2, (size_t)words, st)) ||
+        (rc = dev_upload(&b->d_seqn, seqn, (size_t)(words / 2), st)) ||
         (rc = dev_upload(&b->d_qo, qo.data(), qo.size(), st)) ||
         (rc = dev_upload(&b->d_to, to.data(), to.size(), st)) ||
         (rc = dev_upload(&b->d_qlen, b->qlen.data(), b->qlen.size(), st)) ||
@@ -708,6 +764,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
         (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
         (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64))) {
+        (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
         batch_release(b);
         return rc;
     }

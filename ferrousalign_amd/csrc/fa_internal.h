@@ -12,6 +12,7 @@
 #include "../../include/ferrousalign_amd.h"
 #include "extend_kernel.h"
 #include "mate_kernel.h"
+#include "pack_host.h"
 
 struct fa_ctx {
     int device = 0;
@@ -22,7 +23,9 @@ struct fa_ctx {
     std::string arch;
     int cus = 0;
     double last_dp_ms = 0.0, last_tb_ms = 0.0;  // kernels of the last fa_scalar_banded_swa_batch
-    void *chain_ws = nullptr;  // device workspace of the chain-extension path (fa_pipeline.cpp)
+    void *chain_ws = nullptr;  // device workspace of the chain-extension path (chain_api.cpp)
+    void *pin = nullptr;       // pinned host staging for uploads (grow-only)
+    size_t pin_bytes = 0;
 };
 
 namespace fa_host {
@@ -61,6 +64,8 @@ struct fa_batch {
     int32_t *d_out = nullptr;
     uint32_t *d_cells = nullptr;
     uint32_t *d_queue = nullptr;
+    void *d_sort = nullptr;      // packing sort: keys in/out, pair ids, scratch
+    size_t sort_bytes = 0;
     fa::ExtArgs args{};
     // launch plan for the current packing
     uint32_t packed_flags = 0xFFFFFFFFu;
@@ -105,7 +110,8 @@ bool pair_kernel(int L, bool wide, int elig16, int qlen, int *kind, int *lanes, 
 // 0: the pair needs the 32-bit kernel; 1: its scores provably fit the packed
 // 16-bit kernel (Hmax = h0 + qlen * max(mat) < 4096, Hmax * K < 32768, gap
 // costs in range); 2: also Hmax < 2048, which the wide slot counts need.
-int fits16(const fa_banded_pairwise_sw &sw, int64_t h0, int64_t qlen);
+// matmax = max(0, max(mat)).
+int fits16(const fa_banded_pairwise_sw &sw, int matmax, int64_t h0, int64_t qlen);
 
 // Scoring fields of the kernel argument block (profile bytes, gap costs,
 // z-drop, 16-bit guard).
@@ -114,7 +120,8 @@ void set_scoring(fa::ExtArgs &a, const fa_banded_pairwise_sw &sw);
 // Enqueue one run of a planned batch (buckets over up to 4 streams).
 int enqueue_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
 
-// Releases the chain-extension workspace of a context (fa_pipeline.cpp).
+// Releases the chain-extension workspace of a context (chain_api.cpp).
 void chain_workspace_free(fa_ctx *ctx);
+
 
 }  // namespace fa_host

@@ -604,6 +604,8 @@ int fa_ctx_destroy(fa_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     fa_host::chain_workspace_free(ctx);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    for (void *p : ctx->slot)
+        if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -648,6 +650,31 @@ int pinned_staging(fa_ctx *ctx, size_t bytes) {
     const size_t want = bytes + bytes / 8 + 4096;
     FA_HIP(hipHostMalloc(&ctx->pin, want));
     ctx->pin_bytes = want;
+    return FA_OK;
+}
+
+// Grow-only device buffer slot k of the context (CIGAR-regeneration calls);
+// the previous contents are not kept.
+template <class T>
+int ctx_alloc(fa_ctx *ctx, int k, T **p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (ctx->slot_bytes[k] < bytes) {
+        if (ctx->slot[k]) (void)hipFree(ctx->slot[k]);
+        ctx->slot[k] = nullptr;
+        ctx->slot_bytes[k] = 0;
+        const size_t want = bytes + bytes / 8;
+        FA_HIP(hipMalloc(&ctx->slot[k], want));
+        ctx->slot_bytes[k] = want;
+    }
+    *p = (T *)ctx->slot[k];
+    return FA_OK;
+}
+
+template <class T>
+int ctx_upload(fa_ctx *ctx, int k, T **p, const T *src, size_t count) {
+    int rc = ctx_alloc(ctx, k, p, count);
+    if (rc) return rc;
+    if (count) FA_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     return FA_OK;
 }
 
@@ -934,12 +961,9 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
                 else if (tl < 0 || tl > FA_MAX_TLEN) c = FA_ERR_TLEN;
                 else if (band_width[k] < 0) c = FA_ERR_BAND;
                 else if (h0[k] < 0 || h0[k] > (1 << 19)) c = FA_ERR_H0;
-                else {
-                    for (int32_t j = 0; j < ql && c == FA_OK; ++j)
-                        if (query_seqs[query_offset[k] + j] > 4) c = FA_ERR_CODE;
-                    for (int32_t j = 0; j < tl && c == FA_OK; ++j)
-                        if (ref_seqs[ref_offset[k] + j] > 4) c = FA_ERR_CODE;
-                }
+                else if (!fa_host::codes_ok(query_seqs + query_offset[k], ql) ||
+                         !fa_host::codes_ok(ref_seqs + ref_offset[k], tl))
+                    c = FA_ERR_CODE;
                 if (c != FA_OK) {
                     first_bad[t] = {k, c};
                     return;
@@ -971,38 +995,47 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     std::vector<uint64_t> tbo(m), cgo(m), alo(m);
     uint64_t words = 0, tbw = 0, cgw = 0, alw = 0;
     int tmax = 0, qmax = 0;
-    for (size_t p = 0; p < m; ++p) {
-        const int64_t k = job[p];
-        ql[p] = query_len[k];
-        tl[p] = ref_len[k];
-        ww[p] = band_width[k];
-        hh[p] = h0[k];
-        int L = 16;
-        for (int c : {4, 8, 16})
-            if (fa::swtb_slot_count(c, ql[p]) > 0) {
-                L = c;
-                break;
+    std::vector<int> ttm(16, 0), tqm(16, 0);
+    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned t) {
+        for (int64_t p = lo; p < hi; ++p) {
+            const int64_t k = job[(size_t)p];
+            ql[p] = query_len[k];
+            tl[p] = ref_len[k];
+            ww[p] = band_width[k];
+            hh[p] = h0[k];
+            int L = 16;
+            for (int c : {4, 8, 16})
+                if (fa::swtb_slot_count(c, ql[p]) > 0) {
+                    L = c;
+                    break;
+                }
+            int S = fa::swtb_slot_count(L, ql[p]);
+            lsl[p] = ((uint32_t)L << 8) | (uint32_t)S;
+            int bl, bs;
+            if (fa::swtb_band_config(ql[p], ww[p], L * S, &bl, &bs)) {  // band-window kernel
+                L = bl;
+                S = bs;
+                lsl[p] = fa::kSwtbBand | ((uint32_t)L << 8) | (uint32_t)S;
+                tqm[t] = std::max(tqm[t], ql[p]);
             }
-        int S = fa::swtb_slot_count(L, ql[p]);
-        lsl[p] = ((uint32_t)L << 8) | (uint32_t)S;
-        int bl, bs;
-        if (fa::swtb_band_config(ql[p], ww[p], L * S, &bl, &bs)) {  // band-window kernel
-            L = bl;
-            S = bs;
-            lsl[p] = fa::kSwtbBand | ((uint32_t)L << 8) | (uint32_t)S;
-            qmax = std::max(qmax, ql[p]);
+            tbo[p] = (uint64_t)tl[p] * (uint64_t)L * (uint64_t)((S + 15) / 16);  // size, for now
+            ttm[t] = std::max(ttm[t], tl[p]);
         }
+    });
+    for (int x : ttm) tmax = std::max(tmax, x);
+    for (int x : tqm) qmax = std::max(qmax, x);
+    for (size_t p = 0; p < m; ++p) {
         qo[p] = (uint32_t)words;
         words += seq_words(ql[p]);
         to[p] = (uint32_t)words;
         words += seq_words(tl[p]);
+        const uint64_t tbs = tbo[p];
         tbo[p] = tbw;
-        tbw += (uint64_t)tl[p] * (uint64_t)L * (uint64_t)((S + 15) / 16);
+        tbw += tbs;
         cgo[p] = cgw;
         cgw += (uint64_t)(ql[p] + tl[p] + 4);
         alo[p] = alw;
         alw += (uint64_t)(ql[p] + tl[p] + 2);
-        tmax = std::max(tmax, tl[p]);
         order[p] = (uint32_t)p;
     }
     if (words > 0xFFFFFFF0ull) return FA_ERR_ARG;
@@ -1013,7 +1046,10 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
                      (uint64_t)(4095 - tl[p]);
         order = radix_order(std::move(k64));
     }
-    std::vector<uint32_t> seq2(words), seqn(words / 2);
+    // packed sequences staged in the context's pinned buffer (every word is
+    // written: sequences are packed back to back)
+    if ((rc = pinned_staging(ctx, (size_t)words * 4 + (size_t)(words / 2) * 4))) return rc;
+    uint32_t *seq2 = (uint32_t *)ctx->pin, *seqn = seq2 + words;
     parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
         for (int64_t p = lo; p < hi; ++p) {
             const int64_t k = job[(size_t)p];
@@ -1022,34 +1058,28 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         }
     });
     hipStream_t st = ctx->stream;
-    uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr,
-             *d_order = nullptr, *d_lsl = nullptr, *d_tb = nullptr, *d_cells = nullptr,
-             *d_cig = nullptr;
-    int32_t *d_ql = nullptr, *d_tl = nullptr, *d_w = nullptr, *d_h0 = nullptr, *d_out = nullptr,
-            *d_nc = nullptr, *d_nr = nullptr, *d_nq = nullptr;
-    uint64_t *d_tbo = nullptr, *d_cgo = nullptr, *d_alo = nullptr;
-    uint8_t *d_ra = nullptr, *d_qa = nullptr;
-    auto release = [&] {
-        void *ptrs[] = {d_seq2, d_seqn, d_qo, d_to, d_order, d_lsl, d_tb, d_cells, d_cig, d_ql,
-                        d_tl, d_w, d_h0, d_out, d_nc, d_nr, d_nq, d_tbo, d_cgo, d_alo, d_ra, d_qa};
-        for (void *q : ptrs)
-            if (q) (void)hipFree(q);
-    };
-    if ((rc = dev_upload(&d_seq2, seq2.data(), seq2.size(), st)) ||
-        (rc = dev_upload(&d_seqn, seqn.data(), seqn.size(), st)) ||
-        (rc = dev_upload(&d_qo, qo.data(), m, st)) || (rc = dev_upload(&d_to, to.data(), m, st)) ||
-        (rc = dev_upload(&d_ql, ql.data(), m, st)) || (rc = dev_upload(&d_tl, tl.data(), m, st)) ||
-        (rc = dev_upload(&d_w, ww.data(), m, st)) || (rc = dev_upload(&d_h0, hh.data(), m, st)) ||
-        (rc = dev_upload(&d_order, order.data(), m, st)) ||
-        (rc = dev_upload(&d_lsl, lsl.data(), m, st)) ||
-        (rc = dev_upload(&d_tbo, tbo.data(), m, st)) ||
-        (rc = dev_upload(&d_cgo, cgo.data(), m, st)) ||
-        (rc = dev_upload(&d_alo, alo.data(), m, st)) || (rc = dev_alloc(&d_tb, (size_t)tbw)) ||
-        (rc = dev_alloc(&d_out, m * 6)) || (rc = dev_alloc(&d_cells, m)) ||
-        (rc = dev_alloc(&d_cig, (size_t)cgw)) || (rc = dev_alloc(&d_nc, m)) ||
-        (rc = dev_alloc(&d_nr, m)) || (rc = dev_alloc(&d_nq, m)) ||
-        (rc = dev_alloc(&d_ra, (size_t)alw)) || (rc = dev_alloc(&d_qa, (size_t)alw))) {
-        release();
+    // device buffers: grow-only slots of the context (the traceback block
+    // alone is ~2.5 KB per 150-base pair), reused by the next call
+    uint32_t *d_seq2, *d_seqn, *d_qo, *d_to, *d_order, *d_lsl, *d_tb, *d_cells, *d_cig;
+    int32_t *d_ql, *d_tl, *d_w, *d_h0, *d_out, *d_nc, *d_nr, *d_nq;
+    uint64_t *d_tbo, *d_cgo, *d_alo;
+    uint8_t *d_ra, *d_qa;
+    if ((rc = ctx_upload(ctx, 0, &d_seq2, seq2, (size_t)words)) ||
+        (rc = ctx_upload(ctx, 1, &d_seqn, seqn, (size_t)(words / 2))) ||
+        (rc = ctx_upload(ctx, 2, &d_qo, qo.data(), m)) || (rc = ctx_upload(ctx, 3, &d_to, to.data(), m)) ||
+        (rc = ctx_upload(ctx, 4, &d_ql, ql.data(), m)) || (rc = ctx_upload(ctx, 5, &d_tl, tl.data(), m)) ||
+        (rc = ctx_upload(ctx, 6, &d_w, ww.data(), m)) || (rc = ctx_upload(ctx, 7, &d_h0, hh.data(), m)) ||
+        (rc = ctx_upload(ctx, 8, &d_order, order.data(), m)) ||
+        (rc = ctx_upload(ctx, 9, &d_lsl, lsl.data(), m)) ||
+        (rc = ctx_upload(ctx, 10, &d_tbo, tbo.data(), m)) ||
+        (rc = ctx_upload(ctx, 11, &d_cgo, cgo.data(), m)) ||
+        (rc = ctx_upload(ctx, 12, &d_alo, alo.data(), m)) ||
+        (rc = ctx_alloc(ctx, 13, &d_tb, (size_t)tbw)) || (rc = ctx_alloc(ctx, 14, &d_out, m * 6)) ||
+        (rc = ctx_alloc(ctx, 15, &d_cells, m)) || (rc = ctx_alloc(ctx, 16, &d_cig, (size_t)cgw)) ||
+        (rc = ctx_alloc(ctx, 17, &d_nc, m)) || (rc = ctx_alloc(ctx, 18, &d_nr, m)) ||
+        (rc = ctx_alloc(ctx, 19, &d_nq, m)) || (rc = ctx_alloc(ctx, 20, &d_ra, (size_t)alw)) ||
+        (rc = ctx_alloc(ctx, 21, &d_qa, (size_t)alw))) {
+        (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
         return rc;
     }
     fa::SwtbArgs a{};
@@ -1115,10 +1145,7 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
     float t_dp = 0.f, t_tb = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&t_dp, ctx->ev0, ctx->ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(&t_tb, ctx->ev1, ctx->ev2);
-    if (e != hipSuccess) {
-        release();
-        return hip_fail(e, "fa_scalar_banded_swa_batch");
-    }
+    if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
     std::vector<uint64_t> dense(3 * m);  // per pair: CIGAR, reference-base, query-base offsets
     uint64_t nc_all = 0, nr_all = 0, nq_all = 0;
     for (size_t p = 0; p < m; ++p) {
@@ -1129,27 +1156,26 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         nr_all += (uint64_t)hnr[p];
         nq_all += (uint64_t)hnq[p];
     }
-    uint64_t *d_dense = nullptr;
-    uint32_t *d_ocig = nullptr;
-    uint8_t *d_ora = nullptr, *d_oqa = nullptr;
-    if ((rc = dev_upload(&d_dense, dense.data(), dense.size(), st)) ||
-        (rc = dev_alloc(&d_ocig, (size_t)nc_all + 1)) || (rc = dev_alloc(&d_ora, (size_t)nr_all + 1)) ||
-        (rc = dev_alloc(&d_oqa, (size_t)nq_all + 1))) {
-        for (void *q : {(void *)d_dense, (void *)d_ocig, (void *)d_ora, (void *)d_oqa})
-            if (q) (void)hipFree(q);
-        release();
+    uint64_t *d_dense;
+    uint32_t *d_ocig;
+    uint8_t *d_ora, *d_oqa;
+    if ((rc = ctx_upload(ctx, 22, &d_dense, dense.data(), dense.size())) ||
+        (rc = ctx_alloc(ctx, 23, &d_ocig, (size_t)nc_all + 1)) ||
+        (rc = ctx_alloc(ctx, 24, &d_ora, (size_t)nr_all + 1)) ||
+        (rc = ctx_alloc(ctx, 25, &d_oqa, (size_t)nq_all + 1))) {
+        (void)hipStreamSynchronize(st);
         return rc;
     }
-    std::unique_ptr<uint32_t[]> hcig(new uint32_t[nc_all + 1]);
-    std::unique_ptr<uint8_t[]> hra(new uint8_t[nr_all + 1]), hqa(new uint8_t[nq_all + 1]);
+    // downloads land in the pinned staging buffer (the uploads from it are
+    // complete: the stream was synchronised above)
+    if ((rc = pinned_staging(ctx, (size_t)nc_all * 4 + (size_t)nr_all + (size_t)nq_all + 64))) return rc;
+    uint32_t *hcig = (uint32_t *)ctx->pin;
+    uint8_t *hra = (uint8_t *)(hcig + nc_all), *hqa = hra + nr_all;
     e = fa::swtb_compact_launch(a, (uint32_t)m, d_dense, d_ocig, d_ora, d_oqa, st);
-    if (e == hipSuccess && nc_all)
-        e = hipMemcpyAsync(hcig.get(), d_ocig, nc_all * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && nr_all) e = hipMemcpyAsync(hra.get(), d_ora, nr_all, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && nq_all) e = hipMemcpyAsync(hqa.get(), d_oqa, nq_all, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nc_all) e = hipMemcpyAsync(hcig, d_ocig, nc_all * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nr_all) e = hipMemcpyAsync(hra, d_ora, nr_all, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nq_all) e = hipMemcpyAsync(hqa, d_oqa, nq_all, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    for (void *q : {(void *)d_dense, (void *)d_ocig, (void *)d_ora, (void *)d_oqa}) (void)hipFree(q);
-    release();
     if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
     ctx->last_dp_ms = t_dp;
     ctx->last_tb_ms = t_tb;
@@ -1167,9 +1193,9 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
             n_cigar[k] = hnc[p];
             n_ref_aligned[k] = hnr[p];
             n_query_aligned[k] = hnq[p];
-            std::memcpy(cigar + cigar_offset[k], &hcig[dense[3 * p]], (size_t)hnc[p] * 4);
-            std::memcpy(ref_aligned + aligned_offset[k], &hra[dense[3 * p + 1]], (size_t)hnr[p]);
-            std::memcpy(query_aligned + aligned_offset[k], &hqa[dense[3 * p + 2]], (size_t)hnq[p]);
+            std::memcpy(cigar + cigar_offset[k], hcig + dense[3 * p], (size_t)hnc[p] * 4);
+            std::memcpy(ref_aligned + aligned_offset[k], hra + dense[3 * p + 1], (size_t)hnr[p]);
+            std::memcpy(query_aligned + aligned_offset[k], hqa + dense[3 * p + 2], (size_t)hnq[p]);
         }
     });
     return FA_OK;

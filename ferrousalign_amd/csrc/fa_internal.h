@@ -24,8 +24,10 @@ struct fa_ctx {
     int cus = 0;
     double last_dp_ms = 0.0, last_tb_ms = 0.0;  // kernels of the last fa_scalar_banded_swa_batch
     void *chain_ws = nullptr;  // device workspace of the chain-extension path (chain_api.cpp)
-    void *pin = nullptr;       // pinned host staging for uploads (grow-only)
+    void *pin = nullptr;       // pinned host staging for uploads and downloads (grow-only)
     size_t pin_bytes = 0;
+    void *slot[32] = {};       // grow-only device buffers of the CIGAR-regeneration calls
+    size_t slot_bytes[32] = {};
 };
 
 namespace fa_host {

@@ -23,8 +23,10 @@ the same pairs through the host-buffer entry point
 (fa_execute_batch_simd_scoring: validation, packing, PCIe both ways, run,
 download).
 --workload illumina_150bp_1m times the plain kernel on 1,048,576 pairs from
-150 bp reads instead; mate_rescue_2m, cigar_regen_1m and chains_150bp_1m time
-the mate-rescue kernel, CIGAR regeneration and the chain-extension path.
+150 bp reads instead; mate_rescue_2m (16-bit) and mate_rescue_xbyte_2m (8-bit,
+the mode the reference's mate rescue picks for 151 bp mates), cigar_regen_1m
+and chains_150bp_1m time the mate-rescue kernel, CIGAR regeneration and the
+chain-extension path.
 """
 import argparse
 import hashlib
@@ -148,16 +150,18 @@ def digest_of(out, cells):
     return hashlib.sha256(out.tobytes() + cells.astype(np.uint32).tobytes()).hexdigest()
 
 
-def mate_bench(args, ctx, golden):
-    """Mate-rescue local SW (ksw_align2, 16-bit mode) on 2,097,152 pairs of a
-    151 bp query against a 1,500 bp window."""
+def mate_bench(args, ctx, golden, byte_mode=False):
+    """Mate-rescue local SW (ksw_align2, 16-bit mode; byte_mode: the 8-bit
+    mode, xtra | KSW_XBYTE, which mate_rescue.rs:314-321 selects for 151 bp
+    mates) on 2,097,152 pairs of a 151 bp query against a 1,500 bp window."""
     from ferrousalign_amd import synth, _native as N
     from ferrousalign_amd.banded_swa import DeviceBatch
     from tests import _oracle as O
     ps = synth.mate_pairs()
     n = ps.n
+    xtra = synth.MATE_XTRA | (O.KSW_XBYTE if byte_mode else 0)
     db = DeviceBatch(synth.default_scoring(), *ps.flat(), np.ones(n, np.int32), np.zeros(n, np.int32), ctx=ctx)
-    db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA)
+    db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, xtra)
     flags = N.FA_RUN_ALIGN2
     steps = max(1, min(args.steps, 3))
     ms, ms_ev, nl = timed(db, flags, steps, 1)
@@ -168,15 +172,17 @@ def mate_bench(args, ctx, golden):
     gpath = os.path.join(ROOT, "tests", "golden", "mate_digests.json")
     if os.path.exists(gpath):
         gm = json.load(open(gpath))
-    key = "mate_%d" % n
+    key = ("mate_u8_%d" if byte_mode else "mate_%d") % n
     exact = (digest(out, cells) == gm[key]) if key in gm else None
     roof = roofline(db, flags, cells, ctx)
-    res = {"metric": "mate-rescue pairs/s (ksw_align2 16-bit, bit-exact)", "value": round(n / (ms / 1e3), 1),
+    mode = "8-bit" if byte_mode else "16-bit"
+    res = {"metric": "mate-rescue pairs/s (ksw_align2 %s, bit-exact)" % mode, "value": round(n / (ms / 1e3), 1),
            "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
            "ms_per_step_hip_events": round(ms_ev, 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "int16x2 (saturating)", "data": "synthetic",
-           "config": {"workload": "mate_rescue_2m", "pairs": n, "qlen": 151, "tlen": 1500, "xtra": hex(synth.MATE_XTRA),
-                      "launches_per_step": nl},
+           "vs_baseline": None, "dtype": "u8 in int16x2 halves" if byte_mode else "int16x2 (saturating)",
+           "data": "synthetic",
+           "config": {"workload": "mate_rescue_xbyte_2m" if byte_mode else "mate_rescue_2m", "pairs": n,
+                      "qlen": 151, "tlen": 1500, "xtra": hex(xtra), "launches_per_step": nl},
            "gcups": round(total / (ms / 1e3) / 1e9, 2), "bit_exact_vs_golden": exact,
            "roofline": roof}
     if not args.no_cpu:
@@ -184,12 +190,13 @@ def mate_bench(args, ctx, golden):
         m = 16384
         sub = ps.subset(np.arange(m))
         t0 = time.perf_counter()
-        _, oc, _ = O.ksw_align2_batch(*sub.flat(), synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA, nthreads=threads)
+        _, oc, _ = O.ksw_align2_batch(*sub.flat(), synth.MATE_MATRIX, 6, 1, 6, 1, xtra, nthreads=threads)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "pairs/s", "gcups": round(float(oc.sum()) / dt / 1e9, 3),
                                "cores": threads, "kind": "port",
-                               "sample": "first %d pairs through the C restatement of ksw_align2 16-bit "
-                                         "(ksw_affine_gap.rs:503-861) on %d host threads: %.1f s" % (m, threads, dt)}
+                               "sample": "first %d pairs through the C restatement of ksw_align2 %s "
+                                         "(ksw_affine_gap.rs:267-861) on %d host threads: %.1f s"
+                                         % (m, mode, threads, dt)}
     print(json.dumps(res), flush=True)
 
 
@@ -379,8 +386,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uneven_mix_4m",
-                    choices=["uneven_mix_4m", "illumina_150bp_1m", "mate_rescue_2m", "cigar_regen_1m",
-                             "chains_150bp_1m"])
+                    choices=["uneven_mix_4m", "illumina_150bp_1m", "mate_rescue_2m", "mate_rescue_xbyte_2m",
+                             "cigar_regen_1m", "chains_150bp_1m"])
     ap.add_argument("--train-pairs", type=int, default=16777216)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -406,8 +413,8 @@ def main():
     if os.path.exists(gpath):
         golden = json.load(open(gpath))
 
-    if args.workload == "mate_rescue_2m":
-        mate_bench(args, ctx, golden)
+    if args.workload in ("mate_rescue_2m", "mate_rescue_xbyte_2m"):
+        mate_bench(args, ctx, golden, byte_mode=args.workload == "mate_rescue_xbyte_2m")
         return
     if args.workload == "cigar_regen_1m":
         cigar_bench(args, ctx)

@@ -540,7 +540,7 @@ class DeviceBatch:
         for i in range(min(nl.value, cap)):
             kind, lanes = int(kern[i]) >> 8, int(kern[i]) & 0xFF
             if kind == 2:
-                name = names[2] % int(slots[i])
+                name = names[2] % int(slots[i]) if lanes == 8 else "mate16_kernel<%d,8-bit>" % int(slots[i])
             else:
                 name = names[kind] % (lanes, int(slots[i]))
                 if flags & N.FA_LAUNCH_PERSISTENT:
@@ -579,8 +579,10 @@ class DeviceBatch:
 
 def ksw_align2_batch(qseq, qoff, qlen, tseq, toff, tlen, mat, o_del, e_del, o_ins, e_ins, xtra, m=5, ctx=None):
     """Mate-rescue local SW for a batch: one Kswr record (score, te, qe,
-    score2, te2, tb, qb) per job, exactly ksw_align2's 16-bit mode
-    (ksw_affine_gap.rs:773-861).  Returns (out[n,7], cells[n])."""
+    score2, te2, tb, qb) per job, exactly what ksw_align2
+    (ksw_affine_gap.rs:773-861) returns in the mode xtra selects: 16-bit, or
+    8-bit with KSW_XBYTE (0x10000; mate_rescue.rs:314-321 sets it when
+    qlen * match < 250).  Returns (out[n,7], cells[n])."""
     ctx = ctx or default_context()
     n = len(qlen)
     arrs = [np.ascontiguousarray(x, dt) for x, dt in ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32),

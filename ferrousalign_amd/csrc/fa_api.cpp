@@ -1298,7 +1298,8 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
                 FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
                 if (k < cap) {
                     if (slots) slots[k] = bk.S;
-                    if (kernel) kernel[k] = 2 << 8 | 8;  // mate kernel, 8 lanes per pair
+                    // mate kernel, lanes per pair group: 8 (16-bit mode) or 16 (8-bit)
+                    if (kernel) kernel[k] = 2 << 8 | (b->margs.byte_mode ? 16 : 8);
                     if (first) first[k] = bk.first + off;
                     if (count) count[k] = cnt;
                     if (ms) ms[k] = t;
@@ -1586,31 +1587,39 @@ int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t
                       end_bonus, mat, m, out, q, t);
 }
 
-// ---- mate-rescue local SW (ksw_align2, 16-bit mode) --------------------------------
+// ---- mate-rescue local SW (ksw_align2, 16-bit and 8-bit modes) ---------------------
 
 int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, int32_t o_del,
                         int32_t e_del, int32_t o_ins, int32_t e_ins, uint32_t xtra) {
     if (!ctx || !b || !mat) return FA_ERR_ARG;
     if (m != 5) return FA_ERR_SCORING;
-    if (xtra & fa::kKswXByte) return FA_ERR_SCORING;  // 8-bit mode is not provided on the GPU
+    const bool byte_mode = (xtra & fa::kKswXByte) != 0;
     if (o_del < 0 || e_del < 0 || o_ins < 0 || e_ins < 0 || o_del > 4096 || o_ins > 4096 ||
         e_del > 4096 || e_ins > 4096)
         return FA_ERR_SCORING;
     for (int q = 0; q < 5; ++q)
         if (mat[20 + q] != 0) return FA_ERR_SCORING;  // the N target row must score 0
-    int qmax = 0;
-    for (int k = 0; k < 25; ++k) qmax = std::max(qmax, (int)mat[k]);
+    int qmax = 0, qmin = 127;  // ksw_qinit's scan (ksw_affine_gap.rs:87-99)
+    for (int k = 0; k < 25; ++k) {
+        qmax = std::max(qmax, (int)mat[k]);
+        qmin = std::min(qmin, (int)mat[k]);
+    }
     if (qmax < 1) return FA_ERR_SCORING;
+    // 8-bit mode: profile bytes mat + shift with shift = (u8)(256 - qmin); the zero N
+    // row makes qmin <= 0, so every byte is mat - qmin in 0..255 and never wraps
+    const int shift = (256 - qmin) & 0xFF;
     if (b->tmax > 2032) return FA_ERR_TLEN;
     FA_HIP(hipSetDevice(ctx->device));
     const int64_t n = b->n;
-    // bucket by segments per lane slen = ceil(qlen / 8), longest first: the packed
-    // kernel runs two pairs per lane group in the same stripe geometry
+    // bucket by segments per lane slen = ceil(qlen / lanes), longest first: the
+    // packed kernel runs two pairs per lane group in the reference's stripe
+    // geometry (8 lanes; 16 in the 8-bit mode)
+    const int lanes = byte_mode ? 16 : 8;
     std::vector<int> sk((size_t)n);
     for (int64_t k = 0; k < n; ++k) {
         const int ql = b->qlen[(size_t)k];
-        if (ql < 1 || ql > 8 * fa::kMateMaxSlen) return FA_ERR_QLEN;
-        sk[(size_t)k] = (ql + 7) / 8;
+        if (ql < 1 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
+        sk[(size_t)k] = (ql + lanes - 1) / lanes;
     }
     // stable counting sort by slen, longest first
     std::vector<uint32_t> order((size_t)n);
@@ -1669,6 +1678,8 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     a.e_ins = e_ins;
     a.xtra = xtra;
     a.qmax = qmax;
+    a.byte_mode = byte_mode ? 1 : 0;
+    a.ovf = 255 - shift;
     b->mbuckets.swap(bk);
     b->align2_set = true;
     return FA_OK;

@@ -1,5 +1,6 @@
-// Mate-rescue local Smith-Waterman, packed: two pairs per 8-lane group (kernel
-// template; instantiated by mate16_sym.hip and mate16_kernel.hip).
+// Mate-rescue local Smith-Waterman, packed: two pairs per lane group (kernel
+// template; instantiated by mate16_sym.hip, mate16_kernel.hip and
+// mate8_kernel.hip).
 //
 // Same execution model as the reference's 8-lane SSE ksw_i16
 // (src/core/alignment/ksw_affine_gap.rs:503-735, :773-861; SimdEngine128,
@@ -14,6 +15,17 @@
 // reverse (XSTART) pass runs each half with its own query length, so its
 // segments past a half's slen are masked: they neither feed the row maximum,
 // nor the F chain, nor the lazy-F pass.
+//
+// 8-bit mode (KSW_XBYTE, ksw_u8_impl :267-494): the reference runs 16 unsigned
+// byte lanes, so a group has 16 GPU lanes and slen = ceil(qlen / 16).  Its
+// cell is H = subs(adds(H, mat + shift), shift) with shift = (u8)(256 - the
+// smallest matrix entry), E and F decay by unsigned saturating subtraction
+// (clamped at 0), and a best score reaching 255 - shift stops the pass and is
+// reported as 255.  Here the halves hold the same values unclamped above: the
+// profile adds mat itself (E, F >= 0 make the max at least 0), and a cell the
+// reference would saturate at 255 already carries a row maximum >= 255 - shift,
+// which ends that pass on the same row in both, so no saturated value is ever
+// reported.  E and F use v_pk_sub_u16 with clamp.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -44,6 +56,20 @@ __device__ __forceinline__ uint32_t subsat(uint32_t a, uint32_t b) {
 __device__ __forceinline__ uint32_t sub16(uint32_t a, uint32_t b) {
     return bits(vs(a) - vs(b));
 }
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+// unsigned saturating (clamp at 0) per half: subs_epu8 on values 0..~260
+__device__ __forceinline__ uint32_t usubsat(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, a),
+                                                                      __builtin_bit_cast(u2, b)));
+}
+// the gap decay of the mode: signed saturating (16-bit) or clamped at 0 (8-bit)
+template <bool U8>
+__device__ __forceinline__ uint32_t gsub(uint32_t a, uint32_t b) {
+    if constexpr (U8)
+        return usubsat(a, b);
+    else
+        return subsat(a, b);
+}
 // 0xFFFF in each half that is negative
 __device__ __forceinline__ uint32_t sgnmask(uint32_t d) {
     uint32_t r;
@@ -59,7 +85,8 @@ __device__ __forceinline__ void bfi_into(uint32_t &b, uint32_t m, uint32_t a) {
     asm("v_bfi_b32 %0, %1, %2, %0" : "+v"(b) : "v"(m), "v"(a));
 }
 
-constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowShr1 = 0x111;
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140,
+              kRowShr1 = 0x111;
 
 // DPP move with bound_ctrl (a lane without a source lane reads 0): no old
 // value, so no copy of the source, and 32-bit consumers fuse with it
@@ -67,29 +94,40 @@ template <int CTRL>
 __device__ __forceinline__ uint32_t dmov(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
-// per-half signed max over the 8 lanes of a group
-__device__ __forceinline__ uint32_t grp8_smax2(uint32_t x) {
+// per-half signed max over the G (8 or 16) lanes of a group
+template <int G>
+__device__ __forceinline__ uint32_t grp_smax2(uint32_t x) {
     x = smax(x, dmov<kQuadXor1>(x));
     x = smax(x, dmov<kQuadXor2>(x));
     x = smax(x, dmov<kRowHalfMirror>(x));
+    if constexpr (G == 16) x = smax(x, dmov<kRowMirror>(x));
     return x;
 }
-__device__ __forceinline__ uint32_t grp8_or(uint32_t x) {
+template <int G>
+__device__ __forceinline__ uint32_t grp_or(uint32_t x) {
     x |= dmov<kQuadXor1>(x);
     x |= dmov<kQuadXor2>(x);
     x |= dmov<kRowHalfMirror>(x);
+    if constexpr (G == 16) x |= dmov<kRowMirror>(x);
     return x;
 }
-__device__ __forceinline__ int grp8_imax(int x) {
+template <int G>
+__device__ __forceinline__ int grp_imax(int x) {
     x = max(x, (int)dmov<kQuadXor1>((uint32_t)x));
     x = max(x, (int)dmov<kQuadXor2>((uint32_t)x));
     x = max(x, (int)dmov<kRowHalfMirror>((uint32_t)x));
+    if constexpr (G == 16) x = max(x, (int)dmov<kRowMirror>((uint32_t)x));
     return x;
 }
-// value of lane p-1 of the group (lane 0: 0): the SSE shift by one lane
+// value of lane p-1 of the group (lane 0: 0): the SSE shift by one lane.  A
+// 16-lane group is a DPP row, whose lane 0 reads 0 through bound_ctrl.
+template <int G>
 __device__ __forceinline__ uint32_t shift_up(uint32_t x, int p) {
     const uint32_t y = dmov<kRowShr1>(x);
-    return p == 0 ? 0u : y;
+    if constexpr (G == 16)
+        return y;
+    else
+        return p == 0 ? 0u : y;
 }
 
 __device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *seqn, uint32_t off,
@@ -110,8 +148,8 @@ struct Res {
 
 // Target rows staged per lane group at a time: byte r of a chunk holds row
 // i0 + r of pair A in bits 0-3 and of pair B in bits 4-7.
-constexpr int kTgtChunk = 512;  // the forward staging decodes 4 + 2 words per lane
-static_assert(kTgtChunk == 512, "forward staging reads 64 rows per lane");
+constexpr int kTgtChunk = 512;  // the forward staging decodes 512 / G rows per lane
+static_assert(kTgtChunk == 512, "forward staging reads 64 (G = 8) or 32 (G = 16) rows per lane");
 
 // wave-uniform largest j in [1, SLEN] with pred(j) true for some lane
 template <int SLEN, class P>
@@ -123,7 +161,7 @@ __device__ __forceinline__ int wave_last(P pred) {
     return r;
 }
 
-// One ksw_i16 pass for the group's two pairs (`qrev`: the XSTART reverse
+// One ksw_i16 pass (U8: ksw_u8) for the group's two pairs (`qrev`: the XSTART reverse
 // pass, query position k = original qlen-1-k, target row i = original
 // trev - i while i <= trev).
 // UNIFORM: every pair of the wave has slen = SLEN (the host buckets the
@@ -134,14 +172,15 @@ __device__ __forceinline__ int wave_last(P pred) {
 // logp: lane 0 logs pair A's row maxima, lane 1 pair B's (nullptr: no log).
 // hm: the wave's copy of H at the best row so far (Hmax), word j * 64 + lane.
 // SYM: o_del == o_ins and e_del == e_ins, so E and F share the gap-open term.
-template <int SLEN, bool UNIFORM, bool SYM>
+template <int SLEN, bool UNIFORM, bool SYM, bool U8>
 __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (&hf)[2],
                                           uint8_t *tc2, uint16_t *logp, uint32_t *hm,
                                           bool qrev, Res (&res)[2]) {
+    constexpr int G = U8 ? 16 : 8;  // lanes of the reference's register
     const int lane = (int)(threadIdx.x & 63);
     int slen[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) slen[h] = hf[h].active ? (hf[h].qlen + 7) >> 3 : 0;
+    for (int h = 0; h < 2; ++h) slen[h] = hf[h].active ? (hf[h].qlen + G - 1) / G : 0;
     int smax_w = SLEN, sfull = SLEN;
     if constexpr (!UNIFORM) {
         smax_w = wave_last<SLEN>([&](int j) { return j <= slen[0] || j <= slen[1]; });
@@ -182,39 +221,41 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
     const uint32_t oeins = pk(a.o_ins + a.e_ins, a.o_ins + a.e_ins);
     // per-pair state, packed like the cells: 0xFFFF per live half, best score,
     // its row, rows run, and the stop score min(endsc, 0x7FFF) (a score of 0x7FFF
-    // is saturated and stops too)
+    // is saturated and stops too; 8-bit: min(endsc, 255 - shift))
     uint32_t alivem = (hf[0].active && hf[0].tlen > 0 ? 0xFFFFu : 0u) |
                       (hf[1].active && hf[1].tlen > 0 ? 0xFFFF0000u : 0u);
     uint32_t gmaxp = 0u, tep = 0xFFFFFFFFu, rowsp = 0u, ip = 0u;
-    const uint32_t endscp = pk((int)min(hf[0].endsc, 0x7FFFu), (int)min(hf[1].endsc, 0x7FFFu));
+    const uint32_t sat = U8 ? (uint32_t)a.ovf : 0x7FFFu;
+    const uint32_t endscp = pk((int)min(hf[0].endsc, sat), (int)min(hf[1].endsc, sat));
     const uint32_t tlenp = pk(hf[0].tlen, hf[1].tlen);
     // stage target rows [i0, i0 + kTgtChunk) of both pairs; the reverse pass reads
     // the target reversed up to trev and forward past it (the reference reverses
     // the first te + 1 bases in place, :812-813)
     auto stage = [&](int i0) {
         if (!qrev) {
-            // lane p decodes rows [i0 + 64p, i0 + 64p + 64) of both pairs from four
-            // 2-bit words and two N-bit words per pair (targets start on a 32-base
-            // boundary: `to` counts 16-base words and is even)
-            const int r0 = (kTgtChunk / 8) * p, i = i0 + r0;
-            uint32_t w[2][4], nw[2][2];
+            // lane p decodes rows [i0 + RPL p, i0 + RPL (p + 1)) of both pairs from
+            // RPL / 16 2-bit words and RPL / 32 N-bit words per pair (targets start
+            // on a 32-base boundary: `to` counts 16-base words and is even)
+            constexpr int RPL = kTgtChunk / G, NW2 = RPL / 16, NWN = RPL / 32;
+            const int r0 = RPL * p, i = i0 + r0;
+            uint32_t w[2][NW2], nw[2][NWN];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int nwords = hf[h].active ? (hf[h].tlen + 15) >> 4 : 0;
                 const int nnw = hf[h].active ? (hf[h].tlen + 31) >> 5 : 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < NW2; ++q) {
                     const int x = (i >> 4) + q;
                     w[h][q] = x < nwords ? a.seq2[hf[h].to + x] : 0u;
                 }
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
+                for (int q = 0; q < NWN; ++q) {
                     const int x = (i >> 5) + q;
                     nw[h][q] = x < nnw ? a.seqn[(hf[h].to >> 1) + x] : 0u;
                 }
             }
 #pragma unroll
-            for (int d = 0; d < kTgtChunk / 32; ++d) {
+            for (int d = 0; d < RPL / 4; ++d) {
                 uint32_t v = 0u;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
@@ -233,8 +274,8 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
             return;
         }
 #pragma unroll 8
-        for (int m = 0; m < kTgtChunk / 8; ++m) {
-            const int r = p + 8 * m, i = i0 + r;
+        for (int m = 0; m < kTgtChunk / G; ++m) {
+            const int r = p + G * m, i = i0 + r;
             uint32_t c[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -263,7 +304,7 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         const uint32_t biasp = (nb ^ 0x00010001u) << 7;
         uint32_t f = 0u, mx = 0u;
         {
-            uint32_t h = shift_up(hlast, p);
+            uint32_t h = shift_up<G>(hlast, p);
 #pragma unroll
             for (int j = 0; j < SLEN; ++j) {
                 if (j < smax_w) {
@@ -273,12 +314,12 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
                     hv = smax(smax(hv, e), f);
                     uint32_t fn;
                     if constexpr (SYM) {
-                        const uint32_t t = subsat(hv, oedel);
-                        fn = smax(subsat(f, eins), t);
-                        E[j] = smax(subsat(e, edel), t);
+                        const uint32_t t = gsub<U8>(hv, oedel);
+                        fn = smax(gsub<U8>(f, eins), t);
+                        E[j] = smax(gsub<U8>(e, edel), t);
                     } else {
-                        fn = smax(subsat(f, eins), subsat(hv, oeins));
-                        E[j] = smax(subsat(e, edel), subsat(hv, oedel));
+                        fn = smax(gsub<U8>(f, eins), gsub<U8>(hv, oeins));
+                        E[j] = smax(gsub<U8>(e, edel), gsub<U8>(hv, oedel));
                     }
                     h = Hd[j];  // H(i-1, j): the next segment's diagonal
                     Hn[j] = hv;
@@ -301,7 +342,7 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         // along the chain, so a shifted F <= 0 in every lane of a pair changes no H:
         // the first pass is skipped when that holds for the whole wave.
         uint32_t lazym = alivem;
-        const uint32_t fs = shift_up(f, p);
+        const uint32_t fs = shift_up<G>(f, p);
         if (__builtin_amdgcn_ballot_w64((sgnmask(sub16(0u, fs)) & lazym) != 0u) != 0) {
             f = fs;
             for (int pass = 0;;) {
@@ -310,7 +351,7 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
                 for (int j = 0; j < SLEN; ++j) {
                     if (j < smax_w) {
                         uint32_t hn = smax(Hn[j], f);
-                        const uint32_t fn = subsat(f, eins);
+                        const uint32_t fn = gsub<U8>(f, eins);
                         if (UNIFORM || j < sfull) {
                             f = fn;
                         } else {  // a pair without segment j keeps H and F
@@ -322,11 +363,11 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
                         bfi_into(Hn[j], lazym, hn);
                     }
                 }
-                chg = grp8_or(chg);
+                chg = grp_or<G>(chg);
                 if ((chg & 0xFFFFu) == 0u) lazym &= 0xFFFF0000u;
                 if ((chg >> 16) == 0u) lazym &= 0x0000FFFFu;
-                if (++pass == 8 || __builtin_amdgcn_ballot_w64(lazym != 0u) == 0) break;
-                f = shift_up(f, p);
+                if (++pass == G || __builtin_amdgcn_ballot_w64(lazym != 0u) == 0) break;
+                f = shift_up<G>(f, p);
             }
         }
         // H(i, slen-1) per half for the next row's shift
@@ -340,7 +381,7 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
                 hlast = bfi(eq, Hn[j], hlast);
             }
         }
-        const uint32_t rmax = grp8_smax2(mx);  // >= 0 in both halves
+        const uint32_t rmax = grp_smax2<G>(mx);  // >= 0 in both halves
         if (logp != nullptr && ((alivem >> (16 * (p & 1))) & 1u))
             logp[i] = (uint16_t)(rmax >> (16 * (p & 1)));
         rowsp += alivem & 0x00010001u;
@@ -379,7 +420,9 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
     const int rows[2] = {lo16(rowsp), hi16(rowsp)};
 #pragma unroll
     for (int hh = 0; hh < 2; ++hh) {
-        res[hh].score = gmax[hh];
+        // saturated: 0x7FFF (16-bit), or a best score >= 255 - shift reported as 255
+        const bool satd = U8 ? gmax[hh] >= a.ovf : gmax[hh] == 0x7FFF;
+        res[hh].score = U8 && satd ? 255 : gmax[hh];
         res[hh].te = te[hh];
         res[hh].rows = rows[hh];
         res[hh].qe = -1;
@@ -392,8 +435,8 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
             const int hv = hh ? hi16(hmj) : lo16(hmj);
             if (j < slen[hh] && k < hf[hh].qlen) key = max(key, ((hv + 32769) << 12) | (4095 - k));
         }
-        key = grp8_imax(key);
-        if (gmax[hh] != 0x7FFF) res[hh].qe = key ? 4095 - (key & 4095) : -1;
+        key = grp_imax<G>(key);
+        if (!satd) res[hh].qe = key ? 4095 - (key & 4095) : -1;
     }
 }
 
@@ -404,19 +447,22 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
 // longer ones would spill inside it.
 constexpr int mate16_waves(int slen) { return slen >= 14 && slen <= 20 ? 3 : 1; }
 
-template <int SLEN, bool SYM>
+// U8: the 8-bit mode, 16-lane groups (8 pairs per block); else 8-lane groups
+// (16 pairs per block).
+template <int SLEN, bool SYM, bool U8>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(mate16_waves(SLEN)))) void
 mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
+    constexpr int G = U8 ? 16 : 8, PPB = 128 / G;
     extern __shared__ uint8_t lds[];
     const int lane = (int)(threadIdx.x & 63);
-    const int g = lane >> 3, p = lane & 7;
+    const int g = lane / G, p = lane % G;
     Half hf[2];
     uint32_t pid[2], slot[2];
     bool valid[2];
     uint8_t *const tc2 = lds + g * kTgtChunk;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        slot[h] = blockIdx.x * 16u + 2u * (uint32_t)g + (uint32_t)h;
+        slot[h] = blockIdx.x * (uint32_t)PPB + 2u * (uint32_t)g + (uint32_t)h;
         valid[h] = slot[h] < count;
         pid[h] = valid[h] ? a.order[first + slot[h]] : 0u;
         hf[h].qlen = valid[h] ? a.qlen[pid[h]] : 1;
@@ -438,7 +484,7 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
     hf[0].endsc = hf[1].endsc = endsc;
     Res r[2];
     uint32_t *const hm = (uint32_t *)(lds + 8 * kTgtChunk);
-    ksw_pass2<SLEN, true, SYM>(a, p, hf, tc2, logp, hm, false, r);
+    ksw_pass2<SLEN, true, SYM, U8>(a, p, hf, tc2, logp, hm, false, r);
 
     // suboptimal hits (:639-655, :717-731) from the row log: lane 0 of the group for
     // pair A, lane 1 for pair B (selects, not indexing, keep the state in registers)
@@ -446,7 +492,7 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
     const bool mineB = p == 1;
     const bool mine_valid = mineB ? valid[1] : valid[0];
     const Res mine = mineB ? r[1] : r[0];
-    if (p < 2 && mine_valid && mine.score != 0x7FFF && minsc <= 0xFFFFu) {
+    if (p < 2 && mine_valid && mine.score != (U8 ? 255 : 0x7FFF) && minsc <= 0xFFFFu) {
         const Res rr = mine;
         const uint16_t *log = logp;
         const int iv = (rr.score + a.qmax - 1) / a.qmax;
@@ -488,7 +534,7 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
         hr[h].endsc = (uint32_t)r[h].score & 0xffffu;
     }
     Res rr[2];
-    ksw_pass2<SLEN, false, SYM>(a, p, hr, tc2, nullptr, hm, true, rr);
+    ksw_pass2<SLEN, false, SYM, U8>(a, p, hr, tc2, nullptr, hm, true, rr);
 
     if (p < 2 && mine_valid) {
         const Res f = mine, b = mineB ? rr[1] : rr[0];
@@ -516,26 +562,41 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
     }
 }
 
-// Launch over every stripe length 1..kMateMaxSlen for one gap symmetry (each
-// instantiation set lives in its own translation unit so they build in parallel).
-template <bool SYM>
+// Launch over every stripe length 1..kMateMaxSlen (8-bit: 1..kMate8MaxSlen) for
+// one gap symmetry and mode (each instantiation set lives in its own
+// translation unit so they build in parallel).
+template <bool SYM, bool U8>
 hipError_t mate16_launch_set(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                              hipStream_t st) {
     if (count == 0) return hipSuccess;
-    const dim3 grid((count + 15) / 16);
+    constexpr uint32_t PPB = U8 ? 8u : 16u;
+    const dim3 grid((count + PPB - 1) / PPB);
     switch (slen) {
 #define FA_M16(N)                                                                            \
     case N:                                                                                  \
-        hipLaunchKernelGGL((mate16_kernel<N, SYM>), grid, dim3(64),                          \
+        hipLaunchKernelGGL((mate16_kernel<N, SYM, U8>), grid, dim3(64),                      \
                            (size_t)8 * kTgtChunk + (size_t)256 * N, st, a, first, count); \
         return hipGetLastError();
         FA_M16(1) FA_M16(2) FA_M16(3) FA_M16(4) FA_M16(5) FA_M16(6) FA_M16(7) FA_M16(8)
         FA_M16(9) FA_M16(10) FA_M16(11) FA_M16(12) FA_M16(13) FA_M16(14) FA_M16(15) FA_M16(16)
-        FA_M16(17) FA_M16(18) FA_M16(19) FA_M16(20) FA_M16(21) FA_M16(22) FA_M16(23) FA_M16(24)
-        FA_M16(25) FA_M16(26) FA_M16(27) FA_M16(28) FA_M16(29) FA_M16(30) FA_M16(31) FA_M16(32)
 #undef FA_M16
-        default: return hipErrorInvalidValue;
+        default: break;
     }
+    if constexpr (!U8) {
+        switch (slen) {
+#define FA_M16(N)                                                                            \
+    case N:                                                                                  \
+        hipLaunchKernelGGL((mate16_kernel<N, SYM, false>), grid, dim3(64),                   \
+                           (size_t)8 * kTgtChunk + (size_t)256 * N, st, a, first, count); \
+        return hipGetLastError();
+            FA_M16(17) FA_M16(18) FA_M16(19) FA_M16(20) FA_M16(21) FA_M16(22) FA_M16(23)
+            FA_M16(24) FA_M16(25) FA_M16(26) FA_M16(27) FA_M16(28) FA_M16(29) FA_M16(30)
+            FA_M16(31) FA_M16(32)
+#undef FA_M16
+            default: break;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace fa

@@ -6,7 +6,7 @@ namespace fa {
 
 hipError_t mate16_launch_sym(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                              hipStream_t st) {
-    return mate16_launch_set<true>(a, first, count, slen, st);
+    return mate16_launch_set<true, false>(a, first, count, slen, st);
 }
 
 }  // namespace fa

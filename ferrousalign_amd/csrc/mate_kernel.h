@@ -1,5 +1,5 @@
-// Mate-rescue local SW kernel (ksw_align2, 16-bit mode): argument block and
-// launcher (internal to the library).
+// Mate-rescue local SW kernel (ksw_align2, 16-bit and 8-bit modes): argument
+// block and launcher (internal to the library).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,14 +27,17 @@ struct MateArgs {
     int32_t o_del, e_del, o_ins, e_ins;
     uint32_t xtra;
     int32_t qmax;         // largest matrix entry (>= 1)
+    int32_t byte_mode;    // 1: the 8-bit mode (KSW_XBYTE)
+    int32_t ovf;          // 8-bit mode: 255 - shift, the best score that saturates
 };
 
 // Longest stripe (segments per lane) a launch takes: qlen <= 256.
-constexpr int kMateMaxSlen = 32;
-// Launch over packed positions [first, first+count), two pairs per 8-lane
-// group, 16 pairs per 64-thread block:
-// every pair of [first, first+count) must have slen = ceil(qlen/8) = `slen`
-// (1..32); imax_log must hold count * log_stride entries.
+constexpr int kMateMaxSlen = 32;   // 16-bit mode, 8 lanes
+constexpr int kMate8MaxSlen = 16;  // 8-bit mode, 16 lanes
+// Launch over packed positions [first, first+count), two pairs per lane
+// group (8 lanes, 16 pairs per 64-thread block; 8-bit mode: 16 lanes, 8 pairs):
+// every pair of [first, first+count) must have slen = ceil(qlen/lanes) =
+// `slen` (1..32, 8-bit 1..16); imax_log must hold count * log_stride entries.
 hipError_t mate16_launch(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                          hipStream_t st);
 

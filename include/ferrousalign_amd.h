@@ -266,7 +266,7 @@ int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cel
 int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count);
 int fa_batch_free(fa_ctx *ctx, fa_batch *b);
 
-/* ---- mate-rescue local Smith-Waterman (ksw_align2, 16-bit mode) --------- */
+/* ---- mate-rescue local Smith-Waterman (ksw_align2, 16-bit and 8-bit) ---- */
 /* Kswr, src/core/alignment/ksw_affine_gap.rs:51-61. */
 typedef struct {
     int32_t score, te, qe, score2, te2, tb, qb;
@@ -274,11 +274,15 @@ typedef struct {
 /* Replaces the per-job ksw_align2 call of execute_mate_rescue_batch_scalar
  * (src/pipelines/linear/paired/mate_rescue.rs:640-685; ksw_affine_gap.rs:
  * 773-861) for a whole batch: out[k] is exactly what ksw_align2 with the
- * reference's 8-lane SSE engine returns for job k, including score2/te2 and
- * the XSTART reverse pass.  16-bit mode only (xtra without KSW_XBYTE); the
- * N row of mat must score 0 (true of the mate-rescue matrix,
- * mate_rescue.rs:116-122); 1 <= qlen <= 256, tlen <= 2032.  cells[k]
- * (optional): target rows run times query length over both passes. */
+ * reference's SSE engine returns for job k, including score2/te2 and the
+ * XSTART reverse pass, in the mode xtra selects: 16-bit (8 lanes,
+ * ksw_i16_impl :503-735) or, with KSW_XBYTE, 8-bit (16 lanes, ksw_u8_impl
+ * :267-494; a best score reaching 255 - shift is reported as 255 with
+ * qe = -1, as the reference does; mate_rescue.rs:314-321 sets KSW_XBYTE when
+ * qlen * match < 250).  The N row of mat must score 0 (true of the
+ * mate-rescue matrix, mate_rescue.rs:116-122); 1 <= qlen <= FA_MAX_QLEN,
+ * tlen <= 2032.  cells[k] (optional): target rows run times query length
+ * over both passes. */
 int fa_ksw_align2_batch(fa_ctx *ctx, int64_t n, const uint8_t *query_seqs,
                         const int64_t *query_offset, const int32_t *query_len,
                         const uint8_t *target_seqs, const int64_t *target_offset,

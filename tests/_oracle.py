@@ -124,6 +124,8 @@ def _align_lib():
     if not getattr(l, "_align", False):
         l.ora_ksw_align2_i16.argtypes = [_I32, _P, _I32, _P, _I32, _P, _I32, _I32, _I32, _I32, ctypes.c_uint32, _P, _P]
         l.ora_ksw_align2_i16.restype = ctypes.c_int
+        l.ora_ksw_align2.argtypes = l.ora_ksw_align2_i16.argtypes
+        l.ora_ksw_align2.restype = ctypes.c_int
         l.ora_ksw_align2_batch.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _I32, _P, _I32, _I32, _I32, _I32,
                                            ctypes.c_uint32, _P, _P, _P, ctypes.c_int]
         l.ora_ksw_align2_batch.restype = None
@@ -134,15 +136,24 @@ def _align_lib():
 KSW_XSTOP, KSW_XSUBO, KSW_XSTART, KSW_XBYTE = 0x20000, 0x40000, 0x80000, 0x10000
 
 
-def ksw_align2_i16(query, target, mat, o_del, e_del, o_ins, e_ins, xtra):
+def _align2_call(fn, query, target, mat, o_del, e_del, o_ins, e_ins, xtra):
     qb = np.concatenate([np.asarray(query, np.uint8), np.zeros(1, np.uint8)])
     tb = np.concatenate([np.asarray(target, np.uint8), np.zeros(1, np.uint8)])
     mat = np.ascontiguousarray(mat, np.int8)
     out = np.zeros(7, np.int32)
     cells = np.zeros(1, np.uint64)
-    rc = _align_lib().ora_ksw_align2_i16(len(qb) - 1, P(qb), len(tb) - 1, P(tb), 5, P(mat), o_del, e_del, o_ins,
-                                         e_ins, xtra, P(out), P(cells))
+    rc = fn(len(qb) - 1, P(qb), len(tb) - 1, P(tb), 5, P(mat), o_del, e_del, o_ins, e_ins, xtra, P(out), P(cells))
     return rc, tuple(int(x) for x in out), int(cells[0])
+
+
+def ksw_align2_i16(query, target, mat, o_del, e_del, o_ins, e_ins, xtra):
+    """16-bit mode only (rc -1 when KSW_XBYTE is set)."""
+    return _align2_call(_align_lib().ora_ksw_align2_i16, query, target, mat, o_del, e_del, o_ins, e_ins, xtra)
+
+
+def ksw_align2(query, target, mat, o_del, e_del, o_ins, e_ins, xtra):
+    """The mode xtra selects: 8-bit with KSW_XBYTE, else 16-bit."""
+    return _align2_call(_align_lib().ora_ksw_align2, query, target, mat, o_del, e_del, o_ins, e_ins, xtra)
 
 
 def ksw_align2_batch(q, qoff, qlen, t, toff, tlen, mat, o_del, e_del, o_ins, e_ins, xtra, nthreads=0):

@@ -1,5 +1,5 @@
-"""GPU mate-rescue local SW (ksw_align2, 16-bit mode): every Kswr field and
-the cell count bit-exact against the CPU oracle."""
+"""GPU mate-rescue local SW (ksw_align2, 16-bit and 8-bit modes): every Kswr
+field and the cell count bit-exact against the CPU oracle."""
 import json
 import os
 import random
@@ -41,35 +41,106 @@ def test_mate_golden_fixture():
     _check(out, cells, z["out"], z["cells"].astype(np.uint64), "golden fixture")
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_mate_random_shapes_and_flags(seed):
-    from ferrousalign_amd import synth
-    from ferrousalign_amd.banded_swa import ksw_align2_batch
-    rng = random.Random(seed)
+def _random_cases(rng, n=200, qmax=255, copy_rate=0.92):
     cases = []
-    for _ in range(200):
-        ql = rng.randrange(1, 257)
+    for _ in range(n):
+        ql = rng.randrange(1, qmax + 1)
         tl = rng.randrange(0, 700)
         q = [rng.randrange(5) for _ in range(ql)]
         t = [rng.randrange(5) for _ in range(tl)]
         if tl > ql and rng.random() < 0.7:
             s = rng.randrange(0, tl - ql + 1)
             for k in range(ql):
-                t[s + k] = q[k] if rng.random() < 0.92 else rng.randrange(4)
+                t[s + k] = q[k] if rng.random() < copy_rate else rng.randrange(4)
         cases.append((q, t))
+    return cases
+
+
+def _run_cases(cases, mat, gaps, xtra, label):
+    from ferrousalign_amd.banded_swa import ksw_align2_batch
+    q = np.concatenate([np.asarray(c[0], np.uint8) for c in cases] + [np.zeros(1, np.uint8)])
+    t = np.concatenate([np.asarray(c[1], np.uint8) for c in cases] + [np.zeros(1, np.uint8)])
+    ql = np.array([len(c[0]) for c in cases], np.int32)
+    tl = np.array([len(c[1]) for c in cases], np.int32)
+    qo = np.concatenate([[0], np.cumsum(ql)[:-1]]).astype(np.int64)
+    to = np.concatenate([[0], np.cumsum(tl)[:-1]]).astype(np.int64)
+    print(label, hex(xtra), gaps, flush=True)
+    out, cells = ksw_align2_batch(q, qo, ql, t, to, tl, mat, *gaps, xtra)
+    ora, oc, rc = O.ksw_align2_batch(q, qo, ql, t, to, tl, mat, *gaps, xtra)
+    assert (rc == 0).all()
+    _check(out, cells, ora, oc, "%s xtra=%#x" % (label, xtra))
+    return ora
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_mate_random_shapes_and_flags(seed):
+    from ferrousalign_amd import synth
+    cases = _random_cases(random.Random(seed))
     for xtra, gaps in ((synth.MATE_XTRA, (6, 1, 6, 1)), (O.KSW_XSTART, (5, 2, 3, 1)),
                        (O.KSW_XSUBO | O.KSW_XSTART | 40, (6, 1, 6, 1)), (0, (0, 1, 0, 1))):
-        q = np.concatenate([np.asarray(c[0], np.uint8) for c in cases] + [np.zeros(1, np.uint8)])
-        t = np.concatenate([np.asarray(c[1], np.uint8) for c in cases] + [np.zeros(1, np.uint8)])
-        ql = np.array([len(c[0]) for c in cases], np.int32)
-        tl = np.array([len(c[1]) for c in cases], np.int32)
-        qo = np.concatenate([[0], np.cumsum(ql)[:-1]]).astype(np.int64)
-        to = np.concatenate([[0], np.cumsum(tl)[:-1]]).astype(np.int64)
-        print("mate random", seed, hex(xtra), gaps, flush=True)
-        out, cells = ksw_align2_batch(q, qo, ql, t, to, tl, synth.MATE_MATRIX, *gaps, xtra)
-        ora, oc, rc = O.ksw_align2_batch(q, qo, ql, t, to, tl, synth.MATE_MATRIX, *gaps, xtra)
-        assert (rc == 0).all()
-        _check(out, cells, ora, oc, "random xtra=%#x" % xtra)
+        _run_cases(cases, synth.MATE_MATRIX, gaps, xtra, "mate random %d" % seed)
+
+
+# ---- 8-bit mode (KSW_XBYTE) ------------------------------------------------------
+
+def test_mate_u8_workload_subset():
+    from ferrousalign_amd import synth
+    from ferrousalign_amd.banded_swa import ksw_align2_batch
+    ps = synth.mate_pairs(2048)
+    flat = ps.flat()
+    xtra = synth.MATE_XTRA | O.KSW_XBYTE
+    print("mate u8 subset", flush=True)
+    out, cells = ksw_align2_batch(*flat, synth.MATE_MATRIX, 6, 1, 6, 1, xtra)
+    ora, oc, rc = O.ksw_align2_batch(*flat, synth.MATE_MATRIX, 6, 1, 6, 1, xtra)
+    assert (rc == 0).all()
+    _check(out, cells, ora, oc, "mate u8 workload")
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_mate_u8_random_shapes_and_flags(seed):
+    from ferrousalign_amd import synth
+    cases = _random_cases(random.Random(50 + seed))
+    for xtra, gaps in ((synth.MATE_XTRA, (6, 1, 6, 1)), (O.KSW_XSTART, (5, 2, 3, 1)),
+                       (O.KSW_XSUBO | O.KSW_XSTART | 40, (6, 1, 6, 1)), (0, (0, 1, 0, 1))):
+        _run_cases(cases, synth.MATE_MATRIX, gaps, xtra | O.KSW_XBYTE, "mate u8 random %d" % seed)
+
+
+def test_mate_u8_saturation():
+    # best scores crossing 255 - shift: long near-exact copies under the mate matrix
+    # (shift 4, saturating at 251) and under match 3 / mismatch -2 (shift 2, at 253);
+    # a saturated pair reports score 255, qe -1 and no reverse pass
+    from ferrousalign_amd import synth
+    rng = random.Random(7)
+    cases = _random_cases(rng, n=160, copy_rate=0.995)
+    for k in range(40):  # 30 exact copies of 251..255 bases (saturating), 10 of 240..250
+        ql = 251 + k % 5 if k < 30 else 240 + k % 11
+        tl = rng.randrange(ql, 700)
+        t = [rng.randrange(4) for _ in range(tl)]
+        s = rng.randrange(0, tl - ql + 1)
+        cases.append((t[s:s + ql], t))
+    ora = _run_cases(cases, synth.MATE_MATRIX, (6, 1, 6, 1), synth.MATE_XTRA | O.KSW_XBYTE, "mate u8 sat")
+    assert (ora[:, 0] == 255).sum() >= 30 and ((ora[:, 0] == 255) <= (ora[:, 2] == -1)).all()
+    m3 = [3 if a == b and a < 4 else (0 if a == 4 or b == 4 else -2) for a in range(5) for b in range(5)]
+    ora = _run_cases(cases, m3, (5, 2, 5, 2), O.KSW_XSUBO | O.KSW_XSTART | 30 | O.KSW_XBYTE, "mate u8 sat m3")
+    assert (ora[:, 0] == 255).sum() >= 40
+
+
+def test_mate_u8_full_size_digest():
+    from ferrousalign_amd import synth
+    from tests.golden.make_golden_mate import digest
+    d = json.load(open(os.path.join(HERE, "golden", "mate_digests.json")))
+    key = "mate_u8_%d" % synth.MATE_PAIRS
+    n = synth.MATE_PAIRS
+    from ferrousalign_amd.banded_swa import DeviceBatch
+    from ferrousalign_amd import _native as N
+    ps = synth.mate_pairs(n)
+    print("mate u8 full size", n, flush=True)
+    db = DeviceBatch(synth.default_scoring(), *ps.flat(), np.ones(n, np.int32), np.zeros(n, np.int32))
+    db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA | O.KSW_XBYTE)
+    db.run(N.FA_RUN_ALIGN2)
+    out, cells = db.download_align2()
+    assert digest(out, cells) == d[key]
+    db.free()
 
 
 def test_mate_full_size_digest():

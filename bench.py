@@ -150,6 +150,26 @@ def digest_of(out, cells):
     return hashlib.sha256(out.tobytes() + cells.astype(np.uint32).tobytes()).hexdigest()
 
 
+def cpu_reference_kswv(ps, threads, m=262144):
+    """The reference's own SIMD mate-rescue path restated (oracle/kswv_baseline.c:
+    dispatch_kswv_soa with batch_ksw_align_avx2, 32 jobs per chunk, as
+    execute_mate_rescue_batch_simd runs it by default on x86) over the first m
+    pairs on all host threads.  Its records are not ksw_align2's (approximate
+    tb/qb, its own second-best search), so it is a timing baseline only."""
+    from tests import _oracle as O
+    sub = ps.subset(np.arange(min(m, ps.n)))
+    t0 = time.perf_counter()
+    out = O.kswv_batch(*sub.flat(), simd=True, nthreads=threads)
+    dt = time.perf_counter() - t0
+    if out is None:
+        return None
+    cells = float(sub.n) * 151 * 1500  # every lane sweeps its chunk's full matrix
+    return {"value": round(sub.n / dt, 1), "unit": "pairs/s", "gcups": round(cells / dt / 1e9, 3),
+            "cores": threads, "kind": "port", "isa": "avx2",
+            "sample": "first %d pairs through the restated kswv AVX2 path (kswv_avx2.rs:93-635, "
+                      "dispatch.rs:270-395) on %d host threads: %.2f s" % (sub.n, threads, dt)}
+
+
 def mate_bench(args, ctx, golden, byte_mode=False):
     """Mate-rescue local SW (ksw_align2, 16-bit mode; byte_mode: the 8-bit
     mode, xtra | KSW_XBYTE, which mate_rescue.rs:314-321 selects for 151 bp
@@ -197,6 +217,7 @@ def mate_bench(args, ctx, golden, byte_mode=False):
                                "sample": "first %d pairs through the C restatement of ksw_align2 %s "
                                          "(ksw_affine_gap.rs:267-861) on %d host threads: %.1f s"
                                          % (m, mode, threads, dt)}
+        res["cpu_baseline_reference_simd"] = cpu_reference_kswv(ps, threads)
     print(json.dumps(res), flush=True)
 
 

@@ -257,3 +257,29 @@ def refsimd_batch(sw, qseq, qoff, qlen, tseq, toff, tlen, h0, w, isa="auto", nth
         return None
     names = {v: k for k, v in REFSIMD_ISA.items()}
     return out[:n], names[int(info[0])], bool(info[1])
+
+
+def _kswv_lib():
+    l = lib()
+    if not getattr(l, "_kswv", False):
+        l.kv_batch.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
+                               ctypes.c_int, _P, ctypes.c_int]
+        l.kv_batch.restype = ctypes.c_int
+        l._kswv = True
+    return l
+
+
+def kswv_batch(q, qoff, qlen, t, toff, tlen, match=1, mis=-4, ambig=0, gaps=(6, 1, 6, 1), simd=True,
+               nthreads=0, xtra=None):
+    """The reference's horizontal kswv mate-rescue path restated
+    (oracle/kswv_baseline.c): chunks of 32 jobs through the AVX2 kernel
+    (simd=True) or its plain-C twin.  Returns out[n, 7] (score, te, qe,
+    score2, te2, tb, qb), or None when AVX2 is absent or the input is refused."""
+    n = len(qlen)
+    arrs = [np.ascontiguousarray(x, dt) for x, dt in ((q, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (t, np.uint8), (toff, np.int64), (tlen, np.int32))]
+    xt = None if xtra is None else np.ascontiguousarray(xtra, np.int32)
+    out = np.zeros((n, 7), np.int32)
+    rc = _kswv_lib().kv_batch(n, *[P(a) for a in arrs], P(xt) if xt is not None else None, match, mis, ambig,
+                              *gaps, 1 if simd else 0, P(out), nthreads)
+    return None if rc else out

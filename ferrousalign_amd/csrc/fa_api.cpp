@@ -1298,7 +1298,7 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
                 FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
                 if (k < cap) {
                     if (slots) slots[k] = bk.S;
-                    // mate kernel, lanes per pair group: 8 (16-bit mode) or 16 (8-bit)
+                    // mate kernel, SSE lanes per pair: 8 (16-bit mode) or 16 (8-bit)
                     if (kernel) kernel[k] = 2 << 8 | (b->margs.byte_mode ? 16 : 8);
                     if (first) first[k] = bk.first + off;
                     if (count) count[k] = cnt;
@@ -1612,8 +1612,8 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     FA_HIP(hipSetDevice(ctx->device));
     const int64_t n = b->n;
     // bucket by segments per lane slen = ceil(qlen / lanes), longest first: the
-    // packed kernel runs two pairs per lane group in the reference's stripe
-    // geometry (8 lanes; 16 in the 8-bit mode)
+    // packed kernel runs two pairs per 8-lane group in the reference's stripe
+    // geometry (8 lanes; 16 byte lanes in the 8-bit mode, two per GPU lane)
     const int lanes = byte_mode ? 16 : 8;
     std::vector<int> sk((size_t)n);
     for (int64_t k = 0; k < n; ++k) {

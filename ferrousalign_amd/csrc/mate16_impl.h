@@ -17,7 +17,12 @@
 // nor the F chain, nor the lazy-F pass.
 //
 // 8-bit mode (KSW_XBYTE, ksw_u8_impl :267-494): the reference runs 16 unsigned
-// byte lanes, so a group has 16 GPU lanes and slen = ceil(qlen / 16).  Its
+// byte lanes with slen = ceil(qlen / 16); GPU lane p of an 8-lane group plays
+// SSE lanes 2p and 2p+1 one after the other, so its registers j = 0..2 slen-1
+// hold query positions j + 2p slen (the 16-bit mapping with 2 slen registers):
+// F restarts at 0 where SSE lane 2p+1 begins (register slen), and the lazy-F
+// pass carries both lanes' F chains so that F still moves one SSE lane per
+// pass, as the reference's byte shift does.  Its
 // cell is H = subs(adds(H, mat + shift), shift) with shift = (u8)(256 - the
 // smallest matrix entry), E and F decay by unsigned saturating subtraction
 // (clamped at 0), and a best score reaching 255 - shift stops the pass and is
@@ -85,8 +90,7 @@ __device__ __forceinline__ void bfi_into(uint32_t &b, uint32_t m, uint32_t a) {
     asm("v_bfi_b32 %0, %1, %2, %0" : "+v"(b) : "v"(m), "v"(a));
 }
 
-constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140,
-              kRowShr1 = 0x111;
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowShr1 = 0x111;
 
 // DPP move with bound_ctrl (a lane without a source lane reads 0): no old
 // value, so no copy of the source, and 32-bit consumers fuse with it
@@ -94,40 +98,34 @@ template <int CTRL>
 __device__ __forceinline__ uint32_t dmov(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
-// per-half signed max over the G (8 or 16) lanes of a group
-template <int G>
-__device__ __forceinline__ uint32_t grp_smax2(uint32_t x) {
+// per-half signed max over the 8 lanes of a group
+__device__ __forceinline__ uint32_t grp8_smax2(uint32_t x) {
     x = smax(x, dmov<kQuadXor1>(x));
     x = smax(x, dmov<kQuadXor2>(x));
     x = smax(x, dmov<kRowHalfMirror>(x));
-    if constexpr (G == 16) x = smax(x, dmov<kRowMirror>(x));
     return x;
 }
-template <int G>
-__device__ __forceinline__ uint32_t grp_or(uint32_t x) {
+__device__ __forceinline__ uint32_t grp8_or(uint32_t x) {
     x |= dmov<kQuadXor1>(x);
     x |= dmov<kQuadXor2>(x);
     x |= dmov<kRowHalfMirror>(x);
-    if constexpr (G == 16) x |= dmov<kRowMirror>(x);
     return x;
 }
-template <int G>
-__device__ __forceinline__ int grp_imax(int x) {
+__device__ __forceinline__ int grp8_imax(int x) {
     x = max(x, (int)dmov<kQuadXor1>((uint32_t)x));
     x = max(x, (int)dmov<kQuadXor2>((uint32_t)x));
     x = max(x, (int)dmov<kRowHalfMirror>((uint32_t)x));
-    if constexpr (G == 16) x = max(x, (int)dmov<kRowMirror>((uint32_t)x));
     return x;
 }
-// value of lane p-1 of the group (lane 0: 0): the SSE shift by one lane.  A
-// 16-lane group is a DPP row, whose lane 0 reads 0 through bound_ctrl.
-template <int G>
+// value of lane p-1 of the group (lane 0: 0): the SSE shift by one lane
 __device__ __forceinline__ uint32_t shift_up(uint32_t x, int p) {
     const uint32_t y = dmov<kRowShr1>(x);
-    if constexpr (G == 16)
-        return y;
-    else
-        return p == 0 ? 0u : y;
+    return p == 0 ? 0u : y;
+}
+// 0xFFFF in each half whose value equals j
+__device__ __forceinline__ uint32_t eq16(int j, uint32_t vp) {
+    const uint32_t d = sub16((uint32_t)j * 0x00010001u, vp);
+    return ~(sgnmask(d) | sgnmask(sub16(0u, d)));
 }
 
 __device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *seqn, uint32_t off,
@@ -148,8 +146,8 @@ struct Res {
 
 // Target rows staged per lane group at a time: byte r of a chunk holds row
 // i0 + r of pair A in bits 0-3 and of pair B in bits 4-7.
-constexpr int kTgtChunk = 512;  // the forward staging decodes 512 / G rows per lane
-static_assert(kTgtChunk == 512, "forward staging reads 64 (G = 8) or 32 (G = 16) rows per lane");
+constexpr int kTgtChunk = 512;  // the forward staging decodes 4 + 2 words per lane
+static_assert(kTgtChunk == 512, "forward staging reads 64 rows per lane");
 
 // wave-uniform largest j in [1, SLEN] with pred(j) true for some lane
 template <int SLEN, class P>
@@ -161,7 +159,7 @@ __device__ __forceinline__ int wave_last(P pred) {
     return r;
 }
 
-// One ksw_i16 pass (U8: ksw_u8) for the group's two pairs (`qrev`: the XSTART reverse
+// One ksw_i16 pass (U8: ksw_u8, SLEN = 2 x its slen) for the group's two pairs (`qrev`: the XSTART reverse
 // pass, query position k = original qlen-1-k, target row i = original
 // trev - i while i <= trev).
 // UNIFORM: every pair of the wave has slen = SLEN (the host buckets the
@@ -176,25 +174,32 @@ template <int SLEN, bool UNIFORM, bool SYM, bool U8>
 __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (&hf)[2],
                                           uint8_t *tc2, uint16_t *logp, uint32_t *hm,
                                           bool qrev, Res (&res)[2]) {
-    constexpr int G = U8 ? 16 : 8;  // lanes of the reference's register
+    // R SSE lanes per GPU lane; slen = segments per SSE lane, ext = registers used
+    constexpr int G = 8;  // GPU lanes per pair group
+    constexpr int R = U8 ? 2 : 1, P16 = U8 ? 16 : 8;
+    static_assert(SLEN % R == 0, "8-bit passes run an even register count");
     const int lane = (int)(threadIdx.x & 63);
-    int slen[2];
+    int slen[2], ext[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) slen[h] = hf[h].active ? (hf[h].qlen + G - 1) / G : 0;
+    for (int h = 0; h < 2; ++h) {
+        slen[h] = hf[h].active ? (hf[h].qlen + P16 - 1) / P16 : 0;
+        ext[h] = R * slen[h];
+    }
     int smax_w = SLEN, sfull = SLEN;
     if constexpr (!UNIFORM) {
-        smax_w = wave_last<SLEN>([&](int j) { return j <= slen[0] || j <= slen[1]; });
-        // the smallest slen among the wave's active pairs: slots below it exist
+        smax_w = wave_last<SLEN>([&](int j) { return j <= ext[0] || j <= ext[1]; });
+        // the smallest extent among the wave's active pairs: slots below it exist
         // for every pair
         sfull = SLEN;
 #pragma unroll
         for (int j = SLEN; j >= 1; --j)
-            if (__builtin_amdgcn_ballot_w64((hf[0].active && j > slen[0]) ||
-                                            (hf[1].active && j > slen[1])) != 0)
+            if (__builtin_amdgcn_ballot_w64((hf[0].active && j > ext[0]) ||
+                                            (hf[1].active && j > ext[1])) != 0)
                 sfull = j - 1;
     }
-    const uint32_t slenp = pk(slen[0], slen[1]);
-    const uint32_t slm1p = pk(slen[0] - 1, slen[1] - 1);
+    const uint32_t slenp = pk(ext[0], ext[1]);
+    const uint32_t slm1p = pk(ext[0] - 1, ext[1] - 1);
+    const uint32_t splitp = pk(slen[0], slen[1]);  // 8-bit: first register of SSE lane 2p+1
 
     uint32_t profA[SLEN], profB[SLEN], H[SLEN], H2[SLEN], E[SLEN];
 #pragma unroll
@@ -202,9 +207,9 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         uint32_t pw[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int k = j + p * slen[h];
+            const int k = j + p * ext[h];
             pw[h] = 0x80808080u;  // past the query: score 0 for every target base
-            if (hf[h].active && j < slen[h] && k < hf[h].qlen) {
+            if (hf[h].active && j < ext[h] && k < hf[h].qlen) {
                 const int orig = qrev ? hf[h].qlen - 1 - k : k;
                 pw[h] = a.profw[seq_code(a.seq2, a.seqn, hf[h].qo, orig)];
             }
@@ -302,12 +307,24 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         const uint32_t nb = ((tw & 4u) >> 2) | ((tw & 0x40u) << 10);
         const uint32_t sel = (tw & 3u) | ((tw & 0x30u) << 12) | 0x0C040C00u | (nb * 0x0Cu);
         const uint32_t biasp = (nb ^ 0x00010001u) << 7;
-        uint32_t f = 0u, mx = 0u;
+        uint32_t f = 0u, mx = 0u, fA = 0u;  // fA: 8-bit, SSE lane 2p's F at its end
         {
-            uint32_t h = shift_up<G>(hlast, p);
+            uint32_t h = shift_up(hlast, p);
 #pragma unroll
             for (int j = 0; j < SLEN; ++j) {
                 if (j < smax_w) {
+                    if constexpr (U8) {  // SSE lane 2p+1 begins: its F enters at 0
+                        if constexpr (UNIFORM) {
+                            if (j == SLEN / 2) {
+                                fA = f;
+                                f = 0u;
+                            }
+                        } else if (j >= 1 && j <= SLEN / 2) {
+                            const uint32_t eq = eq16(j, splitp);
+                            fA = bfi(eq, f, fA);
+                            f &= ~eq;
+                        }
+                    }
                     const uint32_t sb = sub16(__builtin_amdgcn_perm(profB[j], profA[j], sel), biasp);
                     uint32_t hv = addsat(h, sb);
                     const uint32_t e = E[j];
@@ -341,15 +358,32 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         // pass without change.  Every lane's F entered its segment 0 at 0 and H >= F
         // along the chain, so a shifted F <= 0 in every lane of a pair changes no H:
         // the first pass is skipped when that holds for the whole wave.
+        // 8-bit: SSE lane 2p takes lane 2p-1's F (GPU lane p-1's second chain) and
+        // lane 2p+1 takes lane 2p's (this lane's first chain), both from the last pass
         uint32_t lazym = alivem;
-        const uint32_t fs = shift_up<G>(f, p);
-        if (__builtin_amdgcn_ballot_w64((sgnmask(sub16(0u, fs)) & lazym) != 0u) != 0) {
+        const uint32_t fs = shift_up(f, p);
+        uint32_t fsB = fA;
+        uint32_t pos = sgnmask(sub16(0u, fs));
+        if constexpr (U8) pos |= sgnmask(sub16(0u, fsB));
+        if (__builtin_amdgcn_ballot_w64((pos & lazym) != 0u) != 0) {
             f = fs;
             for (int pass = 0;;) {
-                uint32_t chg = 0u;
+                uint32_t chg = 0u, fAend = 0u;
 #pragma unroll
                 for (int j = 0; j < SLEN; ++j) {
                     if (j < smax_w) {
+                        if constexpr (U8) {
+                            if constexpr (UNIFORM) {
+                                if (j == SLEN / 2) {
+                                    fAend = f;
+                                    f = fsB;
+                                }
+                            } else if (j >= 1 && j <= SLEN / 2) {
+                                const uint32_t eq = eq16(j, splitp);
+                                fAend = bfi(eq, f, fAend);
+                                f = bfi(eq, fsB, f);
+                            }
+                        }
                         uint32_t hn = smax(Hn[j], f);
                         const uint32_t fn = gsub<U8>(f, eins);
                         if (UNIFORM || j < sfull) {
@@ -363,11 +397,12 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
                         bfi_into(Hn[j], lazym, hn);
                     }
                 }
-                chg = grp_or<G>(chg);
+                chg = grp8_or(chg);
                 if ((chg & 0xFFFFu) == 0u) lazym &= 0xFFFF0000u;
                 if ((chg >> 16) == 0u) lazym &= 0x0000FFFFu;
-                if (++pass == G || __builtin_amdgcn_ballot_w64(lazym != 0u) == 0) break;
-                f = shift_up<G>(f, p);
+                if (++pass == P16 || __builtin_amdgcn_ballot_w64(lazym != 0u) == 0) break;
+                f = shift_up(f, p);
+                fsB = fAend;
             }
         }
         // H(i, slen-1) per half for the next row's shift
@@ -376,12 +411,10 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
             if constexpr (UNIFORM) {
                 if (j == SLEN - 1) hlast = Hn[j];
             } else if (j < smax_w && j + 1 >= sfull) {
-                const uint32_t d = sub16((uint32_t)j * 0x00010001u, slm1p);
-                const uint32_t eq = ~(sgnmask(d) | sgnmask(sub16(0u, d)));  // d == 0
-                hlast = bfi(eq, Hn[j], hlast);
+                hlast = bfi(eq16(j, slm1p), Hn[j], hlast);
             }
         }
-        const uint32_t rmax = grp_smax2<G>(mx);  // >= 0 in both halves
+        const uint32_t rmax = grp8_smax2(mx);  // >= 0 in both halves
         if (logp != nullptr && ((alivem >> (16 * (p & 1))) & 1u))
             logp[i] = (uint16_t)(rmax >> (16 * (p & 1)));
         rowsp += alivem & 0x00010001u;
@@ -430,12 +463,12 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
         int key = 0;
 #pragma unroll
         for (int j = 0; j < SLEN; ++j) {
-            const int k = j + p * slen[hh];
+            const int k = j + p * ext[hh];
             const uint32_t hmj = hm[j * 64 + lane];
             const int hv = hh ? hi16(hmj) : lo16(hmj);
-            if (j < slen[hh] && k < hf[hh].qlen) key = max(key, ((hv + 32769) << 12) | (4095 - k));
+            if (j < ext[hh] && k < hf[hh].qlen) key = max(key, ((hv + 32769) << 12) | (4095 - k));
         }
-        key = grp_imax<G>(key);
+        key = grp8_imax(key);
         if (!satd) res[hh].qe = key ? 4095 - (key & 4095) : -1;
     }
 }
@@ -447,15 +480,15 @@ __device__ __forceinline__ void ksw_pass2(const MateArgs &a, int p, const Half (
 // longer ones would spill inside it.
 constexpr int mate16_waves(int slen) { return slen >= 14 && slen <= 20 ? 3 : 1; }
 
-// U8: the 8-bit mode, 16-lane groups (8 pairs per block); else 8-lane groups
-// (16 pairs per block).
+// Two pairs per 8-lane group, 16 pairs per single-wave block; SLEN registers per
+// lane (U8, the 8-bit mode: SLEN = 2 x its slen).
 template <int SLEN, bool SYM, bool U8>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(mate16_waves(SLEN)))) void
 mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
-    constexpr int G = U8 ? 16 : 8, PPB = 128 / G;
+    constexpr int PPB = 16;
     extern __shared__ uint8_t lds[];
     const int lane = (int)(threadIdx.x & 63);
-    const int g = lane / G, p = lane % G;
+    const int g = lane >> 3, p = lane & 7;
     Half hf[2];
     uint32_t pid[2], slot[2];
     bool valid[2];
@@ -564,39 +597,40 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
 
 // Launch over every stripe length 1..kMateMaxSlen (8-bit: 1..kMate8MaxSlen) for
 // one gap symmetry and mode (each instantiation set lives in its own
-// translation unit so they build in parallel).
+// translation unit so they build in parallel).  The 8-bit kernel of stripe
+// length s runs 2 s registers per lane.
 template <bool SYM, bool U8>
 hipError_t mate16_launch_set(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                              hipStream_t st) {
     if (count == 0) return hipSuccess;
-    constexpr uint32_t PPB = U8 ? 8u : 16u;
-    const dim3 grid((count + PPB - 1) / PPB);
-    switch (slen) {
-#define FA_M16(N)                                                                            \
-    case N:                                                                                  \
-        hipLaunchKernelGGL((mate16_kernel<N, SYM, U8>), grid, dim3(64),                      \
-                           (size_t)8 * kTgtChunk + (size_t)256 * N, st, a, first, count); \
+    const dim3 grid((count + 15) / 16);
+    if constexpr (U8) {
+        switch (slen) {
+#define FA_M8(N)                                                                                 \
+    case N:                                                                                      \
+        hipLaunchKernelGGL((mate16_kernel<2 * N, SYM, true>), grid, dim3(64),                    \
+                           (size_t)8 * kTgtChunk + (size_t)256 * 2 * N, st, a, first, count); \
         return hipGetLastError();
-        FA_M16(1) FA_M16(2) FA_M16(3) FA_M16(4) FA_M16(5) FA_M16(6) FA_M16(7) FA_M16(8)
-        FA_M16(9) FA_M16(10) FA_M16(11) FA_M16(12) FA_M16(13) FA_M16(14) FA_M16(15) FA_M16(16)
-#undef FA_M16
-        default: break;
-    }
-    if constexpr (!U8) {
+            FA_M8(1) FA_M8(2) FA_M8(3) FA_M8(4) FA_M8(5) FA_M8(6) FA_M8(7) FA_M8(8)
+            FA_M8(9) FA_M8(10) FA_M8(11) FA_M8(12) FA_M8(13) FA_M8(14) FA_M8(15) FA_M8(16)
+#undef FA_M8
+            default: return hipErrorInvalidValue;
+        }
+    } else {
         switch (slen) {
 #define FA_M16(N)                                                                            \
     case N:                                                                                  \
         hipLaunchKernelGGL((mate16_kernel<N, SYM, false>), grid, dim3(64),                   \
                            (size_t)8 * kTgtChunk + (size_t)256 * N, st, a, first, count); \
         return hipGetLastError();
-            FA_M16(17) FA_M16(18) FA_M16(19) FA_M16(20) FA_M16(21) FA_M16(22) FA_M16(23)
-            FA_M16(24) FA_M16(25) FA_M16(26) FA_M16(27) FA_M16(28) FA_M16(29) FA_M16(30)
-            FA_M16(31) FA_M16(32)
+            FA_M16(1) FA_M16(2) FA_M16(3) FA_M16(4) FA_M16(5) FA_M16(6) FA_M16(7) FA_M16(8)
+            FA_M16(9) FA_M16(10) FA_M16(11) FA_M16(12) FA_M16(13) FA_M16(14) FA_M16(15) FA_M16(16)
+            FA_M16(17) FA_M16(18) FA_M16(19) FA_M16(20) FA_M16(21) FA_M16(22) FA_M16(23) FA_M16(24)
+            FA_M16(25) FA_M16(26) FA_M16(27) FA_M16(28) FA_M16(29) FA_M16(30) FA_M16(31) FA_M16(32)
 #undef FA_M16
-            default: break;
+            default: return hipErrorInvalidValue;
         }
     }
-    return hipErrorInvalidValue;
 }
 
 }  // namespace fa

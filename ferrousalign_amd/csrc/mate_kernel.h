@@ -34,10 +34,11 @@ struct MateArgs {
 // Longest stripe (segments per lane) a launch takes: qlen <= 256.
 constexpr int kMateMaxSlen = 32;   // 16-bit mode, 8 lanes
 constexpr int kMate8MaxSlen = 16;  // 8-bit mode, 16 lanes
-// Launch over packed positions [first, first+count), two pairs per lane
-// group (8 lanes, 16 pairs per 64-thread block; 8-bit mode: 16 lanes, 8 pairs):
-// every pair of [first, first+count) must have slen = ceil(qlen/lanes) =
-// `slen` (1..32, 8-bit 1..16); imax_log must hold count * log_stride entries.
+// Launch over packed positions [first, first+count), two pairs per 8-lane
+// group, 16 pairs per 64-thread block: every pair of [first, first+count) must
+// have slen = ceil(qlen / 8) = `slen` (1..32), in the 8-bit mode ceil(qlen / 16)
+// (1..16, the reference's 16 byte lanes, two per GPU lane); imax_log must hold
+// count * log_stride entries.
 hipError_t mate16_launch(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                          hipStream_t st);
 

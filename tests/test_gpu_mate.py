@@ -160,3 +160,27 @@ def test_mate_full_size_digest():
     out, cells = db.download_align2()
     assert digest(out, cells) == d[key]
     db.free()
+
+
+@pytest.mark.parametrize("byte_mode", [False, True])
+def test_mate_score2_two_copies(byte_mode):
+    # exact copy + a copy with 4 spaced mismatches far away: score2 = qlen - 20 at
+    # the second copy's last row, on the GPU as in the oracle's closed form
+    from ferrousalign_amd import synth
+    rng = random.Random(31)
+    qlen, cases, want = 151, [], []
+    for s1, s2, first_exact in ((100, 700, True), (120, 900, False), (1200, 300, True)):
+        t = [rng.randrange(4) for _ in range(1500)]
+        q = [rng.randrange(4) for _ in range(qlen)]
+        mut = list(q)
+        for k in (30, 60, 90, 120):
+            mut[k] = (q[k] + 1) % 4
+        a, b = (q, mut) if first_exact else (mut, q)
+        t[s1:s1 + qlen] = a
+        t[s2:s2 + qlen] = b
+        se, so = (s1, s2) if first_exact else (s2, s1)
+        cases.append((q, t))
+        want.append((qlen, se + qlen - 1, qlen - 1, qlen - 20, so + qlen - 1, se, 0))
+    xtra = synth.MATE_XTRA | (O.KSW_XBYTE if byte_mode else 0)
+    ora = _run_cases(cases, synth.MATE_MATRIX, (6, 1, 6, 1), xtra, "mate two copies")
+    assert [tuple(int(x) for x in r) for r in ora] == want

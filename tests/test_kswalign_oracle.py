@@ -215,3 +215,32 @@ def test_u8_matches_numpy_on_mate_workload():
         rc, r, cells = O.ksw_align2(q, t, MATE, 6, 1, 6, 1, XBYTE)
         r2, cells2 = K.ksw_align2(q, t, MATE, 6, 1, 6, 1, XBYTE)
         assert rc == 0 and r == r2 and cells == cells2
+
+
+@pytest.mark.parametrize("byte_mode", [False, True])
+def test_score2_two_copies_closed_form(byte_mode):
+    # an exact copy of the query and, far away, a copy with 4 mismatches spaced 30
+    # apart (score qlen - 5 x 4): the best is the exact copy, the second best is the
+    # other copy's peak row, outside te +- ceil(score / max); each copy's run of
+    # rows with a row maximum >= minsc is one entry of the suboptimal list
+    rng = random.Random(31)
+    qlen = 151
+    xtra = XTRA | (O.KSW_XBYTE if byte_mode else 0)
+    for s1, s2, first_exact in ((100, 700, True), (120, 900, False), (1200, 300, True)):
+        t = [rng.randrange(4) for _ in range(1500)]
+        q = [rng.randrange(4) for _ in range(qlen)]
+        mut = list(q)
+        for k in (30, 60, 90, 120):
+            mut[k] = (q[k] + 1) % 4
+        a, b = (q, mut) if first_exact else (mut, q)
+        t[s1:s1 + qlen] = a
+        t[s2:s2 + qlen] = b
+        se = s1 if first_exact else s2  # the exact copy
+        so = s2 if first_exact else s1
+        rc, r, _ = O.ksw_align2(q, t, MATE, 6, 1, 6, 1, xtra)
+        assert rc == 0
+        assert (r[0], r[1], r[2]) == (qlen, se + qlen - 1, qlen - 1)
+        assert (r[3], r[4]) == (qlen - 20, so + qlen - 1), r
+        assert (r[5], r[6]) == (se, 0)
+        r2, _ = K.ksw_align2(q, t, MATE, 6, 1, 6, 1, xtra)
+        assert r2 == r

@@ -11,7 +11,7 @@
 // fits: with Hmax = h0 + qlen*max(mat), Hmax < 4096 and Hmax*K < 32768
 // (K = max(2, 1 + max(mat))), e_ins < 32 and o+e < 8192, so H, E, F, G stay
 // in (-32768, 16384) and the row-argmax key (H << 4) | slot fits 16 bits;
-// the wide slot counts (S = 20, 24) take only pairs with Hmax < 2048, whose
+// the wide slot counts (S = 18..24) take only pairs with Hmax < 2048, whose
 // key (H << 5) | slot fits; other pairs run the 32-bit kernel.
 //
 // Differences from the 32-bit kernel that keep results identical:
@@ -598,12 +598,12 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
     X(L, 1) X(L, 2) X(L, 3) X(L, 4) X(L, 6) X(L, 8) X(L, 10) X(L, 12) X(L, 14) X(L, 16)
 // wide slot counts (4 and 8 lanes): twice the pairs per wavefront for the same
 // query length, so the per-row bookkeeping is shared by twice as many pairs
-#define FA16_WIDE(X, L) X(L, 20) X(L, 24)
+#define FA16_WIDE(X, L) X(L, 18) X(L, 20) X(L, 22) X(L, 24)
 
 int extend16_slot_count(int L, int qlen, bool wide) {
-    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24};
+    static const int sl[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24};
     if (L != 4 && L != 8 && L != 16) return -1;
-    const int n = wide && L <= 8 ? 12 : 10;
+    const int n = wide && L <= 8 ? 14 : 10;
     for (int k = 0; k < n; ++k)
         if (L * sl[k] >= qlen + 1) return sl[k];
     return -1;

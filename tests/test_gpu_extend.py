@@ -138,7 +138,7 @@ def test_parity_large_band_and_zdrop():
 
 def test_parity_wide_slot_boundary():
     """Pairs on both sides of the score bound the wide slot counts need
-    (Hmax = h0 + qlen * max(mat) < 2048 for S = 20, 24 on 4- and 8-lane
+    (Hmax = h0 + qlen * max(mat) < 2048 for S = 18..24 on 4- and 8-lane
     groups) in one batch, for each lane configuration: the narrow and the wide
     kernels give the oracle's results."""
     from ferrousalign_amd import synth

@@ -226,7 +226,8 @@ def cigar_bench(args, ctx):
     1,048,576 region-like pairs: 150 bp reads against the reference span of
     their alignment region."""
     from ferrousalign_amd import synth
-    from ferrousalign_amd.banded_swa import scalar_banded_swa_batch, scalar_banded_swa_last_ms
+    from ferrousalign_amd.banded_swa import (scalar_banded_swa_batch, scalar_banded_swa_batch_dense,
+                                             scalar_banded_swa_last_ms)
     from tests import _oracle as O
     flat = synth.region_pairs()
     n = len(flat[2])
@@ -245,6 +246,13 @@ def cigar_bench(args, ctx):
         dp.append(a)
         tb.append(b)
     out, cells, cig, coff, nc = res_raw[:5]
+    # the dense entry point: same work, outputs packed back to back
+    dense_wall = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        dres = scalar_banded_swa_batch_dense(sw, *flat, h0, ctx=ctx)
+        dense_wall.append((time.perf_counter() - t0) * 1e3)
+    dense_same = bool(np.array_equal(dres[0], out) and np.array_equal(np.diff(dres[3]), nc))
     ms_k = float(np.mean(dp) + np.mean(tb))
     total_cells = float(cells.astype(np.float64).sum())
     # bit-exactness: records, cell counts and CIGAR lengths of every pair against the
@@ -272,6 +280,9 @@ def cigar_bench(args, ctx):
            "dp_ms": round(float(np.mean(dp)), 4), "traceback_ms": round(float(np.mean(tb)), 4),
            "end_to_end_ms": round(float(np.mean(wall)), 2),
            "end_to_end_pairs_per_s": round(n / (np.mean(wall) / 1e3), 1),
+           "end_to_end_dense_ms": round(float(np.mean(dense_wall)), 2),
+           "end_to_end_dense_pairs_per_s": round(n / (np.mean(dense_wall) / 1e3), 1),
+           "dense_identical": dense_same,
            "bit_exact_vs_oracle": exact,
            "roofline": {"bound": "valu", "achieved": round(dp_tops, 3), "peak": round(VALU_INT32_TOPS, 1),
                         "unit": "TOPS", "frac": round(dp_tops / VALU_INT32_TOPS, 4), "traffic": None,

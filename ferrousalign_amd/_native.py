@@ -117,6 +117,8 @@ def _declare(lib):
                                         _P, _P, ctypes.POINTER(_P)]
     lib.fa_batch_download_targets.argtypes = [_P, _P, _P, _P]
     lib.fa_scalar_banded_swa_batch.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW), _I64] + [_P] * 18
+    lib.fa_scalar_banded_swa_batch_dense.argtypes = ([_P, ctypes.POINTER(BandedPairwiseSW), _I64] + [_P] * 11
+                                                     + [_I64, _P, _P, _P, _P, _P, _I64])
     lib.fa_scalar_banded_swa_last_ms.argtypes = [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     seqargs = [_I64, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.fa_execute_batch_simd_scoring.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
@@ -143,7 +145,7 @@ def _declare(lib):
                  "fa_ref_length", "fa_collect_extension_jobs", "fa_merge_scores_to_regions", "fa_extend_chains",
                  "fa_chains_upload", "fa_chains_extend", "fa_chains_download", "fa_chains_free",
                  "fa_batch_upload_ref", "fa_batch_download_targets", "fa_scalar_banded_swa_batch",
-                 "fa_scalar_banded_swa_last_ms", "fa_execute_batch_simd_scoring",
+                 "fa_scalar_banded_swa_batch_dense", "fa_scalar_banded_swa_last_ms", "fa_execute_batch_simd_scoring",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
                  "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",

@@ -292,6 +292,42 @@ def scalar_banded_swa_batch(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, 
     return out, cigars, ref_al, qry_al, (cells if want_cells else None)
 
 
+def scalar_banded_swa_batch_dense(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, w, h0=None,
+                                  ctx: Optional[Context] = None):
+    """scalar_banded_swa_batch with the CIGARs and aligned bases packed back to
+    back in job order (fa_scalar_banded_swa_batch_dense): returns (out[n,6],
+    cells[n], cigar words, cigar offsets[n+1], ref-aligned bases, their
+    offsets[n+1], query-aligned bases, their offsets[n+1]); job k's entries are
+    [off[k], off[k+1]).  The buffers are sized for the worst case but only
+    their filled part is written, so only those pages are touched."""
+    ctx = ctx or default_context()
+    n = len(qlen)
+    qseq = np.ascontiguousarray(qseq, np.uint8)
+    tseq = np.ascontiguousarray(tseq, np.uint8)
+    qoff = np.ascontiguousarray(qoff, np.int64)
+    toff = np.ascontiguousarray(toff, np.int64)
+    qlen = np.ascontiguousarray(qlen, np.int32)
+    tlen = np.ascontiguousarray(tlen, np.int32)
+    w = np.ascontiguousarray(w, np.int32)
+    h0 = np.ascontiguousarray(np.zeros(n, np.int32) if h0 is None else h0, np.int32)
+    tot = int(qlen.astype(np.int64).sum() + tlen.astype(np.int64).sum())
+    ccap, acap = tot + 4 * n, tot + 2 * n
+    out = np.empty((n, 6), np.int32)
+    cells = np.empty(n, np.uint32)
+    cig = np.empty(max(ccap, 1), np.uint32)
+    ra = np.empty(max(acap, 1), np.uint8)
+    qa = np.empty(max(acap, 1), np.uint8)
+    co = np.empty(n + 1, np.int64)
+    ro = np.empty(n + 1, np.int64)
+    qo = np.empty(n + 1, np.int64)
+    c = sw.to_c()
+    N.check(N.lib().fa_scalar_banded_swa_batch_dense(
+        ctx.handle, ctypes.byref(c), n, N.ptr(qseq), N.ptr(qoff), N.ptr(qlen), N.ptr(tseq), N.ptr(toff),
+        N.ptr(tlen), N.ptr(w), N.ptr(h0), N.ptr(out), N.ptr(cells), N.ptr(cig), ccap, N.ptr(co), N.ptr(ra),
+        N.ptr(ro), N.ptr(qa), N.ptr(qo), acap), "fa_scalar_banded_swa_batch_dense")
+    return out, cells, cig[:co[n]], co, ra[:ro[n]], ro, qa[:qo[n]], qo
+
+
 def scalar_banded_swa_last_ms(ctx: Optional[Context] = None):
     """(DP ms, traceback ms) of the last scalar_banded_swa_batch call (HIP events)."""
     ctx = ctx or default_context()

@@ -931,23 +931,28 @@ int fa_batch_download_targets(fa_ctx *ctx, fa_batch *b, uint8_t *codes, const in
     return FA_OK;
 }
 
-int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
-                               const uint8_t *query_seqs, const int64_t *query_offset,
-                               const int32_t *query_len, const uint8_t *ref_seqs,
-                               const int64_t *ref_offset, const int32_t *ref_len,
-                               const int32_t *band_width, const int32_t *h0, fa_out_score *out,
-                               uint32_t *cells, uint32_t *cigar, const int64_t *cigar_offset,
-                               int32_t *n_cigar, uint8_t *ref_aligned, uint8_t *query_aligned,
-                               const int64_t *aligned_offset, int32_t *n_ref_aligned,
-                               int32_t *n_query_aligned) {
-    if (!ctx || njobs < 0) return FA_ERR_ARG;
-    int rc = check_scoring(sw);
-    if (rc) return rc;
-    if (njobs == 0) return FA_OK;
-    if (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset || !ref_len ||
-        !band_width || !h0 || !out || !cigar || !cigar_offset || !n_cigar || !ref_aligned ||
-        !query_aligned || !aligned_offset || !n_ref_aligned || !n_query_aligned)
-        return FA_ERR_ARG;
+namespace {
+
+// Where fa_scalar_banded_swa_batch{,_dense} put the CIGARs and aligned bases:
+// caller-given slots with counts, or back to back with offsets written.
+struct SwtbOut {
+    uint32_t *cigar;
+    uint8_t *ref_aligned, *query_aligned;
+    bool dense;
+    // slots
+    const int64_t *cigar_offset, *aligned_offset;
+    int32_t *n_cigar, *n_ref_aligned, *n_query_aligned;
+    // dense: capacities (entries) and njobs + 1 offsets
+    int64_t cigar_cap, aligned_cap;
+    int64_t *cigar_off_out, *ref_off_out, *query_off_out;
+};
+
+int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+               const uint8_t *query_seqs, const int64_t *query_offset, const int32_t *query_len,
+               const uint8_t *ref_seqs, const int64_t *ref_offset, const int32_t *ref_len,
+               const int32_t *band_width, const int32_t *h0, fa_out_score *out, uint32_t *cells,
+               const SwtbOut &so) {
+    int rc;
     if (njobs > (int64_t)0x7FFFFFFF) return FA_ERR_ARG;
     // validation (h0 = 0 is allowed here: region.rs:1058 passes 0), in parallel;
     // the code returned is the one of the first invalid job
@@ -985,11 +990,16 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         } else {
             std::memset(&out[k], 0, sizeof(fa_out_score));
             if (cells) cells[k] = 0;
-            n_cigar[k] = n_ref_aligned[k] = n_query_aligned[k] = 0;
+            if (!so.dense) so.n_cigar[k] = so.n_ref_aligned[k] = so.n_query_aligned[k] = 0;
         }
     }
     const size_t m = job.size();
-    if (m == 0) return FA_OK;
+    if (m == 0) {
+        if (so.dense)
+            for (int64_t k = 0; k <= njobs; ++k)
+                so.cigar_off_out[k] = so.ref_off_out[k] = so.query_off_out[k] = 0;
+        return FA_OK;
+    }
     std::vector<uint32_t> qo(m), to(m), lsl(m), order(m);
     std::vector<int32_t> ql(m), tl(m), ww(m), hh(m);
     std::vector<uint64_t> tbo(m), cgo(m), alo(m);
@@ -1156,6 +1166,9 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
         nr_all += (uint64_t)hnr[p];
         nq_all += (uint64_t)hnq[p];
     }
+    if (so.dense && (nc_all > (uint64_t)so.cigar_cap || nr_all > (uint64_t)so.aligned_cap ||
+                     nq_all > (uint64_t)so.aligned_cap))
+        return FA_ERR_ARG;  // the capacities are too small for this batch
     uint64_t *d_dense;
     uint32_t *d_ocig;
     uint8_t *d_ora, *d_oqa;
@@ -1190,15 +1203,120 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
             out[k].global_score = o[4];
             out[k].max_offset = o[5];
             if (cells) cells[k] = hcells[p];
-            n_cigar[k] = hnc[p];
-            n_ref_aligned[k] = hnr[p];
-            n_query_aligned[k] = hnq[p];
-            std::memcpy(cigar + cigar_offset[k], hcig + dense[3 * p], (size_t)hnc[p] * 4);
-            std::memcpy(ref_aligned + aligned_offset[k], hra + dense[3 * p + 1], (size_t)hnr[p]);
-            std::memcpy(query_aligned + aligned_offset[k], hqa + dense[3 * p + 2], (size_t)hnq[p]);
+            if (so.dense) continue;
+            so.n_cigar[k] = hnc[p];
+            so.n_ref_aligned[k] = hnr[p];
+            so.n_query_aligned[k] = hnq[p];
+            std::memcpy(so.cigar + so.cigar_offset[k], hcig + dense[3 * p], (size_t)hnc[p] * 4);
+            std::memcpy(so.ref_aligned + so.aligned_offset[k], hra + dense[3 * p + 1], (size_t)hnr[p]);
+            std::memcpy(so.query_aligned + so.aligned_offset[k], hqa + dense[3 * p + 2],
+                        (size_t)hnq[p]);
         }
     });
+    if (so.dense) {
+        // offsets in job order (an empty job adds nothing), then the gathered
+        // buffers in one piece each: only the pages they fill are touched
+        size_t p = 0;
+        for (int64_t k = 0; k < njobs; ++k) {
+            if (p < m && job[p] == k) {
+                so.cigar_off_out[k] = (int64_t)dense[3 * p];
+                so.ref_off_out[k] = (int64_t)dense[3 * p + 1];
+                so.query_off_out[k] = (int64_t)dense[3 * p + 2];
+                ++p;
+            } else {
+                so.cigar_off_out[k] = p < m ? (int64_t)dense[3 * p] : (int64_t)nc_all;
+                so.ref_off_out[k] = p < m ? (int64_t)dense[3 * p + 1] : (int64_t)nr_all;
+                so.query_off_out[k] = p < m ? (int64_t)dense[3 * p + 2] : (int64_t)nq_all;
+            }
+        }
+        so.cigar_off_out[njobs] = (int64_t)nc_all;
+        so.ref_off_out[njobs] = (int64_t)nr_all;
+        so.query_off_out[njobs] = (int64_t)nq_all;
+        const int64_t pieces = 64;
+        parallel_for(pieces, [&](int64_t lo, int64_t hi, unsigned) {
+            for (int64_t x = lo; x < hi; ++x) {
+                auto part = [&](uint64_t total, uint64_t &a, uint64_t &b) {
+                    a = total * (uint64_t)x / (uint64_t)pieces;
+                    b = total * (uint64_t)(x + 1) / (uint64_t)pieces;
+                };
+                uint64_t a0, a1;
+                part(nc_all, a0, a1);
+                if (a1 > a0) std::memcpy(so.cigar + a0, hcig + a0, (size_t)(a1 - a0) * 4);
+                part(nr_all, a0, a1);
+                if (a1 > a0) std::memcpy(so.ref_aligned + a0, hra + a0, (size_t)(a1 - a0));
+                part(nq_all, a0, a1);
+                if (a1 > a0) std::memcpy(so.query_aligned + a0, hqa + a0, (size_t)(a1 - a0));
+            }
+        });
+    }
     return FA_OK;
+}
+
+}  // namespace
+
+int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                               const uint8_t *query_seqs, const int64_t *query_offset,
+                               const int32_t *query_len, const uint8_t *ref_seqs,
+                               const int64_t *ref_offset, const int32_t *ref_len,
+                               const int32_t *band_width, const int32_t *h0, fa_out_score *out,
+                               uint32_t *cells, uint32_t *cigar, const int64_t *cigar_offset,
+                               int32_t *n_cigar, uint8_t *ref_aligned, uint8_t *query_aligned,
+                               const int64_t *aligned_offset, int32_t *n_ref_aligned,
+                               int32_t *n_query_aligned) {
+    if (!ctx || njobs < 0) return FA_ERR_ARG;
+    int rc = check_scoring(sw);
+    if (rc) return rc;
+    if (njobs == 0) return FA_OK;
+    if (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset || !ref_len ||
+        !band_width || !h0 || !out || !cigar || !cigar_offset || !n_cigar || !ref_aligned ||
+        !query_aligned || !aligned_offset || !n_ref_aligned || !n_query_aligned)
+        return FA_ERR_ARG;
+    SwtbOut so{};
+    so.cigar = cigar;
+    so.ref_aligned = ref_aligned;
+    so.query_aligned = query_aligned;
+    so.cigar_offset = cigar_offset;
+    so.aligned_offset = aligned_offset;
+    so.n_cigar = n_cigar;
+    so.n_ref_aligned = n_ref_aligned;
+    so.n_query_aligned = n_query_aligned;
+    return swtb_batch(ctx, sw, njobs, query_seqs, query_offset, query_len, ref_seqs, ref_offset,
+                      ref_len, band_width, h0, out, cells, so);
+}
+
+int fa_scalar_banded_swa_batch_dense(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                                     const uint8_t *query_seqs, const int64_t *query_offset,
+                                     const int32_t *query_len, const uint8_t *ref_seqs,
+                                     const int64_t *ref_offset, const int32_t *ref_len,
+                                     const int32_t *band_width, const int32_t *h0,
+                                     fa_out_score *out, uint32_t *cells, uint32_t *cigar,
+                                     int64_t cigar_cap, int64_t *cigar_offset,
+                                     uint8_t *ref_aligned, int64_t *ref_aligned_offset,
+                                     uint8_t *query_aligned, int64_t *query_aligned_offset,
+                                     int64_t aligned_cap) {
+    if (!ctx || njobs < 0 || cigar_cap < 0 || aligned_cap < 0) return FA_ERR_ARG;
+    int rc = check_scoring(sw);
+    if (rc) return rc;
+    if (!cigar_offset || !ref_aligned_offset || !query_aligned_offset) return FA_ERR_ARG;
+    if (njobs == 0) {
+        cigar_offset[0] = ref_aligned_offset[0] = query_aligned_offset[0] = 0;
+        return FA_OK;
+    }
+    if (!query_seqs || !query_offset || !query_len || !ref_seqs || !ref_offset || !ref_len ||
+        !band_width || !h0 || !out || !cigar || !ref_aligned || !query_aligned)
+        return FA_ERR_ARG;
+    SwtbOut so{};
+    so.cigar = cigar;
+    so.ref_aligned = ref_aligned;
+    so.query_aligned = query_aligned;
+    so.dense = true;
+    so.cigar_cap = cigar_cap;
+    so.aligned_cap = aligned_cap;
+    so.cigar_off_out = cigar_offset;
+    so.ref_off_out = ref_aligned_offset;
+    so.query_off_out = query_aligned_offset;
+    return swtb_batch(ctx, sw, njobs, query_seqs, query_offset, query_len, ref_seqs, ref_offset,
+                      ref_len, band_width, h0, out, cells, so);
 }
 
 int fa_scalar_banded_swa_last_ms(fa_ctx *ctx, double *dp_ms, double *traceback_ms) {

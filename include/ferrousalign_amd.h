@@ -191,8 +191,28 @@ int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int
                                const int64_t *aligned_offset, int32_t *n_ref_aligned,
                                int32_t *n_query_aligned);
 
+/* Same computation as fa_scalar_banded_swa_batch, with each job's CIGAR and
+ * aligned bases written back to back in job order instead of into caller
+ * slots: the reference returns them as one Vec per alignment
+ * (scalar/implementation.rs:5-13), and a packed buffer plus offsets is their
+ * flat form.  cigar_offset, ref_aligned_offset and query_aligned_offset get
+ * njobs + 1 entries: job k's entries are [off[k], off[k+1]).  Capacities in
+ * entries (CIGAR words; bytes for each aligned buffer); FA_ERR_ARG when a
+ * batch needs more (sum of qlen + tlen + 4 and qlen + tlen + 2 is always
+ * enough).  Only the filled part of each buffer is written. */
+int fa_scalar_banded_swa_batch_dense(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                                     const uint8_t *query_seqs, const int64_t *query_offset,
+                                     const int32_t *query_len, const uint8_t *ref_seqs,
+                                     const int64_t *ref_offset, const int32_t *ref_len,
+                                     const int32_t *band_width, const int32_t *h0,
+                                     fa_out_score *out, uint32_t *cells, uint32_t *cigar,
+                                     int64_t cigar_cap, int64_t *cigar_offset,
+                                     uint8_t *ref_aligned, int64_t *ref_aligned_offset,
+                                     uint8_t *query_aligned, int64_t *query_aligned_offset,
+                                     int64_t aligned_cap);
+
 /* Kernel time (HIP events on the context's stream) of the last
- * fa_scalar_banded_swa_batch call: the DP launches and the traceback launch,
+ * fa_scalar_banded_swa_batch{,_dense} call: the DP launches and the traceback launch,
  * with the inputs already in HBM (measurement aid). */
 int fa_scalar_banded_swa_last_ms(fa_ctx *ctx, double *dp_ms, double *traceback_ms);
 

@@ -135,3 +135,54 @@ def test_band_window_boundaries():
             hs.append(0 if rng.random() < 0.5 else int(rng.integers(1, 60)))
     print("band windows", flush=True)
     _check(qs, ts, ws, hs, "band windows")
+
+
+def test_dense_output_matches_slots():
+    # the dense entry point packs each job's CIGAR and aligned bases back to back;
+    # its contents equal the slot entry point's, empty jobs included, and a batch
+    # whose outputs exceed the capacities is refused
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd.banded_swa import scalar_banded_swa_batch, scalar_banded_swa_batch_dense
+    rng = np.random.default_rng(77)
+    qs, ts, ws, hs = _pairs(rng, 300, n_frac=0.02)
+    qs[3] = np.zeros(0, np.uint8)  # empty query
+    ts[10] = np.zeros(0, np.uint8)  # empty target
+    qs[-1] = np.zeros(0, np.uint8)  # empty last job
+    sw = synth.default_scoring()
+    qf, qo, ql = _flat(qs)
+    tf, to, tl = _flat(ts)
+    print("dense output", len(qs), flush=True)
+    out, cells, cig, coff, nc, ra, qa, aoff, nr, nq = scalar_banded_swa_batch(sw, qf, qo, ql, tf, to, tl, ws, hs,
+                                                                             raw=True)
+    d = scalar_banded_swa_batch_dense(sw, qf, qo, ql, tf, to, tl, ws, hs)
+    dout, dcells, dcig, dco, dra, dro, dqa, dqo = d
+    assert np.array_equal(out, dout) and np.array_equal(cells, dcells)
+    assert dco[0] == dro[0] == dqo[0] == 0
+    assert np.array_equal(np.diff(dco), nc) and np.array_equal(np.diff(dro), nr) and np.array_equal(np.diff(dqo), nq)
+    for k in range(len(qs)):
+        assert np.array_equal(dcig[dco[k]:dco[k + 1]], cig[coff[k]:coff[k] + nc[k]])
+        assert np.array_equal(dra[dro[k]:dro[k + 1]], ra[aoff[k]:aoff[k] + nr[k]])
+        assert np.array_equal(dqa[dqo[k]:dqo[k + 1]], qa[aoff[k]:aoff[k] + nq[k]])
+    # capacities one entry short of what the batch fills
+    import ctypes
+    n = len(qs)
+    c = sw.to_c()
+    o = np.empty((n, 6), np.int32)
+    cl = np.empty(n, np.uint32)
+    buf32 = np.empty(int(dco[n]) + 1, np.uint32)
+    b1 = np.empty(int(max(dro[n], dqo[n])) + 1, np.uint8)
+    b2 = np.empty_like(b1)
+    offs = [np.empty(n + 1, np.int64) for _ in range(3)]
+    keep = [np.ascontiguousarray(x) for x in (qf, qo, ql, tf, to, tl, np.asarray(ws, np.int32),
+                                                np.asarray(hs, np.int32))]  # alive through the call
+    args = [N.ptr(x) for x in keep]
+    rc = N.lib().fa_scalar_banded_swa_batch_dense(
+        _ctx(), ctypes.byref(c), n, *args,
+        N.ptr(o), N.ptr(cl), N.ptr(buf32), int(dco[n]) - 1, N.ptr(offs[0]), N.ptr(b1), N.ptr(offs[1]), N.ptr(b2),
+        N.ptr(offs[2]), int(max(dro[n], dqo[n])))
+    assert rc == N.FA_ERR_ARG
+
+
+def _ctx():
+    from ferrousalign_amd.banded_swa import default_context
+    return default_context().handle

@@ -409,8 +409,9 @@ static void *kv_worker(void *arg) {
         }
         /* workspace.rs:805-824: SoA with 0xFF padding */
         const size_t qn = (size_t)KW * (mq > 0 ? mq : 1), tn = (size_t)KW * (mt > 0 ? mt : 1);
-        if (qn > qcap) { qs = (uint8_t *)realloc(qs, qn); qcap = qn; }
-        if (tn > tcap) { ts = (uint8_t *)realloc(ts, tn); tcap = tn; }
+        if (qn > qcap) { free(qs); qs = (uint8_t *)malloc(qn); qcap = qn; }
+        if (tn > tcap) { free(ts); ts = (uint8_t *)malloc(tn); tcap = tn; }
+        if (!qs || !ts) break; /* out of memory: the remaining records stay as they are */
         memset(qs, 0xFF, qn);
         memset(ts, 0xFF, tn);
         for (int l = 0; l < cnt; ++l) {
@@ -424,12 +425,14 @@ static void *kv_worker(void *arg) {
         } else {
             const size_t hn = (size_t)(mq + 1) * KW, rn = (size_t)(mt + 1) * KW;
             if (hn > hcap) {
-                h0 = (uint8_t *)realloc(h0, hn);
-                h1 = (uint8_t *)realloc(h1, hn);
-                f = (uint8_t *)realloc(f, hn);
+                free(h0); free(h1); free(f);
+                h0 = (uint8_t *)malloc(hn);
+                h1 = (uint8_t *)malloc(hn);
+                f = (uint8_t *)malloc(hn);
                 hcap = hn;
             }
-            if (rn > rcap) { rm = (uint8_t *)realloc(rm, rn); rcap = rn; }
+            if (rn > rcap) { free(rm); rm = (uint8_t *)malloc(rn); rcap = rn; }
+            if (!h0 || !h1 || !f || !rm) break;
             if (jb->simd)
                 kv_chunk_avx2(ts, qs, mt, mq, rlen, xtra, &jb->sc, h0, h1, f, rm, res);
             else

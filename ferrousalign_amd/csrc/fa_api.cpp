@@ -288,7 +288,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint8_t *el = b->elig16.data();
     const bool cost = pack == FA_PACK_COST;
     std::vector<uint64_t> tcnt(16 * nk, 0), ttl(16 * nk, 0);
-    std::vector<int> tmaxq(16 * nk, 1);
+    std::vector<int> tmaxq(16 * nk, 1), tminf(16 * nk, 2);  // max qlen, min fits16 level
     std::vector<double> csum(cost ? 16 * nk * 256 : 0, 0.0);
     std::vector<uint32_t> ccnt(cost ? 16 * nk * 256 : 0, 0u);
     std::vector<int32_t> pk((size_t)n);
@@ -308,6 +308,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             ++tcnt[g];
             ttl[g] += (uint64_t)std::max(tl[k], 1);
             tmaxq[g] = std::max(tmaxq[g], q);
+            tminf[g] = std::min(tminf[g], f);
             if (cost) {
                 csum[g * 256 + (size_t)q] += std::max(0.0f, b->cost[(size_t)k]);
                 ++ccnt[g * 256 + (size_t)q];
@@ -388,11 +389,12 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     uint32_t first = 0;
     for (size_t x = 0; x < nk; ++x) {
         uint64_t count = 0, rows = 0;
-        int maxq = 1;
+        int maxq = 1, minf = 2;
         for (size_t t = 0; t < 16; ++t) {
             count += tcnt[t * nk + x];
             rows += ttl[t * nk + x];
             maxq = std::max(maxq, tmaxq[t * nk + x]);
+            if (tcnt[t * nk + x]) minf = std::min(minf, tminf[t * nk + x]);
         }
         if (!count) continue;
         const int key0 = kinds[x];
@@ -401,9 +403,12 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         bk.first = first;
         bk.count = (uint32_t)count;
         first += (uint32_t)count;
-        if ((key0 & 0xFFFF) == 0) {  // merged bucket: widest slot count of its pairs
+        if ((key0 & 0xFFFF) == 0) {
+            // merged bucket: the slot count of its longest query, wide slots only
+            // when every pair of the bucket is under their bound (fits16 level 2)
             int kind, lanes, S;
-            if (!fa_host::pair_kernel(L, bk.kind == 0, bk.kind, maxq, &kind, &lanes, &S)) return FA_ERR_QLEN;
+            if (!fa_host::pair_kernel(L, bk.kind == 0, bk.kind == 0 ? 0 : minf, maxq, &kind, &lanes, &S))
+                return FA_ERR_QLEN;
             bk.L = lanes;
             bk.S = S;
         } else {

@@ -153,3 +153,38 @@ def test_parity_wide_slot_boundary():
         print("wide slot boundary", lanes, ps.n, flush=True)
         ora, oc, gpu, gc = _run_both(ps, sw, lanes=lanes)
         _assert_same(ora, oc, gpu, gc, "wide slot boundary lanes=%d" % lanes)
+
+
+def test_single_launch_wide_slots():
+    """One launch for a whole batch (FA_LAUNCH_SINGLE) with 8-lane groups:
+    when every pair is under the wide-key bound the launch takes a wide slot
+    count (<8,18> for 150 bp reads); when one pair is over it, the launch
+    stays on the non-wide slot counts.  Both give the oracle's results."""
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd import banded_swa as B
+    sw = synth.default_scoring()
+    ps = synth.workload("illumina_150bp_1m", n=16384)
+    flat = ps.flat()
+    ora, oc, orc = O.extend_batch(sw, *flat)
+    db = B.DeviceBatch(sw, *flat)
+    db.set_launch(8)
+    print("single launch wide", ps.n, flush=True)
+    db.run(N.FA_LAUNCH_SINGLE)
+    kern = db.profile_launches(N.FA_LAUNCH_SINGLE)
+    assert len(kern) == 1 and kern[0]["kernel"].startswith("extend16_plain_kernel<8,"), kern
+    assert kern[0]["slots"] > 16
+    gpu, gc = db.download()
+    db.free()
+    _assert_same(O.to_outscore_order(ora), oc, gpu, gc, "single launch, wide slots")
+    # one pair over the bound (h0 + qlen >= 2048): no wide slots for the batch
+    ps.h0[7] = 2000
+    flat = ps.flat()
+    ora, oc, orc = O.extend_batch(sw, *flat)
+    db = B.DeviceBatch(sw, *flat)
+    db.set_launch(8)
+    db.run(N.FA_LAUNCH_SINGLE)
+    kern = db.profile_launches(N.FA_LAUNCH_SINGLE)
+    assert len(kern) == 1 and kern[0]["slots"] <= 16, kern
+    gpu, gc = db.download()
+    db.free()
+    _assert_same(O.to_outscore_order(ora), oc, gpu, gc, "single launch, one pair over the wide bound")

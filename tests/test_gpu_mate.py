@@ -101,8 +101,19 @@ def test_mate_u8_random_shapes_and_flags(seed):
     from ferrousalign_amd import synth
     cases = _random_cases(random.Random(50 + seed))
     for xtra, gaps in ((synth.MATE_XTRA, (6, 1, 6, 1)), (O.KSW_XSTART, (5, 2, 3, 1)),
-                       (O.KSW_XSUBO | O.KSW_XSTART | 40, (6, 1, 6, 1)), (0, (0, 1, 0, 1))):
+                       (O.KSW_XSUBO | O.KSW_XSTART | 40, (6, 1, 6, 1)), (0, (0, 1, 0, 1)),
+                       (O.KSW_XSTOP | O.KSW_XSTART | 30, (6, 1, 6, 1))):
         _run_cases(cases, synth.MATE_MATRIX, gaps, xtra | O.KSW_XBYTE, "mate u8 random %d" % seed)
+
+
+def test_mate_forward_stop_score():
+    # KSW_XSTOP in the caller's xtra: the forward pass stops at the first best
+    # score >= the stop score, in both modes
+    from ferrousalign_amd import synth
+    cases = _random_cases(random.Random(90), n=160)
+    for xtra in (O.KSW_XSTOP | 30, O.KSW_XSTOP | O.KSW_XSTART | 60):
+        for mode in (0, O.KSW_XBYTE):
+            _run_cases(cases, synth.MATE_MATRIX, (6, 1, 6, 1), xtra | mode, "mate xstop")
 
 
 def test_mate_u8_saturation():

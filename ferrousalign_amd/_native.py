@@ -24,6 +24,7 @@ FA_ERR_DEVICE = -8
 FA_ERR_NODEVICE = -9
 FA_ERR_OVERFLOW = -10
 FA_ERR_SEGMENT = -11
+FA_ERR_SCRATCH = -12
 
 FA_MAX_QLEN = 255
 FA_MAX_TLEN = 2048
@@ -123,6 +124,8 @@ def _declare(lib):
     seqargs = [_I64, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.fa_execute_batch_simd_scoring.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
     lib.fa_batch_upload.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [ctypes.POINTER(_P)]
+    lib.fa_execute_batch_device_noalloc.argtypes = ([_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs +
+                                                    [_I32, _P, _P, _P])
     lib.fa_batch_set_cost.argtypes = [_P, _P, _P]
     lib.fa_batch_set_launch.argtypes = [_P, _P, _I32, _I32, _I32]
     lib.fa_batch_run.argtypes = [_P, _P, _U32]
@@ -146,6 +149,7 @@ def _declare(lib):
                  "fa_chains_upload", "fa_chains_extend", "fa_chains_download", "fa_chains_free",
                  "fa_batch_upload_ref", "fa_batch_download_targets", "fa_scalar_banded_swa_batch",
                  "fa_scalar_banded_swa_batch_dense", "fa_scalar_banded_swa_last_ms", "fa_execute_batch_simd_scoring",
+                 "fa_execute_batch_device_noalloc",
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
                  "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",

@@ -554,6 +554,7 @@ const char *fa_strerror(int code) {
         case FA_ERR_NODEVICE: return "no usable gfx950 device";
         case FA_ERR_OVERFLOW: return "8-bit score overflow";
         case FA_ERR_SEGMENT: return "target segment outside the reference or across the strand boundary";
+        case FA_ERR_SCRATCH: return "batch larger than the allocation-free entry point's fixed scratch";
         default: return "unknown error";
     }
 }
@@ -600,6 +601,9 @@ int fa_ctx_create(int device, fa_ctx **out) {
         fa_ctx_destroy(c);
         return FA_ERR_DEVICE;
     }
+    // device address of the allocation-free entry point's scratch (once per
+    // device; a failure there only disables that entry point)
+    (void)fa_host::noalloc_init(device);
     *out = c;
     return FA_OK;
 }

@@ -125,5 +125,9 @@ int enqueue_run(fa_ctx *ctx, fa_batch *b, uint32_t flags);
 // Releases the chain-extension workspace of a context (chain_api.cpp).
 void chain_workspace_free(fa_ctx *ctx);
 
+// Looks up, with `device` current, the device address of the fixed scratch
+// array of fa_execute_batch_device_noalloc (noalloc.hip); once per device.
+int noalloc_init(int device);
+
 
 }  // namespace fa_host

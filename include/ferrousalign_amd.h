@@ -36,6 +36,7 @@ extern "C" {
 #define FA_ERR_NODEVICE (-9) /* no usable gfx950 device */
 #define FA_ERR_OVERFLOW (-10) /* 8-bit path only: a pair exceeded 8-bit range (FA_SCORE_U8 without rerun) */
 #define FA_ERR_SEGMENT (-11)  /* reference target segment outside [0, 2 l_pac) or across l_pac */
+#define FA_ERR_SCRATCH (-12)  /* batch exceeds the fixed scratch of fa_execute_batch_device_noalloc */
 
 #define FA_MAX_QLEN 255
 #define FA_MAX_TLEN 2048
@@ -121,6 +122,27 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw,
                                   const int32_t *ref_len, const int32_t *h0,
                                   const int32_t *band_width, uint32_t flags,
                                   fa_out_score *out, uint32_t *cells, int32_t *per_job_rc);
+
+/* ---- allocation-free form (no counterpart in the reference) -------------- */
+/* For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * computation and job layout as fa_execute_batch_simd_scoring, but every
+ * pointer is device memory (d_out[k] = job k's OutScore, d_cells and d_job_rc
+ * optional) and the call allocates nothing: validation, 2-bit packing,
+ * packing order and launch metadata live in one fixed per-device scratch
+ * array (64 MiB, set up by fa_ctx_create).  A job is limited to
+ * ref_len <= max_ref_len (<= FA_MAX_TLEN; longer jobs get FA_ERR_TLEN); a
+ * batch needs about njobs * (6 * (16 + 2 * ceil(max_ref_len / 32)) + 52)
+ * bytes of it, and a larger batch is rejected with FA_ERR_SCRATCH before
+ * anything runs.  Synchronous on the context's stream.  Calls on one device
+ * share the scratch: calls that overlap on one device race. */
+int fa_execute_batch_device_noalloc(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                                    const uint8_t *d_query_seqs, const int64_t *d_query_offset,
+                                    const int32_t *d_query_len, const uint8_t *d_ref_seqs,
+                                    const int64_t *d_ref_offset, const int32_t *d_ref_len,
+                                    const int32_t *d_h0, const int32_t *d_band_width,
+                                    int32_t max_ref_len, fa_out_score *d_out, uint32_t *d_cells,
+                                    int32_t *d_job_rc);
 
 /* ---- kernel-level entry points (the banded_swa batch surface) ----------- */
 /* Replace simd_banded_swa_batch{16,32,64}_soa (8-bit; isa_sse_neon.rs:10-16,

@@ -379,16 +379,51 @@ class SoAInputs:
     max_tlen: int = 0
 
 
+@dataclass
+class SoAInputs16:
+    """shared.rs:24-35: the 16-bit kernels' carrier (i16 sequences), int32
+    per-lane parameters."""
+    query_soa: np.ndarray
+    target_soa: np.ndarray
+    qlen: np.ndarray
+    tlen: np.ndarray
+    h0: np.ndarray
+    w: np.ndarray
+    lanes: int
+    max_qlen: int = 0
+    max_tlen: int = 0
+
+
+def pad_batch(batch, lanes):
+    """shared.rs:40-82: pad a list of (qlen, query, tlen, target, w, h0) jobs
+    to `lanes` entries (extra jobs are dropped, missing lanes get empty jobs)
+    and take the per-lane parameters.  Returns (qlen, tlen, h0, w, max_qlen,
+    max_tlen, padded); the parameters are int32 here, where the reference
+    casts them to i8 (lengths clamped at 127)."""
+    padded = list(batch[:lanes]) + [(0, b"", 0, b"", 0, 0)] * max(0, lanes - len(batch))
+    qlen = np.array([int(x[0]) for x in padded], np.int32)
+    tlen = np.array([int(x[2]) for x in padded], np.int32)
+    w = np.array([int(x[4]) for x in padded], np.int32)
+    h0 = np.array([int(x[5]) for x in padded], np.int32)
+    return qlen, tlen, h0, w, int(max(0, qlen.max(initial=0))), int(max(0, tlen.max(initial=0))), padded
+
+
 def soa_transform(batch, lanes, max_len):
     """shared.rs:87-111: lane-interleaved buffers padded with 0xFF.
 
     batch: list of (qlen, query, tlen, target, w, h0) tuples (pad_batch order,
     shared.rs:40-82)."""
+    def codes(x, n):
+        if isinstance(x, (bytes, bytearray)):
+            return np.frombuffer(bytes(x[:n]), np.uint8)
+        return np.asarray(x[:n], np.uint8)
+
     q = np.full(max_len * lanes, 0xFF, np.uint8)
     t = np.full(max_len * lanes, 0xFF, np.uint8)
     for i, (ql, qs, tl, ts, _w, _h) in enumerate(batch[:lanes]):
-        q[np.arange(min(ql, max_len)) * lanes + i] = np.asarray(qs[:max_len], np.uint8)
-        t[np.arange(min(tl, max_len)) * lanes + i] = np.asarray(ts[:max_len], np.uint8)
+        qn, tn = min(ql, max_len), min(tl, max_len)
+        q[np.arange(qn) * lanes + i] = codes(qs, qn)
+        t[np.arange(tn) * lanes + i] = codes(ts, tn)
     return q, t
 
 

@@ -28,6 +28,8 @@ HIP_SOURCES = ["extend_kernel.hip", "extend16_kernel.hip", "predictor.hip", "mat
 
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # the host (x86) pass of every hipcc line reports the GPU-only feature as
+    # unknown; check_device_features() makes sure the device pass knows it
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
                     and "not a recognized feature" not in l)
     if r.returncode != 0:
@@ -35,7 +37,22 @@ def _run(cmd):
     return out
 
 
+def check_device_features():
+    """The device pass alone must accept every -target-feature of HIP_FLAGS:
+    an unknown one would be ignored silently, and packed fp32 VALU forms
+    could enter the code objects."""
+    os.makedirs(OBJ, exist_ok=True)
+    src = os.path.join(OBJ, "feature_probe.hip")
+    with open(src, "w") as f:
+        f.write("#include <hip/hip_runtime.h>\n__global__ void probe(float *x) { x[threadIdx.x] *= 2.0f; }\n")
+    cmd = [HIPCC] + HIP_FLAGS + ["--cuda-device-only", "-c", src, "-o", os.path.join(OBJ, "feature_probe.o")]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0 or "not a recognized feature" in r.stdout:
+        raise RuntimeError("the gfx950 device pass does not accept the build's target features:\n" + r.stdout)
+
+
 def build_hip_library():
+    check_device_features()
     os.makedirs(OBJ, exist_ok=True)
     objs = []
     cmds = []

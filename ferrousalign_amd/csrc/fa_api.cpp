@@ -509,9 +509,13 @@ int run_align2(fa_ctx *ctx, fa_batch *b, int *nlaunch) {
 }
 
 int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
-    if (flags & FA_RUN_ALIGN2) return run_align2(ctx, b, nlaunch);
+    if (flags & FA_RUN_ALIGN2) {
+        b->last_overflow = 0;
+        return run_align2(ctx, b, nlaunch);
+    }
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
+    b->last_overflow = b->n_fallback32;
     if (nlaunch) *nlaunch = (int)b->buckets.size();
     if (!(flags & FA_LAUNCH_GRAPH)) return enqueue_run(ctx, b, flags);
     const uint32_t key = (flags & 0xFF4u) | (uint32_t)b->lanes << 16 | (uint32_t)b->chunk << 21;
@@ -826,6 +830,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     }
     b->qo_host.swap(qo);
     b->to_host.swap(to);
+    for (int64_t k = 0; k < n; ++k) b->n_fallback32 += b->elig16[(size_t)k] == 0;
 
     fa::ExtArgs &a = b->args;
     a.seq2 = b->d_seq2;
@@ -1539,7 +1544,7 @@ int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cel
 
 int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count) {
     if (!ctx || !b || !count) return FA_ERR_ARG;
-    *count = 0;
+    *count = b->last_overflow;
     return FA_OK;
 }
 

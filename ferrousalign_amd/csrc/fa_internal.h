@@ -77,6 +77,8 @@ struct fa_batch {
     int waves_per_cu = 0;
     int streams = 4;
     int tmax = 0;
+    int64_t n_fallback32 = 0;   // pairs whose scores may leave the 16-bit range (fits16 == 0)
+    int64_t last_overflow = 0;  // of the last extension run: pairs that ran the 32-bit kernel for that
     // mate-rescue local SW (FA_RUN_ALIGN2)
     bool align2_set = false;
     fa::MateArgs margs{};

@@ -303,8 +303,13 @@ int fa_hbm_copy(fa_ctx *ctx, int64_t bytes, double *gbytes_per_s);
 /* The pair order (packed position -> job index) used under `flags`. */
 int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order);
 int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cells);
-/* Number of pairs that the 8-bit path handed to the 16-bit rerun on the last
- * run. */
+/* Overflow fallbacks of the last extension run: the number of pairs whose
+ * scores could leave the packed 16-bit kernel's range (h0 + qlen * max(mat)
+ * >= 4096 and the other bounds of the 16-bit route) and that therefore ran
+ * the 32-bit kernel, the reference's i16-after-i8 rerun moved one level up
+ * (dispatch.rs:100-107).  The 8-bit entry points share the 16-bit route
+ * (gfx950 has no packed 8-bit add/max), so no 8-bit rerun is counted; 0
+ * before the first run and after an FA_RUN_ALIGN2 run. */
 int fa_batch_overflow_count(fa_ctx *ctx, fa_batch *b, int64_t *count);
 int fa_batch_free(fa_ctx *ctx, fa_batch *b);
 

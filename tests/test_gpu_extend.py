@@ -188,3 +188,50 @@ def test_single_launch_wide_slots():
     gpu, gc = db.download()
     db.free()
     _assert_same(O.to_outscore_order(ora), oc, gpu, gc, "single launch, one pair over the wide bound")
+
+
+def test_full_illumina_1m_single_launch_matches_golden():
+    """The whole 1,048,576-pair 150 bp Illumina-like workload in its plain
+    configuration -- one launch, length-sorted, 8-lane groups -- against the
+    oracle's committed digest of all records and cells."""
+    import json
+    import os
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd import banded_swa as B
+    from tests.golden.make_golden import output_digest
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "digests.json")))
+    ps = synth.workload("illumina_150bp_1m")
+    assert ps.n == 1048576
+    db = B.DeviceBatch(synth.default_scoring(), *ps.flat())
+    db.set_launch(8)
+    print("illumina 1M single launch", flush=True)
+    db.run(N.FA_LAUNCH_SINGLE | N.FA_PACK_LENGTH)
+    assert len(db.profile_launches(N.FA_LAUNCH_SINGLE | N.FA_PACK_LENGTH)) == 1
+    out, cells = db.download()
+    db.free()
+    assert int(cells.astype(np.uint64).sum()) == golden["illumina_150bp_1m:cells"]
+    assert output_digest(out, cells) == golden["illumina_150bp_1m"]
+
+
+def test_overflow_count_is_the_32bit_fallbacks():
+    """fa_batch_overflow_count: the pairs whose score bound leaves the packed
+    16-bit range (h0 + qlen * max(mat) >= 4096) and so ran the 32-bit kernel;
+    0 before the first run.  Their records still equal the oracle's."""
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd import banded_swa as B
+    sw = synth.default_scoring()
+    ps = synth.workload("uneven_mix_4m", n=4096)
+    rng = np.random.default_rng(11)
+    big = rng.choice(ps.n, 37, replace=False)
+    ps.h0[big] = 4096 - ps.qlen[big] + rng.integers(0, 50, big.size)   # bound >= 4096
+    near = np.setdiff1d(rng.choice(ps.n, 20, replace=False), big)
+    ps.h0[near] = 4095 - ps.qlen[near]                                 # bound 4095: still 16-bit
+    flat = ps.flat()
+    db = B.DeviceBatch(sw, *flat)
+    assert db.overflow_count() == 0
+    db.run()
+    assert db.overflow_count() == big.size
+    gpu, gc = db.download()
+    db.free()
+    ora, oc, orc = O.extend_batch(sw, *flat)
+    _assert_same(O.to_outscore_order(ora), oc, gpu, gc, "overflow fallbacks")

@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libferrousalign_amd.so")
+# FA_LIBRARY selects another build of the same library (the host-sanitized
+# one that `python -m ferrousalign_amd.build --sanitize` tests with)
+LIB_PATH = os.environ.get("FA_LIBRARY") or os.path.join(_PKG, "libferrousalign_amd.so")
 SYNTH_PATH = os.path.join(_PKG, "libfa_synth.so")
 
 FA_OK = 0

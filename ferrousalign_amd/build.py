@@ -51,26 +51,65 @@ def check_device_features():
         raise RuntimeError("the gfx950 device pass does not accept the build's target features:\n" + r.stdout)
 
 
-def build_hip_library():
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer for the sanitized
+# build (host code only: each -fsanitize= goes to the host pass alone).
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined",
+            "-Xarch_host", "-shared-libasan"]
+SAN_LIB = os.path.join(OBJ, "san", "libferrousalign_amd.so")
+
+
+def build_hip_library(sanitize=False):
     check_device_features()
-    os.makedirs(OBJ, exist_ok=True)
+    obj_dir = os.path.join(OBJ, "san") if sanitize else OBJ
+    os.makedirs(obj_dir, exist_ok=True)
+    san = SAN_HOST if sanitize else []
     objs = []
     cmds = []
     for src in HIP_SOURCES:
-        obj = os.path.join(OBJ, src + ".o")
+        obj = os.path.join(obj_dir, src + ".o")
         objs.append(obj)
         if src.endswith(".cc"):  # plain host C++ (x86 intrinsics), no device pass
-            cmds.append(["g++", "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
+            cxx = ["/opt/rocm/llvm/bin/clang++"] + san[1::2] if sanitize else ["g++"]
+            cmds.append(cxx + ["-O3", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
             continue
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + san + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         list(ex.map(_run, cmds))
-    lib = os.path.join(PKG, "libferrousalign_amd.so")
+    lib = SAN_LIB if sanitize else os.path.join(PKG, "libferrousalign_amd.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + san + objs)
     os.replace(tmp, lib)
     return lib
+
+
+def asan_runtime():
+    """clang's shared ASan runtime (it carries the UBSan handlers too)."""
+    r = subprocess.run(["/opt/rocm/llvm/bin/clang++", "-print-file-name=libclang_rt.asan-x86_64.so"],
+                       stdout=subprocess.PIPE, text=True)
+    path = r.stdout.strip()
+    if not os.path.isabs(path) or not os.path.exists(path):
+        import glob
+        hits = glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so")
+        path = hits[0] if hits else ""
+    return path
+
+
+def run_sanitized_cpu_tests(pytest_args=()):
+    """Builds the sanitized library and runs the CPU test suite against it
+    (FA_LIBRARY points the ctypes loader there; the runtime is preloaded
+    because the interpreter itself is not instrumented).  Host only: the
+    sanitized library never goes to a GPU."""
+    lib = build_hip_library(sanitize=True)
+    env = dict(os.environ)
+    env["FA_LIBRARY"] = lib
+    env["LD_PRELOAD"] = asan_runtime()
+    env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1:halt_on_error=1"
+    env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
+    cmd = [sys.executable, "-m", "pytest", os.path.join(ROOT, "tests"), "-q", "-x", "-m", "not gpu",
+           "-p", "no:cacheprovider"] + list(pytest_args)
+    return subprocess.run(cmd, env=env, cwd=ROOT).returncode
 
 
 def build_synth_library():
@@ -96,5 +135,7 @@ def build_all(verbose=True):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "--sanitize":
+        sys.exit(run_sanitized_cpu_tests(sys.argv[2:]))
     build_all()
     sys.exit(0)

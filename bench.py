@@ -104,6 +104,17 @@ def timed(db, flags, steps, warmup):
     return wall * 1e3 / steps, ms_ev, nlaunch
 
 
+def wall_steps(step, steps, warmup):
+    """Mean wall time (ms) of `step()`, each call synchronous, after `warmup`
+    untimed calls: a whole per-batch step, host work included."""
+    for _ in range(warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
 def traffic_of(kernel, cells):
     """HBM bytes per launch of `kernel` over `cells` DP cells, from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (rows 'traffic: <kernel>
@@ -516,16 +527,18 @@ def main():
                   flush=True)
         return
 
-    # plain GPU kernel: one lane group (16 lanes) per pair, length-sorted, single launch
+    # plain GPU kernel: one lane group (16 lanes) per pair, length-sorted, single
+    # launch; a step plans the packing (device sort by length) and runs it
     db.set_launch(16)
     db.set_streams(1)
     plain_flags = N.FA_PACK_LENGTH | N.FA_LAUNCH_SINGLE
     plain_steps = max(2, args.steps // 2)
-    ms_plain, ms_plain_ev, _ = timed(db, plain_flags, plain_steps, 1)
+    ms_plain_run, _, _ = timed(db, plain_flags, plain_steps, 1)
+    ms_plain = wall_steps(lambda: db.run(plain_flags | N.FA_PLAN_FRESH), plain_steps, 1)
     out_plain, cells = db.download()
-    log("plain single-launch: %.2f ms/step" % ms_plain)
+    log("plain single-launch: %.2f ms/step (%.2f run only)" % (ms_plain, ms_plain_run))
 
-    # learned packing
+    # learned packing: the predictions stay on the device as the packing cost
     cpred = pred.predict(db, set_cost=True)
     pred_mae = mae(cpred, cells)
     if args.no_tune:
@@ -536,15 +549,31 @@ def main():
     else:
         best, flags, _ = autotune.tune(db, ps.qlen, ps.tlen, ps.w, pack_flags=N.FA_PACK_COST, log=log,
                                        use_cache=False)
-    ms, ms_ev, nlaunch = timed(db, flags, args.steps, args.warmup)
+    # run only: the plan of this batch is cached
+    ms_run, ms_ev, nlaunch = timed(db, flags, args.steps, args.warmup)
+    out_run, cells_run = db.download()
+    roof = roofline(db, flags, cells_run, ctx)
+
+    # the headline step, as a new batch costs it: features computed again, the
+    # network's forward pass, the packing plan on the device, the run
+    def learned_step():
+        pred.predict(db, set_cost=True, fresh_features=True, download=False)
+        db.run(flags)
+    ms = wall_steps(learned_step, args.steps, args.warmup)
     out, cells2 = db.download()
     total_cells = float(cells2.astype(np.float64).sum())
-    roof = roofline(db, flags, cells2, ctx)
 
     # ablation: same launch configuration, length-sorted instead of learned
+    # (a step plans the packing again), and the packing planned from the true
+    # cell counts (what a perfect predictor could give; run only)
     lflags = (flags & ~0xF0) | N.FA_PACK_LENGTH
-    ms_len, ms_len_ev, _ = timed(db, lflags, max(2, args.steps // 2), 1)
+    len_steps = max(2, args.steps // 2)
+    ms_len_run, _, _ = timed(db, lflags, len_steps, 1)
+    ms_len = wall_steps(lambda: db.run(lflags | N.FA_PLAN_FRESH), len_steps, 1)
     out_len, cells_len = db.download()
+    db.set_cost(cells2.astype(np.float32))
+    ms_perfect_run, _, _ = timed(db, flags, len_steps, 1)
+    out_perf, cells_perf = db.download()
 
     # the same pairs through the host-buffer entry point (validation, 2-bit
     # packing, PCIe both ways, packing plan, run, download): the rate a caller
@@ -560,8 +589,8 @@ def main():
 
     dg = digest_of(out, cells2)
     exact = (dg == golden.get(args.workload)) if golden.get(args.workload) else None
-    same = bool(np.array_equal(out, out_plain) and np.array_equal(cells2, cells)
-                and np.array_equal(out_len, out_plain) and np.array_equal(cells_len, cells))
+    same = all(np.array_equal(o, out_plain) and np.array_equal(c, cells)
+               for o, c in ((out, cells2), (out_run, cells_run), (out_len, cells_len), (out_perf, cells_perf)))
     res = {
         "metric": "extension pairs/s + GCUPS, learned-packed vs plain GPU kernel, bit-exact",
         "value": round(ps.n / (ms / 1e3), 1),
@@ -570,13 +599,14 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
-        "ms_per_step_hip_events": round(ms_ev, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int16x2",
         "data": "synthetic",
-        "config": {"workload": args.workload, "pairs": ps.n, "packing": "learned (predicted cells)",
+        "config": {"workload": args.workload, "pairs": ps.n,
+                   "packing": "learned (predicted cells); a step = features + forward pass + device packing "
+                              "plan + run",
                    "lanes_per_pair": best["lanes"] or "narrowest group per query length",
                    "chunk_pairs": best["chunk"], "streams": best.get("streams", 4),
                    "launch": ("persistent queue" if flags & N.FA_LAUNCH_PERSISTENT else "bucket launches")
@@ -589,6 +619,12 @@ def main():
         "speedup_vs_plain": round(ms_plain / ms, 3),
         "length_sorted_pairs_per_s": round(ps.n / (ms_len / 1e3), 1),
         "learned_gain_vs_length_sorted": round(ms_len / ms, 3),
+        "learned_path_ms": round(ms, 4),
+        "run_only_ms": {"learned": round(ms_run, 4), "learned_hip_events": round(ms_ev, 4),
+                        "length_sorted": round(ms_len_run, 4), "plain": round(ms_plain_run, 4),
+                        "perfect_costs": round(ms_perfect_run, 4)},
+        "perfect_cost_ms": round(ms_perfect_run, 4),
+        "learned_gain_run_only": round(ms_len_run / ms_run, 3),
         "bit_exact_vs_golden": exact,
         "identical_across_packings": same,
         "host_buffer_ms": round(ms_host, 1),

@@ -24,6 +24,7 @@
 #include "valu_probe.h"
 #include "ref_kernel.h"
 #include "swtb_kernel.h"
+#include "plan_kernel.h"
 
 using fa_host::Bucket;
 
@@ -237,30 +238,28 @@ void batch_release(fa_batch *b) {
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
-                    b->d_mout, b->d_imax_log, b->d_sort};
+                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
 }
 
-// Build the pair order and the per-slot-count buckets for `flags`.  Buckets
-// run widest first; inside a bucket pairs run from the most to the least
-// expensive (by (qlen, tlen), or by the caller's cost), so that lane groups
-// of one wavefront hold pairs of similar cost and the long pairs start first.
-__global__ __launch_bounds__(256) void iota_kernel(uint32_t *v, uint32_t n) {
-    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k < n) v[k] = k;
-}
-
+// Build the pair order and the per-slot-count buckets for `flags`, on the
+// device (plan_kernel.hip): a (fit level, qlen) histogram, one sort key per
+// pair, a stable radix sort of (key, pair id) and the launch metadata; the
+// host reads the 9 KB histogram back and cuts the order into buckets.
+// Buckets run widest first; inside a bucket pairs run from the most to the
+// least expensive (by (qlen, tlen), or by the cost), so that lane groups of
+// one wavefront hold pairs of similar cost and the long pairs start first.
 int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t pack = flags & 0xF0u;
     const uint32_t single = flags & FA_LAUNCH_SINGLE;
     const uint32_t wide = flags & FA_SCORE_I32;
     const int L = b->lanes;
     const uint32_t key = pack | single | wide | (uint32_t)L << 16;
-    if (b->packed_flags == key) return FA_OK;
+    if (b->packed_flags == key && !(flags & FA_PLAN_FRESH) && !b->plan_dirty) return FA_OK;
     const int64_t n = b->n;
-    if (pack == FA_PACK_COST && (int64_t)b->cost.size() != n) return FA_ERR_ARG;
+    if (pack == FA_PACK_COST && !b->cost_set) return FA_ERR_ARG;
     // kernel of a pair, (kind << 16) | (lanes << 8) | slots, as a function of
     // its fits16 level and qlen (-1: no kernel holds it); merged buckets (one
     // per kernel kind) take the kind alone
@@ -277,125 +276,74 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     std::sort(kinds.begin(), kinds.end(), [](int x, int y) { return x > y; });
     kinds.erase(std::unique(kinds.begin(), kinds.end()), kinds.end());
     const size_t nk = kinds.size();
-    int kidx[3][256];
+    fa::PlanTable t;
     for (int f = 0; f < 3; ++f)
         for (int q = 0; q < 256; ++q)
-            kidx[f][q] = tab[f][q] < 0 ? -1
-                                       : (int)(std::find(kinds.begin(), kinds.end(), tab[f][q]) - kinds.begin());
-    // per-thread statistics per kernel key: pairs, sum of tlen, max qlen, and
-    // (cost packing) the cost sum and count per (key, qlen)
-    const int32_t *ql = b->qlen.data(), *tl = b->tlen.data();
-    const uint8_t *el = b->elig16.data();
-    const bool cost = pack == FA_PACK_COST;
-    std::vector<uint64_t> tcnt(16 * nk, 0), ttl(16 * nk, 0);
-    std::vector<int> tmaxq(16 * nk, 1), tminf(16 * nk, 2);  // max qlen, min fits16 level
-    std::vector<double> csum(cost ? 16 * nk * 256 : 0, 0.0);
-    std::vector<uint32_t> ccnt(cost ? 16 * nk * 256 : 0, 0u);
-    std::vector<int32_t> pk((size_t)n);
-    std::vector<int> bad(16, 0);
-    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned t) {
-        for (int64_t k = lo; k < hi; ++k) {
-            const int q = ql[k];
-            const int f = el[k] > 2 ? 2 : el[k];
-            const int x = (q > 0 && q < 256) ? kidx[f][q] : -1;
-            if (x < 0) {
-                bad[t] = 1;
-                pk[(size_t)k] = 0;
-                continue;
-            }
-            pk[(size_t)k] = x;
-            const size_t g = (size_t)t * nk + (size_t)x;
-            ++tcnt[g];
-            ttl[g] += (uint64_t)std::max(tl[k], 1);
-            tmaxq[g] = std::max(tmaxq[g], q);
-            tminf[g] = std::min(tminf[g], f);
-            if (cost) {
-                csum[g * 256 + (size_t)q] += std::max(0.0f, b->cost[(size_t)k]);
-                ++ccnt[g * 256 + (size_t)q];
-            }
-        }
-    });
-    for (int v : bad)
-        if (v) return FA_ERR_QLEN;
-    std::vector<double> gsum(cost ? nk * 256 : 0, 0.0);
-    std::vector<uint64_t> gcnt(cost ? nk * 256 : 0, 0);
-    if (cost)
-        for (size_t t = 0; t < 16; ++t)
-            for (size_t g = 0; g < nk * 256; ++g) {
-                gsum[g] += csum[t * nk * 256 + g];
-                gcnt[g] += ccnt[t * nk * 256 + g];
-            }
-    // One sortable key per pair, ascending: kernel key rank (the widest kernel
-    // first), then -- cost packing -- the early-stop flag and the cost class
-    // of an early stopper, then qlen and tlen descending.  A wavefront runs
-    // until the last of its pairs stops, so what must match inside a
-    // wavefront is the row count, which length order groups.  What the costs
-    // add is the pairs that stop early (z-drop, non-homologous targets): a
-    // pair whose cost is below half the mean cost of its (kernel, qlen) group
-    // goes to a group of its own, ordered by cost class (a factor e^(1/2)
-    // apart) and length.  Sorting on the raw costs would let their noise
-    // scatter pairs of equal length over many wavefronts.
-    std::vector<uint64_t> k64((size_t)n);
-    std::vector<uint64_t> tor(16, 0);
-    parallel_for(n, [&](int64_t lo, int64_t hi, unsigned t) {
-        uint64_t acc = 0;
-        for (int64_t k = lo; k < hi; ++k) {
-            uint64_t x = (uint64_t)pk[(size_t)k] << 28;
-            if (cost) {
-                const size_t g = (size_t)pk[(size_t)k] * 256 + (size_t)ql[k];
-                const double c = std::max(0.0f, b->cost[(size_t)k]);
-                if (c * 2.0 * (double)gcnt[g] < gsum[g]) {
-                    const int cls = (int)std::lrint(2.0 * std::log1p(c));
-                    x |= ((uint64_t)1 << 27) | (uint64_t)(127 - std::min(cls, 127)) << 20;
-                }
-            }
-            if (pack != FA_PACK_NONE) x |= ((uint64_t)(255 - ql[k]) << 12) | (uint64_t)(4095 - tl[k]);
-            k64[(size_t)k] = x;
-            acc |= x;
-        }
-        tor[t] |= acc;
-    });
-    uint64_t any = 0;
-    for (uint64_t v : tor) any |= v;
-    // stable device radix sort of (key, pair id): the pair order (ties keep
-    // the input order)
+            t.kidx[f][q] = (int16_t)(tab[f][q] < 0 ? -1
+                                                   : (int)(std::find(kinds.begin(), kinds.end(), tab[f][q]) -
+                                                           kinds.begin()));
+    // device buffers (grow-only): histogram, bad flag, keys in/out, ids, sort scratch
+    constexpr size_t kBins = 768;
+    size_t tb = 0;
+    FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (unsigned long long *)nullptr,
+                                              (unsigned long long *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (int)std::max<int64_t>(n, 1), 0, 34,
+                                              ctx->stream));
+    const size_t hist_bytes = kBins * (4 + 8 + 8) + 256;
+    const size_t need = hist_bytes + 16 * (size_t)n + 4 * (size_t)n + tb + 512;
+    if (b->sort_bytes < need) {
+        if (b->d_sort) (void)hipFree(b->d_sort);
+        b->d_sort = nullptr;
+        b->sort_bytes = 0;
+        FA_HIP(hipMalloc(&b->d_sort, need));
+        b->sort_bytes = need;
+    }
+    uint8_t *base = (uint8_t *)b->d_sort;
+    fa::PlanHist h;
+    h.rows = (unsigned long long *)base;
+    h.csum = h.rows + kBins;
+    h.cnt = (uint32_t *)(h.csum + kBins);
+    uint32_t *bad = h.cnt + kBins;
+    unsigned long long *kin = (unsigned long long *)(base + hist_bytes);
+    unsigned long long *kout = kin + n;
+    uint32_t *vin = (uint32_t *)(kout + n);
+    void *temp = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
+    fa::PlanArgs pa;
+    pa.n = (uint32_t)n;
+    pa.mode = pack == FA_PACK_COST ? fa::kPlanCost : (pack == FA_PACK_NONE ? fa::kPlanNone : fa::kPlanLength);
+    pa.qlen = b->d_qlen;
+    pa.tlen = b->d_tlen;
+    pa.elig = b->d_elig;
+    pa.cost = pack == FA_PACK_COST ? b->d_pred : nullptr;
+    FA_HIP(hipMemsetAsync(base, 0, hist_bytes, ctx->stream));
+    FA_HIP(fa::plan_hist(pa, h, ctx->cus, ctx->stream));
+    std::vector<uint32_t> cnt(kBins + 1);
+    std::vector<unsigned long long> rows(kBins);
+    FA_HIP(hipMemcpyAsync(rows.data(), h.rows, kBins * 8, hipMemcpyDeviceToHost, ctx->stream));
     if (n > 0) {
-        const int end_bit = any ? 64 - __builtin_clzll(any) : 1;
-        size_t tb = 0;
-        FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                                  (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0,
-                                                  end_bit, ctx->stream));
-        const size_t need = 16 * (size_t)n + 4 * (size_t)n + tb + 256;
-        if (b->sort_bytes < need) {
-            if (b->d_sort) (void)hipFree(b->d_sort);
-            b->d_sort = nullptr;
-            b->sort_bytes = 0;
-            FA_HIP(hipMalloc(&b->d_sort, need));
-            b->sort_bytes = need;
-        }
-        uint8_t *base = (uint8_t *)b->d_sort;
-        uint64_t *kin = (uint64_t *)base, *kout = kin + n;
-        uint32_t *vin = (uint32_t *)(kout + n);
-        void *temp = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
-        FA_HIP(hipMemcpyAsync(kin, k64.data(), (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, vin,
-                           (uint32_t)n);
-        FA_HIP(hipGetLastError());
-        FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, b->d_order, (int)n, 0, end_bit,
+        FA_HIP(fa::plan_keys(pa, h, t, kin, vin, bad, ctx->stream));
+        FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, b->d_order, (int)n, 0, 34,
                                                   ctx->stream));
     }
+    FA_HIP(hipMemcpyAsync(cnt.data(), h.cnt, (kBins + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(fa::meta_gather(b->args, (uint32_t)n, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    if (cnt[kBins]) return FA_ERR_QLEN;  // a pair no kernel holds
     // buckets: the kernel keys in rank order with their pair counts
     std::vector<Bucket> buckets;
     uint32_t first = 0;
     for (size_t x = 0; x < nk; ++x) {
-        uint64_t count = 0, rows = 0;
+        uint64_t count = 0, nrows = 0;
         int maxq = 1, minf = 2;
-        for (size_t t = 0; t < 16; ++t) {
-            count += tcnt[t * nk + x];
-            rows += ttl[t * nk + x];
-            maxq = std::max(maxq, tmaxq[t * nk + x]);
-            if (tcnt[t * nk + x]) minf = std::min(minf, tminf[t * nk + x]);
-        }
+        for (int f = 0; f < 3; ++f)
+            for (int q = 1; q < 256; ++q) {
+                const uint32_t c = cnt[(size_t)f * 256 + (size_t)q];
+                if (!c || t.kidx[f][q] != (int)x) continue;
+                count += c;
+                nrows += rows[(size_t)f * 256 + (size_t)q];
+                maxq = std::max(maxq, q);
+                minf = std::min(minf, f);
+            }
         if (!count) continue;
         const int key0 = kinds[x];
         Bucket bk;
@@ -418,16 +366,22 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         // work: rows x VALU instructions per wavefront-row / pairs per
         // wavefront (31 per slot for the packed kernel's two pairs, ~40 for
         // the 32-bit kernel's one, ~175 per row of bookkeeping)
-        bk.work = (double)rows *
+        bk.work = (double)nrows *
                   (bk.kind == 1 ? (31.0 * bk.S + 175.0) * bk.L / 128.0 : (40.0 * bk.S + 175.0) * bk.L / 64.0);
         buckets.push_back(bk);
     }
     if (buckets.size() > 64) return FA_ERR_ARG;
-    FA_HIP(fa::meta_gather(b->args, (uint32_t)n, ctx->stream));
-    FA_HIP(hipStreamSynchronize(ctx->stream));
+    // a re-plan that gives the same launches keeps the captured graph (it
+    // reads the order and metadata from device memory at replay)
+    bool same = key == b->packed_flags && buckets.size() == b->buckets.size();
+    for (size_t x = 0; same && x < buckets.size(); ++x)
+        same = buckets[x].first == b->buckets[x].first && buckets[x].count == b->buckets[x].count &&
+               buckets[x].L == b->buckets[x].L && buckets[x].S == b->buckets[x].S &&
+               buckets[x].kind == b->buckets[x].kind;
     b->buckets.swap(buckets);
     b->packed_flags = key;
-    drop_graph(b);
+    b->plan_dirty = false;
+    if (!same) drop_graph(b);
     return FA_OK;
 }
 
@@ -801,6 +755,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         (rc = dev_upload(&b->d_tlen, b->tlen.data(), b->tlen.size(), st)) ||
         (rc = dev_upload(&b->d_h0, b->h0.data(), b->h0.size(), st)) ||
         (rc = dev_upload(&b->d_w, b->w.data(), b->w.size(), st)) ||
+        (rc = dev_upload(&b->d_elig, b->elig16.data(), b->elig16.size(), st)) ||
         (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
         (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
         (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64))) {
@@ -1342,9 +1297,15 @@ int fa_scalar_banded_swa_last_ms(fa_ctx *ctx, double *dp_ms, double *traceback_m
 
 int fa_batch_set_cost(fa_ctx *ctx, fa_batch *b, const float *cost) {
     if (!ctx || !b || (!cost && b->n > 0)) return FA_ERR_ARG;
-    b->cost.assign(cost, cost + b->n);
-    b->packed_flags = 0xFFFFFFFFu;
-    drop_graph(b);
+    FA_HIP(hipSetDevice(ctx->device));
+    int rc;
+    if (!b->d_pred && (rc = dev_alloc(&b->d_pred, (size_t)b->n))) return rc;
+    if (b->n)
+        FA_HIP(hipMemcpyAsync(b->d_pred, cost, (size_t)b->n * sizeof(float), hipMemcpyHostToDevice,
+                              ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    b->cost_set = true;
+    b->plan_dirty = true;  // the next run re-plans
     return FA_OK;
 }
 
@@ -1945,18 +1906,23 @@ int fa_batch_features(fa_ctx *ctx, fa_batch *b, float *feat) {
 int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32_t set_cost) {
     if (!ctx || !m || !b) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
-    int rc = ensure_features(ctx, b);
-    if (rc) return rc;
+    int rc;
+    if ((set_cost & 2) && b->d_feat) {  // features of a new batch: computed again, in place
+        FA_HIP(fa::mlp_launch_features(b->args, (uint32_t)b->n, b->d_feat, ctx->stream));
+    } else if ((rc = ensure_features(ctx, b))) {
+        return rc;
+    }
     if (!b->d_pred && (rc = dev_alloc(&b->d_pred, (size_t)b->n))) return rc;
     FA_HIP(fa::mlp_launch_forward(m->wh, m->norm, b->d_feat, (uint32_t)b->n, b->d_pred, nullptr,
                                   ctx->stream));
-    std::vector<float> host((size_t)b->n);
-    if (b->n)
-        FA_HIP(hipMemcpyAsync(host.data(), b->d_pred, (size_t)b->n * sizeof(float),
-                              hipMemcpyDeviceToHost, ctx->stream));
+    if (cells_pred && b->n)
+        FA_HIP(hipMemcpyAsync(cells_pred, b->d_pred, (size_t)b->n * sizeof(float), hipMemcpyDeviceToHost,
+                              ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
-    if (cells_pred) std::memcpy(cells_pred, host.data(), host.size() * sizeof(float));
-    if (set_cost) return fa_batch_set_cost(ctx, b, host.data());
+    if (set_cost & 1) {  // the predictions stay on the device as the packing cost
+        b->cost_set = true;
+        b->plan_dirty = true;
+    }
     return FA_OK;
 }
 

@@ -56,11 +56,13 @@ struct fa_batch {
     // host copies kept for re-packing
     std::vector<int32_t> qlen, tlen, h0, w;
     std::vector<uint8_t> elig16;  // fits16 of each pair
-    std::vector<float> cost;
+    bool cost_set = false;         // d_pred holds the packing cost (FA_PACK_COST)
+    bool plan_dirty = false;       // the cost changed since the last plan
     std::vector<uint32_t> qo_host, to_host;  // word offsets of each query / target in d_seq2
     // device
     uint32_t *d_seq2 = nullptr, *d_seqn = nullptr, *d_qo = nullptr, *d_to = nullptr;
     int32_t *d_qlen = nullptr, *d_tlen = nullptr, *d_h0 = nullptr, *d_w = nullptr;
+    uint8_t *d_elig = nullptr;   // fits16 of each pair (device planner)
     uint32_t *d_order = nullptr;
     uint32_t *d_meta = nullptr;  // per packed slot: pid, qo, to, qlen, tlen, h0, w, 0
     int32_t *d_out = nullptr;
@@ -87,7 +89,7 @@ struct fa_batch {
     uint32_t log_slots = 0;
     std::vector<fa_host::Bucket> mbuckets;
     float *d_feat = nullptr;       // predictor features [n][6]
-    float *d_pred = nullptr;       // predicted cells [n]
+    float *d_pred = nullptr;       // predicted cells [n], or the caller's cost (fa_batch_set_cost)
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_key = 0;
     int graph_wpc = 0;

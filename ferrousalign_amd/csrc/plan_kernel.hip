@@ -1,0 +1,107 @@
+// Packing plan on the device: the sort keys that order a batch's pairs into
+// lane groups (the host needs only a 768-bin histogram back to cut the sorted
+// order into per-kernel launches).
+//
+// A pair's kernel follows from its 16-bit fit level f (0..2) and query
+// length q, so the statistics are kept per (f, q) bin:
+//   plan_hist_kernel  pairs, target rows and (cost packing) fixed-point cost
+//                     sums per bin, block histograms in LDS, one atomic per
+//                     bin and block;
+//   plan_key_kernel   one 64-bit key per pair, ascending = launch order:
+//                     kernel rank, then (cost packing) the early-stop flag and
+//                     cost class of a pair whose cost is below half the mean
+//                     of its (kernel, qlen) group, then qlen and tlen
+//                     descending; the device radix sort of (key, pair id) is
+//                     stable, so equal keys keep the input order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "plan_kernel.h"
+
+namespace fa {
+
+namespace {
+
+constexpr int kBins = 3 * 256;
+
+__global__ __launch_bounds__(1024) void plan_hist_kernel(PlanArgs a, PlanHist h) {
+    __shared__ uint32_t cnt[kBins];
+    __shared__ unsigned long long rows[kBins];
+    __shared__ unsigned long long csum[kBins];
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) {
+        cnt[b] = 0u;
+        rows[b] = 0ull;
+        csum[b] = 0ull;
+    }
+    __syncthreads();
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += gridDim.x * blockDim.x) {
+        const int q = a.qlen[k];
+        const int f = a.elig[k] > 2 ? 2 : a.elig[k];
+        if (q < 1 || q > 255) continue;  // validated at upload
+        const int b = f * 256 + q;
+        atomicAdd(&cnt[b], 1u);
+        atomicAdd(&rows[b], (unsigned long long)max(a.tlen[k], 1));
+        if (a.cost) atomicAdd(&csum[b], cost_fixed(a.cost[k]));
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) {
+        if (!cnt[b]) continue;
+        atomicAdd(&h.cnt[b], cnt[b]);
+        atomicAdd(&h.rows[b], rows[b]);
+        if (a.cost) atomicAdd(&h.csum[b], csum[b]);
+    }
+}
+
+__global__ __launch_bounds__(256) void plan_key_kernel(PlanArgs a, PlanHist h, PlanTable t,
+                                                       unsigned long long *keys, uint32_t *ids,
+                                                       uint32_t *bad) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= a.n) return;
+    ids[k] = k;
+    const int q = a.qlen[k];
+    const int f = a.elig[k] > 2 ? 2 : a.elig[k];
+    const int x = (q >= 1 && q <= 255) ? t.kidx[f][q] : -1;
+    if (x < 0) {
+        atomicOr(bad, 1u);
+        keys[k] = 0ull;
+        return;
+    }
+    unsigned long long key = (unsigned long long)x << 28;
+    if (a.mode == kPlanCost) {
+        // the (kernel, qlen) group: every fit level whose kernel is x at this qlen
+        unsigned long long gs = 0ull, gc = 0ull;
+        for (int g = 0; g < 3; ++g)
+            if (t.kidx[g][q] == x) {
+                gs += h.csum[g * 256 + q];
+                gc += h.cnt[g * 256 + q];
+            }
+        const float c = fmaxf(a.cost[k], 0.0f);
+        if ((unsigned long long)cost_fixed(c) * 2ull * gc < gs) {
+            const int cls = (int)rint(2.0 * log1p((double)c));
+            key |= (1ull << 27) | (unsigned long long)(127 - min(cls, 127)) << 20;
+        }
+    }
+    if (a.mode != kPlanNone)
+        key |= ((unsigned long long)(255 - q) << 12) | (unsigned long long)(4095 - min(a.tlen[k], 4095));
+    keys[k] = key;
+}
+
+}  // namespace
+
+hipError_t plan_hist(const PlanArgs &a, const PlanHist &h, int cus, hipStream_t st) {
+    if (a.n == 0) return hipSuccess;
+    uint32_t blocks = (a.n + 1023) / 1024;
+    const uint32_t cap = (uint32_t)(cus > 0 ? cus : 256);
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(plan_hist_kernel, dim3(blocks), dim3(1024), 0, st, a, h);
+    return hipGetLastError();
+}
+
+hipError_t plan_keys(const PlanArgs &a, const PlanHist &h, const PlanTable &t, unsigned long long *keys,
+                     uint32_t *ids, uint32_t *bad, hipStream_t st) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(plan_key_kernel, dim3((a.n + 255) / 256), dim3(256), 0, st, a, h, t, keys, ids, bad);
+    return hipGetLastError();
+}
+
+}  // namespace fa

@@ -26,12 +26,6 @@ static_assert(OFF_B3 + 1 == kMlpParams, "parameter layout");
 
 __device__ __forceinline__ float r16(float v) { return __half2float(__float2half(v)); }
 
-__device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *seqn, uint32_t off,
-                                        int p) {
-    const uint32_t c = (seq2[off + (p >> 4)] >> ((p & 15) * 2)) & 3u;
-    return ((seqn[(off >> 1) + (p >> 5)] >> (p & 31)) & 1u) ? 4 : (int)c;
-}
-
 // Forward of one pair with weights `w` (fp16, LDS).  Fills x, h1, h2 (fp16
 // values held in float) and the pre-activations' signs.
 __device__ __forceinline__ float mlp_forward1(const __half *w, const float *raw, const float *norm,
@@ -65,7 +59,19 @@ __device__ __forceinline__ float mlp_forward1(const __half *w, const float *raw,
 
 }  // namespace
 
-// Raw features per pair from the batch's 2-bit sequences in HBM.
+// Even bits 0, 2, .., 30 of x moved to bits 0 .. 15.
+__device__ __forceinline__ uint32_t even_bits(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0x0000FFFFu;
+}
+
+// Raw features per pair from the batch's 2-bit sequences in HBM, word-wide:
+// the first 32 bases of query and target are two 2-bit words and one N-mask
+// word each; base j mismatches when exactly one of the two is N, or neither
+// is and the 2-bit codes differ.
 __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n, float *feat) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -74,8 +80,16 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n
     int lim = qlen < tlen ? qlen : tlen;
     if (lim > 32) lim = 32;
     int mm = 0;
-    for (int j = 0; j < lim; ++j)
-        mm += seq_code(a.seq2, a.seqn, qo, j) != seq_code(a.seq2, a.seqn, to, j);
+    if (lim > 0) {  // both sequences have at least one 32-base block (two words)
+        const uint2 q = *reinterpret_cast<const uint2 *>(a.seq2 + qo);
+        const uint2 t = *reinterpret_cast<const uint2 *>(a.seq2 + to);
+        const uint32_t qn = a.seqn[qo >> 1], tn = a.seqn[to >> 1];
+        const uint32_t d0 = q.x ^ t.x, d1 = q.y ^ t.y;
+        const uint32_t code = even_bits(d0 | (d0 >> 1)) | (even_bits(d1 | (d1 >> 1)) << 16);
+        const uint32_t mism = (qn ^ tn) | (~(qn | tn) & code);
+        const uint32_t lm = lim == 32 ? 0xFFFFFFFFu : ((1u << lim) - 1u);
+        mm = __popc(mism & lm);
+    }
     // C (01) and G (10) are the 2-bit codes whose two bits differ; N is 00
     int gc = 0;
     const int nw = (qlen + 15) >> 4;
@@ -92,22 +106,56 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n
     o[5] = qlen > 0 ? (float)gc / (float)qlen : 0.f;
 }
 
-// Predicted cells (expm1 of the network output) per pair.
-__global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *wg, const float *normg,
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2v pack_h2(float a, float b) {
+    h2v r;
+    r.x = (_Float16)a;
+    r.y = (_Float16)b;
+    return r;
+}
+
+// Predicted cells (expm1 of the network output) per pair.  Same network as
+// mlp_forward1, two fp16 products per v_dot2_f32_f16 with fp32 accumulation;
+// the weights are read at wave-uniform addresses (scalar loads).
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *__restrict__ wg,
+                                                          const float *__restrict__ normg,
                                                           const float *feat, uint32_t n,
                                                           float *cells_pred, float *yout) {
-    __shared__ __half w[kMlpParams];
-    __shared__ float norm[2 * NIN];
-    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = wg[i];
-    if (threadIdx.x < 2 * NIN) norm[threadIdx.x] = normg[threadIdx.x];
-    __syncthreads();
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    float raw[NIN], x[NIN], h1[NH], h2[NH];
-    uint32_t p1, p2;
+    const h2v *w2 = reinterpret_cast<const h2v *>(wg);  // half pairs; every layer block starts even
+    h2v x[NIN / 2];
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) raw[i] = feat[(size_t)k * NIN + i];
-    const float y = mlp_forward1(w, raw, norm, x, h1, h2, p1, p2);
+    for (int i = 0; i < NIN / 2; ++i) {
+        const float f0 = feat[(size_t)k * NIN + 2 * i], f1 = feat[(size_t)k * NIN + 2 * i + 1];
+        x[i] = pack_h2((f0 - normg[2 * i]) * normg[NIN + 2 * i], (f1 - normg[2 * i + 1]) * normg[NIN + 2 * i + 1]);
+    }
+    h2v h1[NH / 2];
+#pragma unroll
+    for (int i = 0; i < NH; i += 2) {
+        float a0 = __half2float(wg[OFF_B1 + i]), a1 = __half2float(wg[OFF_B1 + i + 1]);
+#pragma unroll
+        for (int m = 0; m < NIN / 2; ++m) {
+            a0 = __builtin_amdgcn_fdot2(w2[(i * NIN) / 2 + m], x[m], a0, false);
+            a1 = __builtin_amdgcn_fdot2(w2[((i + 1) * NIN) / 2 + m], x[m], a1, false);
+        }
+        h1[i / 2] = pack_h2(a0 > 0.f ? a0 : 0.f, a1 > 0.f ? a1 : 0.f);
+    }
+    h2v h2[NH / 2];
+#pragma unroll
+    for (int i = 0; i < NH; i += 2) {
+        float a0 = __half2float(wg[OFF_B2 + i]), a1 = __half2float(wg[OFF_B2 + i + 1]);
+#pragma unroll
+        for (int m = 0; m < NH / 2; ++m) {
+            a0 = __builtin_amdgcn_fdot2(w2[(OFF_W2 + i * NH) / 2 + m], h1[m], a0, false);
+            a1 = __builtin_amdgcn_fdot2(w2[(OFF_W2 + (i + 1) * NH) / 2 + m], h1[m], a1, false);
+        }
+        h2[i / 2] = pack_h2(a0 > 0.f ? a0 : 0.f, a1 > 0.f ? a1 : 0.f);
+    }
+    float y = __half2float(wg[OFF_B3]);
+#pragma unroll
+    for (int m = 0; m < NH / 2; ++m) y = __builtin_amdgcn_fdot2(w2[OFF_W3 / 2 + m], h2[m], y, false);
     if (yout) yout[k] = y;
     float c = expm1f(y);
     cells_pred[k] = c > 0.f ? c : 0.f;

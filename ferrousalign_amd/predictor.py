@@ -100,10 +100,14 @@ class WorkPredictor:
         N.check(self.lib.fa_batch_features(self.ctx.handle, db._h, N.ptr(f)), "fa_batch_features")
         return f
 
-    def predict(self, db, set_cost=False):
-        out = np.zeros(db.n, np.float32)
-        N.check(self.lib.fa_mlp_predict(self.ctx.handle, self._h, db._h, N.ptr(out), int(set_cost)),
-                "fa_mlp_predict")
+    def predict(self, db, set_cost=False, fresh_features=False, download=True):
+        """Predicted cells of every pair of a device batch; set_cost makes them
+        the batch's packing cost (kept in device memory), fresh_features
+        computes the features again first (the per-batch cost of a new batch);
+        download=False skips the copy to the host and returns None."""
+        out = np.zeros(db.n, np.float32) if download else None
+        flags = (1 if set_cost else 0) | (2 if fresh_features else 0)
+        N.check(self.lib.fa_mlp_predict(self.ctx.handle, self._h, db._h, N.ptr(out), flags), "fa_mlp_predict")
         return out
 
     def gradient(self, db, pairs):

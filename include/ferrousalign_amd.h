@@ -63,6 +63,8 @@ extern "C" {
 #define FA_RUN_ALIGN2 0x1000u  /* run/time the batch's mate-rescue local SW (fa_batch_set_align2) */
 #define FA_LAUNCH_SINGLE 0x400u /* one launch for the whole batch at the slot count of its
                                    longest query (no query-length buckets) */
+#define FA_PLAN_FRESH 0x2000u  /* plan the packing again even when the batch's plan for these
+                                  flags is current (per-batch planning cost in a timed step) */
 
 /* ---- records ------------------------------------------------------------ */
 /* BandedPairWiseSW, src/core/alignment/banded_swa/types.rs:106-199 */
@@ -358,8 +360,10 @@ int fa_mlp_get_params(fa_ctx *ctx, fa_mlp *m, float *params);
 /* Raw features of every pair of a resident batch, computed on the device
  * from its 2-bit sequences; copies them to feat[n][6] when feat != NULL. */
 int fa_batch_features(fa_ctx *ctx, fa_batch *b, float *feat);
-/* Predicted cells per pair (to cells_pred when non-NULL); with set_cost the
- * predictions become the batch's packing cost. */
+/* Predicted cells per pair (to cells_pred when non-NULL).  set_cost bit 0:
+ * the predictions become the batch's packing cost (they stay in device
+ * memory, where the packing plan reads them); bit 1: compute the batch's
+ * features again first (what a new batch costs). */
 int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32_t set_cost);
 /* Mean loss 0.5*(y - log1p(cells))^2 over pairs[0..count) of the batch and
  * its gradient (labels: the cells recorded by the batch's last run). */

@@ -25,11 +25,12 @@ static int32_t f64_as_i32(double x) {
     return (int32_t)x; /* C truncates toward zero, like Rust */
 }
 
-int ora_ksw_extend2(int32_t qlen, const uint8_t *query, int32_t tlen,
-                    const uint8_t *target, int32_t m, const int8_t *mat,
-                    int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
-                    int32_t w, int32_t end_bonus, int32_t zdrop, int32_t h0,
-                    ora_ext_result *out, uint64_t *cells) {
+/* rows (optional): target rows the row loop ran, the break row included. */
+static int extend2_core(int32_t qlen, const uint8_t *query, int32_t tlen,
+                        const uint8_t *target, int32_t m, const int8_t *mat,
+                        int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                        int32_t w, int32_t end_bonus, int32_t zdrop, int32_t h0,
+                        ora_ext_result *out, uint64_t *cells, int32_t *rows) {
     if (h0 <= 0) return ORA_ERR_H0;   /* :968 assert!(h0 > 0) */
     if (qlen <= 0) return ORA_ERR_QLEN; /* :993 eh[1] needs qlen >= 1 */
     if (tlen < 0) return ORA_ERR_TLEN;
@@ -85,8 +86,10 @@ int ora_ksw_extend2(int32_t qlen, const uint8_t *query, int32_t tlen,
     int32_t beg = 0, end = qlen;
     uint64_t ncell = 0;
 
+    int32_t nrows = 0;
     for (int32_t i = 0; i < tlen; ++i) {
         int32_t f = 0, m_row = 0, mj = -1;
+        nrows = i + 1;
         const int8_t *q = qp + (size_t)target[i] * qn;
 
         if (beg < i - w) beg = i - w;         /* :1038-1040 */
@@ -173,7 +176,17 @@ int ora_ksw_extend2(int32_t qlen, const uint8_t *query, int32_t tlen,
     out->gscore = gscore;
     out->max_off = max_off;
     if (cells) *cells = ncell;
+    if (rows) *rows = nrows;
     return ORA_OK;
+}
+
+int ora_ksw_extend2(int32_t qlen, const uint8_t *query, int32_t tlen,
+                    const uint8_t *target, int32_t m, const int8_t *mat,
+                    int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                    int32_t w, int32_t end_bonus, int32_t zdrop, int32_t h0,
+                    ora_ext_result *out, uint64_t *cells) {
+    return extend2_core(qlen, query, tlen, target, m, mat, o_del, e_del, o_ins, e_ins, w,
+                        end_bonus, zdrop, h0, out, cells, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -190,6 +203,7 @@ typedef struct {
     ora_ext_result *out;
     uint64_t *cells;
     int32_t *rc;
+    int32_t *rows;
     int64_t nbad;
 } batch_job_t;
 
@@ -198,10 +212,11 @@ static void *batch_worker(void *arg) {
     for (int64_t k = b->lo; k < b->hi; ++k) {
         ora_ext_result r = {0, 0, 0, 0, 0, 0};
         uint64_t c = 0;
-        int code = ora_ksw_extend2(b->qlen[k], b->qseq + b->qoff[k], b->tlen[k],
-                                   b->tseq + b->toff[k], b->m, b->mat, b->o_del,
-                                   b->e_del, b->o_ins, b->e_ins, b->w[k],
-                                   b->end_bonus, b->zdrop, b->h0[k], &r, &c);
+        int32_t nr = 0;
+        int code = extend2_core(b->qlen[k], b->qseq + b->qoff[k], b->tlen[k],
+                                b->tseq + b->toff[k], b->m, b->mat, b->o_del,
+                                b->e_del, b->o_ins, b->e_ins, b->w[k],
+                                b->end_bonus, b->zdrop, b->h0[k], &r, &c, &nr);
         if (code != ORA_OK) {
             ++b->nbad;
             memset(&r, 0, sizeof r);
@@ -210,6 +225,7 @@ static void *batch_worker(void *arg) {
         b->out[k] = r;
         if (b->cells) b->cells[k] = c;
         if (b->rc) b->rc[k] = code;
+        if (b->rows) b->rows[k] = code == ORA_OK ? nr : 0;
     }
     return NULL;
 }
@@ -222,6 +238,19 @@ int64_t ora_extend_batch(int64_t n, const uint8_t *qseq, const int64_t *qoff,
                          int32_t o_ins, int32_t e_ins, int32_t end_bonus,
                          int32_t zdrop, ora_ext_result *out, uint64_t *cells,
                          int32_t *rc, int nthreads) {
+    return ora_extend_batch_rows(n, qseq, qoff, qlen, tseq, toff, tlen, h0, w, m, mat, o_del,
+                                 e_del, o_ins, e_ins, end_bonus, zdrop, out, cells, rc, NULL,
+                                 nthreads);
+}
+
+int64_t ora_extend_batch_rows(int64_t n, const uint8_t *qseq, const int64_t *qoff,
+                              const int32_t *qlen, const uint8_t *tseq,
+                              const int64_t *toff, const int32_t *tlen,
+                              const int32_t *h0, const int32_t *w, int32_t m,
+                              const int8_t *mat, int32_t o_del, int32_t e_del,
+                              int32_t o_ins, int32_t e_ins, int32_t end_bonus,
+                              int32_t zdrop, ora_ext_result *out, uint64_t *cells,
+                              int32_t *rc, int32_t *rows, int nthreads) {
     if (n <= 0) return 0;
     if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     if (nthreads < 1) nthreads = 1;
@@ -253,6 +282,7 @@ int64_t ora_extend_batch(int64_t n, const uint8_t *qseq, const int64_t *qoff,
         b->out = out;
         b->cells = cells;
         b->rc = rc;
+        b->rows = rows;
         b->nbad = 0;
     }
     if (nthreads == 1) {

@@ -51,6 +51,16 @@ int64_t ora_extend_batch(int64_t n, const uint8_t *qseq, const int64_t *qoff,
                          int32_t o_ins, int32_t e_ins, int32_t end_bonus,
                          int32_t zdrop, ora_ext_result *out, uint64_t *cells,
                          int32_t *rc, int nthreads);
+/* As ora_extend_batch, plus rows[k] (optional): target rows job k's row loop
+ * ran, the row it broke on included (0 for an invalid job). */
+int64_t ora_extend_batch_rows(int64_t n, const uint8_t *qseq, const int64_t *qoff,
+                              const int32_t *qlen, const uint8_t *tseq,
+                              const int64_t *toff, const int32_t *tlen,
+                              const int32_t *h0, const int32_t *w, int32_t m,
+                              const int8_t *mat, int32_t o_del, int32_t e_del,
+                              int32_t o_ins, int32_t e_ins, int32_t end_bonus,
+                              int32_t zdrop, ora_ext_result *out, uint64_t *cells,
+                              int32_t *rc, int32_t *rows, int nthreads);
 
 /* Target segment of a 2-bit .pac reference (pac_oracle.c): `len` bases from
  * `start` in the bidirectional coordinates [0, 2 l_pac) (>= l_pac: reverse

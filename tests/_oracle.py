@@ -24,6 +24,9 @@ def lib():
         l.ora_extend_batch.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P] + [_I32] * 6 + [_P, _P, _P,
                                                                                                         ctypes.c_int]
         l.ora_extend_batch.restype = _I64
+        l.ora_extend_batch_rows.argtypes = ([_I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P] + [_I32] * 6 +
+                                            [_P, _P, _P, _P, ctypes.c_int])
+        l.ora_extend_batch_rows.restype = _I64
         _lib = l
     return _lib
 
@@ -61,6 +64,23 @@ def extend_batch(sw, qseq, qoff, qlen, tseq, toff, tlen, h0, w, nthreads=0):
     lib().ora_extend_batch(n, *[P(a) for a in arrs], 5, P(mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins,
                            sw.end_bonus, sw.zdrop, P(out), P(cells), P(rc), nthreads)
     return out, cells, rc
+
+
+def extend_rows(sw, qseq, qoff, qlen, tseq, toff, tlen, h0, w, nthreads=0):
+    """Target rows each job's row loop ran (the break row included), with the
+    cells: (rows int32[n], cells uint64[n])."""
+    n = len(qlen)
+    out = np.zeros((n, 6), np.int32)
+    cells = np.zeros(n, np.uint64)
+    rc = np.zeros(n, np.int32)
+    rows = np.zeros(n, np.int32)
+    arrs = [np.ascontiguousarray(x, dt) for x, dt in ((qseq, np.uint8), (qoff, np.int64), (qlen, np.int32),
+                                                      (tseq, np.uint8), (toff, np.int64), (tlen, np.int32),
+                                                      (h0, np.int32), (w, np.int32))]
+    mat = np.ascontiguousarray(sw.mat, np.int8)
+    lib().ora_extend_batch_rows(n, *[P(a) for a in arrs], 5, P(mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins,
+                                sw.end_bonus, sw.zdrop, P(out), P(cells), P(rc), P(rows), nthreads)
+    return rows, cells
 
 
 def to_outscore_order(ora_out):

@@ -39,8 +39,9 @@ def capped_w(qlen, mat, o_del, e_del, o_ins, e_ins, w, end_bonus):
     return min(w, md)
 
 
-def extend(query, target, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0, m=5):
-    """Returns ((score, qle, tle, gtle, gscore, max_off), cells)."""
+def extend(query, target, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0, m=5, with_rows=False):
+    """Returns ((score, qle, tle, gtle, gscore, max_off), cells), and the rows
+    the row loop ran (the break row included) when with_rows."""
     qlen, tlen = len(query), len(target)
     assert h0 > 0 and qlen > 0
     oe_del, oe_ins = o_del + e_del, o_ins + e_ins
@@ -52,8 +53,10 @@ def extend(query, target, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, 
     best, max_i, max_j, max_ie, gscore, max_off = h0, -1, -1, -1, -1, 0
     beg, end = 0, qlen
     cells = 0
+    rows = 0
     NEG = -(1 << 40)
     for i in range(tlen):
+        rows = i + 1
         rb = max(beg, i - w)
         re = min(end, i + w + 1, qlen)
         cells += max(re - rb, 0)
@@ -103,4 +106,5 @@ def extend(query, target, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, 
         nb = first[0] if first else re
         ne = max(nb, nz[-1]) if nz else nb
         beg, end = nb, min(ne + 2, qlen)
-    return (best, max_j + 1, max_i + 1, max_ie + 1, gscore, max_off), cells
+    res = (best, max_j + 1, max_i + 1, max_ie + 1, gscore, max_off)
+    return (res, cells, rows) if with_rows else (res, cells)

@@ -93,3 +93,60 @@ def test_training_and_learned_packing(setup):
     db.run(N.FA_PACK_COST | N.FA_LAUNCH_PERSISTENT | N.FA_LAUNCH_GRAPH)
     out, c2 = db.download()
     assert np.array_equal(out, ref_out) and np.array_equal(c2, ref_cells)
+
+
+def _wavefront_work(db, flags, rows):
+    """Sum over the wavefronts of a packing of (max true rows of its pairs x
+    VALU instructions per wavefront-row of its kernel): a wavefront runs until
+    its slowest pair stops, so this is the work a packing decides (31 per slot
+    and ~175 per row for the packed kernel's two pairs per lane group, ~40 and
+    ~175 for the 32-bit kernel's one)."""
+    order = db.order(flags)
+    total = 0.0
+    for l in db.profile_launches(flags):
+        packed = l["kernel"].startswith("extend16")
+        per_wave = (128 if packed else 64) // l["lanes"]
+        per_row = (31.0 if packed else 40.0) * l["slots"] + 175.0
+        r = rows[order[l["first"]:l["first"] + l["count"]]]
+        pad = (-len(r)) % per_wave
+        r = np.concatenate([r, np.zeros(pad, r.dtype)]).reshape(-1, per_wave)
+        total += float(r.max(axis=1).sum()) * per_row
+    return total
+
+
+def test_learned_packing_lowers_wavefront_work():
+    """What the trained network gains over length sorting.  A network trained
+    on one seed's uneven mix packs held-out pairs (another seed); the work
+    model above, with the oracle's true row counts, must be lower for the
+    learned packing than for length sorting, and lowest when the packing uses
+    the true cell counts (a perfect predictor).  Deterministic: fixed seeds,
+    deterministic training reductions and an integer-keyed packing plan."""
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd.banded_swa import DeviceBatch, Context
+    from ferrousalign_amd.predictor import WorkPredictor
+    ctx = Context(0)
+    sw = synth.default_scoring()
+    tr = synth.workload("uneven_mix_train_16m", n=262144)
+    dbt = DeviceBatch(sw, *tr.flat(), ctx=ctx)
+    dbt.run(0)
+    pred = WorkPredictor(ctx, seed=0)
+    print("train 262144", flush=True)
+    pred.fit(dbt, epochs=3, minibatch=8192, lr=3e-3)
+    dbt.free()
+    ps = synth.workload("uneven_mix_4m", n=131072)
+    rows, ocells = O.extend_rows(sw, *ps.flat())
+    db = DeviceBatch(sw, *ps.flat(), ctx=ctx)
+    db.set_launch(0)
+    length = _wavefront_work(db, N.FA_PACK_LENGTH, rows)
+    pred.predict(db, set_cost=True, download=False)
+    learned = _wavefront_work(db, N.FA_PACK_COST, rows)
+    db.set_cost(ocells.astype(np.float32))
+    perfect = _wavefront_work(db, N.FA_PACK_COST, rows)
+    print("wavefront work: length %.4g learned %.4g (%.4f) perfect %.4g (%.4f)"
+          % (length, learned, learned / length, perfect, perfect / length), flush=True)
+    assert learned < length
+    assert perfect <= learned
+    out, cells = db.download()
+    assert np.array_equal(cells.astype(np.uint64), ocells)
+    db.free()
+    ctx.close()

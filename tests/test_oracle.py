@@ -235,3 +235,40 @@ def test_invariants(name):
     # thread count independence
     out3, cells3, _ = O.extend_batch(sw, *ps.flat(), nthreads=1)
     assert (out3 == out).all() and (cells3 == cells).all()
+
+
+def test_rows_match_rowpar_and_bounds():
+    """Rows of the row loop (what a wavefront's run time follows): the C
+    oracle against the independent row-parallel restatement on random pairs,
+    plus closed forms -- tlen = 0 runs no row; an exact match of a query
+    against itself plus a random tail runs past the query end; a random
+    non-homologous target breaks early."""
+    from ferrousalign_amd import synth
+    sw = synth.default_scoring()
+    rng = np.random.default_rng(3)
+    qs, ts, ql, tl, h0, w = [], [], [], [], [], []
+    for _ in range(300):
+        q = rng.integers(0, 4, int(rng.integers(1, 40))).astype(np.uint8)
+        t = rng.integers(0, 4, int(rng.integers(0, 60))).astype(np.uint8)
+        if rng.random() < 0.5 and len(t) >= len(q):
+            t[:len(q)] = q
+        qs.append(q), ts.append(t), ql.append(len(q)), tl.append(len(t))
+        h0.append(int(rng.integers(1, 40))), w.append(int(rng.integers(0, 60)))
+    qo = np.concatenate([[0], np.cumsum(ql)[:-1]]).astype(np.int64)
+    to = np.concatenate([[0], np.cumsum(tl)[:-1]]).astype(np.int64)
+    rows, cells = O.extend_rows(sw, np.concatenate(qs), qo, ql, np.concatenate(ts + [np.zeros(1, np.uint8)]), to, tl,
+                                h0, w)
+    for k in range(len(ql)):
+        _, c, r = rowpar.extend(list(qs[k]), list(ts[k]), list(sw.mat), sw.o_del, sw.e_del, sw.o_ins, sw.e_ins,
+                                w[k], sw.end_bonus, sw.zdrop, h0[k], with_rows=True)
+        assert (int(rows[k]), int(cells[k])) == (r, c), k
+        assert 0 <= rows[k] <= tl[k] and (tl[k] == 0) == (rows[k] == 0)
+    # exact match, then a long random tail: the loop runs beyond the query
+    q = rng.integers(0, 4, 50).astype(np.uint8)
+    t = np.concatenate([q, rng.integers(0, 4, 200).astype(np.uint8)])
+    r, _ = O.extend_rows(sw, q, [0], [50], t, [0], [250], [20], [100])
+    assert 50 < r[0] <= 250
+    # a random target against a short seed score: breaks within a few rows
+    t2 = rng.integers(0, 4, 250).astype(np.uint8)
+    r2, _ = O.extend_rows(sw, q, [0], [50], t2, [0], [250], [5], [100])
+    assert r2[0] < 50

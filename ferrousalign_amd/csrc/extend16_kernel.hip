@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "extend_kernel.h"
 
 namespace fa {
@@ -109,6 +111,8 @@ __device__ __forceinline__ uint32_t bfi1(uint32_t m, uint32_t a, uint32_t b) {
 
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr = 0x110;
+// row forms of the packed kernel (see the row loop of extend16_wave)
+constexpr int kFull = 0, kWkill = 1, kFast = 2;
 constexpr int kRowNewBcast15 = 0x15F;  // every lane of a 16-lane row reads its lane 15
 
 // DPP move with bound_ctrl: a lane without a source lane (row_shr at the
@@ -378,16 +382,34 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     const int capins = a.e_ins > 0 ? 32767 / a.e_ins : 32767;
     const uint32_t capdelp = pk(capdel, capdel), capinsp = pk(capins, capins);
 
+    // per-lane constants of the unmasked row forms: column qlen (the gscore
+    // column, never in a row's [rb, re)) is the last slot of the last lane for
+    // both halves, and slot 0 of lane 0 is always dead there (c <= 0 < rb)
+    const uint32_t notlast = l == L - 1 ? 0u : 0xFFFFFFFFu;
+    const uint32_t notlane0 = l == 0 ? 0u : 0xFFFFFFFFu;
+
     uint32_t ip = 0u;  // i in both halves
     for (int i = 0;; ++i, ip += 0x00010001u) {
         if (__builtin_amdgcn_ballot_w64(P.alive != 0u) == 0) break;
         // :1038-1046; a finished pair gets the empty band [4000, -3)
         const uint32_t rbp = bfi1(P.alive, smax(P.beg, sub16(ip, P.w)), 0x0FA00FA0u);
         const uint32_t rep = bfi1(P.alive, smin(smin(P.end, add16(ip, P.w1)), P.qlen), 0xFFFDFFFDu);
-        // :1049-1054 h1 = max(h0 - (o_del + e_del*(i+1)), 0) when rb == 0 (h0 < 4096,
-        // so clamping the subtrahend at 32767 keeps the difference in range)
-        const int dsub = min(a.o_del + a.e_del * (i + 1), 32767);
-        const uint32_t h1ip = smax(sub16(P.h0, pk(dsub, dsub)), 0u) & lt16(rbp, 0x00010001u);
+        // Row form, the same for the whole wavefront.  Every column left of the
+        // band holds h = e = 0 at the start of a row: the band start moves past a
+        // column by the beg rule only when its h and e are 0 (:1148-1152), and a
+        // masked row writes 0 into every column it leaves behind.  A zero column
+        // computed as if it were in the band stays 0 (M <= 0, E = 0, F <= 0 from
+        // zero columns on its left) and adds nothing to the F prefix, the row
+        // maximum or the non-zero scan.  So when every live pair has rb > 0 (no
+        // first-column value h1 to place), re == qlen (no stale columns on the
+        // right to keep) and rb == beg (no column dropped by the i - w bound
+        // this row), no band mask is needed (kFast); when only the i - w bound
+        // drops a column, the left mask alone is (kWkill); other rows keep
+        // every mask (kFull).
+        const uint32_t live = P.alive;
+        const bool any_short = __builtin_amdgcn_ballot_w64((live & lt16(rep, P.qlen)) != 0u) != 0;
+        const bool any_rb0 = __builtin_amdgcn_ballot_w64((live & lt16(rbp, 0x00010001u)) != 0u) != 0;
+        const bool any_wk = __builtin_amdgcn_ballot_w64((live & lt16(P.beg, rbp)) != 0u) != 0;
         // both pairs' target codes of this row (a row past a pair's target reads
         // its code 0, and a finished pair's half is never used)
         const uint32_t tw = tc2[i];
@@ -403,70 +425,132 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         // at slot s-1.
         const uint32_t trb = sub16(rbp, cbp), tre = sub16(rep, cbp);
 
-        // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
-        uint32_t Mv[S], pre[S], mrb[S];
-        uint32_t run = negs;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
-            const uint32_t M = smin(sub16(add16(Hh[s], sb), biasp), mul16(Hh[s], kp));
-            Mv[s] = M;
-            mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
-            const uint32_t G = bfi(mrb[s], negs, smax(add16(M, seo[s]), se[s]));
-            pre[s] = run;
-            run = smax(run, G);
-        }
-        // carry relative to this lane's cb*e_ins; the scanned values run + cb*e_ins
-        // lie in (-16384, 16384), so with 0x4000 added they scan as unsigned with
-        // identity 0 (the empty prefix of lane 0 comes back as -16384 - cb*e_ins)
-        const uint32_t carry = sub16(excl_scan_u2<L>(add16(run, cbeb), l), cbeb);
-
-        // pass 2: H, E, row-max key, write-back, non-zero columns
-        // acc: one bit per slot and half; with S > 16 the first S - 16 slots
-        // go to accH (the high bits of a 32-bit mask per half)
-        uint32_t pkey = 0, acc = 0, accH = 0, Hprev = 0;
-        uint32_t mle_rb0 = 0, mle_re0 = 0, wr0 = 0;
-        uint32_t mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
-        uint32_t mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const uint32_t sp = (uint32_t)s * 0x00010001u;
-            const uint32_t F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
-            const uint32_t H = smax(smax(Mv[s], Ee[s]), F);
-            const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
-            const uint32_t mlt_rb = mrb[s], mle_rb = mlt_rb_prev;
-            const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
-            mlt_rb_prev = mlt_rb;
-            mlt_re_prev = mlt_re;
-            const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
-            pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
-            Ee[s] = bfi1(mle_re, En & mlt_re, Ee[s]);  // E' in band, 0 at c = re (:1106-1107)
-            const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
-            if (s == 0) {
-                mle_rb0 = mle_rb;
-                mle_re0 = mle_re;
-                wr0 = wr;
-            } else {
-                const uint32_t hv = bfi1(mle_rb, h1ip, Hprev);  // eh[c].h <- H(i, c-1) or h1
-                Hh[s] = bfi1(mle_re, hv, Hh[s]);
-                const uint32_t x = (Hh[s] | Ee[s]) & wr;
-                if (S > 16 && s < S - 16)
-                    accH = dbl_add(accH, nz01(x));
-                else
-                    acc = dbl_add(acc, nz01(x));
+        uint32_t pkey = 0, acc = 0, accH = 0;
+        auto row = [&](auto form) {
+            constexpr int MODE = decltype(form)::value;
+            constexpr bool LEFT = MODE != kFast;   // left band mask
+            constexpr bool FULL = MODE == kFull;   // right band mask and h1
+            // :1049-1054 h1 = max(h0 - (o_del + e_del*(i+1)), 0) when rb == 0 (h0 < 4096,
+            // so clamping the subtrahend at 32767 keeps the difference in range)
+            uint32_t h1ip = 0u;
+            if constexpr (FULL) {
+                const int dsub = min(a.o_del + a.e_del * (i + 1), 32767);
+                h1ip = smax(sub16(P.h0, pk(dsub, dsub)), 0u) & lt16(rbp, 0x00010001u);
             }
-            Hprev = H;
-        }
-        {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
-            const uint32_t hl = dmov<kRowShr + 1>(Hprev);
-            const uint32_t hv = bfi1(mle_rb0, h1ip, hl);
-            Hh[0] = bfi1(mle_re0, hv, Hh[0]);
-            const uint32_t x = (Hh[0] | Ee[0]) & wr0;
-            if constexpr (S > 16)
-                accH = mad_s(nz01(x), (1u << (S - 17)) * 0x00010001u, accH);
-            else
-                acc = mad_s(nz01(x), (1u << (S - 1)) * 0x00010001u, acc);
-        }
+            // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
+            uint32_t Mv[S], pre[S], mrb[S];
+            uint32_t run = negs;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
+                const uint32_t M = smin(sub16(add16(Hh[s], sb), biasp), mul16(Hh[s], kp));
+                Mv[s] = M;
+                const uint32_t G0 = smax(add16(M, seo[s]), se[s]);
+                uint32_t G = G0;
+                if constexpr (LEFT) {
+                    mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
+                    G = bfi(mrb[s], negs, G0);
+                }
+                pre[s] = run;
+                run = smax(run, G);
+            }
+            // carry relative to this lane's cb*e_ins; the scanned values run + cb*e_ins
+            // lie in (-16384, 16384), so with 0x4000 added they scan as unsigned with
+            // identity 0 (the empty prefix of lane 0 comes back as -16384 - cb*e_ins)
+            const uint32_t carry = sub16(excl_scan_u2<L>(add16(run, cbeb), l), cbeb);
+
+            // pass 2: H, E, row-max key, write-back, non-zero columns
+            // acc: one bit per slot and half; with S > 16 the first S - 16 slots
+            // go to accH (the high bits of a 32-bit mask per half)
+            uint32_t Hprev = 0;
+            uint32_t mle_rb0 = 0, mle_re0 = 0, wr0 = 0, mlt_rb0 = 0;
+            uint32_t mlt_rb_prev = 0, mlt_re_prev = 0;
+            if constexpr (LEFT) mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
+            if constexpr (FULL) mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const uint32_t sp = (uint32_t)s * 0x00010001u;
+                const uint32_t F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
+                const uint32_t H = smax(smax(Mv[s], Ee[s]), F);
+                const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+                uint32_t x;
+                if constexpr (FULL) {
+                    const uint32_t mlt_rb = mrb[s], mle_rb = mlt_rb_prev;
+                    const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
+                    mlt_rb_prev = mlt_rb;
+                    mlt_re_prev = mlt_re;
+                    const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
+                    pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+                    // E' in band, 0 at c = re (:1106-1107) and left of the band
+                    Ee[s] = bfi1(mle_re, En & mlt_re & ~mlt_rb, Ee[s]);
+                    const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
+                    if (s == 0) {
+                        mle_rb0 = mle_rb;
+                        mle_re0 = mle_re;
+                        wr0 = wr;
+                        mlt_rb0 = mlt_rb;
+                        x = 0u;
+                    } else {
+                        // eh[c].h <- H(i, c-1), h1 at c = rb, 0 left of the band
+                        const uint32_t hv = bfi1(mle_rb, h1ip, Hprev) & ~mlt_rb;
+                        Hh[s] = bfi1(mle_re, hv, Hh[s]);
+                        x = (Hh[s] | Ee[s]) & wr;
+                    }
+                } else {
+                    // re == qlen: every column but qlen is below re
+                    const uint32_t keep = s == S - 1 ? notlast : 0xFFFFFFFFu;
+                    uint32_t Hm = H, Ev = En;
+                    if constexpr (LEFT) {
+                        Hm = H & ~mrb[s] & keep;
+                        Ev = En & ~mrb[s] & keep;
+                    } else if (s == S - 1) {
+                        Hm = H & keep;
+                        Ev = En & keep;
+                    }
+                    pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+                    Ee[s] = Ev;
+                    if (s == 0) {
+                        x = 0u;
+                    } else {
+                        // H(i, c-1); c <= rb takes h1 = 0 (rb > 0), which a zero
+                        // column on the left already gives in the unmasked form
+                        Hh[s] = LEFT ? (Hprev & ~mrb[s - 1]) : Hprev;
+                        x = Hh[s] | Ee[s];
+                    }
+                }
+                if (s > 0) {
+                    if (S > 16 && s < S - 16)
+                        accH = dbl_add(accH, nz01(x));
+                    else
+                        acc = dbl_add(acc, nz01(x));
+                }
+                Hprev = H;
+            }
+            {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
+                const uint32_t hl = dmov<kRowShr + 1>(Hprev);
+                uint32_t x0;
+                if constexpr (FULL) {
+                    const uint32_t hv = bfi1(mle_rb0, h1ip, hl) & ~mlt_rb0;
+                    Hh[0] = bfi1(mle_re0, hv, Hh[0]);
+                    x0 = (Hh[0] | Ee[0]) & wr0;
+                } else {
+                    // lane 0's slot 0 is dead; c <= rb takes 0
+                    Hh[0] = LEFT ? (hl & ~sgnmask(sub16(0xFFFFFFFFu, trb))) : (hl & notlane0);
+                    x0 = Hh[0] | Ee[0];
+                }
+                if constexpr (S > 16)
+                    accH = mad_s(nz01(x0), (1u << (S - 17)) * 0x00010001u, accH);
+                else
+                    acc = mad_s(nz01(x0), (1u << (S - 1)) * 0x00010001u, acc);
+            }
+        };
+        if (any_short || any_rb0)
+            row(std::integral_constant<int, kFull>());
+        else if (any_wk)
+            row(std::integral_constant<int, kWkill>());
+        else
+            row(std::integral_constant<int, kFast>());
+
         // acc: bit S-1-s of each half set where column cb+s in [rb, re] has
         // eh.h or eh.e non-zero after the row.  First such column as 1023 - c,
         // last as c + 1, 0 when none (:1148-1158).

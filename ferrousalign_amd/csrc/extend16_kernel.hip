@@ -114,6 +114,7 @@ constexpr int kRowShr = 0x110;
 // row forms of the packed kernel (see the row loop of extend16_wave)
 constexpr int kFull = 0, kWkill = 1, kFast = 2;
 constexpr int kRowNewBcast15 = 0x15F;  // every lane of a 16-lane row reads its lane 15
+constexpr int kQuadBcast3 = 0xFF;      // quad_perm [3,3,3,3]: every lane of a quad reads its lane 3
 
 // DPP move with bound_ctrl: a lane without a source lane (row_shr at the
 // start of a 16-lane row) reads 0, and no old value is needed, so the
@@ -214,8 +215,17 @@ struct PairPack {
     uint32_t alive;                     // 0xFFFF per pair still running
     uint32_t qlen, tlen, w, w1, h0;     // w1 = w + 1
     uint32_t beg, end, best, max_i, max_j, max_ie, gscore, max_off;
+    uint32_t nacc;                      // cells of the rows since the last flush, both pairs
     uint32_t ncell[2];                  // cells per pair (32-bit)
 };
+
+// A row adds at most 255 cells per pair, so nacc takes 256 rows without
+// wrapping; it is folded into the 32-bit counts every 256 rows and at the end.
+__device__ __forceinline__ void flush_cells(PairPack &P) {
+    P.ncell[0] += P.nacc & 0xFFFFu;
+    P.ncell[1] += P.nacc >> 16;
+    P.nacc = 0u;
+}
 
 // 0xFFFF per half where a < b (signed, |a - b| < 32768)
 __device__ __forceinline__ uint32_t lt16(uint32_t a, uint32_t b) { return sgnmask(sub16(a, b)); }
@@ -229,9 +239,7 @@ __device__ __forceinline__ void pair_row_end2(const ExtArgs &a, PairPack &P, uin
                                               uint32_t fenc, uint32_t lastp1, uint32_t capdelp,
                                               uint32_t capinsp) {
     const uint32_t alive = P.alive;
-    const uint32_t d = smax(sub16(rep, rbp), 0u) & alive;
-    P.ncell[0] += d & 0xFFFFu;
-    P.ncell[1] += d >> 16;
+    P.nacc = add16(P.nacc, smax(sub16(rep, rbp), 0u) & alive);
     // :1110-1120 (re <= qlen, so re == qlen is !(re < qlen))
     const uint32_t g = alive & ~lt16(rep, P.qlen) & lt16(P.gscore, hend);
     P.gscore = bfi1(g, hend, P.gscore);
@@ -242,15 +250,23 @@ __device__ __forceinline__ void pair_row_end2(const ExtArgs &a, PairPack &P, uin
     const uint32_t better = lt16(P.best, mrow);  // :1126 (best >= h0 > 0, so m_row != 0)
     // :1134-1145 z-drop: gap = (di - dj) * e_del if di > dj else (dj - di) * e_ins,
     // with |di - dj| clamped to cap = 32767 / e (a clamped gap exceeds any
-    // best - m_row - zdrop, so the test is unchanged and the product fits)
-    const uint32_t dd = sub16(sub16(ip, P.max_i), sub16(mj, P.max_j));
-    const uint32_t pos = lt16(0u, dd);
-    const uint32_t ad = smax(dd, sub16(0u, dd));
-    const uint32_t gap = mul16(umin(ad, bfi1(pos, capdelp, capinsp)),
-                               bfi1(pos, pk(a.e_del, a.e_del), pk(a.e_ins, a.e_ins)));
+    // best - m_row - zdrop, so the test is unchanged and the product fits).
+    // The gap is never negative, so a pair can break only when
+    // best - m_row > zdrop; the gap is computed only in rows where some pair
+    // of the wavefront is there.
     // zdrop >= 4096 never breaks (best < 4096); clamped at 8191 it stays in range
     const int zd = min(a.zdrop, 8191);
-    const uint32_t zbrk = zd > 0 ? nz & ~better & lt16(gap, sub16(sub16(P.best, mrow), pk(zd, zd))) : 0u;
+    const uint32_t slack = sub16(sub16(P.best, mrow), pk(zd, zd));  // best - m_row - zdrop
+    const uint32_t cand = zd > 0 ? alive & nz & ~better & lt16(0u, slack) : 0u;
+    uint32_t zbrk = 0u;
+    if (__builtin_amdgcn_ballot_w64(cand != 0u) != 0) {
+        const uint32_t dd = sub16(sub16(ip, P.max_i), sub16(mj, P.max_j));
+        const uint32_t pos = lt16(0u, dd);
+        const uint32_t ad = smax(dd, sub16(0u, dd));
+        const uint32_t gap = mul16(umin(ad, bfi1(pos, capdelp, capinsp)),
+                                   bfi1(pos, pk(a.e_del, a.e_del), pk(a.e_ins, a.e_ins)));
+        zbrk = cand & lt16(gap, slack);
+    }
     const uint32_t upd = alive & better;
     P.best = bfi1(upd, mrow, P.best);
     P.max_i = bfi1(upd, ip, P.max_i);
@@ -328,6 +344,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     P.max_i = P.max_j = P.max_ie = P.gscore = 0xFFFFFFFFu;  // -1
     P.max_off = 0u;
     P.ncell[0] = P.ncell[1] = 0u;
+    P.nacc = 0u;
     stage_targets2(a, tc2, tl, to, l, L);
 
     const int oe_ins = a.o_ins + a.e_ins, oe_del = a.o_del + a.e_del;
@@ -572,16 +589,25 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         uint32_t flast = lenc[0] | (lenc[1] << 16);
         // column qlen (the group's last slot): H(i, qlen-1) when re == qlen
         uint32_t hend;
-        if constexpr (L == 16)
+        if constexpr (L == 16) {
             hend = dmov<kRowNewBcast15>(Hh[S - 1]);  // lane 15 of the 16-lane row
-        else
-            hend = allmax_u2<L>(l == L - 1 ? Hh[S - 1] : 0u);
+        } else if constexpr (L == 8) {
+            // lane 3 of each quad, then the other quad's through the half-row mirror
+            const uint32_t x = dmov<kQuadBcast3>(Hh[S - 1]);
+            const uint32_t y = dmov<kRowHalfMirror>(x);  // every lane, before the select
+            hend = l < 4 ? y : x;
+        } else {
+            hend = dmov<kQuadBcast3>(Hh[S - 1]);  // lane 3 of the 4-lane group
+        }
+        // flush the packed cell counts before they could wrap
+        if ((i & 255) == 255) flush_cells(P);
         const int kA = allmax_i<L>(kk[0]);
         const int kB = allmax_i<L>(kk[1]);
         ffirst = allmax_u2<L>(ffirst);
         flast = allmax_u2<L>(flast);
         pair_row_end2(a, P, ip, rbp, rep, kA, kB, hend, ffirst, flast, capdelp, capinsp);
     }
+    flush_cells(P);
     if (l == 0) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {

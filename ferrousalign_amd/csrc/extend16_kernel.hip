@@ -295,87 +295,68 @@ __device__ __forceinline__ void pair_row_end2(const ExtArgs &a, PairPack &P, uin
 // (rb = max(beg, i - w) with beg non-decreasing), so those registers may hold
 // anything; every reader masks them (G in pass 1, the argmax, the non-zero
 // scan) or overwrites them before use (eh.h at c = rb is h1).
+// Slot count a wavefront moves to once every running pair's live columns
+// [beg, qlen] fit it: about 2/3 of S, none below 4 (the per-row bookkeeping
+// dominates there).
+constexpr int next_slots16(int S) {
+    return S >= 24 ? 16 : S >= 22 ? 14 : S >= 20 ? 12 : S >= 18 ? 12 : S >= 16 ? 10 : S >= 14 ? 8
+         : S >= 12 ? 8 : S >= 10 ? 6 : S >= 8 ? 4 : S >= 6 ? 4 : 0;
+}
+
+// Wavefront state carried from one slot count to the next.
+struct WaveCtx {
+    int i;          // next row
+    uint32_t ip;    // i in both halves
+    int l;          // lane in the group
+    int ql[2];      // qlen per half
+    uint8_t *tc2;   // the group's staged target codes
+    uint32_t *xfer; // the wavefront's restripe buffer in LDS (64 x S words)
+};
+
+// Column position p = c + pad holds in lane p / S, slot p % S; with S' slots
+// the pad shrinks by L (S - S') for both halves, so position p' of the new
+// layout is position p' + L (S - S') of the old one.  The positions that fall
+// off hold columns below every running pair's beg.
+template <int L, int S, int NS>
+__device__ __forceinline__ void restripe(const WaveCtx &w, const uint32_t (&src)[S], uint32_t (&dst)[NS]) {
+    const int g = (int)(threadIdx.x & 63) / L;
+    uint32_t *buf = w.xfer + g * (L * S);
+#pragma unroll
+    for (int s = 0; s < S; ++s) buf[w.l * S + s] = src[s];
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    constexpr int D = L * (S - NS);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) dst[s] = buf[w.l * NS + s + D];
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// LDS bytes of a wavefront's restripe buffer at S slots.
+constexpr size_t xfer_bytes16(int S) { return next_slots16(S) > 0 ? (size_t)256 * S : 0; }
+
+// The row loop at S slots per lane, from row w.i until every pair of the
+// wavefront has stopped (false) or, when next_slots16(S) > 0, until the next
+// row's live columns fit that many slots (true, w at the next row).
 template <int L, int S>
-__device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, uint32_t nslots,
-                                              uint8_t *lds_wave) {
-    // lds_wave: tpad bytes of staged target codes per lane group
+__device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, WaveCtx &w,
+                                              const uint32_t (&profA)[S], const uint32_t (&profB)[S],
+                                              uint32_t (&Hh)[S], uint32_t (&Ee)[S]) {
+    constexpr int NS = next_slots16(S);
     // row-argmax key per half: (H << KB) | (KM - s); with S > 16 the host
     // routes only pairs whose scores stay below 2048 here, so the key fits
     constexpr int KB = S <= 8 ? 3 : (S <= 16 ? 4 : 5);
     constexpr uint32_t KM = (1u << KB) - 1u;
-    const int lane = (int)(threadIdx.x & 63);
-    const int g = lane / L;
-    const int l = lane % L;
-    PairPack P;
-    bool valid[2];
-    uint32_t pid[2], qo[2], to[2];
-    int tl[2], ql[2], h0v[2];
-    uint8_t *const tc2 = lds_wave + g * a.tpad;
+    const int l = w.l;
+    const uint8_t *const tc2 = w.tc2;
     int cbh[2];
-    int wv[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
-        valid[h] = slot < nslots;
-        uint4 m0 = {0u, 0u, 0u, 1u}, m1 = {0u, 1u, 0u, 0u};  // empty slot: qlen 1, tlen 0, h0 1
-        if (valid[h]) {
-            const uint4 *mp = reinterpret_cast<const uint4 *>(a.meta) + 2 * (size_t)slot;
-            m0 = mp[0];
-            m1 = mp[1];
-        }
-        pid[h] = m0.x;
-        ql[h] = (int)m0.w;
-        tl[h] = (int)m1.x;
-        h0v[h] = (int)m1.y;
-        wv[h] = min((int)m1.z, 4095);
-        qo[h] = m0.y;
-        to[h] = m0.z;
-        cbh[h] = l * S - (L * S - 1 - ql[h]);  // column of slot 0 in this lane
-    }
-    P.alive = (valid[0] && tl[0] > 0 ? 0xFFFFu : 0u) | (valid[1] && tl[1] > 0 ? 0xFFFF0000u : 0u);
-    P.qlen = pk(ql[0], ql[1]);
-    P.tlen = pk(tl[0], tl[1]);
-    P.w = pk(wv[0], wv[1]);
-    P.w1 = add16(P.w, 0x00010001u);
-    P.h0 = pk(h0v[0], h0v[1]);
-    P.beg = 0u;
-    P.end = P.qlen;
-    P.best = P.h0;
-    P.max_i = P.max_j = P.max_ie = P.gscore = 0xFFFFFFFFu;  // -1
-    P.max_off = 0u;
-    P.ncell[0] = P.ncell[1] = 0u;
-    P.nacc = 0u;
-    stage_targets2(a, tc2, tl, to, l, L);
-
+    for (int h = 0; h < 2; ++h) cbh[h] = l * S - (L * S - 1 - w.ql[h]);  // column of slot 0 in this lane
     const int oe_ins = a.o_ins + a.e_ins, oe_del = a.o_del + a.e_del;
     const uint32_t einsp = pk(a.e_ins, a.e_ins), edelp = pk(a.e_del, a.e_del);
     const uint32_t oedelp = pk(oe_del, oe_del);
     const uint32_t kp = pk(a.mguard, a.mguard);
     const uint32_t cbp = pk(cbh[0], cbh[1]);
-
-    uint32_t profA[S], profB[S], Hh[S], Ee[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        int hv[2];
-        uint32_t pw[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = cbh[h] + s;
-            int code = 4;
-            if (valid[h] && c >= 0 && c < ql[h]) code = decode_base(a.seq2, a.seqn, qo[h], c);
-            pw[h] = a.profw[code];
-            const int h0 = h0v[h];
-            const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
-            const int x = c == 0 ? h0 : max(v1 - (c - 1) * a.e_ins, 0);  // :992-998
-            hv[h] = c >= 0 ? x : 0;
-        }
-        profA[s] = pw[0];
-        profB[s] = pw[1];
-        Hh[s] = pk(hv[0], hv[1]);
-        Ee[s] = 0u;
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
     // G(c) = max(M - oe_ins, 0) + c*e_ins is split into the lane's cb*e_ins
     // and a lane-local part max(M + (s*e_ins - oe_ins), s*e_ins) whose
     // constants are wave-uniform.
@@ -405,9 +386,10 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     const uint32_t notlast = l == L - 1 ? 0u : 0xFFFFFFFFu;
     const uint32_t notlane0 = l == 0 ? 0u : 0xFFFFFFFFu;
 
-    uint32_t ip = 0u;  // i in both halves
-    for (int i = 0;; ++i, ip += 0x00010001u) {
-        if (__builtin_amdgcn_ballot_w64(P.alive != 0u) == 0) break;
+    for (;; ++w.i, w.ip += 0x00010001u) {
+        const int i = w.i;
+        const uint32_t ip = w.ip;
+        if (__builtin_amdgcn_ballot_w64(P.alive != 0u) == 0) return false;
         // :1038-1046; a finished pair gets the empty band [4000, -3)
         const uint32_t rbp = bfi1(P.alive, smax(P.beg, sub16(ip, P.w)), 0x0FA00FA0u);
         const uint32_t rep = bfi1(P.alive, smin(smin(P.end, add16(ip, P.w1)), P.qlen), 0xFFFDFFFDu);
@@ -606,7 +588,122 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         ffirst = allmax_u2<L>(ffirst);
         flast = allmax_u2<L>(flast);
         pair_row_end2(a, P, ip, rbp, rep, kA, kB, hend, ffirst, flast, capdelp, capinsp);
+        if constexpr (NS > 0) {
+            // the next row's live columns [beg, qlen] of every running pair fit
+            // NS slots per lane: move to that layout and go on there
+            const uint32_t need = sub16(add16(P.qlen, 0x00010001u), P.beg);
+            const uint32_t big = P.alive & ~lt16(need, (uint32_t)(L * NS + 1) * 0x00010001u);
+            if (__builtin_amdgcn_ballot_w64(big != 0u) == 0) {
+                ++w.i;
+                w.ip += 0x00010001u;
+                return true;
+            }
+        }
     }
+}
+
+// Rows at S slots, then at the next slot counts as the band shrinks.
+template <int L, int S>
+__device__ __forceinline__ void extend16_phase(const ExtArgs &a, PairPack &P, WaveCtx &w,
+                                               const uint32_t (&profA)[S], const uint32_t (&profB)[S],
+                                               uint32_t (&Hh)[S], uint32_t (&Ee)[S]) {
+    const bool shrink = extend16_rows<L, S>(a, P, w, profA, profB, Hh, Ee);
+    constexpr int NS = next_slots16(S);
+    if constexpr (NS > 0) {
+        if (shrink) {
+            uint32_t nA[NS], nB[NS], nH[NS], nE[NS];
+            restripe<L, S, NS>(w, profA, nA);
+            restripe<L, S, NS>(w, profB, nB);
+            restripe<L, S, NS>(w, Hh, nH);
+            restripe<L, S, NS>(w, Ee, nE);
+            extend16_phase<L, NS>(a, P, w, nA, nB, nH, nE);
+        }
+    }
+}
+
+template <int L, int S>
+__device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, uint32_t nslots,
+                                              uint8_t *lds_wave) {
+    // lds_wave: tpad bytes of staged target codes per lane group, then the
+    // restripe buffer (64 x S words)
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane / L;
+    const int l = lane % L;
+    PairPack P;
+    bool valid[2];
+    uint32_t pid[2], qo[2], to[2];
+    int tl[2], ql[2], h0v[2];
+    uint8_t *const tc2 = lds_wave + g * a.tpad;
+    int cbh[2];
+    int wv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
+        valid[h] = slot < nslots;
+        uint4 m0 = {0u, 0u, 0u, 1u}, m1 = {0u, 1u, 0u, 0u};  // empty slot: qlen 1, tlen 0, h0 1
+        if (valid[h]) {
+            const uint4 *mp = reinterpret_cast<const uint4 *>(a.meta) + 2 * (size_t)slot;
+            m0 = mp[0];
+            m1 = mp[1];
+        }
+        pid[h] = m0.x;
+        ql[h] = (int)m0.w;
+        tl[h] = (int)m1.x;
+        h0v[h] = (int)m1.y;
+        wv[h] = min((int)m1.z, 4095);
+        qo[h] = m0.y;
+        to[h] = m0.z;
+        cbh[h] = l * S - (L * S - 1 - ql[h]);  // column of slot 0 in this lane
+    }
+    P.alive = (valid[0] && tl[0] > 0 ? 0xFFFFu : 0u) | (valid[1] && tl[1] > 0 ? 0xFFFF0000u : 0u);
+    P.qlen = pk(ql[0], ql[1]);
+    P.tlen = pk(tl[0], tl[1]);
+    P.w = pk(wv[0], wv[1]);
+    P.w1 = add16(P.w, 0x00010001u);
+    P.h0 = pk(h0v[0], h0v[1]);
+    P.beg = 0u;
+    P.end = P.qlen;
+    P.best = P.h0;
+    P.max_i = P.max_j = P.max_ie = P.gscore = 0xFFFFFFFFu;  // -1
+    P.max_off = 0u;
+    P.ncell[0] = P.ncell[1] = 0u;
+    P.nacc = 0u;
+    stage_targets2(a, tc2, tl, to, l, L);
+
+    const int oe_ins = a.o_ins + a.e_ins;
+
+    uint32_t profA[S], profB[S], Hh[S], Ee[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        int hv[2];
+        uint32_t pw[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = cbh[h] + s;
+            int code = 4;
+            if (valid[h] && c >= 0 && c < ql[h]) code = decode_base(a.seq2, a.seqn, qo[h], c);
+            pw[h] = a.profw[code];
+            const int h0 = h0v[h];
+            const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
+            const int x = c == 0 ? h0 : max(v1 - (c - 1) * a.e_ins, 0);  // :992-998
+            hv[h] = c >= 0 ? x : 0;
+        }
+        profA[s] = pw[0];
+        profB[s] = pw[1];
+        Hh[s] = pk(hv[0], hv[1]);
+        Ee[s] = 0u;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    WaveCtx w;
+    w.i = 0;
+    w.ip = 0u;
+    w.l = l;
+    w.ql[0] = ql[0];
+    w.ql[1] = ql[1];
+    w.tc2 = tc2;
+    w.xfer = reinterpret_cast<uint32_t *>(lds_wave + (64 / L) * a.tpad);
+    extend16_phase<L, S>(a, P, w, profA, profB, Hh, Ee);
     flush_cells(P);
     if (l == 0) {
 #pragma unroll
@@ -647,7 +744,7 @@ __global__ __launch_bounds__(256) void extend16_plain_kernel(ExtArgs a, uint32_t
     constexpr int G = 64 / L;
     const int wave = (int)(threadIdx.x >> 6);
     const uint32_t slot0 = (blockIdx.x * (blockDim.x >> 6) + (uint32_t)wave) * (2u * G);
-    uint8_t *lds_wave = lds + wave * G * a.tpad;
+    uint8_t *lds_wave = lds + wave * ((size_t)G * a.tpad + xfer_bytes16(S));
     if (slot0 >= count) return;
     extend16_wave<L, S>(a, first + slot0, first + count, lds_wave);
 }
@@ -658,7 +755,7 @@ __global__ __launch_bounds__(256) void extend16_persistent_kernel(ExtArgs a, uin
     extern __shared__ uint8_t lds[];
     constexpr int G = 64 / L;
     const int wave = (int)(threadIdx.x >> 6);
-    uint8_t *lds_wave = lds + wave * G * a.tpad;
+    uint8_t *lds_wave = lds + wave * ((size_t)G * a.tpad + xfer_bytes16(S));
     const uint32_t ntasks = (count + 2 * G - 1) / (2 * G);
     const uint32_t chunk = (uint32_t)a.chunk;
     uint32_t *counter = a.queue + qidx;
@@ -678,8 +775,9 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
                            uint32_t qidx, int cus, int blocks_per_cu, hipStream_t st) {
     constexpr int G = 64 / L;
     if (count == 0) return hipSuccess;
-    const size_t lds = (size_t)4 * G * a.tpad;
-    if (lds > 65536) return hipErrorInvalidValue;
+    const size_t lds_wave = (size_t)G * a.tpad + xfer_bytes16(S);
+    const size_t lds = (size_t)4 * lds_wave;  // persistent: 4 wavefronts per workgroup
+    if ((persistent ? lds : lds_wave) > 65536) return hipErrorInvalidValue;
     const uint32_t per_block = 4u * 2u * G;
     const uint32_t need = (count + per_block - 1) / per_block;
     if (!persistent) {
@@ -688,7 +786,7 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
         // short wavefronts hand their slot on at once
         const uint32_t pw = 2u * G;
         hipLaunchKernelGGL((extend16_plain_kernel<L, S>), dim3((count + pw - 1) / pw), dim3(64),
-                           (size_t)G * a.tpad, st, a, first, count);
+                           lds_wave, st, a, first, count);
         return hipGetLastError();
     }
     int occ = 0;

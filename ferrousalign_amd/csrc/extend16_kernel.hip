@@ -296,11 +296,13 @@ __device__ __forceinline__ void pair_row_end2(const ExtArgs &a, PairPack &P, uin
 // anything; every reader masks them (G in pass 1, the argmax, the non-zero
 // scan) or overwrites them before use (eh.h at c = rb is h1).
 // Slot count a wavefront moves to once every running pair's live columns
-// [beg, qlen] fit it: about 2/3 of S, none below 4 (the per-row bookkeeping
-// dominates there).
+// [beg, qlen] fit it: one or two slots fewer, none below 3 (the per-row
+// bookkeeping dominates there).  A restripe (two LDS instructions per slot
+// and array) costs a few slot-rows of DP (about 30 instructions each), so
+// the smallest step pays once the band stays inside the smaller stripe for
+// a few rows; measured on uneven_mix_4m it beats steps of 2/3 and 3/4.
 constexpr int next_slots16(int S) {
-    return S >= 24 ? 16 : S >= 22 ? 14 : S >= 20 ? 12 : S >= 18 ? 12 : S >= 16 ? 10 : S >= 14 ? 8
-         : S >= 12 ? 8 : S >= 10 ? 6 : S >= 8 ? 4 : S >= 6 ? 4 : 0;
+    return S > 12 ? S - 2 : S > 3 ? S - 1 : 0;
 }
 
 // Wavefront state carried from one slot count to the next.

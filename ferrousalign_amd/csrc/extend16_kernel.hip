@@ -109,10 +109,25 @@ __device__ __forceinline__ uint32_t bfi1(uint32_t m, uint32_t a, uint32_t b) {
     return r;
 }
 
+// In-place forms for the loop-carried column arrays: the result takes the
+// register of `dst`, whose old value must be dead after the instruction.
+// Without them the register allocator lets each row's new H array drift by
+// one register (H(c-1) is stored at column c) and copies it back every row.
+__device__ __forceinline__ void bfi_in(uint32_t &dst, uint32_t m, uint32_t a) {
+    asm("v_bfi_b32 %0, %1, %2, %0" : "+v"(dst) : "v"(m), "v"(a));
+}
+__device__ __forceinline__ uint32_t smax_to(uint32_t &dst, uint32_t a, uint32_t b) {
+    asm("v_pk_max_i16 %0, %1, %2" : "=v"(dst) : "v"(a), "v"(b), "0"(dst));
+    return dst;
+}
+__device__ __forceinline__ void and_to(uint32_t &dst, uint32_t a, uint32_t b) {
+    asm("v_and_b32 %0, %1, %2" : "=v"(dst) : "v"(a), "v"(b), "0"(dst));
+}
+
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr = 0x110;
-// row forms of the packed kernel (see the row loop of extend16_wave)
-constexpr int kFull = 0, kWkill = 1, kFast = 2;
+// row forms of the packed kernel (see the row loop of extend16_rows)
+constexpr int kFull = 0, kFast = 1;
 constexpr int kRowNewBcast15 = 0x15F;  // every lane of a 16-lane row reads its lane 15
 constexpr int kQuadBcast3 = 0xFF;      // quad_perm [3,3,3,3]: every lane of a quad reads its lane 3
 
@@ -388,219 +403,243 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
     const uint32_t notlast = l == L - 1 ? 0u : 0xFFFFFFFFu;
     const uint32_t notlane0 = l == 0 ? 0u : 0xFFFFFFFFu;
 
-    for (;; ++w.i, w.ip += 0x00010001u) {
-        const int i = w.i;
-        const uint32_t ip = w.ip;
-        if (__builtin_amdgcn_ballot_w64(P.alive != 0u) == 0) return false;
-        // :1038-1046; a finished pair gets the empty band [4000, -3)
-        const uint32_t rbp = bfi1(P.alive, smax(P.beg, sub16(ip, P.w)), 0x0FA00FA0u);
-        const uint32_t rep = bfi1(P.alive, smin(smin(P.end, add16(ip, P.w1)), P.qlen), 0xFFFDFFFDu);
-        // Row form, the same for the whole wavefront.  Every column left of the
-        // band holds h = e = 0 at the start of a row: the band start moves past a
-        // column by the beg rule only when its h and e are 0 (:1148-1152), and a
-        // masked row writes 0 into every column it leaves behind.  A zero column
-        // computed as if it were in the band stays 0 (M <= 0, E = 0, F <= 0 from
-        // zero columns on its left) and adds nothing to the F prefix, the row
-        // maximum or the non-zero scan.  So when every live pair has rb > 0 (no
-        // first-column value h1 to place), re == qlen (no stale columns on the
-        // right to keep) and rb == beg (no column dropped by the i - w bound
-        // this row), no band mask is needed (kFast); when only the i - w bound
-        // drops a column, the left mask alone is (kWkill); other rows keep
-        // every mask (kFull).
-        const uint32_t live = P.alive;
-        const bool any_short = __builtin_amdgcn_ballot_w64((live & lt16(rep, P.qlen)) != 0u) != 0;
-        const bool any_rb0 = __builtin_amdgcn_ballot_w64((live & lt16(rbp, 0x00010001u)) != 0u) != 0;
-        const bool any_wk = __builtin_amdgcn_ballot_w64((live & lt16(P.beg, rbp)) != 0u) != 0;
-        // both pairs' target codes of this row (a row past a pair's target reads
-        // its code 0, and a finished pair's half is never used)
-        const uint32_t tw = tc2[i];
-        // score byte mat[t][q] + 128 from each pair's profile word; an N target
-        // (uniform N row) selects the byte 0x00 (selector 0x0C) and its half's
-        // bias becomes 128 - (mat[20] + 128)
-        const uint32_t nb = ((tw & 4u) >> 2) | ((tw & 0x40u) << 10);
-        const uint32_t sel = (tw & 3u) | ((tw & 0x30u) << 12) | 0x0C040C00u | (nb * 0x0Cu);
-        const uint32_t biasp = 0x00800080u - nb * (a.nrow & 0xFFu);  // (24-bit multiply)
+    // Rows of one form until the next row needs the other form (0), every
+    // pair has stopped (1), or the next row's live columns fit NS slots (2).
+    // One loop per form keeps the column arrays in their registers from row
+    // to row (a loop body that joins both forms makes the compiler copy them
+    // at every row).
+    auto rows_in = [&](auto kind) -> int {
+        constexpr bool FULLROWS = decltype(kind)::value == kFull;
+        for (;; ++w.i, w.ip += 0x00010001u) {
+            const int i = w.i;
+            const uint32_t ip = w.ip;
+            if (__builtin_amdgcn_ballot_w64(P.alive != 0u) == 0) return 1;
+            // :1038-1046; a finished pair gets the empty band [4000, -3)
+            const uint32_t rbp = bfi1(P.alive, smax(P.beg, sub16(ip, P.w)), 0x0FA00FA0u);
+            const uint32_t rep = bfi1(P.alive, smin(smin(P.end, add16(ip, P.w1)), P.qlen), 0xFFFDFFFDu);
+            // Row form, the same for the whole wavefront.  Every column left of the
+            // band holds h = e = 0 at the start of a row: the band start moves past a
+            // column by the beg rule only when its h and e are 0 (:1148-1152), a
+            // column the i - w bound drops is zeroed before its row, and a masked
+            // row writes 0 into every column it leaves behind.  A zero column
+            // computed as if it were in the band stays 0 (M <= 0, E = 0, F <= 0 from
+            // zero columns on its left) and adds nothing to the F prefix, the row
+            // maximum or the non-zero scan.  So when every live pair has rb > 0 (no
+            // first-column value h1 to place) and re == qlen (no stale columns on
+            // the right to keep), no band mask is needed (kFast), after zeroing the
+            // columns that the i - w bound drops this row; other rows keep every
+            // mask (kFull).
+            const uint32_t live = P.alive;
+            const bool any_short = __builtin_amdgcn_ballot_w64((live & lt16(rep, P.qlen)) != 0u) != 0;
+            const bool any_rb0 = __builtin_amdgcn_ballot_w64((live & lt16(rbp, 0x00010001u)) != 0u) != 0;
+            const bool any_wk = __builtin_amdgcn_ballot_w64((live & lt16(P.beg, rbp)) != 0u) != 0;
+            if ((any_short || any_rb0) != FULLROWS) return 0;
+            // both pairs' target codes of this row (a row past a pair's target reads
+            // its code 0, and a finished pair's half is never used)
+            const uint32_t tw = tc2[i];
+            // score byte mat[t][q] + 128 from each pair's profile word; an N target
+            // (uniform N row) selects the byte 0x00 (selector 0x0C) and its half's
+            // bias becomes 128 - (mat[20] + 128)
+            const uint32_t nb = ((tw & 4u) >> 2) | ((tw & 0x40u) << 10);
+            const uint32_t sel = (tw & 3u) | ((tw & 0x30u) << 12) | 0x0C040C00u | (nb * 0x0Cu);
+            const uint32_t biasp = 0x00800080u - nb * (a.nrow & 0xFFu);  // (24-bit multiply)
 
-        // Band tests are lane-relative: with t = bound - cb, column cb+s is
-        // below the bound iff s - t < 0, and c <= bound at slot s is c < bound
-        // at slot s-1.
-        const uint32_t trb = sub16(rbp, cbp), tre = sub16(rep, cbp);
+            // Band tests are lane-relative: with t = bound - cb, column cb+s is
+            // below the bound iff s - t < 0, and c <= bound at slot s is c < bound
+            // at slot s-1.
+            const uint32_t trb = sub16(rbp, cbp), tre = sub16(rep, cbp);
 
-        uint32_t pkey = 0, acc = 0, accH = 0;
-        auto row = [&](auto form) {
-            constexpr int MODE = decltype(form)::value;
-            constexpr bool LEFT = MODE != kFast;   // left band mask
-            constexpr bool FULL = MODE == kFull;   // right band mask and h1
-            // :1049-1054 h1 = max(h0 - (o_del + e_del*(i+1)), 0) when rb == 0 (h0 < 4096,
-            // so clamping the subtrahend at 32767 keeps the difference in range)
-            uint32_t h1ip = 0u;
-            if constexpr (FULL) {
-                const int dsub = min(a.o_del + a.e_del * (i + 1), 32767);
-                h1ip = smax(sub16(P.h0, pk(dsub, dsub)), 0u) & lt16(rbp, 0x00010001u);
-            }
-            // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
-            uint32_t Mv[S], pre[S], mrb[S];
-            uint32_t run = negs;
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
-                const uint32_t M = smin(sub16(add16(Hh[s], sb), biasp), mul16(Hh[s], kp));
-                Mv[s] = M;
-                const uint32_t G0 = smax(add16(M, seo[s]), se[s]);
-                uint32_t G = G0;
-                if constexpr (LEFT) {
-                    mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
-                    G = bfi(mrb[s], negs, G0);
-                }
-                pre[s] = run;
-                run = smax(run, G);
-            }
-            // carry relative to this lane's cb*e_ins; the scanned values run + cb*e_ins
-            // lie in (-16384, 16384), so with 0x4000 added they scan as unsigned with
-            // identity 0 (the empty prefix of lane 0 comes back as -16384 - cb*e_ins)
-            const uint32_t carry = sub16(excl_scan_u2<L>(add16(run, cbeb), l), cbeb);
-
-            // pass 2: H, E, row-max key, write-back, non-zero columns
-            // acc: one bit per slot and half; with S > 16 the first S - 16 slots
-            // go to accH (the high bits of a 32-bit mask per half)
-            uint32_t Hprev = 0;
-            uint32_t mle_rb0 = 0, mle_re0 = 0, wr0 = 0, mlt_rb0 = 0;
-            uint32_t mlt_rb_prev = 0, mlt_re_prev = 0;
-            if constexpr (LEFT) mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
-            if constexpr (FULL) mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const uint32_t sp = (uint32_t)s * 0x00010001u;
-                const uint32_t F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
-                const uint32_t H = smax(smax(Mv[s], Ee[s]), F);
-                const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
-                uint32_t x;
+            uint32_t pkey = 0, acc = 0, accH = 0;
+            auto row = [&](auto form) {
+                constexpr int MODE = decltype(form)::value;
+                constexpr bool FULL = MODE == kFull;   // band masks and h1
+                // :1049-1054 h1 = max(h0 - (o_del + e_del*(i+1)), 0) when rb == 0 (h0 < 4096,
+                // so clamping the subtrahend at 32767 keeps the difference in range)
+                uint32_t h1ip = 0u;
                 if constexpr (FULL) {
-                    const uint32_t mlt_rb = mrb[s], mle_rb = mlt_rb_prev;
-                    const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
-                    mlt_rb_prev = mlt_rb;
-                    mlt_re_prev = mlt_re;
-                    const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
-                    pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
-                    // E' in band, 0 at c = re (:1106-1107) and left of the band
-                    Ee[s] = bfi1(mle_re, En & mlt_re & ~mlt_rb, Ee[s]);
-                    const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
-                    if (s == 0) {
-                        mle_rb0 = mle_rb;
-                        mle_re0 = mle_re;
-                        wr0 = wr;
-                        mlt_rb0 = mlt_rb;
-                        x = 0u;
-                    } else {
-                        // eh[c].h <- H(i, c-1), h1 at c = rb, 0 left of the band
-                        const uint32_t hv = bfi1(mle_rb, h1ip, Hprev) & ~mlt_rb;
-                        Hh[s] = bfi1(mle_re, hv, Hh[s]);
-                        x = (Hh[s] | Ee[s]) & wr;
-                    }
-                } else {
-                    // re == qlen: every column but qlen is below re
-                    const uint32_t keep = s == S - 1 ? notlast : 0xFFFFFFFFu;
-                    uint32_t Hm = H, Ev = En;
-                    if constexpr (LEFT) {
-                        Hm = H & ~mrb[s] & keep;
-                        Ev = En & ~mrb[s] & keep;
-                    } else if (s == S - 1) {
-                        Hm = H & keep;
-                        Ev = En & keep;
-                    }
-                    pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
-                    Ee[s] = Ev;
-                    if (s == 0) {
-                        x = 0u;
-                    } else {
-                        // H(i, c-1); c <= rb takes h1 = 0 (rb > 0), which a zero
-                        // column on the left already gives in the unmasked form
-                        Hh[s] = LEFT ? (Hprev & ~mrb[s - 1]) : Hprev;
-                        x = Hh[s] | Ee[s];
-                    }
+                    const int dsub = min(a.o_del + a.e_del * (i + 1), 32767);
+                    h1ip = smax(sub16(P.h0, pk(dsub, dsub)), 0u) & lt16(rbp, 0x00010001u);
                 }
-                if (s > 0) {
-                    if (S > 16 && s < S - 16)
-                        accH = dbl_add(accH, nz01(x));
+                // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
+                uint32_t Mv[S], pre[S], mrb[S];
+                uint32_t run = negs;
+    #pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
+                    const uint32_t M = smin(sub16(add16(Hh[s], sb), biasp), mul16(Hh[s], kp));
+                    Mv[s] = M;
+                    // G without the clamp of M - oe_ins at 0: that clamp only lifts F
+                    // to >= 0, and H = max(M, E, F) with E >= 0 does not see it
+                    const uint32_t G0 = add16(M, seo[s]);
+                    uint32_t G = G0;
+                    if constexpr (FULL) {
+                        mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
+                        G = bfi(mrb[s], negs, G0);
+                    }
+                    pre[s] = run;
+                    run = smax(run, G);
+                }
+                // carry relative to this lane's cb*e_ins; the scanned values run + cb*e_ins
+                // lie in (-16384, 16384), so with 0x4000 added they scan as unsigned with
+                // identity 0 (the empty prefix of lane 0 comes back as -16384 - cb*e_ins)
+                const uint32_t carry = sub16(excl_scan_u2<L>(add16(run, cbeb), l), cbeb);
+
+                // pass 2: H, E, row-max key, write-back, non-zero columns
+                // acc: one bit per slot and half; with S > 16 the first S - 16 slots
+                // go to accH (the high bits of a 32-bit mask per half)
+                uint32_t Hprev = 0;
+                uint32_t mle_rb0 = 0, mle_re0 = 0, wr0 = 0, mlt_rb0 = 0;
+                uint32_t mlt_rb_prev = 0, mlt_re_prev = 0;
+                if constexpr (FULL) {
+                    mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
+                    mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
+                }
+    #pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const uint32_t sp = (uint32_t)s * 0x00010001u;
+                    const uint32_t F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
+                    const uint32_t me = smax(Mv[s], Ee[s]);
+                    uint32_t x, H;
+                    if constexpr (FULL) {
+                        H = smax(me, F);
+                        const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+                        const uint32_t mlt_rb = mrb[s], mle_rb = mlt_rb_prev;
+                        const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
+                        mlt_rb_prev = mlt_rb;
+                        mlt_re_prev = mlt_re;
+                        const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
+                        pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+                        // E' in band, 0 at c = re (:1106-1107) and left of the band
+                        bfi_in(Ee[s], mle_re, En & mlt_re & ~mlt_rb);
+                        const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
+                        if (s == 0) {
+                            mle_rb0 = mle_rb;
+                            mle_re0 = mle_re;
+                            wr0 = wr;
+                            mlt_rb0 = mlt_rb;
+                            x = 0u;
+                        } else {
+                            // eh[c].h <- H(i, c-1), h1 at c = rb, 0 left of the band
+                            const uint32_t hv = bfi1(mle_rb, h1ip, Hprev) & ~mlt_rb;
+                            bfi_in(Hh[s], mle_re, hv);
+                            x = (Hh[s] | Ee[s]) & wr;
+                        }
+                    } else {
+                        // eh[c+1].h <- H(i, c): H goes straight into the next slot's
+                        // register (its old value was read in pass 1 only)
+                        if (s < S - 1)
+                            H = smax_to(Hh[s + 1], me, F);
+                        else
+                            H = smax(me, F);
+                        smax_to(Ee[s], subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+                        // re == qlen: every column but qlen is below re
+                        uint32_t Hm = H;
+                        if (s == S - 1) {
+                            Hm = H & notlast;
+                            and_to(Ee[s], Ee[s], notlast);
+                        }
+                        pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+                        // H(i, c-1) at s > 0; c <= rb takes h1 = 0 (rb > 0), which a
+                        // zero column on the left already gives
+                        x = s == 0 ? 0u : (Hh[s] | Ee[s]);
+                    }
+                    if (s > 0) {
+                        if (S > 16 && s < S - 16)
+                            accH = dbl_add(accH, nz01(x));
+                        else
+                            acc = dbl_add(acc, nz01(x));
+                    }
+                    Hprev = H;
+                }
+                {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
+                    const uint32_t hl = dmov<kRowShr + 1>(Hprev);
+                    uint32_t x0;
+                    if constexpr (FULL) {
+                        const uint32_t hv = bfi1(mle_rb0, h1ip, hl) & ~mlt_rb0;
+                        bfi_in(Hh[0], mle_re0, hv);
+                        x0 = (Hh[0] | Ee[0]) & wr0;
+                    } else {
+                        // lane 0's slot 0 is dead; c <= rb takes 0
+                        and_to(Hh[0], hl, notlane0);
+                        x0 = Hh[0] | Ee[0];
+                    }
+                    if constexpr (S > 16)
+                        accH = mad_s(nz01(x0), (1u << (S - 17)) * 0x00010001u, accH);
                     else
-                        acc = dbl_add(acc, nz01(x));
+                        acc = mad_s(nz01(x0), (1u << (S - 1)) * 0x00010001u, acc);
                 }
-                Hprev = H;
-            }
-            {  // slot 0: its H(i, c-1) comes from the last slot of lane l-1
-                const uint32_t hl = dmov<kRowShr + 1>(Hprev);
-                uint32_t x0;
-                if constexpr (FULL) {
-                    const uint32_t hv = bfi1(mle_rb0, h1ip, hl) & ~mlt_rb0;
-                    Hh[0] = bfi1(mle_re0, hv, Hh[0]);
-                    x0 = (Hh[0] | Ee[0]) & wr0;
-                } else {
-                    // lane 0's slot 0 is dead; c <= rb takes 0
-                    Hh[0] = LEFT ? (hl & ~sgnmask(sub16(0xFFFFFFFFu, trb))) : (hl & notlane0);
-                    x0 = Hh[0] | Ee[0];
+            };
+            if constexpr (FULLROWS) {
+                row(kind);
+            } else {
+                if (any_wk) {
+                    // columns [beg, rb) leave the band by the i - w bound: zero them
+                    // (every column left of rb is below the band from now on)
+    #pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const uint32_t m = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
+                        bfi_in(Hh[s], m, 0u);
+                        bfi_in(Ee[s], m, 0u);
+                    }
                 }
-                if constexpr (S > 16)
-                    accH = mad_s(nz01(x0), (1u << (S - 17)) * 0x00010001u, accH);
-                else
-                    acc = mad_s(nz01(x0), (1u << (S - 1)) * 0x00010001u, acc);
+                row(kind);
             }
-        };
-        if (any_short || any_rb0)
-            row(std::integral_constant<int, kFull>());
-        else if (any_wk)
-            row(std::integral_constant<int, kWkill>());
-        else
-            row(std::integral_constant<int, kFast>());
 
-        // acc: bit S-1-s of each half set where column cb+s in [rb, re] has
-        // eh.h or eh.e non-zero after the row.  First such column as 1023 - c,
-        // last as c + 1, 0 when none (:1148-1158).
-        uint32_t fenc[2], lenc[2];
-        int kk[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            uint32_t bitsh = h ? (acc >> 16) : (acc & 0xFFFFu);
-            if constexpr (S > 16) bitsh |= (h ? (accH >> 16) : (accH & 0xFFFFu)) << 16;
-            const int clz = __builtin_clz(bitsh | 1u);
-            const int ctz = __builtin_ctz(bitsh | (S > 16 ? 0x80000000u : 0x10000u));
-            fenc[h] = bitsh ? (uint32_t)(1023 - (cbh[h] + S - 32 + clz)) : 0u;
-            lenc[h] = bitsh ? (uint32_t)(cbh[h] + S - ctz) : 0u;
-            const int key = (int)(h ? (pkey >> 16) : (pkey & 0xFFFFu));
-            const int c = max(cbh[h] + (int)KM - (key & (int)KM), 0);
-            kk[h] = ((key >> KB) << 10) | (1023 - c);
-        }
-        uint32_t ffirst = fenc[0] | (fenc[1] << 16);
-        uint32_t flast = lenc[0] | (lenc[1] << 16);
-        // column qlen (the group's last slot): H(i, qlen-1) when re == qlen
-        uint32_t hend;
-        if constexpr (L == 16) {
-            hend = dmov<kRowNewBcast15>(Hh[S - 1]);  // lane 15 of the 16-lane row
-        } else if constexpr (L == 8) {
-            // lane 3 of each quad, then the other quad's through the half-row mirror
-            const uint32_t x = dmov<kQuadBcast3>(Hh[S - 1]);
-            const uint32_t y = dmov<kRowHalfMirror>(x);  // every lane, before the select
-            hend = l < 4 ? y : x;
-        } else {
-            hend = dmov<kQuadBcast3>(Hh[S - 1]);  // lane 3 of the 4-lane group
-        }
-        // flush the packed cell counts before they could wrap
-        if ((i & 255) == 255) flush_cells(P);
-        const int kA = allmax_i<L>(kk[0]);
-        const int kB = allmax_i<L>(kk[1]);
-        ffirst = allmax_u2<L>(ffirst);
-        flast = allmax_u2<L>(flast);
-        pair_row_end2(a, P, ip, rbp, rep, kA, kB, hend, ffirst, flast, capdelp, capinsp);
-        if constexpr (NS > 0) {
-            // the next row's live columns [beg, qlen] of every running pair fit
-            // NS slots per lane: move to that layout and go on there
-            const uint32_t need = sub16(add16(P.qlen, 0x00010001u), P.beg);
-            const uint32_t big = P.alive & ~lt16(need, (uint32_t)(L * NS + 1) * 0x00010001u);
-            if (__builtin_amdgcn_ballot_w64(big != 0u) == 0) {
-                ++w.i;
-                w.ip += 0x00010001u;
-                return true;
+            // acc: bit S-1-s of each half set where column cb+s in [rb, re] has
+            // eh.h or eh.e non-zero after the row.  First such column as 1023 - c,
+            // last as c + 1, 0 when none (:1148-1158).
+            uint32_t fenc[2], lenc[2];
+            int kk[2];
+    #pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint32_t bitsh = h ? (acc >> 16) : (acc & 0xFFFFu);
+                if constexpr (S > 16) bitsh |= (h ? (accH >> 16) : (accH & 0xFFFFu)) << 16;
+                const int clz = __builtin_clz(bitsh | 1u);
+                const int ctz = __builtin_ctz(bitsh | (S > 16 ? 0x80000000u : 0x10000u));
+                fenc[h] = bitsh ? (uint32_t)(1023 - (cbh[h] + S - 32 + clz)) : 0u;
+                lenc[h] = bitsh ? (uint32_t)(cbh[h] + S - ctz) : 0u;
+                const int key = (int)(h ? (pkey >> 16) : (pkey & 0xFFFFu));
+                const int c = max(cbh[h] + (int)KM - (key & (int)KM), 0);
+                kk[h] = ((key >> KB) << 10) | (1023 - c);
+            }
+            uint32_t ffirst = fenc[0] | (fenc[1] << 16);
+            uint32_t flast = lenc[0] | (lenc[1] << 16);
+            // column qlen (the group's last slot): H(i, qlen-1) when re == qlen
+            uint32_t hend;
+            if constexpr (L == 16) {
+                hend = dmov<kRowNewBcast15>(Hh[S - 1]);  // lane 15 of the 16-lane row
+            } else if constexpr (L == 8) {
+                // lane 3 of each quad, then the other quad's through the half-row mirror
+                const uint32_t x = dmov<kQuadBcast3>(Hh[S - 1]);
+                const uint32_t y = dmov<kRowHalfMirror>(x);  // every lane, before the select
+                hend = l < 4 ? y : x;
+            } else {
+                hend = dmov<kQuadBcast3>(Hh[S - 1]);  // lane 3 of the 4-lane group
+            }
+            // flush the packed cell counts before they could wrap
+            if ((i & 255) == 255) flush_cells(P);
+            const int kA = allmax_i<L>(kk[0]);
+            const int kB = allmax_i<L>(kk[1]);
+            ffirst = allmax_u2<L>(ffirst);
+            flast = allmax_u2<L>(flast);
+            pair_row_end2(a, P, ip, rbp, rep, kA, kB, hend, ffirst, flast, capdelp, capinsp);
+            if constexpr (NS > 0) {
+                // the next row's live columns [beg, qlen] of every running pair fit
+                // NS slots per lane: move to that layout and go on there
+                const uint32_t need = sub16(add16(P.qlen, 0x00010001u), P.beg);
+                const uint32_t big = P.alive & ~lt16(need, (uint32_t)(L * NS + 1) * 0x00010001u);
+                if (__builtin_amdgcn_ballot_w64(big != 0u) == 0) {
+                    ++w.i;
+                    w.ip += 0x00010001u;
+                    return 2;
+                }
             }
         }
+    };
+    for (;;) {
+        int r = rows_in(std::integral_constant<int, kFast>());
+        if (r == 0) r = rows_in(std::integral_constant<int, kFull>());
+        if (r != 0) return r == 2;
     }
 }
 

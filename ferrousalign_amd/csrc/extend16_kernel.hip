@@ -109,21 +109,6 @@ __device__ __forceinline__ uint32_t bfi1(uint32_t m, uint32_t a, uint32_t b) {
     return r;
 }
 
-// In-place forms for the loop-carried column arrays: the result takes the
-// register of `dst`, whose old value must be dead after the instruction.
-// Without them the register allocator lets each row's new H array drift by
-// one register (H(c-1) is stored at column c) and copies it back every row.
-__device__ __forceinline__ void bfi_in(uint32_t &dst, uint32_t m, uint32_t a) {
-    asm("v_bfi_b32 %0, %1, %2, %0" : "+v"(dst) : "v"(m), "v"(a));
-}
-__device__ __forceinline__ uint32_t smax_to(uint32_t &dst, uint32_t a, uint32_t b) {
-    asm("v_pk_max_i16 %0, %1, %2" : "=v"(dst) : "v"(a), "v"(b), "0"(dst));
-    return dst;
-}
-__device__ __forceinline__ void and_to(uint32_t &dst, uint32_t a, uint32_t b) {
-    asm("v_and_b32 %0, %1, %2" : "=v"(dst) : "v"(a), "v"(b), "0"(dst));
-}
-
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr = 0x110;
 // row forms of the packed kernel (see the row loop of extend16_rows)
@@ -510,7 +495,7 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                         const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
                         pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
                         // E' in band, 0 at c = re (:1106-1107) and left of the band
-                        bfi_in(Ee[s], mle_re, En & mlt_re & ~mlt_rb);
+                        Ee[s] = bfi1(mle_re, En & mlt_re & ~mlt_rb, Ee[s]);
                         const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
                         if (s == 0) {
                             mle_rb0 = mle_rb;
@@ -521,22 +506,19 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                         } else {
                             // eh[c].h <- H(i, c-1), h1 at c = rb, 0 left of the band
                             const uint32_t hv = bfi1(mle_rb, h1ip, Hprev) & ~mlt_rb;
-                            bfi_in(Hh[s], mle_re, hv);
+                            Hh[s] = bfi1(mle_re, hv, Hh[s]);
                             x = (Hh[s] | Ee[s]) & wr;
                         }
                     } else {
-                        // eh[c+1].h <- H(i, c): H goes straight into the next slot's
-                        // register (its old value was read in pass 1 only)
-                        if (s < S - 1)
-                            H = smax_to(Hh[s + 1], me, F);
-                        else
-                            H = smax(me, F);
-                        smax_to(Ee[s], subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+                        // eh[c+1].h <- H(i, c) (the next slot's old h was read in pass 1 only)
+                        H = smax(me, F);
+                        if (s < S - 1) Hh[s + 1] = H;
+                        Ee[s] = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
                         // re == qlen: every column but qlen is below re
                         uint32_t Hm = H;
                         if (s == S - 1) {
                             Hm = H & notlast;
-                            and_to(Ee[s], Ee[s], notlast);
+                            Ee[s] = Ee[s] & notlast;
                         }
                         pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
                         // H(i, c-1) at s > 0; c <= rb takes h1 = 0 (rb > 0), which a
@@ -556,11 +538,11 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                     uint32_t x0;
                     if constexpr (FULL) {
                         const uint32_t hv = bfi1(mle_rb0, h1ip, hl) & ~mlt_rb0;
-                        bfi_in(Hh[0], mle_re0, hv);
+                        Hh[0] = bfi1(mle_re0, hv, Hh[0]);
                         x0 = (Hh[0] | Ee[0]) & wr0;
                     } else {
                         // lane 0's slot 0 is dead; c <= rb takes 0
-                        and_to(Hh[0], hl, notlane0);
+                        Hh[0] = hl & notlane0;
                         x0 = Hh[0] | Ee[0];
                     }
                     if constexpr (S > 16)
@@ -578,8 +560,8 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
     #pragma unroll
                     for (int s = 0; s < S; ++s) {
                         const uint32_t m = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
-                        bfi_in(Hh[s], m, 0u);
-                        bfi_in(Ee[s], m, 0u);
+                        Hh[s] = bfi1(m, 0u, Hh[s]);
+                        Ee[s] = bfi1(m, 0u, Ee[s]);
                     }
                 }
                 row(kind);

@@ -181,6 +181,26 @@ def cpu_reference_kswv(ps, threads, m=262144):
                       "dispatch.rs:270-395) on %d host threads: %.2f s" % (sub.n, threads, dt)}
 
 
+def mate_roofline(db, flags, cells, ctx, reps=3):
+    """Mate rescue: the forward launches and the separate XSTART reverse
+    launches of one run, each replayed alone (reps times, mean), together:
+    achieved = 16 ops x all cells of the run / the summed launch time."""
+    runs = [db.profile_launches(flags) for _ in range(reps)]
+    fwd = float(np.mean([sum(x["ms"] for x in r if x["kernel"].startswith("mate16_kernel")) for r in runs]))
+    rev = float(np.mean([sum(x["ms"] for x in r if x["kernel"].startswith("mate16_rev")) for r in runs]))
+    ms = fwd + rev
+    tops = cells * OPS_PER_CELL / (ms / 1e3) / 1e12
+    res = {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_INT16X2_TOPS, 1), "unit": "TOPS",
+           "frac": round(tops / VALU_INT16X2_TOPS, 4), "traffic": None,
+           "kernel": "mate16_kernel + mate16_rev_kernel (every launch of one run)", "peak_dtype": "int16x2",
+           "launch_ms": round(ms, 4), "forward_ms": round(fwd, 4), "reverse_ms": round(rev, 4),
+           "launches": len(runs[0]), "launch_cells": cells, "ops_per_cell": OPS_PER_CELL}
+    m = ctx.valu_peak(True)
+    res["peak_measured"] = round(m, 2)
+    res["frac_of_measured"] = round(tops / m, 4)
+    return res
+
+
 def mate_bench(args, ctx, golden, byte_mode=False):
     """Mate-rescue local SW (ksw_align2, 16-bit mode; byte_mode: the 8-bit
     mode, xtra | KSW_XBYTE, which mate_rescue.rs:314-321 selects for 151 bp
@@ -198,6 +218,10 @@ def mate_bench(args, ctx, golden, byte_mode=False):
     ms, ms_ev, nl = timed(db, flags, steps, 1)
     out, cells = db.download_align2()
     total = float(cells.astype(np.float64).sum())
+    # the same run with each XSTART pass inside the forward kernel
+    ms_fused, _, _ = timed(db, flags | N.FA_ALIGN2_FUSED, steps, 1)
+    out_f, cells_f = db.download_align2()
+    same_fused = bool((out_f == out).all() and (cells_f == cells).all())
     from tests.golden.make_golden_mate import digest
     gm = {}
     gpath = os.path.join(ROOT, "tests", "golden", "mate_digests.json")
@@ -205,7 +229,7 @@ def mate_bench(args, ctx, golden, byte_mode=False):
         gm = json.load(open(gpath))
     key = ("mate_u8_%d" if byte_mode else "mate_%d") % n
     exact = (digest(out, cells) == gm[key]) if key in gm else None
-    roof = roofline(db, flags, cells, ctx)
+    roof = mate_roofline(db, flags, total, ctx)
     mode = "8-bit" if byte_mode else "16-bit"
     res = {"metric": "mate-rescue pairs/s (ksw_align2 %s, bit-exact)" % mode, "value": round(n / (ms / 1e3), 1),
            "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
@@ -215,6 +239,7 @@ def mate_bench(args, ctx, golden, byte_mode=False):
            "config": {"workload": "mate_rescue_xbyte_2m" if byte_mode else "mate_rescue_2m", "pairs": n,
                       "qlen": 151, "tlen": 1500, "xtra": hex(xtra), "launches_per_step": nl},
            "gcups": round(total / (ms / 1e3) / 1e9, 2), "bit_exact_vs_golden": exact,
+           "fused_reverse_ms_per_step": round(ms_fused, 3), "identical_to_fused": same_fused,
            "roofline": roof}
     if not args.no_cpu:
         threads = os.cpu_count() or 1

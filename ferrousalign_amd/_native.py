@@ -44,6 +44,7 @@ FA_LAUNCH_GRAPH = 0x200
 FA_LAUNCH_SINGLE = 0x400
 FA_RUN_ALIGN2 = 0x1000
 FA_PLAN_FRESH = 0x2000
+FA_ALIGN2_FUSED = 0x4000
 
 
 class FaError(RuntimeError):

@@ -606,12 +606,15 @@ class DeviceBatch:
         N.check(N.lib().fa_batch_profile_launches(self.ctx.handle, self._h, flags, cap, ctypes.byref(nl),
                                                   N.ptr(slots), N.ptr(kern), N.ptr(first), N.ptr(count), N.ptr(ms)),
                 "fa_batch_profile_launches")
-        names = {0: "extend_plain_kernel<%d,%d>", 1: "extend16_plain_kernel<%d,%d>", 2: "mate16_kernel<%d>"}
+        names = {0: "extend_plain_kernel<%d,%d>", 1: "extend16_plain_kernel<%d,%d>"}
         out = []
         for i in range(min(nl.value, cap)):
             kind, lanes = int(kern[i]) >> 8, int(kern[i]) & 0xFF
-            if kind == 2:
-                name = names[2] % int(slots[i]) if lanes == 8 else "mate16_kernel<%d,8-bit>" % int(slots[i])
+            if kind in (2, 3):
+                # mate rescue: forward (2) or separate XSTART reverse (3) launch;
+                # `first` of a reverse launch indexes the reverse order
+                base = "mate16_kernel" if kind == 2 else "mate16_rev_kernel"
+                name = "%s<%d%s>" % (base, int(slots[i]), "" if lanes == 8 else ",8-bit")
             else:
                 name = names[kind] % (lanes, int(slots[i]))
                 if flags & N.FA_LAUNCH_PERSISTENT:

@@ -238,7 +238,7 @@ void batch_release(fa_batch *b) {
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
-                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig};
+                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -448,14 +448,89 @@ using fa_host::launch_bucket;
 
 constexpr uint32_t kMateChunk = 262144;  // pairs per mate launch (row-log scratch reuse)
 
-int run_align2(fa_ctx *ctx, fa_batch *b, int *nlaunch) {
+// One launch of a mate-rescue run, for fa_batch_profile_launches: kernel
+// (2 << 8: forward, 3 << 8: XSTART reverse; | SSE lanes per pair), stripe
+// length, range in its order and duration.
+struct MateLaunch {
+    int kernel, slen;
+    uint32_t first, count;
+    double ms;
+};
+
+// One mate-rescue run: the forward launches over the slen buckets (chunks of
+// kMateChunk pairs, the row log's size), then, when KSW_XSTART is set and
+// FA_ALIGN2_FUSED is not, the reverse pass as launches of its own over only
+// the pairs that need it: sort keys from the forward records (bin = stripe
+// length of qe + 1, then forward score descending), a stable radix sort, the
+// bins' counts read back, one unmasked launch per stripe length.  A pair's
+// reverse pass in the forward kernel would keep its wavefront's 16 pairs
+// until the slowest pass ends, masked to the longest stripe among them, and
+// 20-25% of the pairs of a mate window need none.  With `rec`, every launch
+// is timed alone (a synchronisation after each).
+int run_align2(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch,
+               std::vector<MateLaunch> *rec = nullptr) {
     if (!b->align2_set) return FA_ERR_ARG;
+    const bool split = !(flags & FA_ALIGN2_FUSED) && (b->margs.xtra & fa::kKswXStart) && b->n > 0;
+    fa::MateArgs a = b->margs;
+    a.split_rev = split ? 1 : 0;
+    const int lanes = a.byte_mode ? 16 : 8;
     int launches = 0;
+    auto timed = [&](int kernel, int slen, uint32_t first, uint32_t count, auto &&go) -> int {
+        if (rec) FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        FA_HIP(go());
+        ++launches;
+        if (rec) {
+            FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+            FA_HIP(hipEventSynchronize(ctx->ev1));
+            float t = 0.f;
+            FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+            rec->push_back(MateLaunch{kernel, slen, first, count, (double)t});
+        }
+        return FA_OK;
+    };
+    int rc;
     for (const Bucket &bk : b->mbuckets) {
         for (uint32_t off = 0; off < bk.count; off += kMateChunk) {
             const uint32_t cnt = std::min<uint32_t>(kMateChunk, bk.count - off);
-            FA_HIP(fa::mate16_launch(b->margs, bk.first + off, cnt, bk.slen, ctx->stream));
-            ++launches;
+            rc = timed(2 << 8 | lanes, bk.slen, bk.first + off, cnt,
+                       [&] { return fa::mate16_launch(a, bk.first + off, cnt, bk.slen, ctx->stream); });
+            if (rc) return rc;
+        }
+    }
+    if (split) {
+        const int64_t n = b->n;
+        size_t tb = 0;
+        FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                  (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0,
+                                                  fa::kMateRevKeyBits, ctx->stream));
+        const size_t need = 16 * (size_t)n + 64 * 4 + tb + 512;
+        if (b->mrev_bytes < need) {
+            if (b->d_mrev) (void)hipFree(b->d_mrev);
+            b->d_mrev = nullptr;
+            b->mrev_bytes = 0;
+            FA_HIP(hipMalloc(&b->d_mrev, need));
+            b->mrev_bytes = need;
+        }
+        uint32_t *kin = (uint32_t *)b->d_mrev;
+        uint32_t *kout = kin + n, *ids = kout + n, *rorder = ids + n, *hist = rorder + n;
+        void *temp = (void *)(((uintptr_t)(hist + 64) + 255) & ~(uintptr_t)255);
+        FA_HIP(hipMemsetAsync(hist, 0, 64 * 4, ctx->stream));
+        FA_HIP(fa::mate_rev_keys(a, (uint32_t)n, kin, ids, hist, ctx->stream));
+        FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, ids, rorder, (int)n, 0,
+                                                  fa::kMateRevKeyBits, ctx->stream));
+        uint32_t cnt[64];
+        FA_HIP(hipMemcpyAsync(cnt, hist, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+        FA_HIP(hipStreamSynchronize(ctx->stream));
+        a.rorder = rorder;
+        uint32_t start = 0;
+        for (int bin = 0; bin < fa::kMateMaxSlen; ++bin) {
+            const uint32_t c = cnt[bin];
+            const int slen = fa::kMateMaxSlen - bin;
+            if (c == 0) continue;
+            rc = timed(3 << 8 | lanes, slen, start, c,
+                       [&] { return fa::mate16_rev_launch(a, start, c, slen, ctx->stream); });
+            if (rc) return rc;
+            start += c;
         }
     }
     if (nlaunch) *nlaunch = launches;
@@ -465,7 +540,7 @@ int run_align2(fa_ctx *ctx, fa_batch *b, int *nlaunch) {
 int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
     if (flags & FA_RUN_ALIGN2) {
         b->last_overflow = 0;
-        return run_align2(ctx, b, nlaunch);
+        return run_align2(ctx, b, flags, nlaunch);
     }
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
@@ -1378,29 +1453,18 @@ int fa_batch_profile_launches(fa_ctx *ctx, fa_batch *b, uint32_t flags, int32_t 
     if (!ctx || !b || !nlaunch || cap < 0) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     if (flags & FA_RUN_ALIGN2) {
-        if (!b->align2_set) return FA_ERR_ARG;
-        int k = 0;
-        for (const Bucket &bk : b->mbuckets) {
-            for (uint32_t off = 0; off < bk.count; off += kMateChunk) {
-                const uint32_t cnt = std::min<uint32_t>(kMateChunk, bk.count - off);
-                FA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-                FA_HIP(fa::mate16_launch(b->margs, bk.first + off, cnt, bk.slen, ctx->stream));
-                FA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-                FA_HIP(hipEventSynchronize(ctx->ev1));
-                float t = 0.f;
-                FA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
-                if (k < cap) {
-                    if (slots) slots[k] = bk.S;
-                    // mate kernel, SSE lanes per pair: 8 (16-bit mode) or 16 (8-bit)
-                    if (kernel) kernel[k] = 2 << 8 | (b->margs.byte_mode ? 16 : 8);
-                    if (first) first[k] = bk.first + off;
-                    if (count) count[k] = cnt;
-                    if (ms) ms[k] = t;
-                }
-                ++k;
-            }
+        std::vector<MateLaunch> rec;
+        int nl = 0;
+        const int rc = run_align2(ctx, b, flags, &nl, &rec);
+        if (rc) return rc;
+        for (size_t k = 0; k < rec.size() && (int)k < cap; ++k) {
+            if (slots) slots[k] = rec[k].slen;
+            if (kernel) kernel[k] = rec[k].kernel;
+            if (first) first[k] = rec[k].first;
+            if (count) count[k] = rec[k].count;
+            if (ms) ms[k] = rec[k].ms;
         }
-        *nlaunch = k;
+        *nlaunch = (int)rec.size();
         return FA_OK;
     }
     int rc = plan_packing(ctx, b, flags);

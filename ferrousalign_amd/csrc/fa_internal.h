@@ -87,6 +87,8 @@ struct fa_batch {
     int32_t *d_mout = nullptr;     // 7 int32 per pair
     uint16_t *d_imax_log = nullptr;
     uint32_t log_slots = 0;
+    void *d_mrev = nullptr;        // reverse launches: keys in/out, ids, order, histogram, sort scratch
+    size_t mrev_bytes = 0;
     std::vector<fa_host::Bucket> mbuckets;
     float *d_feat = nullptr;       // predictor features [n][6]
     float *d_pred = nullptr;       // predicted cells [n], or the caller's cost (fa_batch_set_cost)

@@ -144,6 +144,14 @@ struct Res {
     int score, te, qe, rows;
 };
 
+// Whether ksw_align2 runs the XSTART reverse pass after a forward result
+// (:804-810): KSW_XSTART set, the score not below KSW_XSUBO's minimum, and a
+// hit (qe, te >= 0; a saturated 8-bit pass reports qe = -1).
+__device__ __forceinline__ bool xstart_needed(uint32_t xtra, int score, int te, int qe) {
+    return (xtra & kKswXStart) && !((xtra & kKswXSubo) && score < (int)(xtra & 0xffffu)) &&
+           qe >= 0 && te >= 0;
+}
+
 // Target rows staged per lane group at a time: byte r of a chunk holds row
 // i0 + r of pair A in bits 0-3 and of pair B in bits 4-7.
 constexpr int kTgtChunk = 512;  // the forward staging decodes 4 + 2 words per lane
@@ -552,14 +560,30 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
         }
     }
 
+    if (a.split_rev) {
+        // the XSTART pass runs in mate16_rev_kernel, over the pairs that need it
+        if (p < 2 && mine_valid) {
+            const uint32_t id = mineB ? pid[1] : pid[0];
+            const int ql = mineB ? hf[1].qlen : hf[0].qlen;
+            int32_t *o = a.out + (size_t)id * 7;
+            o[0] = mine.score;
+            o[1] = mine.te;
+            o[2] = mine.qe;
+            o[3] = score2;
+            o[4] = te2;
+            o[5] = -1;
+            o[6] = -1;
+            if (a.cells) a.cells[id] = (uint32_t)mine.rows * (uint32_t)ql;
+        }
+        return;
+    }
+
     // XSTART reverse pass (:804-855), per pair
     Half hr[2];
     bool rev[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const bool find_start =
-            (xtra & kKswXStart) && !((xtra & kKswXSubo) && r[h].score < (int)(xtra & 0xffffu));
-        rev[h] = valid[h] && find_start && r[h].qe >= 0 && r[h].te >= 0;
+        rev[h] = valid[h] && xstart_needed(xtra, r[h].score, r[h].te, r[h].qe);
         hr[h] = hf[h];
         hr[h].active = rev[h];
         hr[h].qlen = rev[h] ? r[h].qe + 1 : 1;
@@ -592,6 +616,94 @@ mate16_kernel(MateArgs a, uint32_t first, uint32_t count) {
         o[5] = tb;
         o[6] = qb;
         if (a.cells) a.cells[id] = cells;
+    }
+}
+
+// The XSTART reverse pass (:804-855) alone, two pairs per 8-lane group, for
+// pairs whose reverse query length qe + 1 has the stripe length of SLEN (the
+// host buckets them by it), so the pass runs unmasked.  The reference runs
+// ksw_i16 / ksw_u8 on the first qe + 1 query bases and the first te + 1
+// target bases, both reversed, with KSW_XSTOP at the forward score.
+template <int SLEN, bool SYM, bool U8>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(mate16_waves(SLEN)))) void
+mate16_rev_kernel(MateArgs a, uint32_t first, uint32_t count) {
+    constexpr int PPB = 16;
+    extern __shared__ uint8_t lds[];
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane >> 3, p = lane & 7;
+    Half hr[2];
+    Res f[2];
+    uint32_t pid[2];
+    bool valid[2];
+    uint8_t *const tc2 = lds + g * kTgtChunk;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t slot = blockIdx.x * (uint32_t)PPB + 2u * (uint32_t)g + (uint32_t)h;
+        valid[h] = slot < count;
+        pid[h] = valid[h] ? a.rorder[first + slot] : 0u;
+        const int32_t *o = a.out + (size_t)pid[h] * 7;
+        f[h].score = valid[h] ? o[0] : 0;
+        f[h].te = valid[h] ? o[1] : -1;
+        f[h].qe = valid[h] ? o[2] : -1;
+        f[h].rows = 0;
+        hr[h].active = valid[h];
+        hr[h].qlen = valid[h] ? f[h].qe + 1 : 1;
+        hr[h].tlen = valid[h] ? a.tlen[pid[h]] : 0;
+        hr[h].trev = f[h].te;
+        hr[h].qo = valid[h] ? a.qo[pid[h]] : 0u;
+        hr[h].to = valid[h] ? a.to[pid[h]] : 0u;
+        hr[h].endsc = (uint32_t)f[h].score & 0xffffu;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    uint32_t *const hm = (uint32_t *)(lds + 8 * kTgtChunk);
+    Res rr[2];
+    ksw_pass2<SLEN, true, SYM, U8>(a, p, hr, tc2, nullptr, hm, true, rr);
+    // lane 0 of the group finishes pair A, lane 1 pair B (:846-855)
+    const bool mineB = p == 1;
+    if (p < 2 && (mineB ? valid[1] : valid[0])) {
+        const Res fw = mineB ? f[1] : f[0], b = mineB ? rr[1] : rr[0];
+        const uint32_t id = mineB ? pid[1] : pid[0];
+        int32_t *o = a.out + (size_t)id * 7;
+        if (b.score >= fw.score) {
+            o[5] = fw.te - b.te;
+            o[6] = fw.qe - b.qe;
+        }
+        if (a.cells) a.cells[id] += (uint32_t)b.rows * (uint32_t)(fw.qe + 1);
+    }
+}
+
+template <bool SYM, bool U8>
+hipError_t mate16_rev_launch_set(const MateArgs &a, uint32_t first, uint32_t count, int slen,
+                                 hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    const dim3 grid((count + 15) / 16);
+    if constexpr (U8) {
+        switch (slen) {
+#define FA_M8(N)                                                                                 \
+    case N:                                                                                      \
+        hipLaunchKernelGGL((mate16_rev_kernel<2 * N, SYM, true>), grid, dim3(64),                \
+                           (size_t)8 * kTgtChunk + (size_t)256 * 2 * N, st, a, first, count); \
+        return hipGetLastError();
+            FA_M8(1) FA_M8(2) FA_M8(3) FA_M8(4) FA_M8(5) FA_M8(6) FA_M8(7) FA_M8(8)
+            FA_M8(9) FA_M8(10) FA_M8(11) FA_M8(12) FA_M8(13) FA_M8(14) FA_M8(15) FA_M8(16)
+#undef FA_M8
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (slen) {
+#define FA_M16(N)                                                                            \
+    case N:                                                                                  \
+        hipLaunchKernelGGL((mate16_rev_kernel<N, SYM, false>), grid, dim3(64),               \
+                           (size_t)8 * kTgtChunk + (size_t)256 * N, st, a, first, count); \
+        return hipGetLastError();
+            FA_M16(1) FA_M16(2) FA_M16(3) FA_M16(4) FA_M16(5) FA_M16(6) FA_M16(7) FA_M16(8)
+            FA_M16(9) FA_M16(10) FA_M16(11) FA_M16(12) FA_M16(13) FA_M16(14) FA_M16(15) FA_M16(16)
+            FA_M16(17) FA_M16(18) FA_M16(19) FA_M16(20) FA_M16(21) FA_M16(22) FA_M16(23) FA_M16(24)
+            FA_M16(25) FA_M16(26) FA_M16(27) FA_M16(28) FA_M16(29) FA_M16(30) FA_M16(31) FA_M16(32)
+#undef FA_M16
+            default: return hipErrorInvalidValue;
+        }
     }
 }
 

@@ -9,4 +9,9 @@ hipError_t mate16_launch_sym(const MateArgs &a, uint32_t first, uint32_t count, 
     return mate16_launch_set<true, false>(a, first, count, slen, st);
 }
 
+hipError_t mate16_rev_launch_sym(const MateArgs &a, uint32_t first, uint32_t count, int slen,
+                                 hipStream_t st) {
+    return mate16_rev_launch_set<true, false>(a, first, count, slen, st);
+}
+
 }  // namespace fa

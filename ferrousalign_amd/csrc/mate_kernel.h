@@ -29,6 +29,8 @@ struct MateArgs {
     int32_t qmax;         // largest matrix entry (>= 1)
     int32_t byte_mode;    // 1: the 8-bit mode (KSW_XBYTE)
     int32_t ovf;          // 8-bit mode: 255 - shift, the best score that saturates
+    int32_t split_rev;    // 1: the forward kernel leaves the XSTART pass to mate16_rev_launch
+    const uint32_t *rorder;  // reverse launches: pair ids in launch order
 };
 
 // Longest stripe (segments per lane) a launch takes: qlen <= 256.
@@ -41,5 +43,18 @@ constexpr int kMate8MaxSlen = 16;  // 8-bit mode, 16 lanes
 // count * log_stride entries.
 hipError_t mate16_launch(const MateArgs &a, uint32_t first, uint32_t count, int slen,
                          hipStream_t st);
+// The XSTART reverse pass alone, for the pairs rorder[first, first+count),
+// whose forward records (score, te, qe) are in out: every pair of the range
+// must need the pass and have a reverse stripe length ceil((qe + 1) / lanes)
+// equal to `slen`.  Writes tb and qb and adds the pass's cells.
+hipError_t mate16_rev_launch(const MateArgs &a, uint32_t first, uint32_t count, int slen,
+                             hipStream_t st);
+// Reverse-pass sort keys of pairs [0, n) from their forward records: bin
+// (kMateMaxSlen - reverse stripe length, 63 for a pair without the pass) in
+// bits 15-20, 32767 - forward score below; ids[k] = k; hist[bin] counts the
+// pairs of each bin (64 entries, zero before the call).
+constexpr int kMateRevKeyBits = 21;
+hipError_t mate_rev_keys(const MateArgs &a, uint32_t n, uint32_t *keys, uint32_t *ids,
+                         uint32_t *hist, hipStream_t st);
 
 }  // namespace fa

@@ -65,6 +65,9 @@ extern "C" {
                                    longest query (no query-length buckets) */
 #define FA_PLAN_FRESH 0x2000u  /* plan the packing again even when the batch's plan for these
                                   flags is current (per-batch planning cost in a timed step) */
+#define FA_ALIGN2_FUSED 0x4000u /* with FA_RUN_ALIGN2: run each pair's XSTART reverse pass in the
+                                    forward kernel (default: a second launch over the pairs that
+                                    need it, bucketed by the stripe length of qe + 1; same results) */
 
 /* ---- records ------------------------------------------------------------ */
 /* BandedPairWiseSW, src/core/alignment/banded_swa/types.rs:106-199 */

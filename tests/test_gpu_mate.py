@@ -81,6 +81,36 @@ def test_mate_random_shapes_and_flags(seed):
         _run_cases(cases, synth.MATE_MATRIX, gaps, xtra, "mate random %d" % seed)
 
 
+@pytest.mark.parametrize("byte_mode", [False, True])
+def test_mate_reverse_launches_match_fused(byte_mode):
+    # random query lengths spread the XSTART passes over many reverse stripe
+    # lengths; both run forms against the oracle
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd.banded_swa import DeviceBatch
+    cases = _random_cases(random.Random(7), n=600)
+    q = np.concatenate([np.asarray(c[0], np.uint8) for c in cases] + [np.zeros(1, np.uint8)])
+    t = np.concatenate([np.asarray(c[1], np.uint8) for c in cases] + [np.zeros(1, np.uint8)])
+    ql = np.array([len(c[0]) for c in cases], np.int32)
+    tl = np.array([len(c[1]) for c in cases], np.int32)
+    qo = np.concatenate([[0], np.cumsum(ql)[:-1]]).astype(np.int64)
+    to = np.concatenate([[0], np.cumsum(tl)[:-1]]).astype(np.int64)
+    n = len(cases)
+    xtra = synth.MATE_XTRA | (O.KSW_XBYTE if byte_mode else 0)
+    print("mate reverse launches", byte_mode, flush=True)
+    db = DeviceBatch(synth.default_scoring(), q, qo, ql, t, to, tl, np.ones(n, np.int32), np.zeros(n, np.int32))
+    db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, xtra)
+    ora, oc, rc = O.ksw_align2_batch(q, qo, ql, t, to, tl, synth.MATE_MATRIX, 6, 1, 6, 1, xtra)
+    assert (rc == 0).all()
+    for flags in (N.FA_RUN_ALIGN2, N.FA_RUN_ALIGN2 | N.FA_ALIGN2_FUSED, N.FA_RUN_ALIGN2):
+        db.run(flags)
+        out, cells = db.download_align2()
+        _check(out, cells, ora, oc, "flags %#x" % flags)
+    # the reverse launches are timed apart from the forward ones
+    launches = db.profile_launches(N.FA_RUN_ALIGN2)
+    assert any(x["kernel"].startswith("mate16_rev") for x in launches)
+    db.free()
+
+
 # ---- 8-bit mode (KSW_XBYTE) ------------------------------------------------------
 
 def test_mate_u8_workload_subset():
@@ -148,9 +178,12 @@ def test_mate_u8_full_size_digest():
     print("mate u8 full size", n, flush=True)
     db = DeviceBatch(synth.default_scoring(), *ps.flat(), np.ones(n, np.int32), np.zeros(n, np.int32))
     db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA | O.KSW_XBYTE)
-    db.run(N.FA_RUN_ALIGN2)
-    out, cells = db.download_align2()
-    assert digest(out, cells) == d[key]
+    # the separate reverse launches (default) and the reverse pass inside the
+    # forward kernel give the same records
+    for flags in (N.FA_RUN_ALIGN2, N.FA_RUN_ALIGN2 | N.FA_ALIGN2_FUSED):
+        db.run(flags)
+        out, cells = db.download_align2()
+        assert digest(out, cells) == d[key], hex(flags)
     db.free()
 
 
@@ -167,9 +200,12 @@ def test_mate_full_size_digest():
     print("mate full size", n, flush=True)
     db = DeviceBatch(synth.default_scoring(), *ps.flat(), np.ones(n, np.int32), np.zeros(n, np.int32))
     db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA)
-    db.run(N.FA_RUN_ALIGN2)
-    out, cells = db.download_align2()
-    assert digest(out, cells) == d[key]
+    # the separate reverse launches (default) and the reverse pass inside the
+    # forward kernel give the same records
+    for flags in (N.FA_RUN_ALIGN2, N.FA_RUN_ALIGN2 | N.FA_ALIGN2_FUSED):
+        db.run(flags)
+        out, cells = db.download_align2()
+        assert digest(out, cells) == d[key], hex(flags)
     db.free()
 
 

@@ -522,15 +522,33 @@ int run_align2(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch,
         FA_HIP(hipMemcpyAsync(cnt, hist, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
         FA_HIP(hipStreamSynchronize(ctx->stream));
         a.rorder = rorder;
+        // the bins run on the context's four streams, largest first on the
+        // least-loaded one, so that the small bins (a single wavefront's pass
+        // each) overlap the large ones (timed runs stay on one stream)
+        hipStream_t st[4] = {ctx->stream, ctx->side[0], ctx->side[1], ctx->side[2]};
+        const int ns = rec ? 1 : 4;
+        if (ns > 1) {
+            FA_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+            for (int j = 1; j < ns; ++j) FA_HIP(hipStreamWaitEvent(st[j], ctx->ev_fork, 0));
+        }
+        double load[4] = {0, 0, 0, 0};
         uint32_t start = 0;
         for (int bin = 0; bin < fa::kMateMaxSlen; ++bin) {
             const uint32_t c = cnt[bin];
             const int slen = fa::kMateMaxSlen - bin;
             if (c == 0) continue;
+            int j = 0;
+            for (int q = 1; q < ns; ++q)
+                if (load[q] < load[j]) j = q;
+            load[j] += (double)c * slen;
             rc = timed(3 << 8 | lanes, slen, start, c,
-                       [&] { return fa::mate16_rev_launch(a, start, c, slen, ctx->stream); });
+                       [&] { return fa::mate16_rev_launch(a, start, c, slen, st[j]); });
             if (rc) return rc;
             start += c;
+        }
+        for (int j = 1; j < ns; ++j) {
+            FA_HIP(hipEventRecord(ctx->ev_join[j - 1], st[j]));
+            FA_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join[j - 1], 0));
         }
     }
     if (nlaunch) *nlaunch = launches;

@@ -85,13 +85,18 @@ __device__ __forceinline__ uint32_t dbl_add(uint32_t a, uint32_t b) {
     asm("v_pk_mad_u16 %0, %1, 2, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
-// a * MUL + b per half, MUL an inline constant (0..64), b wave-uniform (the
-// "n" constraint: the host pass checks "I" against x86, whose range is 0..31)
+// acc = max(acc, a * MUL + b) per half (unsigned), MUL an inline constant, b
+// wave-uniform.  One statement, so the running maximum is taken slot by slot:
+// with the multiply-add and the maximum apart, the compiler sinks the
+// maximum to the end of the row and keeps every slot's term in a register.
+// The packed maximum reads the multiply-add's result on the next
+// instruction, which needs one wait state (s_nop 0) on gfx950.
 template <int MUL>
-__device__ __forceinline__ uint32_t mad_imm(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_pk_mad_u16 %0, %1, %2, %3 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "n"(MUL), "s"(b));
-    return r;
+__device__ __forceinline__ void key_acc(uint32_t &acc, uint32_t a, uint32_t b) {
+    uint32_t t;
+    asm("v_pk_mad_u16 %1, %2, %3, %4 op_sel_hi:[1,0,1]\n\ts_nop 0\n\tv_pk_max_u16 %0, %0, %1"
+        : "+v"(acc), "=&v"(t)
+        : "v"(a), "n"(MUL), "s"(b));
 }
 // a * m + b per half, m wave-uniform
 __device__ __forceinline__ uint32_t mad_s(uint32_t a, uint32_t m, uint32_t b) {
@@ -345,6 +350,10 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                                               const uint32_t (&profA)[S], const uint32_t (&profB)[S],
                                               uint32_t (&Hh)[S], uint32_t (&Ee)[S]) {
     constexpr int NS = next_slots16(S);
+    // one register per slot fewer for one instruction per slot more where that
+    // decides the occupancy: S = 18 then fits 3 waves per SIMD instead of 2
+    // (167 VGPRs; measured 3.5% faster); S = 20..24 stay at 2 either way
+    constexpr bool GONLY = S == 18;
     // row-argmax key per half: (H << KB) | (KM - s); with S > 16 the host
     // routes only pairs whose scores stay below 2048 here, so the key fits
     constexpr int KB = S <= 8 ? 3 : (S <= 16 ? 4 : 5);
@@ -445,14 +454,18 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                     const int dsub = min(a.o_del + a.e_del * (i + 1), 32767);
                     h1ip = smax(sub16(P.h0, pk(dsub, dsub)), 0u) & lt16(rbp, 0x00010001u);
                 }
-                // pass 1: M, the band-start masks and the lane-local exclusive prefix of G
-                uint32_t Mv[S], pre[S], mrb[S];
+                // pass 1: M, the band-start masks and the lane-local exclusive prefix of G.
+                // With GONLY only G is kept per slot: pass 2 takes M back as G - seo
+                // (a masked column's value is wrong, but every use of it is masked too)
+                // and rebuilds the prefix from the carry, one more instruction per slot
+                // for one register fewer.
+                uint32_t Mv[GONLY ? 1 : S], pre[GONLY ? 1 : S], Gv[GONLY ? S : 1], mrb[S];
                 uint32_t run = negs;
     #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const uint32_t sb = __builtin_amdgcn_perm(profB[s], profA[s], sel);
                     const uint32_t M = smin(sub16(add16(Hh[s], sb), biasp), mul16(Hh[s], kp));
-                    Mv[s] = M;
+                    if constexpr (!GONLY) Mv[s] = M;
                     // G without the clamp of M - oe_ins at 0: that clamp only lifts F
                     // to >= 0, and H = max(M, E, F) with E >= 0 does not see it
                     const uint32_t G0 = add16(M, seo[s]);
@@ -461,7 +474,11 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                         mrb[s] = sgnmask(sub16((uint32_t)s * 0x00010001u, trb));  // c < rb
                         G = bfi(mrb[s], negs, G0);
                     }
-                    pre[s] = run;
+                    if constexpr (GONLY) {
+                        Gv[s] = G;
+                    } else {
+                        pre[s] = run;
+                    }
                     run = smax(run, G);
                 }
                 // carry relative to this lane's cb*e_ins; the scanned values run + cb*e_ins
@@ -479,21 +496,32 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                     mlt_rb_prev = sgnmask(sub16(0xFFFFFFFFu, trb));  // column cb-1
                     mlt_re_prev = sgnmask(sub16(0xFFFFFFFFu, tre));
                 }
+                // GONLY: max(carry, G of the slots left of s); it differs from
+                // max(carry, pre[s]) only below the identity, where F < 0 <= E
+                uint32_t runp = carry;
     #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const uint32_t sp = (uint32_t)s * 0x00010001u;
-                    const uint32_t F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
-                    const uint32_t me = smax(Mv[s], Ee[s]);
+                    uint32_t F, Ms;
+                    if constexpr (GONLY) {
+                        F = sub16(runp, s ? se[s - 1] : se_m1);  // - (c-1)*e_ins
+                        runp = smax(runp, Gv[s]);
+                        Ms = sub16(Gv[s], seo[s]);
+                    } else {
+                        F = sub16(smax(carry, pre[s]), s ? se[s - 1] : se_m1);
+                        Ms = Mv[s];
+                    }
+                    const uint32_t me = smax(Ms, Ee[s]);
                     uint32_t x, H;
                     if constexpr (FULL) {
                         H = smax(me, F);
-                        const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+                        const uint32_t En = smax(subsat(Ee[s], edelp), sub16(Ms, oedelp));
                         const uint32_t mlt_rb = mrb[s], mle_rb = mlt_rb_prev;
                         const uint32_t mlt_re = sgnmask(sub16(sp, tre)), mle_re = mlt_re_prev;
                         mlt_rb_prev = mlt_rb;
                         mlt_re_prev = mlt_re;
                         const uint32_t Hm = H & mlt_re & ~mlt_rb;  // H inside [rb, re), else 0
-                        pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+                        key_acc<(1 << KB)>(pkey, Hm, (KM - (uint32_t)s) * 0x00010001u);
                         // E' in band, 0 at c = re (:1106-1107) and left of the band
                         Ee[s] = bfi1(mle_re, En & mlt_re & ~mlt_rb, Ee[s]);
                         const uint32_t wr = mle_re & ~mlt_rb;      // rb <= c <= re
@@ -513,14 +541,14 @@ __device__ __forceinline__ bool extend16_rows(const ExtArgs &a, PairPack &P, Wav
                         // eh[c+1].h <- H(i, c) (the next slot's old h was read in pass 1 only)
                         H = smax(me, F);
                         if (s < S - 1) Hh[s + 1] = H;
-                        Ee[s] = smax(subsat(Ee[s], edelp), sub16(Mv[s], oedelp));
+                        Ee[s] = smax(subsat(Ee[s], edelp), sub16(Ms, oedelp));
                         // re == qlen: every column but qlen is below re
                         uint32_t Hm = H;
                         if (s == S - 1) {
                             Hm = H & notlast;
                             Ee[s] = Ee[s] & notlast;
                         }
-                        pkey = umax(pkey, mad_imm<(1 << KB)>(Hm, (KM - (uint32_t)s) * 0x00010001u));
+                        key_acc<(1 << KB)>(pkey, Hm, (KM - (uint32_t)s) * 0x00010001u);
                         // H(i, c-1) at s > 0; c <= rb takes h1 = 0 (rb > 0), which a
                         // zero column on the left already gives
                         x = s == 0 ? 0u : (Hh[s] | Ee[s]);

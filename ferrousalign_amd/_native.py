@@ -28,8 +28,10 @@ FA_ERR_OVERFLOW = -10
 FA_ERR_SEGMENT = -11
 FA_ERR_SCRATCH = -12
 
-FA_MAX_QLEN = 255
-FA_MAX_TLEN = 2048
+FA_MAX_QLEN = 32767
+FA_MAX_TLEN = 65536
+FA_MAX_QLEN_PACKED = 255
+FA_MAX_TLEN_PACKED = 2048
 
 FA_SCORE_I16 = 0x0
 FA_SCORE_U8 = 0x1

@@ -607,6 +607,7 @@ class DeviceBatch:
                                                   N.ptr(slots), N.ptr(kern), N.ptr(first), N.ptr(count), N.ptr(ms)),
                 "fa_batch_profile_launches")
         names = {0: "extend_plain_kernel<%d,%d>", 1: "extend16_plain_kernel<%d,%d>"}
+        # kind 4: the long-pair kernel (one wavefront per pair, no lane groups)
         out = []
         for i in range(min(nl.value, cap)):
             kind, lanes = int(kern[i]) >> 8, int(kern[i]) & 0xFF
@@ -615,6 +616,8 @@ class DeviceBatch:
                 # `first` of a reverse launch indexes the reverse order
                 base = "mate16_kernel" if kind == 2 else "mate16_rev_kernel"
                 name = "%s<%d%s>" % (base, int(slots[i]), "" if lanes == 8 else ",8-bit")
+            elif kind == 4:
+                name = "extend_long_kernel"
             else:
                 name = names[kind] % (lanes, int(slots[i]))
                 if flags & N.FA_LAUNCH_PERSISTENT:

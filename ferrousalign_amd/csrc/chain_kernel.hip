@@ -61,8 +61,8 @@ __device__ __forceinline__ uint32_t seq_words(int32_t len) { return (uint32_t)((
 // validate_job's order for everything but the base codes: h0, qlen, tlen, band, h0 range
 __device__ __forceinline__ int job_error(int32_t h0, int32_t ql, int32_t tl, int32_t w) {
     if (h0 <= 0) return FA_ERR_H0;
-    if (ql <= 0 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
-    if (tl < 0 || tl > FA_MAX_TLEN) return FA_ERR_TLEN;
+    if (ql <= 0 || ql > FA_MAX_QLEN_PACKED) return FA_ERR_QLEN;
+    if (tl < 0 || tl > FA_MAX_TLEN_PACKED) return FA_ERR_TLEN;
     if (w < 0) return FA_ERR_BAND;
     if (h0 > (1 << 19)) return FA_ERR_SCORING;
     return FA_OK;
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(256) void chain_jobs_kernel(ChainIn in, ChainOpt o,
                 rq = qlen - qend;
                 const int32_t tl = (int32_t)(wlen - tmp);
                 int e = FA_OK;
-                if (rq > FA_MAX_QLEN) e = FA_ERR_QLEN;
-                else if (tl < 0 || tl > FA_MAX_TLEN) e = FA_ERR_TLEN;
+                if (rq > FA_MAX_QLEN_PACKED) e = FA_ERR_QLEN;
+                else if (tl < 0 || tl > FA_MAX_TLEN_PACKED) e = FA_ERR_TLEN;
                 else if (o.w < 0) e = FA_ERR_BAND;
                 rerr = (uint8_t)(-e);
                 R.okey[slot] = okey;

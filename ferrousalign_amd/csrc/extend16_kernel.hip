@@ -828,7 +828,10 @@ static hipError_t launch16(const ExtArgs &a, uint32_t first, uint32_t count, boo
     if (count == 0) return hipSuccess;
     const size_t lds_wave = (size_t)G * a.tpad + xfer_bytes16(S);
     const size_t lds = (size_t)4 * lds_wave;  // persistent: 4 wavefronts per workgroup
-    if ((persistent ? lds : lds_wave) > 65536) return hipErrorInvalidValue;
+    if (lds_wave > 65536) return hipErrorInvalidValue;
+    // four wavefronts' staged targets (targets near 2048 bases) may not fit a
+    // persistent workgroup's LDS: the plain form runs the same work then
+    if (lds > 65536) persistent = false;
     const uint32_t per_block = 4u * 2u * G;
     const uint32_t need = (count + per_block - 1) / per_block;
     if (!persistent) {

@@ -65,4 +65,11 @@ hipError_t extend16_launch(int L, int S, const ExtArgs &a, uint32_t first, uint3
                            bool persistent, uint32_t qidx, int cus, int blocks_per_cu,
                            hipStream_t st);
 
+// Long pairs (query > 255 or target > 2048 bases): one wavefront per pair,
+// eh[] in `scratch` (extend_long_scratch bytes for `count` pairs whose
+// longest query is qmax), the query's codes in LDS (qmax <= 65519).
+size_t extend_long_scratch(int qmax, uint32_t count);
+hipError_t extend_long_launch(const ExtArgs &a, uint32_t first, uint32_t count, int qmax,
+                              void *scratch, size_t scratch_bytes, hipStream_t st);
+
 }  // namespace fa

@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void extend_plain_kernel(ExtArgs a, uint32_t f
     extern __shared__ uint8_t lds[];
     constexpr int G = 64 / L;
     const int wave = (int)(threadIdx.x >> 6);
-    const uint32_t slot0 = (blockIdx.x * 4u + (uint32_t)wave) * G;
+    const uint32_t slot0 = (blockIdx.x * (blockDim.x >> 6) + (uint32_t)wave) * G;
     uint8_t *lds_wave = lds + wave * G * a.tpad;
     if (slot0 >= count) return;  // whole wave idle; no block barrier follows
     extend_wave<L, S>(a, first + slot0, first + count, lds_wave);
@@ -342,16 +342,23 @@ __global__ void queue_reset_kernel(uint32_t *q, uint32_t n) {
 
 // ---- host-side launch table ------------------------------------------------
 
+// LDS a workgroup may take (the staged targets of its wavefronts)
+constexpr size_t kLdsPerGroup = 65536;
+
 template <int L, int S>
 static hipError_t launch_plain(const ExtArgs &a, uint32_t first, uint32_t count,
                                hipStream_t st) {
     constexpr int G = 64 / L;
-    const uint32_t per_block = 4u * G;
+    // four wavefronts per workgroup, or one when four wavefronts' staged
+    // targets (targets near 2048 bases) would not fit a workgroup's LDS
+    const uint32_t waves = (size_t)4 * G * a.tpad <= kLdsPerGroup ? 4u : 1u;
+    const uint32_t per_block = waves * G;
     const uint32_t blocks = (count + per_block - 1) / per_block;
     if (blocks == 0) return hipSuccess;
-    const size_t lds = (size_t)4 * G * a.tpad;
-    hipLaunchKernelGGL((extend_plain_kernel<L, S>), dim3(blocks), dim3(256), lds, st, a, first,
-                       count);
+    const size_t lds = (size_t)waves * G * a.tpad;
+    if (lds > kLdsPerGroup) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((extend_plain_kernel<L, S>), dim3(blocks), dim3(64 * waves), lds, st, a,
+                       first, count);
     return hipGetLastError();
 }
 
@@ -361,6 +368,9 @@ static hipError_t launch_persistent(const ExtArgs &a, uint32_t first, uint32_t c
     constexpr int G = 64 / L;
     if (count == 0) return hipSuccess;
     const size_t lds = (size_t)4 * G * a.tpad;
+    // a persistent workgroup holds four wavefronts: when their staged targets
+    // do not fit its LDS, the plain form runs the same work
+    if (lds > kLdsPerGroup) return launch_plain<L, S>(a, first, count, st);
     int occ = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &occ, reinterpret_cast<const void *>(&extend_persistent_kernel<L, S>), 256, lds);

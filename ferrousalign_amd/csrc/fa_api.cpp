@@ -238,7 +238,7 @@ void batch_release(fa_batch *b) {
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
-                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev};
+                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev, b->d_long};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -315,6 +315,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     pa.tlen = b->d_tlen;
     pa.elig = b->d_elig;
     pa.cost = pack == FA_PACK_COST ? b->d_pred : nullptr;
+    pa.long_rank = (int)nk;
     FA_HIP(hipMemsetAsync(base, 0, hist_bytes, ctx->stream));
     FA_HIP(fa::plan_hist(pa, h, ctx->cus, ctx->stream));
     std::vector<uint32_t> cnt(kBins + 1);
@@ -370,6 +371,17 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
                   (bk.kind == 1 ? (31.0 * bk.S + 175.0) * bk.L / 128.0 : (40.0 * bk.S + 175.0) * bk.L / 64.0);
         buckets.push_back(bk);
     }
+    if (b->nlong > 0) {
+        // long pairs sort after every lane-group kernel: one launch of their own
+        Bucket bk;
+        bk.kind = 4;
+        bk.first = first;
+        bk.count = (uint32_t)b->nlong;
+        bk.L = 64;
+        bk.S = 0;
+        bk.work = b->long_work;
+        buckets.push_back(bk);
+    }
     if (buckets.size() > 64) return FA_ERR_ARG;
     // a re-plan that gives the same launches keeps the captured graph (it
     // reads the order and metadata from device memory at replay)
@@ -392,6 +404,8 @@ namespace fa_host {
 hipError_t launch_bucket(fa_ctx *ctx, fa_batch *b, uint32_t flags, size_t k, hipStream_t st) {
     const Bucket &bk = b->buckets[k];
     fa::ExtArgs a = b->args;
+    if (bk.kind == 4)
+        return fa::extend_long_launch(a, bk.first, bk.count, b->long_qmax, b->d_long, b->long_bytes, st);
     a.chunk = std::max(1, b->chunk / (64 / bk.L) / (bk.kind == 1 ? 2 : 1));
     if (bk.kind == 1)
         return fa::extend16_launch(bk.L, bk.S, a, bk.first, bk.count,
@@ -596,9 +610,9 @@ const char *fa_strerror(int code) {
         case FA_OK: return "ok";
         case FA_ERR_ARG: return "invalid argument";
         case FA_ERR_H0: return "h0 must be positive";
-        case FA_ERR_QLEN: return "query length out of range (1..FA_MAX_QLEN)";
+        case FA_ERR_QLEN: return "query length out of range (1..FA_MAX_QLEN; 1..FA_MAX_QLEN_PACKED on the paths without long pairs)";
         case FA_ERR_CODE: return "base code out of range (0..4)";
-        case FA_ERR_TLEN: return "target length out of range (0..FA_MAX_TLEN)";
+        case FA_ERR_TLEN: return "target length out of range (0..FA_MAX_TLEN; 0..FA_MAX_TLEN_PACKED on the paths without long pairs)";
         case FA_ERR_BAND: return "band width must be non-negative";
         case FA_ERR_SCORING: return "unsupported scoring parameters";
         case FA_ERR_DEVICE: return "HIP device error";
@@ -773,9 +787,13 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     std::vector<int8_t> vrc((size_t)n);
     int matmax = 0;  // max(0, max(mat))
     for (int x = 0; x < 25; ++x) matmax = std::max(matmax, (int)sw->mat[x]);
-    std::vector<int> ttmax(16, 0);
+    std::vector<int> ttmax(16, 0), tqmax(16, 0);
+    std::vector<int64_t> tnlong(16, 0);
+    std::vector<double> twork(16, 0.0);
     parallel_for(n, [&](int64_t lo, int64_t hi, unsigned t) {
-        int tm = 0;
+        int tm = 0, qm = 0;
+        int64_t nl = 0;
+        double wk = 0.0;
         for (int64_t k = lo; k < hi; ++k) {
             int r = ts.validated ? FA_OK
                                  : validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
@@ -790,13 +808,29 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             vrc[(size_t)k] = (int8_t)r;
             if (r == FA_OK) {
                 b->w[(size_t)k] = capped_w(*sw, matmax, query_len[k], band_width[k]);
-                b->elig16[(size_t)k] = (uint8_t)fa_host::fits16(*sw, matmax, h0[k], query_len[k]);
-                tm = std::max(tm, ref_len[k]);
+                if (query_len[k] > FA_MAX_QLEN_PACKED || ref_len[k] > FA_MAX_TLEN_PACKED) {
+                    // beyond the lane-group kernels: the long-pair kernel
+                    b->elig16[(size_t)k] = fa_host::kEligLong;
+                    qm = std::max(qm, query_len[k]);
+                    ++nl;
+                    wk += (double)query_len[k] * (double)ref_len[k];
+                } else {
+                    b->elig16[(size_t)k] = (uint8_t)fa_host::fits16(*sw, matmax, h0[k], query_len[k]);
+                    tm = std::max(tm, ref_len[k]);
+                }
             }
         }
         ttmax[t] = std::max(ttmax[t], tm);
+        tqmax[t] = std::max(tqmax[t], qm);
+        tnlong[t] += nl;
+        twork[t] += wk;
     });
     for (int v : ttmax) tmax = std::max(tmax, v);
+    for (int x = 0; x < 16; ++x) {
+        b->long_qmax = std::max(b->long_qmax, tqmax[(size_t)x]);
+        b->nlong += tnlong[(size_t)x];
+        b->long_work += twork[(size_t)x];
+    }
     for (int64_t k = 0; k < n; ++k) {
         int r = vrc[(size_t)k];
         if (r != FA_OK) {
@@ -851,7 +885,10 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         (rc = dev_upload(&b->d_elig, b->elig16.data(), b->elig16.size(), st)) ||
         (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
         (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
-        (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64))) {
+        (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64)) ||
+        (b->nlong > 0 &&
+         (rc = dev_alloc((uint8_t **)&b->d_long,
+                         b->long_bytes = fa::extend_long_scratch(b->long_qmax, (uint32_t)b->nlong))))) {
         (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
         batch_release(b);
         return rc;
@@ -1024,8 +1061,8 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             for (int64_t k = lo; k < hi; ++k) {
                 const int32_t ql = query_len[k], tl = ref_len[k];
                 int c = FA_OK;
-                if (ql < 0 || ql > FA_MAX_QLEN) c = FA_ERR_QLEN;
-                else if (tl < 0 || tl > FA_MAX_TLEN) c = FA_ERR_TLEN;
+                if (ql < 0 || ql > FA_MAX_QLEN_PACKED) c = FA_ERR_QLEN;
+                else if (tl < 0 || tl > FA_MAX_TLEN_PACKED) c = FA_ERR_TLEN;
                 else if (band_width[k] < 0) c = FA_ERR_BAND;
                 else if (h0[k] < 0 || h0[k] > (1 << 19)) c = FA_ERR_H0;
                 else if (!fa_host::codes_ok(query_seqs + query_offset[k], ql) ||
@@ -1793,7 +1830,7 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     std::vector<int> sk((size_t)n);
     for (int64_t k = 0; k < n; ++k) {
         const int ql = b->qlen[(size_t)k];
-        if (ql < 1 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
+        if (ql < 1 || ql > FA_MAX_QLEN_PACKED) return FA_ERR_QLEN;
         sk[(size_t)k] = (ql + lanes - 1) / lanes;
     }
     // stable counting sort by slen, longest first

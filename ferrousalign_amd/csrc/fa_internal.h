@@ -36,10 +36,15 @@ struct Bucket {
     int S;
     uint32_t first, count;
     double work;   // estimated work: target rows x VALU instructions per row / pairs per wavefront
-    int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group)
+    int kind = 0;  // 0: 32-bit kernel, 1: packed 16-bit kernel (two pairs per lane group),
+                   // 4: long pairs (one wavefront per pair)
     int L = 16;    // lanes per pair of this bucket
     int slen = 0;  // mate rescue: segments per lane ceil(qlen / 8) of every pair
 };
+
+// fits16 level of a pair that only the long-pair kernel holds (query over
+// FA_MAX_QLEN_PACKED or target over FA_MAX_TLEN_PACKED bases)
+constexpr uint8_t kEligLong = 3;
 
 }  // namespace fa_host
 
@@ -80,6 +85,11 @@ struct fa_batch {
     int streams = 4;
     int tmax = 0;
     int64_t n_fallback32 = 0;   // pairs whose scores may leave the 16-bit range (fits16 == 0)
+    int64_t nlong = 0;          // pairs for the long-pair kernel (fits16 level kEligLong)
+    int long_qmax = 0;          // their longest query
+    double long_work = 0.0;     // their band-free cell bound (sum of qlen x tlen), for stream balance
+    void *d_long = nullptr;     // eh[] slices of the long-pair kernel
+    size_t long_bytes = 0;
     int64_t last_overflow = 0;  // of the last extension run: pairs that ran the 32-bit kernel for that
     // mate-rescue local SW (FA_RUN_ALIGN2)
     bool align2_set = false;

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(64) void noalloc_prep_kernel(PrepArgs a) {
     // validate_job (fa_api.cpp), same order of checks
     int rc = FA_OK;
     if (h0 <= 0) rc = FA_ERR_H0;
-    else if (ql <= 0 || ql > FA_MAX_QLEN) rc = FA_ERR_QLEN;
+    else if (ql <= 0 || ql > FA_MAX_QLEN_PACKED) rc = FA_ERR_QLEN;
     else if (tl < 0 || tl > a.max_tlen) rc = FA_ERR_TLEN;
     else if (w < 0) rc = FA_ERR_BAND;
     else if (h0 > (1 << 19)) rc = FA_ERR_SCORING;
@@ -219,7 +219,7 @@ extern "C" int fa_execute_batch_device_noalloc(
     const int64_t *d_ref_offset, const int32_t *d_ref_len, const int32_t *d_h0,
     const int32_t *d_band_width, int32_t max_ref_len, fa_out_score *d_out, uint32_t *d_cells,
     int32_t *d_job_rc) {
-    if (!ctx || njobs < 0 || max_ref_len < 0 || max_ref_len > FA_MAX_TLEN) return FA_ERR_ARG;
+    if (!ctx || njobs < 0 || max_ref_len < 0 || max_ref_len > FA_MAX_TLEN_PACKED) return FA_ERR_ARG;
     int rc = fa_host::check_scoring(sw);
     if (rc) return rc;
     if (njobs == 0) return FA_OK;
@@ -308,7 +308,7 @@ extern "C" int fa_execute_batch_device_noalloc(
     Bin bins[3 * 256];
     int nb = 0;
     for (int lvl = 2; lvl >= 0; --lvl)
-        for (int q = FA_MAX_QLEN; q >= 1; --q) {
+        for (int q = FA_MAX_QLEN_PACKED; q >= 1; --q) {
             if (!hist[lvl][q]) continue;
             int kind, lanes, S;
             if (!fa_host::pair_kernel(0, false, lvl, q, &kind, &lanes, &S)) return FA_ERR_QLEN;

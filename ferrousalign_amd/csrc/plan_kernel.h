@@ -13,8 +13,9 @@ struct PlanArgs {
     uint32_t n;
     int mode;
     const int32_t *qlen, *tlen;
-    const uint8_t *elig;  // 16-bit fit level per pair (0..2)
+    const uint8_t *elig;  // 16-bit fit level per pair (0..2), 3: long pair
     const float *cost;    // per-pair packing cost (kPlanCost), else null
+    int long_rank;        // launch rank of the long pairs (after every lane-group kernel)
 };
 
 // Per (fit level, qlen) bin, 3 x 256 entries each; zero before plan_hist.

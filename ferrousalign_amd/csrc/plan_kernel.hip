@@ -11,8 +11,9 @@
 //                     kernel rank, then (cost packing) the early-stop flag and
 //                     cost class of a pair whose cost is below half the mean
 //                     of its (kernel, qlen) group, then qlen and tlen
-//                     descending; the device radix sort of (key, pair id) is
-//                     stable, so equal keys keep the input order.
+//                     descending; long pairs (fit level 3) last, longest
+//                     query first; the device radix sort of (key, pair id)
+//                     is stable, so equal keys keep the input order.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,8 +37,8 @@ __global__ __launch_bounds__(1024) void plan_hist_kernel(PlanArgs a, PlanHist h)
     __syncthreads();
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += gridDim.x * blockDim.x) {
         const int q = a.qlen[k];
-        const int f = a.elig[k] > 2 ? 2 : a.elig[k];
-        if (q < 1 || q > 255) continue;  // validated at upload
+        const int f = a.elig[k];
+        if (f > 2 || q < 1 || q > 255) continue;  // long pairs: one launch of their own
         const int b = f * 256 + q;
         atomicAdd(&cnt[b], 1u);
         atomicAdd(&rows[b], (unsigned long long)max(a.tlen[k], 1));
@@ -59,7 +60,13 @@ __global__ __launch_bounds__(256) void plan_key_kernel(PlanArgs a, PlanHist h, P
     if (k >= a.n) return;
     ids[k] = k;
     const int q = a.qlen[k];
-    const int f = a.elig[k] > 2 ? 2 : a.elig[k];
+    const int f = a.elig[k];
+    if (f > 2) {
+        // long pair: after every lane-group kernel, longest query first
+        keys[k] = ((unsigned long long)a.long_rank << 28) | ((unsigned long long)(65535 - min(q, 65535)) << 12) |
+                  (unsigned long long)(4095 - min(a.tlen[k], 4095));
+        return;
+    }
     const int x = (q >= 1 && q <= 255) ? t.kidx[f][q] : -1;
     if (x < 0) {
         atomicOr(bad, 1u);

@@ -38,8 +38,17 @@ extern "C" {
 #define FA_ERR_SEGMENT (-11)  /* reference target segment outside [0, 2 l_pac) or across l_pac */
 #define FA_ERR_SCRATCH (-12)  /* batch exceeds the fixed scratch of fa_execute_batch_device_noalloc */
 
-#define FA_MAX_QLEN 255
-#define FA_MAX_TLEN 2048
+/* Lengths.  The extension entry points (fa_execute_batch_simd_scoring,
+ * fa_batch_upload and the SoA entry points) take queries up to FA_MAX_QLEN
+ * and targets up to FA_MAX_TLEN bases: pairs up to the _PACKED limits run in
+ * the lane-group kernels, longer ones in a kernel of one wavefront per pair
+ * (the reference's ksw_extend2 has no length limit; these keep a pair's
+ * cell count within 32 bits).  The chain path, the allocation-free entry
+ * point, CIGAR regeneration and ksw_align2 take the _PACKED limits. */
+#define FA_MAX_QLEN 32767
+#define FA_MAX_TLEN 65536
+#define FA_MAX_QLEN_PACKED 255
+#define FA_MAX_TLEN_PACKED 2048
 
 /* ---- flags -------------------------------------------------------------- */
 /* score path.  FA_SCORE_I16 (default): pairs whose scores provably fit run
@@ -136,7 +145,8 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw,
  * optional) and the call allocates nothing: validation, 2-bit packing,
  * packing order and launch metadata live in one fixed per-device scratch
  * array (64 MiB, set up by fa_ctx_create).  A job is limited to
- * ref_len <= max_ref_len (<= FA_MAX_TLEN; longer jobs get FA_ERR_TLEN); a
+ * ref_len <= max_ref_len (<= FA_MAX_TLEN_PACKED; longer jobs get FA_ERR_TLEN,
+ * and so do queries over FA_MAX_QLEN_PACKED); a
  * batch needs about njobs * (6 * (16 + 2 * ceil(max_ref_len / 32)) + 52)
  * bytes of it, and a larger batch is rejected with FA_ERR_SCRATCH before
  * anything runs.  Synchronous on the context's stream.  Calls on one device
@@ -207,7 +217,8 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
  * query_len + ref_len + 2 codes each (lengths n_ref_aligned[j],
  * n_query_aligned[j]).  An empty query or reference gives the reference's
  * zero record and an empty CIGAR.  cells may be NULL.  Limits as for the
- * extension (query <= 255, reference <= 2048, uniform N row). */
+ * lane-group kernels (query <= FA_MAX_QLEN_PACKED, reference <=
+ * FA_MAX_TLEN_PACKED, uniform N row). */
 int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                                const uint8_t *query_seqs, const int64_t *query_offset,
                                const int32_t *query_len, const uint8_t *ref_seqs,
@@ -332,7 +343,7 @@ typedef struct {
  * :267-494; a best score reaching 255 - shift is reported as 255 with
  * qe = -1, as the reference does; mate_rescue.rs:314-321 sets KSW_XBYTE when
  * qlen * match < 250).  The N row of mat must score 0 (true of the
- * mate-rescue matrix, mate_rescue.rs:116-122); 1 <= qlen <= FA_MAX_QLEN,
+ * mate-rescue matrix, mate_rescue.rs:116-122); 1 <= qlen <= FA_MAX_QLEN_PACKED,
  * tlen <= 2032.  cells[k] (optional): target rows run times query length
  * over both passes. */
 int fa_ksw_align2_batch(fa_ctx *ctx, int64_t n, const uint8_t *query_seqs,

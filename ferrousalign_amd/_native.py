@@ -147,6 +147,10 @@ def _declare(lib):
     lib.fa_batch_get_order.argtypes = [_P, _P, _U32, _P]
     lib.fa_batch_overflow_count.argtypes = [_P, _P, ctypes.POINTER(_I64)]
     lib.fa_batch_free.argtypes = [_P, _P]
+    lib.fa_agg_create.argtypes = [ctypes.c_int, _I32, _I32, ctypes.POINTER(_P)]
+    lib.fa_agg_destroy.argtypes = [_P]
+    lib.fa_agg_execute.argtypes = [_P, ctypes.POINTER(BandedPairwiseSW)] + seqargs + [_U32, _P, _P, _P]
+    lib.fa_agg_stats.argtypes = [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]
     soa = [_P, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P]
     lib.fa_simd_banded_swa_batch_soa.argtypes = soa
     lib.fa_simd_banded_swa_batch_int16_soa.argtypes = soa
@@ -159,7 +163,8 @@ def _declare(lib):
                  "fa_batch_upload", "fa_batch_set_cost", "fa_batch_set_launch", "fa_batch_set_streams", "fa_batch_run", "fa_batch_set_align2",
                  "fa_batch_download_align2", "fa_ksw_align2_batch",
                  "fa_batch_time", "fa_batch_download", "fa_batch_profile_launches", "fa_batch_get_order", "fa_batch_overflow_count", "fa_batch_free",
-                 "fa_simd_banded_swa_batch_soa", "fa_simd_banded_swa_batch_int16_soa"):
+                 "fa_simd_banded_swa_batch_soa", "fa_simd_banded_swa_batch_int16_soa",
+                 "fa_agg_create", "fa_agg_destroy", "fa_agg_execute", "fa_agg_stats"):
         getattr(lib, name).restype = ctypes.c_int
 
 

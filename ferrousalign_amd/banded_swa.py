@@ -245,6 +245,60 @@ def extend_arrays(sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, h0, 
     return out, cells
 
 
+class Aggregator:
+    """Coalesces the small batches of concurrent host threads (fa_agg_*): the
+    reference calls execute_batch_simd_scoring from every rayon worker at once
+    (single_end.rs:92-110).  `execute` takes the arguments of extend_arrays
+    and returns the same records; calls from several threads at once are run
+    as one GPU batch per (scoring, flags) once `min_jobs` jobs are pending or
+    `max_wait_us` after the oldest call arrived."""
+
+    def __init__(self, device=0, min_jobs=4096, max_wait_us=2000):
+        self._h = ctypes.c_void_p()
+        N.check(N.lib().fa_agg_create(int(device), int(min_jobs), int(max_wait_us), ctypes.byref(self._h)),
+                "fa_agg_create")
+
+    def execute(self, sw: BandedPairWiseSW, qseq, qoff, qlen, tseq, toff, tlen, h0, w, flags=0,
+                want_cells=False, per_job_rc=False):
+        n = len(qlen)
+        qseq = np.ascontiguousarray(qseq, np.uint8)
+        tseq = np.ascontiguousarray(tseq, np.uint8)
+        qoff = np.ascontiguousarray(qoff, np.int64)
+        toff = np.ascontiguousarray(toff, np.int64)
+        qlen = np.ascontiguousarray(qlen, np.int32)
+        tlen = np.ascontiguousarray(tlen, np.int32)
+        h0 = np.ascontiguousarray(h0, np.int32)
+        w = np.ascontiguousarray(w, np.int32)
+        out = np.zeros((n, 6), np.int32)
+        cells = np.zeros(n, np.uint32) if want_cells else None
+        rc = np.zeros(n, np.int32) if per_job_rc else None
+        c = sw.to_c()
+        code = N.lib().fa_agg_execute(self._h, ctypes.byref(c), n, N.ptr(qseq), N.ptr(qoff), N.ptr(qlen),
+                                      N.ptr(tseq), N.ptr(toff), N.ptr(tlen), N.ptr(h0), N.ptr(w), flags,
+                                      N.ptr(out), N.ptr(cells), N.ptr(rc))
+        if per_job_rc:
+            return out, cells, rc, code
+        N.check(code, "fa_agg_execute")
+        return out, cells
+
+    def stats(self):
+        """(calls served, batches run, jobs scored)."""
+        v = [ctypes.c_int64() for _ in range(3)]
+        N.check(N.lib().fa_agg_stats(self._h, *[ctypes.byref(x) for x in v]), "fa_agg_stats")
+        return tuple(x.value for x in v)
+
+    def close(self):
+        if self._h:
+            N.lib().fa_agg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 CIGAR_OPS = "MIDNSHP=X"
 
 

@@ -23,7 +23,7 @@ HIP_FLAGS = [
 ]
 HIP_SOURCES = ["extend_kernel.hip", "extend16_kernel.hip", "predictor.hip", "mate16_kernel.hip", "mate16_sym.hip", "mate8_kernel.hip", "valu_probe.hip", "extend_long_kernel.hip",
                "ref_kernel.hip", "swtb_kernel.hip", "chain_kernel.hip", "noalloc.hip", "plan_kernel.hip", "fa_api.cpp",
-               "fa_pipeline.cpp", "chain_api.cpp", "pack_host.cc"]
+               "fa_pipeline.cpp", "chain_api.cpp", "aggregator.cpp", "pack_host.cc"]
 
 
 def _run(cmd):

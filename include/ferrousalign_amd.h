@@ -137,6 +137,30 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw,
                                   const int32_t *band_width, uint32_t flags,
                                   fa_out_score *out, uint32_t *cells, int32_t *per_job_rc);
 
+/* ---- concurrent callers: coalescing aggregator ---------------------------- */
+/* The reference calls execute_batch_simd_scoring from every rayon worker at
+ * once, each call synchronous (src/pipelines/linear/single_end.rs:92-110),
+ * and a GPU pays off only on large batches (src/core/compute/mod.rs:277-287).
+ * An aggregator owns one context on `device` and one worker thread: calls of
+ * fa_agg_execute from any number of host threads are coalesced, the worker
+ * taking every job submitted so far once min_jobs jobs are pending or
+ * max_wait_us after the oldest pending call arrived, and running them as one
+ * batch per (scoring, flags).  Each call returns when its own jobs are done,
+ * with exactly what fa_execute_batch_simd_scoring would return for them (same
+ * arguments and outputs; the return code is the call's own first invalid
+ * job's, or a device error of the run).  Destroy only when no call is in
+ * flight; calls still pending then are run first. */
+typedef struct fa_agg fa_agg;
+int fa_agg_create(int device, int32_t min_jobs, int32_t max_wait_us, fa_agg **out);
+int fa_agg_destroy(fa_agg *agg);
+int fa_agg_execute(fa_agg *agg, const fa_banded_pairwise_sw *sw, int64_t njobs,
+                   const uint8_t *query_seqs, const int64_t *query_offset,
+                   const int32_t *query_len, const uint8_t *ref_seqs, const int64_t *ref_offset,
+                   const int32_t *ref_len, const int32_t *h0, const int32_t *band_width,
+                   uint32_t flags, fa_out_score *out, uint32_t *cells, int32_t *per_job_rc);
+/* Calls served, batches run and jobs scored since fa_agg_create. */
+int fa_agg_stats(fa_agg *agg, int64_t *calls, int64_t *runs, int64_t *jobs);
+
 /* ---- allocation-free form (no counterpart in the reference) -------------- */
 /* For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc): the same

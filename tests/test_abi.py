@@ -35,6 +35,11 @@ def test_no_device_is_an_error_not_a_crash():
     assert N.lib().fa_ctx_create(0, ctypes.byref(h)) == N.FA_ERR_NODEVICE
     assert N.lib().fa_strerror(N.FA_ERR_H0) == b"h0 must be positive"
     assert N.lib().fa_ctx_destroy(None) == N.FA_ERR_ARG
+    # the aggregator opens its own context: no device, no aggregator
+    assert N.lib().fa_agg_create(0, 1, 0, ctypes.byref(h)) == N.FA_ERR_NODEVICE
+    assert N.lib().fa_agg_create(0, 0, 0, ctypes.byref(h)) == N.FA_ERR_ARG
+    assert N.lib().fa_agg_destroy(None) == N.FA_ERR_ARG
+    assert N.lib().fa_agg_stats(None, None, None, None) == N.FA_ERR_ARG
 
 
 @pytest.mark.parametrize("seed", range(4))

@@ -540,7 +540,7 @@ def main():
         pred.predict(db, set_cost=True)
         db.set_launch(0)
         db.set_streams(1)
-        flags = N.FA_PACK_COST
+        flags = N.FA_PACK_COST_ORDER
         db.run(flags)
         _, pcells = db.download()
         print(json.dumps({"roofline": roofline(db, flags, pcells, ctx)}), flush=True)
@@ -570,9 +570,9 @@ def main():
         best = dict(lanes=8, chunk=0, wpc=0, streams=4, launch=N.FA_LAUNCH_GRAPH)
         db.set_launch(8)
         db.set_streams(4)
-        flags = N.FA_PACK_COST | N.FA_LAUNCH_GRAPH
+        flags = N.FA_PACK_COST_ORDER | N.FA_LAUNCH_GRAPH
     else:
-        best, flags, _ = autotune.tune(db, ps.qlen, ps.tlen, ps.w, pack_flags=N.FA_PACK_COST, log=log,
+        best, flags, _ = autotune.tune(db, ps.qlen, ps.tlen, ps.w, pack_flags=N.FA_PACK_COST_ORDER, log=log,
                                        use_cache=False)
     # run only: the plan of this batch is cached
     ms_run, ms_ev, nlaunch = timed(db, flags, args.steps, args.warmup)
@@ -630,7 +630,8 @@ def main():
         "dtype": "int16x2",
         "data": "synthetic",
         "config": {"workload": args.workload, "pairs": ps.n,
-                   "packing": "learned (predicted cells); a step = features + forward pass + device packing "
+                   "packing": "learned (descending predicted cells per kernel); a step = features + forward "
+                              "pass + device packing "
                               "plan + run",
                    "lanes_per_pair": best["lanes"] or "narrowest group per query length",
                    "chunk_pairs": best["chunk"], "streams": best.get("streams", 4),

@@ -40,6 +40,7 @@ FA_SCORE_I32 = 0x4
 FA_PACK_LENGTH = 0x00
 FA_PACK_NONE = 0x10
 FA_PACK_COST = 0x20
+FA_PACK_COST_ORDER = 0x30
 FA_LAUNCH_PLAIN = 0x000
 FA_LAUNCH_PERSISTENT = 0x100
 FA_LAUNCH_GRAPH = 0x200

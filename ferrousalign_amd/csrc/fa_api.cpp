@@ -259,7 +259,8 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     const uint32_t key = pack | single | wide | (uint32_t)L << 16;
     if (b->packed_flags == key && !(flags & FA_PLAN_FRESH) && !b->plan_dirty) return FA_OK;
     const int64_t n = b->n;
-    if (pack == FA_PACK_COST && !b->cost_set) return FA_ERR_ARG;
+    const bool by_cost = pack == FA_PACK_COST || pack == FA_PACK_COST_ORDER;
+    if (by_cost && !b->cost_set) return FA_ERR_ARG;
     // kernel of a pair, (kind << 16) | (lanes << 8) | slots, as a function of
     // its fits16 level and qlen (-1: no kernel holds it); merged buckets (one
     // per kernel kind) take the kind alone
@@ -310,11 +311,26 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     void *temp = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
     fa::PlanArgs pa;
     pa.n = (uint32_t)n;
-    pa.mode = pack == FA_PACK_COST ? fa::kPlanCost : (pack == FA_PACK_NONE ? fa::kPlanNone : fa::kPlanLength);
+    pa.mode = pack == FA_PACK_COST         ? fa::kPlanCost
+              : pack == FA_PACK_COST_ORDER ? fa::kPlanCostOrder
+              : pack == FA_PACK_NONE       ? fa::kPlanNone
+                                           : fa::kPlanLength;
     pa.qlen = b->d_qlen;
     pa.tlen = b->d_tlen;
     pa.elig = b->d_elig;
-    pa.cost = pack == FA_PACK_COST ? b->d_pred : nullptr;
+    pa.cost = by_cost ? b->d_pred : nullptr;
+    // cost order where a lane group holds 64 or more columns: there a
+    // wavefront of pairs with equal cost ends together; in the narrow kernels
+    // (short queries, a few hundred cells) the prediction's error is too large
+    // a part of the cost and length order does better
+    for (size_t x = 0; x < 64; ++x) {
+        int L = 0, S = 0;
+        if (x < nk && (kinds[x] & 0xFFFF) != 0) {
+            L = (kinds[x] >> 8) & 0xFF;
+            S = kinds[x] & 0xFF;
+        }
+        t.cost_order[x] = (uint8_t)(L * S >= 64 ? 1 : 0);
+    }
     pa.long_rank = (int)nk;
     FA_HIP(hipMemsetAsync(base, 0, hist_bytes, ctx->stream));
     FA_HIP(fa::plan_hist(pa, h, ctx->cus, ctx->stream));

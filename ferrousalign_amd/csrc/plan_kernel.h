@@ -8,6 +8,7 @@ namespace fa {
 constexpr int kPlanLength = 0;  // (kernel, qlen desc, tlen desc)
 constexpr int kPlanNone = 1;    // kernel only: input order inside a kernel's range
 constexpr int kPlanCost = 2;    // length order with early stoppers apart, by cost class
+constexpr int kPlanCostOrder = 3;  // descending cost inside the kernels marked in PlanTable::cost_order
 
 struct PlanArgs {
     uint32_t n;
@@ -29,6 +30,7 @@ struct PlanHist {
 // the launch order, -1 where no kernel holds it.
 struct PlanTable {
     int16_t kidx[3][256];
+    uint8_t cost_order[64];  // kPlanCostOrder: kernel x orders its pairs by descending cost
 };
 
 // Packing cost in 1/16 cells, non-negative and saturating at 2^24 cells

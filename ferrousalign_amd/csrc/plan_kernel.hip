@@ -11,7 +11,8 @@
 //                     kernel rank, then (cost packing) the early-stop flag and
 //                     cost class of a pair whose cost is below half the mean
 //                     of its (kernel, qlen) group, then qlen and tlen
-//                     descending; long pairs (fit level 3) last, longest
+//                     descending (cost-order packing: descending cost inside
+//                     the kernels the host marks); long pairs (fit level 3) last, longest
 //                     query first; the device radix sort of (key, pair id)
 //                     is stable, so equal keys keep the input order.
 #include <hip/hip_runtime.h>
@@ -88,8 +89,14 @@ __global__ __launch_bounds__(256) void plan_key_kernel(PlanArgs a, PlanHist h, P
             key |= (1ull << 27) | (unsigned long long)(127 - min(cls, 127)) << 20;
         }
     }
-    if (a.mode != kPlanNone)
+    if (a.mode == kPlanCostOrder && t.cost_order[x]) {
+        // descending cost (cells, saturating at 2^20 - 1; a pair has at most
+        // 256 x 2048), then qlen descending
+        const float c = fminf(fmaxf(a.cost[k], 0.0f), 1048575.0f);
+        key |= ((unsigned long long)(1048575u - (uint32_t)rintf(c)) << 8) | (unsigned long long)(255 - q);
+    } else if (a.mode != kPlanNone) {
         key |= ((unsigned long long)(255 - q) << 12) | (unsigned long long)(4095 - min(a.tlen[k], 4095));
+    }
     keys[k] = key;
 }
 

@@ -3,9 +3,11 @@
 A 6-32-32-1 MLP (FP16 storage, FP32 accumulation) trained with the
 library's own HIP kernels (features, forward, backward, reduce + Adam) on the
 cell counts that the plain extension kernel records for every pair.  Its
-prediction only orders pairs inside each query-length bucket (FA_PACK_COST),
-so that the lane groups of one wavefront hold pairs that retire together; it
-never changes an alignment result.  Length-sorted packing (FA_PACK_LENGTH) is
+prediction only orders pairs inside each kernel's bucket (FA_PACK_COST_ORDER:
+descending predicted cells where a lane group holds 64 or more columns, so
+that the lane groups of one wavefront hold pairs that retire together, and
+their bands shrink together; FA_PACK_COST: length order with predicted early
+stoppers apart); it never changes an alignment result.  Length-sorted packing (FA_PACK_LENGTH) is
 the fallback when no trained network is available.
 
 Features (computed on the device from the resident 2-bit sequences): qlen,

@@ -63,6 +63,10 @@ extern "C" {
 /* packing of pairs into lane groups (output order never changes) */
 #define FA_PACK_LENGTH 0x00u    /* sort by (qlen, tlen): the plain kernel */
 #define FA_PACK_NONE 0x10u      /* input order */
+#define FA_PACK_COST_ORDER 0x30u /* caller-supplied per-pair cost (fa_batch_set_cost), e.g. predicted
+                                   DP cells: inside each kernel whose lane groups hold 64 or more
+                                   columns, pairs in descending cost, so that a wavefront's pairs
+                                   end together; length order for the narrower kernels */
 #define FA_PACK_COST 0x20u      /* group by a caller-supplied per-pair cost (fa_batch_set_cost): cost
                                    classes e^(1/2) apart, length order inside a class */
 /* launch strategy */

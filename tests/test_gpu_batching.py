@@ -78,6 +78,9 @@ def test_order_invariance(mix):
     db.set_cost(cost)
     db.run(N.FA_PACK_COST | N.FA_LAUNCH_PERSISTENT)
     _check(db, ora[perm], oc[perm], "random cost packing")
+    for flags in (N.FA_PACK_COST_ORDER, N.FA_PACK_COST_ORDER | N.FA_LAUNCH_GRAPH):
+        db.run(flags)
+        _check(db, ora[perm], oc[perm], "random cost-order packing flags=%#x" % flags)
     db.free()
 
 

@@ -150,3 +150,57 @@ def test_learned_packing_lowers_wavefront_work():
     assert np.array_equal(cells.astype(np.uint64), ocells)
     db.free()
     ctx.close()
+
+
+def _wavefront_idle_cells(db, flags, cells):
+    """Sum over the wavefronts of (pairs x the most cells of any of its pairs
+    - their cells): the cell-slots a wavefront's lane groups idle while its
+    costliest pair runs on, over the kernels whose lane groups hold 64 or
+    more columns (where descending-cost packing applies)."""
+    order = db.order(flags)
+    total = 0.0
+    for l in db.profile_launches(flags):
+        if not l["kernel"].startswith("extend16") or l["lanes"] * l["slots"] < 64:
+            continue
+        per_wave = 128 // l["lanes"]
+        c = cells[order[l["first"]:l["first"] + l["count"]]].astype(np.float64)
+        pad = (-len(c)) % per_wave
+        c = np.concatenate([c, np.zeros(pad)]).reshape(-1, per_wave)
+        total += float((c.max(axis=1, keepdims=True) - c).sum())
+    return total
+
+
+def test_cost_order_packing_lowers_idle_cells():
+    """Descending predicted cells per kernel (FA_PACK_COST_ORDER, the learned
+    path's packing): on held-out pairs the idle cell-slots of the wavefronts
+    must fall below length sorting's (measured 0.85 of it), and lowest with
+    the true cells."""
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd.banded_swa import DeviceBatch, Context
+    from ferrousalign_amd.predictor import WorkPredictor
+    ctx = Context(0)
+    sw = synth.default_scoring()
+    tr = synth.workload("uneven_mix_train_16m", n=262144)
+    dbt = DeviceBatch(sw, *tr.flat(), ctx=ctx)
+    dbt.run(0)
+    pred = WorkPredictor(ctx, seed=0)
+    pred.fit(dbt, epochs=3, minibatch=8192, lr=3e-3)
+    dbt.free()
+    ps = synth.workload("uneven_mix_4m", n=131072)
+    rows, ocells = O.extend_rows(sw, *ps.flat())
+    db = DeviceBatch(sw, *ps.flat(), ctx=ctx)
+    db.set_launch(0)
+    length = _wavefront_idle_cells(db, N.FA_PACK_LENGTH, ocells)
+    pred.predict(db, set_cost=True, download=False)
+    learned = _wavefront_idle_cells(db, N.FA_PACK_COST_ORDER, ocells)
+    db.run(N.FA_PACK_COST_ORDER)
+    out, cells = db.download()
+    assert np.array_equal(cells.astype(np.uint64), ocells)
+    db.set_cost(ocells.astype(np.float32))
+    perfect = _wavefront_idle_cells(db, N.FA_PACK_COST_ORDER, ocells)
+    print("idle cell-slots: length %.4g learned %.4g (%.3f) perfect %.4g (%.3f)"
+          % (length, learned, learned / length, perfect, perfect / length), flush=True)
+    assert learned < 0.9 * length
+    assert perfect <= learned
+    db.free()
+    ctx.close()

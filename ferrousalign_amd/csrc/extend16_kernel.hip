@@ -778,13 +778,34 @@ __global__ __launch_bounds__(256) void meta_gather_kernel(ExtArgs a, uint32_t n)
     if (k >= n) return;
     const uint32_t p = a.order[k];
     uint4 *mp = reinterpret_cast<uint4 *>(a.meta) + 2 * (size_t)k;
+    if (a.params) {
+        // the pair's 32-byte record: one line instead of six scattered words
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.params) + 2 * (size_t)p;
+        mp[0] = src[0];
+        mp[1] = src[1];
+        return;
+    }
     mp[0] = make_uint4(p, a.qo[p], a.to[p], (uint32_t)a.qlen[p]);
     mp[1] = make_uint4((uint32_t)a.tlen[p], (uint32_t)a.h0[p], (uint32_t)a.w[p], 0u);
+}
+
+__global__ __launch_bounds__(256) void params_pack_kernel(ExtArgs a, uint32_t n, uint32_t *params) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint4 *mp = reinterpret_cast<uint4 *>(params) + 2 * (size_t)k;
+    mp[0] = make_uint4(k, a.qo[k], a.to[k], (uint32_t)a.qlen[k]);
+    mp[1] = make_uint4((uint32_t)a.tlen[k], (uint32_t)a.h0[k], (uint32_t)a.w[k], 0u);
 }
 
 hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(meta_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    return hipGetLastError();
+}
+
+hipError_t params_pack(const ExtArgs &a, uint32_t n, uint32_t *params, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(params_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n, params);
     return hipGetLastError();
 }
 

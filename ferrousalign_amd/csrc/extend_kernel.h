@@ -27,6 +27,8 @@ struct ExtArgs {
     const int32_t *w;
     const uint32_t *order;
     uint32_t *meta;      // 8 words per packed position: pid, qo, to, qlen, tlen, h0, w, 0
+    const uint32_t *params;  // optional: the same 8 words per pair in job order (one 32-byte
+                             // read per pair for meta_gather instead of six scattered ones)
     int32_t *out;
     uint32_t *cells;
     uint32_t *queue;     // persistent launch: work counter (one per launch)
@@ -53,9 +55,11 @@ hipError_t extend_launch_persistent(int L, int S, const ExtArgs &a, uint32_t fir
 hipError_t queue_reset(uint32_t *q, uint32_t n, hipStream_t st);
 
 // Fills a.meta for packed positions [0, n) from a.order and the per-pair
-// arrays, so that a wavefront reads its pairs' parameters as one contiguous
-// block.
+// arrays (or a.params when set), so that a wavefront reads its pairs'
+// parameters as one contiguous block.
 hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st);
+// Writes params (8 words per pair, job order) from the per-pair arrays of a.
+hipError_t params_pack(const ExtArgs &a, uint32_t n, uint32_t *params, hipStream_t st);
 
 // 16-bit packed kernel (two pairs per lane group): slot count for L lanes,
 // or -1 (wide: the pair may take the slot counts above 16, which need

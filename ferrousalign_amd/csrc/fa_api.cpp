@@ -238,7 +238,7 @@ void batch_release(fa_batch *b) {
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
-                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev, b->d_long};
+                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev, b->d_long, b->d_params};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -900,6 +900,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         (rc = dev_upload(&b->d_w, b->w.data(), b->w.size(), st)) ||
         (rc = dev_upload(&b->d_elig, b->elig16.data(), b->elig16.size(), st)) ||
         (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
+        (rc = dev_alloc(&b->d_params, (size_t)n * 8)) ||
         (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
         (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64)) ||
         (b->nlong > 0 &&
@@ -952,6 +953,14 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     b->tmax = tmax;
     a.tpad = ((tmax + 16) / 16) * 16;
     a.chunk = 0;
+    // the per-pair records meta_gather reads, one 32-byte line per pair
+    a.params = nullptr;
+    if (fa::params_pack(a, (uint32_t)n, b->d_params, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        batch_release(b);
+        return FA_ERR_DEVICE;
+    }
+    a.params = b->d_params;
     *out = b;
     return FA_OK;
 }

@@ -70,6 +70,7 @@ struct fa_batch {
     uint8_t *d_elig = nullptr;   // fits16 of each pair (device planner)
     uint32_t *d_order = nullptr;
     uint32_t *d_meta = nullptr;  // per packed slot: pid, qo, to, qlen, tlen, h0, w, 0
+    uint32_t *d_params = nullptr;  // the same 8 words per pair in job order (meta_gather's source)
     int32_t *d_out = nullptr;
     uint32_t *d_cells = nullptr;
     uint32_t *d_queue = nullptr;

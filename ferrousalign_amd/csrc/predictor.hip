@@ -68,10 +68,24 @@ __device__ __forceinline__ uint32_t even_bits(uint32_t x) {
     return (x | (x >> 8)) & 0x0000FFFFu;
 }
 
+// 32 bases of a sequence from base d (0..3) on: 2-bit words w0, w1, w2 (bases
+// 0-15, 16-31, 32-47) as 64 bits, N-mask words m0, m1 (bases 0-31, 32-63)
+// as 32 bits.
+__device__ __forceinline__ uint64_t window2(uint32_t w0, uint32_t w1, uint32_t w2, int d) {
+    const uint64_t x = ((uint64_t)w1 << 32) | w0;
+    return d ? (x >> (2 * d)) | ((uint64_t)w2 << (64 - 2 * d)) : x;
+}
+__device__ __forceinline__ uint32_t window_n(uint32_t m0, uint32_t m1, int d) {
+    return d ? (m0 >> d) | (m1 << (32 - d)) : m0;
+}
+
 // Raw features per pair from the batch's 2-bit sequences in HBM, word-wide:
-// the first 32 bases of query and target are two 2-bit words and one N-mask
-// word each; base j mismatches when exactly one of the two is N, or neither
-// is and the 2-bit codes differ.
+// qlen, tlen, h0, w, mismatches among the first lim = min(32, qlen, tlen)
+// bases of the best diagonal within three of the main one (query window from
+// base max(0, -d), target window from base max(0, d), taken when both lie
+// inside their sequences; an indel near the start leaves few mismatches on a
+// neighbouring diagonal), query GC fraction.  Base j mismatches when exactly
+// one of the two is N, or neither is and the 2-bit codes differ.
 __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n, float *feat) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -83,12 +97,24 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n
     if (lim > 0) {  // both sequences have at least one 32-base block (two words)
         const uint2 q = *reinterpret_cast<const uint2 *>(a.seq2 + qo);
         const uint2 t = *reinterpret_cast<const uint2 *>(a.seq2 + to);
-        const uint32_t qn = a.seqn[qo >> 1], tn = a.seqn[to >> 1];
-        const uint32_t d0 = q.x ^ t.x, d1 = q.y ^ t.y;
-        const uint32_t code = even_bits(d0 | (d0 >> 1)) | (even_bits(d1 | (d1 >> 1)) << 16);
-        const uint32_t mism = (qn ^ tn) | (~(qn | tn) & code);
+        // the next block only where the sequence has one (shifted windows reach
+        // at most base 34; below a sequence's end the missing bits are masked)
+        const uint32_t q2 = qlen > 32 ? a.seq2[qo + 2] : 0u, t2 = tlen > 32 ? a.seq2[to + 2] : 0u;
+        const uint32_t qn0 = a.seqn[qo >> 1], tn0 = a.seqn[to >> 1];
+        const uint32_t qn1 = qlen > 32 ? a.seqn[(qo >> 1) + 1] : 0u, tn1 = tlen > 32 ? a.seqn[(to >> 1) + 1] : 0u;
         const uint32_t lm = lim == 32 ? 0xFFFFFFFFu : ((1u << lim) - 1u);
-        mm = __popc(mism & lm);
+        mm = 32;
+#pragma unroll
+        for (int d = -3; d <= 3; ++d) {
+            const int qs = d < 0 ? -d : 0, ts = d > 0 ? d : 0;
+            if (qs + lim > qlen || ts + lim > tlen) continue;
+            const uint64_t x = window2(q.x, q.y, q2, qs) ^ window2(t.x, t.y, t2, ts);
+            const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+            const uint32_t code = even_bits(lo | (lo >> 1)) | (even_bits(hi | (hi >> 1)) << 16);
+            const uint32_t qn = window_n(qn0, qn1, qs), tn = window_n(tn0, tn1, ts);
+            const uint32_t mism = (qn ^ tn) | (~(qn | tn) & code);
+            mm = min(mm, __popc(mism & lm));
+        }
     }
     // C (01) and G (10) are the 2-bit codes whose two bits differ; N is 00
     int gc = 0;

@@ -12,7 +12,8 @@ the fallback when no trained network is available.
 
 Features (computed on the device from the resident 2-bit sequences): qlen,
 tlen, h0, w after end-bonus capping, mismatches in the first 32 bp of the
-ungapped diagonal, query GC fraction.
+best diagonal within three of the main one (an indel near the start leaves
+the main diagonal looking like a random target), query GC fraction.
 """
 import ctypes
 import os

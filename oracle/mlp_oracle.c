@@ -128,14 +128,24 @@ double ora_mlp_grad(const float *params, const float *norm, const float *feat,
 }
 
 /* Raw features of one extension pair from u8 codes:
- * qlen, tlen, h0, w, mismatches among the first min(32, qlen, tlen) positions
- * of the ungapped diagonal, query GC fraction. */
+ * qlen, tlen, h0, w, mismatches among the first lim = min(32, qlen, tlen)
+ * positions of the best diagonal within three of the main one, query GC
+ * fraction.  Diagonal d >= 0 compares q[j] with t[j + d], d < 0 compares
+ * q[j - d] with t[j], j < lim, and is taken only when both windows lie inside
+ * their sequences (an indel near the start leaves few mismatches on a
+ * neighbouring diagonal, where the main one looks like a random target). */
 void ora_pair_features(const uint8_t *q, int32_t qlen, const uint8_t *t, int32_t tlen, int32_t h0,
                        int32_t w, float *out) {
     int lim = qlen < tlen ? qlen : tlen;
     if (lim > 32) lim = 32;
-    int mm = 0;
-    for (int j = 0; j < lim; ++j) mm += q[j] != t[j];
+    int mm = 1 << 30;
+    for (int d = -3; d <= 3; ++d) {
+        const int qo = d < 0 ? -d : 0, to = d > 0 ? d : 0;
+        if (qo + lim > qlen || to + lim > tlen) continue;
+        int c = 0;
+        for (int j = 0; j < lim; ++j) c += q[qo + j] != t[to + j];
+        if (c < mm) mm = c;
+    }
     int gc = 0;
     for (int j = 0; j < qlen; ++j) gc += q[j] == 1 || q[j] == 2;
     out[0] = (float)qlen;

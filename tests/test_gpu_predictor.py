@@ -173,7 +173,7 @@ def _wavefront_idle_cells(db, flags, cells):
 def test_cost_order_packing_lowers_idle_cells():
     """Descending predicted cells per kernel (FA_PACK_COST_ORDER, the learned
     path's packing): on held-out pairs the idle cell-slots of the wavefronts
-    must fall below length sorting's (measured 0.85 of it), and lowest with
+    must fall below length sorting's (measured 0.73 of it), and lowest with
     the true cells."""
     from ferrousalign_amd import synth, _native as N
     from ferrousalign_amd.banded_swa import DeviceBatch, Context

@@ -74,6 +74,34 @@ def test_features_direct():
         t = rng.integers(0, 5, tl).astype(np.uint8)
         f = O.pair_features(q, t, 33, 100)
         lim = min(32, ql, tl)
-        assert f[4] == np.count_nonzero(q[:lim] != t[:lim])
+        assert f[4] <= np.count_nonzero(q[:lim] != t[:lim])  # the main diagonal is a candidate
         assert f[5] == np.float32(np.count_nonzero((q == 1) | (q == 2))) / np.float32(ql)
         assert tuple(f[:4]) == (ql, tl, 33, 100)
+
+
+def test_pair_features_best_diagonal():
+    """Feature 4: mismatches in the first min(32, qlen, tlen) bases on the
+    best diagonal within three of the main one.  An exact copy gives 0; a
+    read with two bases deleted after base 3 mismatches on the main diagonal
+    (a random-looking count) and matches from base 3 on the diagonal two to
+    the right; a random target stays high on every diagonal."""
+    rng = np.random.default_rng(9)
+    t = rng.integers(0, 4, 200).astype(np.uint8)
+    f = O.pair_features(t[:120], t, 30, 100)
+    assert f[4] == 0 and f[0] == 120 and f[1] == 200
+    q = np.concatenate([t[:3], t[5:125]])          # bases 3, 4 of the target deleted in the read
+    main = int((q[:32] != t[:32]).sum())
+    f = O.pair_features(q, t, 30, 100)
+    best = int((q[3:35] != t[5:37]).sum())          # diagonal +2 from base 3 on (q[j] vs t[j + 2])
+    assert main >= 12 and f[4] == int((q[:32] != t[2:34]).sum()) <= 3 + best
+    r = rng.integers(0, 4, 200).astype(np.uint8)
+    assert O.pair_features(r[:120], t, 30, 100)[4] >= 12
+    # brute force over random short pairs, N bases included
+    for _ in range(300):
+        ql, tl = int(rng.integers(0, 45)), int(rng.integers(0, 45))
+        qq = rng.integers(0, 5, ql).astype(np.uint8)
+        tt = rng.integers(0, 5, tl).astype(np.uint8)
+        lim = min(32, ql, tl)
+        want = min(int((qq[max(0, -d):max(0, -d) + lim] != tt[max(0, d):max(0, d) + lim]).sum())
+                   for d in range(-3, 4) if max(0, -d) + lim <= ql and max(0, d) + lim <= tl)
+        assert O.pair_features(qq, tt, 1, 1)[4] == want

@@ -241,6 +241,7 @@ void batch_release(fa_batch *b) {
                     b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev, b->d_long, b->d_params};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (b->h_hist) (void)hipHostFree(b->h_hist);
     delete b;
 }
 
@@ -263,33 +264,40 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     if (by_cost && !b->cost_set) return FA_ERR_ARG;
     // kernel of a pair, (kind << 16) | (lanes << 8) | slots, as a function of
     // its fits16 level and qlen (-1: no kernel holds it); merged buckets (one
-    // per kernel kind) take the kind alone
+    // per kernel kind) take the kind alone.  Kept between plans.
     const bool merged = pack == FA_PACK_NONE || single;
-    int tab[3][256];
-    std::vector<int> kinds;  // distinct keys, descending: the bucket order
-    for (int f = 0; f < 3; ++f)
-        for (int q = 0; q < 256; ++q) {
-            int kind, lanes, S;
-            const bool ok = q > 0 && fa_host::pair_kernel(L, wide != 0, f, q, &kind, &lanes, &S);
-            tab[f][q] = !ok ? -1 : (merged ? kind << 16 : (kind << 16) | (lanes << 8) | S);
-            if (tab[f][q] >= 0) kinds.push_back(tab[f][q]);
-        }
-    std::sort(kinds.begin(), kinds.end(), [](int x, int y) { return x > y; });
-    kinds.erase(std::unique(kinds.begin(), kinds.end()), kinds.end());
+    const uint32_t tkey = (uint32_t)L << 8 | (wide ? 2u : 0u) | (merged ? 1u : 0u);
+    if (b->tab_key != tkey) {
+        std::vector<int> kinds;  // distinct keys, descending: the bucket order
+        for (int f = 0; f < 3; ++f)
+            for (int q = 0; q < 256; ++q) {
+                int kind, lanes, S;
+                const bool ok = q > 0 && fa_host::pair_kernel(L, wide != 0, f, q, &kind, &lanes, &S);
+                b->tab[f][q] = !ok ? -1 : (merged ? kind << 16 : (kind << 16) | (lanes << 8) | S);
+                if (b->tab[f][q] >= 0) kinds.push_back(b->tab[f][q]);
+            }
+        std::sort(kinds.begin(), kinds.end(), [](int x, int y) { return x > y; });
+        kinds.erase(std::unique(kinds.begin(), kinds.end()), kinds.end());
+        for (int f = 0; f < 3; ++f)
+            for (int q = 0; q < 256; ++q)
+                if (b->tab[f][q] >= 0)
+                    b->tab[f][q] = (int)(std::find(kinds.begin(), kinds.end(), b->tab[f][q]) - kinds.begin());
+        b->kinds.swap(kinds);
+        b->tab_key = tkey;
+    }
+    const std::vector<int> &kinds = b->kinds;  // b->tab now holds indices into kinds
     const size_t nk = kinds.size();
-    fa::PlanTable t;
-    for (int f = 0; f < 3; ++f)
-        for (int q = 0; q < 256; ++q)
-            t.kidx[f][q] = (int16_t)(tab[f][q] < 0 ? -1
-                                                   : (int)(std::find(kinds.begin(), kinds.end(), tab[f][q]) -
-                                                           kinds.begin()));
-    // device buffers (grow-only): histogram, bad flag, keys in/out, ids, sort scratch
+    // device buffers (grow-only): histogram, keys in/out (64-bit room), ids, sort scratch
     constexpr size_t kBins = 768;
-    size_t tb = 0;
-    FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (unsigned long long *)nullptr,
+    size_t tb32 = 0, tb64 = 0;
+    FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb32, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (int)std::max<int64_t>(n, 1), 0, 32, ctx->stream));
+    FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb64, (unsigned long long *)nullptr,
                                               (unsigned long long *)nullptr, (uint32_t *)nullptr,
                                               (uint32_t *)nullptr, (int)std::max<int64_t>(n, 1), 0, 34,
                                               ctx->stream));
+    size_t tb = std::max(tb32, tb64);
     const size_t hist_bytes = kBins * (4 + 8 + 8) + 256;
     const size_t need = hist_bytes + 16 * (size_t)n + 4 * (size_t)n + tb + 512;
     if (b->sort_bytes < need) {
@@ -299,15 +307,14 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
         FA_HIP(hipMalloc(&b->d_sort, need));
         b->sort_bytes = need;
     }
+    if (!b->h_hist) FA_HIP(hipHostMalloc(&b->h_hist, kBins * 12, hipHostMallocDefault));
     uint8_t *base = (uint8_t *)b->d_sort;
     fa::PlanHist h;
     h.rows = (unsigned long long *)base;
     h.csum = h.rows + kBins;
     h.cnt = (uint32_t *)(h.csum + kBins);
-    uint32_t *bad = h.cnt + kBins;
-    unsigned long long *kin = (unsigned long long *)(base + hist_bytes);
-    unsigned long long *kout = kin + n;
-    uint32_t *vin = (uint32_t *)(kout + n);
+    uint8_t *kbase = base + hist_bytes;
+    uint32_t *vin = (uint32_t *)(kbase + 16 * (size_t)n);
     void *temp = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
     fa::PlanArgs pa;
     pa.n = (uint32_t)n;
@@ -319,49 +326,75 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     pa.tlen = b->d_tlen;
     pa.elig = b->d_elig;
     pa.cost = by_cost ? b->d_pred : nullptr;
+    // the histogram first: the launches, their ranks and the key width come
+    // from it, so the rest of the plan needs no further readback
+    FA_HIP(hipMemsetAsync(base, 0, hist_bytes, ctx->stream));
+    FA_HIP(fa::plan_hist(pa, h, ctx->cus, ctx->stream));
+    uint32_t *cnt = (uint32_t *)b->h_hist;
+    unsigned long long *rows = (unsigned long long *)(cnt + kBins);
+    FA_HIP(hipMemcpyAsync(cnt, h.cnt, kBins * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipMemcpyAsync(rows, h.rows, kBins * 8, hipMemcpyDeviceToHost, ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    // launch ranks of the kernels the batch uses, in kernel order
+    std::vector<int> rank(nk, -1);
+    std::vector<uint64_t> kcount(nk, 0), krows(nk, 0);
+    std::vector<int> kmaxq(nk, 1), kminf(nk, 2);
+    for (int f = 0; f < 3; ++f)
+        for (int q = 1; q < 256; ++q) {
+            const uint32_t c = cnt[(size_t)f * 256 + (size_t)q];
+            if (!c) continue;
+            const int x = b->tab[f][q];
+            if (x < 0) return FA_ERR_QLEN;  // a pair no kernel holds
+            kcount[(size_t)x] += c;
+            krows[(size_t)x] += rows[(size_t)f * 256 + (size_t)q];
+            kmaxq[(size_t)x] = std::max(kmaxq[(size_t)x], q);
+            kminf[(size_t)x] = std::min(kminf[(size_t)x], f);
+        }
+    int nr = 0;
+    for (size_t x = 0; x < nk; ++x)
+        if (kcount[x]) rank[x] = nr++;
+    pa.long_rank = nr;
+    const int nranks = nr + (b->nlong > 0 ? 1 : 0);
+    if (nranks > 64) return FA_ERR_ARG;
+    fa::PlanTable t;
+    for (int f = 0; f < 3; ++f)
+        for (int q = 0; q < 256; ++q) t.kidx[f][q] = (int16_t)(b->tab[f][q] < 0 ? -1 : rank[(size_t)b->tab[f][q]]);
     // cost order where a lane group holds 64 or more columns: there a
     // wavefront of pairs with equal cost ends together; in the narrow kernels
     // (short queries, a few hundred cells) the prediction's error is too large
     // a part of the cost and length order does better
-    for (size_t x = 0; x < 64; ++x) {
-        int L = 0, S = 0;
-        if (x < nk && (kinds[x] & 0xFFFF) != 0) {
-            L = (kinds[x] >> 8) & 0xFF;
-            S = kinds[x] & 0xFF;
+    std::memset(t.cost_order, 0, sizeof(t.cost_order));
+    for (size_t x = 0; x < nk; ++x)
+        if (rank[x] >= 0 && (kinds[x] & 0xFFFF) != 0) {
+            const int kl = (kinds[x] >> 8) & 0xFF, ks = kinds[x] & 0xFF;
+            t.cost_order[rank[x]] = (uint8_t)(kl * ks >= 64 ? 1 : 0);
         }
-        t.cost_order[x] = (uint8_t)(L * S >= 64 ? 1 : 0);
-    }
-    pa.long_rank = (int)nk;
-    FA_HIP(hipMemsetAsync(base, 0, hist_bytes, ctx->stream));
-    FA_HIP(fa::plan_hist(pa, h, ctx->cus, ctx->stream));
-    std::vector<uint32_t> cnt(kBins + 1);
-    std::vector<unsigned long long> rows(kBins);
-    FA_HIP(hipMemcpyAsync(rows.data(), h.rows, kBins * 8, hipMemcpyDeviceToHost, ctx->stream));
+    // keys: rank << 27 | order inside the launch; 32 bits hold 32 ranks
+    int rbits = 0;
+    while ((1 << rbits) < nranks) ++rbits;
+    const int end_bit = fa::kRankShift + rbits;
     if (n > 0) {
-        FA_HIP(fa::plan_keys(pa, h, t, kin, vin, bad, ctx->stream));
-        FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, b->d_order, (int)n, 0, 34,
-                                                  ctx->stream));
+        if (end_bit <= 32) {
+            uint32_t *kin = (uint32_t *)kbase, *kout = kin + n;
+            FA_HIP(fa::plan_keys(pa, h, t, kin, vin, ctx->stream));
+            FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, b->d_order, (int)n, 0, end_bit,
+                                                      ctx->stream));
+        } else {
+            unsigned long long *kin = (unsigned long long *)kbase, *kout = kin + n;
+            FA_HIP(fa::plan_keys(pa, h, t, kin, vin, ctx->stream));
+            FA_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, kin, kout, vin, b->d_order, (int)n, 0, end_bit,
+                                                      ctx->stream));
+        }
     }
-    FA_HIP(hipMemcpyAsync(cnt.data(), h.cnt, (kBins + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    // the launch metadata; the launches that follow on this stream wait for it
     FA_HIP(fa::meta_gather(b->args, (uint32_t)n, ctx->stream));
-    FA_HIP(hipStreamSynchronize(ctx->stream));
-    if (cnt[kBins]) return FA_ERR_QLEN;  // a pair no kernel holds
-    // buckets: the kernel keys in rank order with their pair counts
+    // buckets: the used kernels in rank order with their pair counts
     std::vector<Bucket> buckets;
     uint32_t first = 0;
     for (size_t x = 0; x < nk; ++x) {
-        uint64_t count = 0, nrows = 0;
-        int maxq = 1, minf = 2;
-        for (int f = 0; f < 3; ++f)
-            for (int q = 1; q < 256; ++q) {
-                const uint32_t c = cnt[(size_t)f * 256 + (size_t)q];
-                if (!c || t.kidx[f][q] != (int)x) continue;
-                count += c;
-                nrows += rows[(size_t)f * 256 + (size_t)q];
-                maxq = std::max(maxq, q);
-                minf = std::min(minf, f);
-            }
-        if (!count) continue;
+        if (rank[x] < 0) continue;
+        const uint64_t count = kcount[x], nrows = krows[x];
+        const int maxq = kmaxq[x], minf = kminf[x];
         const int key0 = kinds[x];
         Bucket bk;
         bk.kind = key0 >> 16;
@@ -1627,9 +1660,11 @@ int fa_batch_get_order(fa_ctx *ctx, fa_batch *b, uint32_t flags, uint32_t *order
     }
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
+    // the plan's sort runs on the context's stream (non-blocking): copy there
     if (b->n)
-        FA_HIP(hipMemcpy(order, b->d_order, (size_t)b->n * sizeof(uint32_t),
-                         hipMemcpyDeviceToHost));
+        FA_HIP(hipMemcpyAsync(order, b->d_order, (size_t)b->n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
     return FA_OK;
 }
 

@@ -76,6 +76,13 @@ struct fa_batch {
     uint32_t *d_queue = nullptr;
     void *d_sort = nullptr;      // packing sort: keys in/out, pair ids, scratch
     size_t sort_bytes = 0;
+    void *h_hist = nullptr;      // pinned host copy of the packing histogram
+    // kernel table of the packing plan for (lanes, FA_SCORE_I32, merged),
+    // kept between plans: kernel key per (fit level, qlen) and the distinct
+    // keys in launch order
+    uint32_t tab_key = 0xFFFFFFFFu;
+    int tab[3][256];
+    std::vector<int> kinds;
     fa::ExtArgs args{};
     // launch plan for the current packing
     uint32_t packed_flags = 0xFFFFFFFFu;

@@ -15,7 +15,7 @@ struct PlanArgs {
     int mode;
     const int32_t *qlen, *tlen;
     const uint8_t *elig;  // 16-bit fit level per pair (0..2), 3: long pair
-    const float *cost;    // per-pair packing cost (kPlanCost), else null
+    const float *cost;    // per-pair packing cost (kPlanCost, kPlanCostOrder), else null
     int long_rank;        // launch rank of the long pairs (after every lane-group kernel)
 };
 
@@ -26,12 +26,15 @@ struct PlanHist {
     unsigned long long *csum;  // sum of cost_fixed(cost)
 };
 
-// Kernel index of each (fit level, qlen): position of the pair's kernel in
-// the launch order, -1 where no kernel holds it.
+// Launch rank of each (fit level, qlen): the position of the pair's kernel
+// among the kernels the batch uses, -1 where no kernel holds it.
 struct PlanTable {
     int16_t kidx[3][256];
-    uint8_t cost_order[64];  // kPlanCostOrder: kernel x orders its pairs by descending cost
+    uint8_t cost_order[64];  // kPlanCostOrder: rank x orders its pairs by descending cost
 };
+
+// Sort keys: launch rank << kRankShift | a 27-bit order inside the launch.
+constexpr int kRankShift = 27;
 
 // Packing cost in 1/16 cells, non-negative and saturating at 2^24 cells
 // (a pair has at most 256 x 2048), so a sum over 2^31 pairs cannot wrap.
@@ -42,8 +45,11 @@ __host__ __device__ inline unsigned long long cost_fixed(float c) {
 }
 
 hipError_t plan_hist(const PlanArgs &a, const PlanHist &h, int cus, hipStream_t st);
-// keys[k] and ids[k] = k for the stable sort; *bad |= 1 for a pair no kernel holds.
+// keys[k] and ids[k] = k for the stable sort (32-bit keys when the batch
+// uses at most 32 launch ranks, else 64-bit).
+hipError_t plan_keys(const PlanArgs &a, const PlanHist &h, const PlanTable &t, uint32_t *keys,
+                     uint32_t *ids, hipStream_t st);
 hipError_t plan_keys(const PlanArgs &a, const PlanHist &h, const PlanTable &t, unsigned long long *keys,
-                     uint32_t *ids, uint32_t *bad, hipStream_t st);
+                     uint32_t *ids, hipStream_t st);
 
 }  // namespace fa

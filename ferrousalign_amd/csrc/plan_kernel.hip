@@ -7,8 +7,10 @@
 //   plan_hist_kernel  pairs, target rows and (cost packing) fixed-point cost
 //                     sums per bin, block histograms in LDS, one atomic per
 //                     bin and block;
-//   plan_key_kernel   one 64-bit key per pair, ascending = launch order:
-//                     kernel rank, then (cost packing) the early-stop flag and
+//   plan_key_kernel   one key per pair, ascending = launch order: launch rank
+//                     (the kernel's position among the kernels the batch
+//                     uses, read from the histogram on the host; 32-bit keys
+//                     for at most 32 ranks), then (cost packing) the early-stop flag and
 //                     cost class of a pair whose cost is below half the mean
 //                     of its (kernel, qlen) group, then qlen and tlen
 //                     descending (cost-order packing: descending cost inside
@@ -54,27 +56,23 @@ __global__ __launch_bounds__(1024) void plan_hist_kernel(PlanArgs a, PlanHist h)
     }
 }
 
-__global__ __launch_bounds__(256) void plan_key_kernel(PlanArgs a, PlanHist h, PlanTable t,
-                                                       unsigned long long *keys, uint32_t *ids,
-                                                       uint32_t *bad) {
+template <class K>
+__global__ __launch_bounds__(256) void plan_key_kernel(PlanArgs a, PlanHist h, PlanTable t, K *keys,
+                                                       uint32_t *ids) {
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
     if (k >= a.n) return;
     ids[k] = k;
     const int q = a.qlen[k];
     const int f = a.elig[k];
+    const uint32_t len20 = ((uint32_t)(255 - min(q, 255)) << 12) | (uint32_t)(4095 - min(a.tlen[k], 4095));
     if (f > 2) {
         // long pair: after every lane-group kernel, longest query first
-        keys[k] = ((unsigned long long)a.long_rank << 28) | ((unsigned long long)(65535 - min(q, 65535)) << 12) |
-                  (unsigned long long)(4095 - min(a.tlen[k], 4095));
+        keys[k] = ((K)a.long_rank << kRankShift) | ((K)(65535 - min(q, 65535)) << 11) |
+                  (K)(2047 - min(a.tlen[k] >> 5, 2047));
         return;
     }
-    const int x = (q >= 1 && q <= 255) ? t.kidx[f][q] : -1;
-    if (x < 0) {
-        atomicOr(bad, 1u);
-        keys[k] = 0ull;
-        return;
-    }
-    unsigned long long key = (unsigned long long)x << 28;
+    const int x = t.kidx[f][q];  // >= 0: the host checked the histogram
+    uint32_t low = 0;
     if (a.mode == kPlanCost) {
         // the (kernel, qlen) group: every fit level whose kernel is x at this qlen
         unsigned long long gs = 0ull, gc = 0ull;
@@ -84,20 +82,22 @@ __global__ __launch_bounds__(256) void plan_key_kernel(PlanArgs a, PlanHist h, P
                 gc += h.cnt[g * 256 + q];
             }
         const float c = fmaxf(a.cost[k], 0.0f);
+        low = len20;
         if ((unsigned long long)cost_fixed(c) * 2ull * gc < gs) {
+            // an early stopper: after the others, by cost class (e^(1/2) apart,
+            // at most 28 for 2^20 cells), highest first
             const int cls = (int)rint(2.0 * log1p((double)c));
-            key |= (1ull << 27) | (unsigned long long)(127 - min(cls, 127)) << 20;
+            low |= (1u << 25) | ((uint32_t)(31 - min(cls, 31)) << 20);
         }
-    }
-    if (a.mode == kPlanCostOrder && t.cost_order[x]) {
-        // descending cost (cells, saturating at 2^20 - 1; a pair has at most
+    } else if (a.mode == kPlanCostOrder && t.cost_order[x]) {
+        // descending cost (cells, saturating at 2^19 - 1; a pair has at most
         // 256 x 2048), then qlen descending
-        const float c = fminf(fmaxf(a.cost[k], 0.0f), 1048575.0f);
-        key |= ((unsigned long long)(1048575u - (uint32_t)rintf(c)) << 8) | (unsigned long long)(255 - q);
+        const float c = fminf(fmaxf(a.cost[k], 0.0f), 524287.0f);
+        low = ((524287u - (uint32_t)rintf(c)) << 8) | (uint32_t)(255 - q);
     } else if (a.mode != kPlanNone) {
-        key |= ((unsigned long long)(255 - q) << 12) | (unsigned long long)(4095 - min(a.tlen[k], 4095));
+        low = len20;
     }
-    keys[k] = key;
+    keys[k] = ((K)x << kRankShift) | (K)low;
 }
 
 }  // namespace
@@ -111,10 +111,18 @@ hipError_t plan_hist(const PlanArgs &a, const PlanHist &h, int cus, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t plan_keys(const PlanArgs &a, const PlanHist &h, const PlanTable &t, unsigned long long *keys,
-                     uint32_t *ids, uint32_t *bad, hipStream_t st) {
+hipError_t plan_keys(const PlanArgs &a, const PlanHist &h, const PlanTable &t, uint32_t *keys, uint32_t *ids,
+                     hipStream_t st) {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(plan_key_kernel, dim3((a.n + 255) / 256), dim3(256), 0, st, a, h, t, keys, ids, bad);
+    hipLaunchKernelGGL(plan_key_kernel<uint32_t>, dim3((a.n + 255) / 256), dim3(256), 0, st, a, h, t, keys, ids);
+    return hipGetLastError();
+}
+
+hipError_t plan_keys(const PlanArgs &a, const PlanHist &h, const PlanTable &t, unsigned long long *keys,
+                     uint32_t *ids, hipStream_t st) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(plan_key_kernel<unsigned long long>, dim3((a.n + 255) / 256), dim3(256), 0, st, a, h, t,
+                       keys, ids);
     return hipGetLastError();
 }
 

@@ -2086,13 +2086,12 @@ int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32
     if (!ctx || !m || !b) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     int rc;
-    if ((set_cost & 2) && b->d_feat) {  // features of a new batch: computed again, in place
-        FA_HIP(fa::mlp_launch_features(b->args, (uint32_t)b->n, b->d_feat, ctx->stream));
-    } else if ((rc = ensure_features(ctx, b))) {
-        return rc;
-    }
     if (!b->d_pred && (rc = dev_alloc(&b->d_pred, (size_t)b->n))) return rc;
-    FA_HIP(fa::mlp_launch_forward(m->wh, m->norm, b->d_feat, (uint32_t)b->n, b->d_pred, nullptr,
+    // a new batch's features (set_cost & 2, or none yet) are computed in the
+    // same kernel as the forward pass and kept for later calls
+    const bool fresh = (set_cost & 2) || !b->d_feat;
+    if (!b->d_feat && (rc = dev_alloc(&b->d_feat, (size_t)b->n * fa::kMlpFeatures))) return rc;
+    FA_HIP(fa::mlp_launch_predict(b->args, m->wh, m->norm, b->d_feat, !fresh, (uint32_t)b->n, b->d_pred,
                                   ctx->stream));
     if (cells_pred && b->n)
         FA_HIP(hipMemcpyAsync(cells_pred, b->d_pred, (size_t)b->n * sizeof(float), hipMemcpyDeviceToHost,

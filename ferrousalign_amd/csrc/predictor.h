@@ -12,8 +12,11 @@ constexpr int kMlpFeatures = 6;
 constexpr int kMlpBlock = 64;     // pairs per backward block
 
 hipError_t mlp_launch_features(const ExtArgs &a, uint32_t n, float *feat, hipStream_t st);
-hipError_t mlp_launch_forward(const void *wh, const float *norm, const float *feat, uint32_t n,
-                              float *cells_pred, float *y, hipStream_t st);
+// Predicted cells of pairs [0, n): from the raw features in feat
+// (from_feat), or computed from the batch's sequences in a (and written to
+// feat when it is not null).
+hipError_t mlp_launch_predict(const ExtArgs &a, const void *wh, const float *norm, float *feat,
+                              bool from_feat, uint32_t n, float *cells_pred, hipStream_t st);
 hipError_t mlp_launch_backward(const void *wh, const float *norm, const float *feat,
                                const uint32_t *cells, const uint32_t *perm, uint32_t first,
                                uint32_t count, float *partial, hipStream_t st);

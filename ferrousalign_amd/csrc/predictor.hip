@@ -86,9 +86,7 @@ __device__ __forceinline__ uint32_t window_n(uint32_t m0, uint32_t m1, int d) {
 // inside their sequences; an indel near the start leaves few mismatches on a
 // neighbouring diagonal), query GC fraction.  Base j mismatches when exactly
 // one of the two is N, or neither is and the 2-bit codes differ.
-__global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n, float *feat) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void pair_features(const ExtArgs &a, uint32_t k, float *o) {
     const int qlen = a.qlen[k], tlen = a.tlen[k];
     const uint32_t qo = a.qo[k], to = a.to[k];
     int lim = qlen < tlen ? qlen : tlen;
@@ -123,7 +121,6 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n
         const uint32_t w2 = a.seq2[qo + x];
         gc += __popc((w2 ^ (w2 >> 1)) & 0x55555555u);
     }
-    float *o = feat + (size_t)k * NIN;
     o[0] = (float)qlen;
     o[1] = (float)tlen;
     o[2] = (float)a.h0[k];
@@ -132,59 +129,122 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n
     o[5] = qlen > 0 ? (float)gc / (float)qlen : 0.f;
 }
 
-typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ h2v pack_h2(float a, float b) {
-    h2v r;
-    r.x = (_Float16)a;
-    r.y = (_Float16)b;
-    return r;
-}
-
-// Predicted cells (expm1 of the network output) per pair.  Same network as
-// mlp_forward1, two fp16 products per v_dot2_f32_f16 with fp32 accumulation;
-// the weights are read at wave-uniform addresses (scalar loads).
-__global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *__restrict__ wg,
-                                                          const float *__restrict__ normg,
-                                                          const float *feat, uint32_t n,
-                                                          float *cells_pred, float *yout) {
+__global__ __launch_bounds__(256) void mlp_features_kernel(ExtArgs a, uint32_t n, float *feat) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const h2v *w2 = reinterpret_cast<const h2v *>(wg);  // half pairs; every layer block starts even
-    h2v x[NIN / 2];
+    pair_features(a, k, feat + (size_t)k * NIN);
+}
+
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+
+// Row of accumulator r (0..15) of lane l in the 32 x 32 output of
+// v_mfma_f32_32x32x8_f16; its column is l % 32.  The A operand (32 x 8) of
+// lane l is row l % 32, columns 4 (l / 32) .. +3; the B operand (8 x 32) is
+// rows 4 (l / 32) .. +3 of column l % 32.  So accumulators 4s .. 4s+3 of a
+// 32 x 32 output are exactly the B operand of K-step s of the next product.
+__device__ __forceinline__ int mfma_row(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {
+    const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+}
+__device__ __forceinline__ half4v as_half4(uint32_t lo, uint32_t hi) {
+    const uint64_t v = (uint64_t)lo | ((uint64_t)hi << 32);
+    return __builtin_bit_cast(half4v, v);
+}
+
+// Predicted cells (expm1 of the network output) of 64 pairs per wavefront,
+// the 6-32-32-1 network on the matrix cores: with the pairs as the N
+// dimension of v_mfma_f32_32x32x8_f16, layer 1 is one product (K = 6 padded
+// to 8), layer 2 four (K = 32), and layer 1's ReLU'd, fp16-rounded
+// accumulators are layer 2's B operand in place; layer 3 is a 16-term dot
+// per lane and one cross-half add.  Two passes of 32 pairs per wavefront.
+// FROM_FEAT: the raw features come from feat; otherwise they are computed
+// here from the batch's sequences (and written to feat when it is set).
+template <bool FROM_FEAT>
+__global__ __launch_bounds__(256) void mlp_forward_kernel(ExtArgs a, const __half *__restrict__ wg,
+                                                          const float *__restrict__ normg, float *feat,
+                                                          uint32_t n, float *cells_pred) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t tile = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+    if (tile >= n) return;  // whole wavefront idle
+    const int m = lane & 31, kh = 4 * (lane >> 5);
+    // this lane's operands of the weights (same for every pass)
+    half4v A1;
 #pragma unroll
-    for (int i = 0; i < NIN / 2; ++i) {
-        const float f0 = feat[(size_t)k * NIN + 2 * i], f1 = feat[(size_t)k * NIN + 2 * i + 1];
-        x[i] = pack_h2((f0 - normg[2 * i]) * normg[NIN + 2 * i], (f1 - normg[2 * i + 1]) * normg[NIN + 2 * i + 1]);
+    for (int j = 0; j < 4; ++j) A1[j] = kh + j < NIN ? (_Float16)wg[m * NIN + kh + j] : (_Float16)0.0f;
+    half4v A2[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) A2[st][j] = (_Float16)wg[OFF_W2 + m * NH + 8 * st + kh + j];
+    float b1r[16], b2r[16], w3r[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = mfma_row(r, lane);
+        b1r[r] = __half2float(wg[OFF_B1 + row]);
+        b2r[r] = __half2float(wg[OFF_B2 + row]);
+        w3r[r] = __half2float(wg[OFF_W3 + row]);
     }
-    h2v h1[NH / 2];
+    const float b3 = __half2float(wg[OFF_B3]);
+
+    // this lane's pair: raw features, normalised, fp16, two halves per word
+    const uint32_t k = tile + (uint32_t)lane;
+    float f[NIN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (k < n) {
+        if constexpr (FROM_FEAT) {
 #pragma unroll
-    for (int i = 0; i < NH; i += 2) {
-        float a0 = __half2float(wg[OFF_B1 + i]), a1 = __half2float(wg[OFF_B1 + i + 1]);
+            for (int i = 0; i < NIN; ++i) f[i] = feat[(size_t)k * NIN + i];
+        } else {
+            pair_features(a, k, f);
+            if (feat) {
 #pragma unroll
-        for (int m = 0; m < NIN / 2; ++m) {
-            a0 = __builtin_amdgcn_fdot2(w2[(i * NIN) / 2 + m], x[m], a0, false);
-            a1 = __builtin_amdgcn_fdot2(w2[((i + 1) * NIN) / 2 + m], x[m], a1, false);
+                for (int i = 0; i < NIN; ++i) feat[(size_t)k * NIN + i] = f[i];
+            }
         }
-        h1[i / 2] = pack_h2(a0 > 0.f ? a0 : 0.f, a1 > 0.f ? a1 : 0.f);
     }
-    h2v h2[NH / 2];
+    float xn[NIN];
 #pragma unroll
-    for (int i = 0; i < NH; i += 2) {
-        float a0 = __half2float(wg[OFF_B2 + i]), a1 = __half2float(wg[OFF_B2 + i + 1]);
+    for (int i = 0; i < NIN; ++i) xn[i] = (f[i] - normg[i]) * normg[NIN + i];
+    const uint32_t x01 = pack_h2(xn[0], xn[1]), x23 = pack_h2(xn[2], xn[3]), x45 = pack_h2(xn[4], xn[5]);
+
 #pragma unroll
-        for (int m = 0; m < NH / 2; ++m) {
-            a0 = __builtin_amdgcn_fdot2(w2[(OFF_W2 + i * NH) / 2 + m], h1[m], a0, false);
-            a1 = __builtin_amdgcn_fdot2(w2[(OFF_W2 + (i + 1) * NH) / 2 + m], h1[m], a1, false);
+    for (int pass = 0; pass < 2; ++pass) {
+        // B operand of layer 1: features 0-3 (lanes 0-31) or 4-7 (lanes
+        // 32-63) of pair 32 pass + lane % 32
+        const int src = 32 * pass + m;
+        const uint32_t lo0 = (uint32_t)__shfl((int)x01, src, 64), lo1 = (uint32_t)__shfl((int)x23, src, 64);
+        const uint32_t hi0 = (uint32_t)__shfl((int)x45, src, 64);
+        const half4v B1 = lane < 32 ? as_half4(lo0, lo1) : as_half4(hi0, 0u);
+        float16v acc = {};
+        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(A1, B1, acc, 0, 0, 0);
+        // layer 1 bias and ReLU, rounded to fp16: layer 2's B operand in place
+        half4v H1[4];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float v = acc[r] + b1r[r];
+            H1[r >> 2][r & 3] = (_Float16)(v > 0.f ? v : 0.f);
         }
-        h2[i / 2] = pack_h2(a0 > 0.f ? a0 : 0.f, a1 > 0.f ? a1 : 0.f);
-    }
-    float y = __half2float(wg[OFF_B3]);
+        float16v acc2 = {};
 #pragma unroll
-    for (int m = 0; m < NH / 2; ++m) y = __builtin_amdgcn_fdot2(w2[OFF_W3 / 2 + m], h2[m], y, false);
-    if (yout) yout[k] = y;
-    float c = expm1f(y);
-    cells_pred[k] = c > 0.f ? c : 0.f;
+        for (int st = 0; st < 4; ++st) acc2 = __builtin_amdgcn_mfma_f32_32x32x8f16(A2[st], H1[st], acc2, 0, 0, 0);
+        // layer 2 bias, ReLU and fp16 rounding, then layer 3 over this lane's
+        // 16 neurons and the other half's 16
+        float y = 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float v = acc2[r] + b2r[r];
+            y += w3r[r] * (float)(_Float16)(v > 0.f ? v : 0.f);
+        }
+        y += __shfl_xor(y, 32, 64);
+        y += b3;
+        const uint32_t pk = tile + (uint32_t)src;
+        if (lane < 32 && pk < n) {
+            const float c = expm1f(y);
+            cells_pred[pk] = c > 0.f ? c : 0.f;
+        }
+    }
 }
 
 // One minibatch slice of kMlpBlock pairs per block: forward, backward, and the
@@ -316,11 +376,16 @@ hipError_t mlp_launch_features(const ExtArgs &a, uint32_t n, float *feat, hipStr
     return hipGetLastError();
 }
 
-hipError_t mlp_launch_forward(const void *wh, const float *norm, const float *feat, uint32_t n,
-                              float *cells_pred, float *y, hipStream_t st) {
+hipError_t mlp_launch_predict(const ExtArgs &a, const void *wh, const float *norm, float *feat,
+                              bool from_feat, uint32_t n, float *cells_pred, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
-                       (const __half *)wh, norm, feat, n, cells_pred, y);
+    const dim3 grid((n + 255) / 256), block(256);  // 64 pairs per wavefront, 4 wavefronts
+    if (from_feat)
+        hipLaunchKernelGGL(mlp_forward_kernel<true>, grid, block, 0, st, a, (const __half *)wh, norm, feat, n,
+                           cells_pred);
+    else
+        hipLaunchKernelGGL(mlp_forward_kernel<false>, grid, block, 0, st, a, (const __half *)wh, norm, feat, n,
+                           cells_pred);
     return hipGetLastError();
 }
 

@@ -65,6 +65,11 @@ def test_forward_and_gradient_vs_oracle(setup):
     y_ora = O.mlp_forward(pred.params, pred.norm, feat[:4000], fp16=True).astype(np.float64)
     ok = y_ora > 0  # expm1 clamps negative outputs to 0 cells
     assert np.max(np.abs(y_gpu[ok] - y_ora[ok])) < Y_TOL
+    # features computed inside the forward kernel (a new batch) give the same
+    # predictions, bit for bit, as features read back from the feature buffer
+    cf = pred.predict(db, fresh_features=True)
+    assert np.array_equal(cf, cp)
+    assert np.array_equal(pred.features(db), feat)
     print("gradient", flush=True)
     pairs = np.arange(1000, 5000, dtype=np.uint32)
     g, loss = pred.gradient(db, pairs)

@@ -164,19 +164,22 @@ __device__ __forceinline__ uint32_t excl_scan_u2(uint32_t x, int l) {
     return y;
 }
 
+// nfree: the sequence has no N base, its N-mask word is not read
 __device__ __forceinline__ int decode_base(const uint32_t *seq2, const uint32_t *seqn,
-                                           uint32_t off, int pos) {
+                                           uint32_t off, int pos, bool nfree) {
     const uint32_t w2 = seq2[off + (pos >> 4)];
-    const uint32_t wn = seqn[(off >> 1) + (pos >> 5)];
+    const uint32_t wn = nfree ? 0u : seqn[(off >> 1) + (pos >> 5)];
     const int code = (int)((w2 >> ((pos & 15) * 2)) & 3u);
     return ((wn >> (pos & 31)) & 1u) ? 4 : code;
 }
 
 // Stage the target codes of a lane group's two pairs: byte i = code of row i
 // of pair A in bits 0-3 and of pair B in bits 4-7 (0 past a pair's target).
-// Lane l of the group decodes 16-base words l, l+L, ...
+// Lane l of the group decodes 16-base words l, l+L, ...  A target without N
+// (tfree) reads no N-mask words.
 __device__ __forceinline__ void stage_targets2(const ExtArgs &a, uint8_t *tc2, const int (&tlen)[2],
-                                               const uint32_t (&to)[2], int l, int L) {
+                                               const uint32_t (&to)[2], const bool (&tfree)[2], int l,
+                                               int L) {
     const int nw0 = (tlen[0] + 15) >> 4, nw1 = (tlen[1] + 15) >> 4;
     const int nw = max(nw0, nw1);
     for (int k = l; k < nw; k += L) {
@@ -185,7 +188,7 @@ __device__ __forceinline__ void stage_targets2(const ExtArgs &a, uint8_t *tc2, c
         for (int h = 0; h < 2; ++h) {
             const bool in = k < (h ? nw1 : nw0);
             w2[h] = in ? a.seq2[to[h] + k] : 0u;
-            wn[h] = in ? (a.seqn[(to[h] >> 1) + (k >> 1)] >> ((k & 1) * 16)) & 0xFFFFu : 0u;
+            wn[h] = in && !tfree[h] ? (a.seqn[(to[h] >> 1) + (k >> 1)] >> ((k & 1) * 16)) & 0xFFFFu : 0u;
         }
         uint32_t v[4];
 #pragma unroll
@@ -687,6 +690,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     uint8_t *const tc2 = lds_wave + g * a.tpad;
     int cbh[2];
     int wv[2];
+    bool qfree[2], tfree[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const uint32_t slot = slot0 + 2u * (uint32_t)g + (uint32_t)h;
@@ -704,6 +708,8 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         wv[h] = min((int)m1.z, 4095);
         qo[h] = m0.y;
         to[h] = m0.z;
+        qfree[h] = (m1.w & 1u) != 0u;
+        tfree[h] = (m1.w & 2u) != 0u;
         cbh[h] = l * S - (L * S - 1 - ql[h]);  // column of slot 0 in this lane
     }
     P.alive = (valid[0] && tl[0] > 0 ? 0xFFFFu : 0u) | (valid[1] && tl[1] > 0 ? 0xFFFF0000u : 0u);
@@ -719,7 +725,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
     P.max_off = 0u;
     P.ncell[0] = P.ncell[1] = 0u;
     P.nacc = 0u;
-    stage_targets2(a, tc2, tl, to, l, L);
+    stage_targets2(a, tc2, tl, to, tfree, l, L);
 
     const int oe_ins = a.o_ins + a.e_ins;
 
@@ -732,7 +738,7 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         for (int h = 0; h < 2; ++h) {
             const int c = cbh[h] + s;
             int code = 4;
-            if (valid[h] && c >= 0 && c < ql[h]) code = decode_base(a.seq2, a.seqn, qo[h], c);
+            if (valid[h] && c >= 0 && c < ql[h]) code = decode_base(a.seq2, a.seqn, qo[h], c, qfree[h]);
             pw[h] = a.profw[code];
             const int h0 = h0v[h];
             const int v1 = h0 > oe_ins ? h0 - oe_ins : 0;
@@ -761,14 +767,9 @@ __device__ __forceinline__ void extend16_wave(const ExtArgs &a, uint32_t slot0, 
         for (int h = 0; h < 2; ++h) {
             if (!valid[h]) continue;
             const int sh = 16 * h;
-            int32_t *o = a.out + (size_t)pid[h] * 6;
-            o[0] = (int)(short)(P.best >> sh);
-            o[1] = (int)(short)(P.max_i >> sh) + 1;
-            o[2] = (int)(short)(P.max_ie >> sh) + 1;
-            o[3] = (int)(short)(P.max_j >> sh) + 1;
-            o[4] = (int)(short)(P.gscore >> sh);
-            o[5] = (int)(short)(P.max_off >> sh);
-            if (a.cells) a.cells[pid[h]] = P.ncell[h];
+            store_result(a, pid[h], (int)(short)(P.best >> sh), (int)(short)(P.max_i >> sh) + 1,
+                         (int)(short)(P.max_ie >> sh) + 1, (int)(short)(P.max_j >> sh) + 1,
+                         (int)(short)(P.gscore >> sh), (int)(short)(P.max_off >> sh), P.ncell[h]);
         }
     }
 }
@@ -789,12 +790,39 @@ __global__ __launch_bounds__(256) void meta_gather_kernel(ExtArgs a, uint32_t n)
     mp[1] = make_uint4((uint32_t)a.tlen[p], (uint32_t)a.h0[p], (uint32_t)a.w[p], 0u);
 }
 
+// 1 when none of the first len bases of the sequence at word offset off is N.
+__device__ __forceinline__ uint32_t n_free(const uint32_t *seqn, uint32_t off, int len) {
+    uint32_t any = 0u;
+    const int nw = (len + 31) >> 5;
+    for (int j = 0; j < nw; ++j) {
+        uint32_t m = seqn[(off >> 1) + j];
+        if (j == nw - 1 && (len & 31)) m &= (1u << (len & 31)) - 1u;
+        any |= m;
+    }
+    return any ? 0u : 1u;
+}
+
 __global__ __launch_bounds__(256) void params_pack_kernel(ExtArgs a, uint32_t n, uint32_t *params) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    const uint32_t qo = a.qo[k], to = a.to[k];
+    const int qlen = a.qlen[k], tlen = a.tlen[k];
+    const uint32_t nfree = n_free(a.seqn, qo, qlen) | n_free(a.seqn, to, tlen) << 1;
     uint4 *mp = reinterpret_cast<uint4 *>(params) + 2 * (size_t)k;
-    mp[0] = make_uint4(k, a.qo[k], a.to[k], (uint32_t)a.qlen[k]);
-    mp[1] = make_uint4((uint32_t)a.tlen[k], (uint32_t)a.h0[k], (uint32_t)a.w[k], 0u);
+    mp[0] = make_uint4(k, qo, to, (uint32_t)qlen);
+    mp[1] = make_uint4((uint32_t)tlen, (uint32_t)a.h0[k], (uint32_t)a.w[k], nfree);
+}
+
+__global__ __launch_bounds__(256) void records_split_kernel(const int4 *__restrict__ rec, uint32_t n,
+                                                            int2 *__restrict__ out,
+                                                            uint32_t *__restrict__ cells) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int4 r0 = rec[2 * (size_t)k], r1 = rec[2 * (size_t)k + 1];
+    out[3 * (size_t)k] = make_int2(r0.x, r0.y);
+    out[3 * (size_t)k + 1] = make_int2(r0.z, r0.w);
+    out[3 * (size_t)k + 2] = make_int2(r1.x, r1.y);
+    cells[k] = (uint32_t)r1.z;
 }
 
 hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st) {
@@ -806,6 +834,13 @@ hipError_t meta_gather(const ExtArgs &a, uint32_t n, hipStream_t st) {
 hipError_t params_pack(const ExtArgs &a, uint32_t n, uint32_t *params, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(params_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n, params);
+    return hipGetLastError();
+}
+
+hipError_t records_split(const int32_t *rec, uint32_t n, int32_t *out, uint32_t *cells, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(records_split_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<const int4 *>(rec), n, reinterpret_cast<int2 *>(out), cells);
     return hipGetLastError();
 }
 

@@ -284,16 +284,8 @@ __device__ __forceinline__ void extend_wave(const ExtArgs &a, uint32_t slot0, ui
         }
     }
 
-    if (valid && l == 0) {
-        int32_t *o = a.out + (size_t)pid * 6;
-        o[0] = best;        // score
-        o[1] = max_i + 1;   // target_end_pos (tle)
-        o[2] = max_ie + 1;  // gtarget_end_pos (gtle)
-        o[3] = max_j + 1;   // query_end_pos (qle)
-        o[4] = gscore;      // global_score
-        o[5] = max_off;     // max_offset
-        if (a.cells) a.cells[pid] = ncell;
-    }
+    if (valid && l == 0)  // score, tle, gtle, qle, global score, max offset
+        store_result(a, pid, best, max_i + 1, max_ie + 1, max_j + 1, gscore, max_off, ncell);
 }
 
 template <int L, int S>

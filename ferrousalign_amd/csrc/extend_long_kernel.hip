@@ -228,16 +228,8 @@ __global__ __launch_bounds__(64) void extend_long_kernel(ExtArgs a, uint32_t fir
             beg = nb;
             end = ne + 2 < qlen ? ne + 2 : qlen;
         }
-        if (lane == 0) {
-            int32_t *o = a.out + (size_t)pid * 6;
-            o[0] = best;        // score
-            o[1] = max_i + 1;   // tle
-            o[2] = max_ie + 1;  // gtle
-            o[3] = max_j + 1;   // qle
-            o[4] = gscore;
-            o[5] = max_off;
-            if (a.cells) a.cells[pid] = ncell;
-        }
+        if (lane == 0)  // score, tle, gtle, qle, global score, max offset
+            store_result(a, pid, best, max_i + 1, max_ie + 1, max_j + 1, gscore, max_off, ncell);
         __syncthreads();  // the next pair reuses the LDS codes and the eh slice
     }
 }

@@ -232,13 +232,23 @@ void drop_graph(fa_batch *b) {
     b->graph_exec = nullptr;
 }
 
+// The extension records of the last run as OutScore and cell arrays (one
+// coalesced pass on the context's stream, where every run ends).
+int ensure_records(fa_ctx *ctx, fa_batch *b) {
+    if (!b->rec_pending) return FA_OK;
+    FA_HIP(fa::records_split(b->d_rec, (uint32_t)b->n, b->d_out, b->d_cells, ctx->stream));
+    b->rec_pending = false;
+    return FA_OK;
+}
+
 void batch_release(fa_batch *b) {
     if (!b) return;
     drop_graph(b);
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
-                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev, b->d_long, b->d_params};
+                    b->d_mout, b->d_imax_log, b->d_sort, b->d_elig, b->d_mrev, b->d_long, b->d_params,
+                    b->d_rec, b->d_mcells};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->h_hist) (void)hipHostFree(b->h_hist);
@@ -453,6 +463,7 @@ namespace fa_host {
 hipError_t launch_bucket(fa_ctx *ctx, fa_batch *b, uint32_t flags, size_t k, hipStream_t st) {
     const Bucket &bk = b->buckets[k];
     fa::ExtArgs a = b->args;
+    if (a.orec) b->rec_pending = true;
     if (bk.kind == 4)
         return fa::extend_long_launch(a, bk.first, bk.count, b->long_qmax, b->d_long, b->long_bytes, st);
     a.chunk = std::max(1, b->chunk / (64 / bk.L) / (bk.kind == 1 ? 2 : 1));
@@ -626,6 +637,7 @@ int run_batch(fa_ctx *ctx, fa_batch *b, uint32_t flags, int *nlaunch) {
     int rc = plan_packing(ctx, b, flags);
     if (rc) return rc;
     b->last_overflow = b->n_fallback32;
+    if (b->args.orec) b->rec_pending = true;  // also for a graph replay
     if (nlaunch) *nlaunch = (int)b->buckets.size();
     if (!(flags & FA_LAUNCH_GRAPH)) return enqueue_run(ctx, b, flags);
     const uint32_t key = (flags & 0xFF4u) | (uint32_t)b->lanes << 16 | (uint32_t)b->chunk << 21;
@@ -886,6 +898,11 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
             first_err = r;
             break;
         }
+        // a pair's query and target words, back to back, start where they
+        // touch the fewest 128-byte lines (32 words): the kernels read each
+        // pair's words alone, in packed order, and HBM is read in whole lines
+        const uint64_t blk = (uint64_t)seq_words(query_len[k]) + seq_words(ref_len[k]);
+        if (((words & 31u) + blk + 31u) / 32u > (blk + 31u) / 32u) words = (words + 31u) & ~(uint64_t)31u;
         qo[(size_t)k] = (uint32_t)words;
         words += seq_words(query_len[k]);
         to[(size_t)k] = (uint32_t)words;
@@ -909,6 +926,13 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     uint32_t *seq2 = (uint32_t *)ctx->pin, *seqn = seq2 + words;
     parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
         for (int64_t k = lo; k < hi; ++k) {
+            // the alignment gap before the pair (never read; zeroed so that
+            // uploads are deterministic)
+            const uint32_t gap0 = k ? to[(size_t)k - 1] + seq_words(ref_len[k - 1]) : 0u;
+            if (gap0 < qo[(size_t)k]) {
+                std::memset(&seq2[gap0], 0, (qo[(size_t)k] - gap0) * sizeof(uint32_t));
+                std::memset(&seqn[gap0 / 2], 0, (qo[(size_t)k] - gap0) / 2 * sizeof(uint32_t));
+            }
             pack_seq(query_seqs + query_offset[k], query_len[k], &seq2[qo[(size_t)k]],
                      &seqn[qo[(size_t)k] / 2]);
             if (!from_ref) {
@@ -934,7 +958,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         (rc = dev_upload(&b->d_elig, b->elig16.data(), b->elig16.size(), st)) ||
         (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
         (rc = dev_alloc(&b->d_params, (size_t)n * 8)) ||
-        (rc = dev_alloc(&b->d_out, (size_t)n * 6)) ||
+        (rc = dev_alloc(&b->d_out, (size_t)n * 6)) || (rc = dev_alloc(&b->d_rec, (size_t)n * 8)) ||
         (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64)) ||
         (b->nlong > 0 &&
          (rc = dev_alloc((uint8_t **)&b->d_long,
@@ -978,7 +1002,10 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     a.w = b->d_w;
     a.order = b->d_order;
     a.meta = b->d_meta;
-    a.out = b->d_out;
+    // the kernels write one 32-byte record per pair (a single HBM sector),
+    // split into the OutScore and cell arrays when they are read
+    a.out = b->d_rec;
+    a.orec = 1;
     a.cells = b->d_cells;
     a.queue = b->d_queue;
     a.ovf = nullptr;
@@ -1672,6 +1699,8 @@ int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cel
     if (!ctx || !b) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     static_assert(sizeof(fa_out_score) == 24, "OutScore layout");
+    int rc = ensure_records(ctx, b);
+    if (rc) return rc;
     if (out && b->n)
         FA_HIP(hipMemcpyAsync(out, b->d_out, (size_t)b->n * sizeof(fa_out_score),
                               hipMemcpyDeviceToHost, ctx->stream));
@@ -1914,6 +1943,7 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     }
     int rc;
     if (!b->d_mout && (rc = dev_alloc(&b->d_mout, (size_t)n * 7))) return rc;
+    if (!b->d_mcells && (rc = dev_alloc(&b->d_mcells, (size_t)n))) return rc;
     const uint32_t stride = (uint32_t)std::max(16, ((b->tmax + 15) / 16) * 16);
     const uint32_t slots = (uint32_t)std::min<int64_t>(n, kMateChunk);
     if (b->log_slots < slots || !b->d_imax_log) {
@@ -1936,7 +1966,7 @@ int fa_batch_set_align2(fa_ctx *ctx, fa_batch *b, const int8_t *mat, int32_t m, 
     a.tlen = b->d_tlen;
     a.order = d_morder;
     a.out = b->d_mout;
-    a.cells = b->d_cells;
+    a.cells = b->d_mcells;
     a.imax_log = b->d_imax_log;
     a.log_stride = stride;
     for (int q = 0; q < 5; ++q) {
@@ -1965,7 +1995,7 @@ int fa_batch_download_align2(fa_ctx *ctx, fa_batch *b, fa_kswr *out, uint32_t *c
         FA_HIP(hipMemcpyAsync(out, b->d_mout, (size_t)b->n * sizeof(fa_kswr), hipMemcpyDeviceToHost,
                               ctx->stream));
     if (cells && b->n)
-        FA_HIP(hipMemcpyAsync(cells, b->d_cells, (size_t)b->n * sizeof(uint32_t),
+        FA_HIP(hipMemcpyAsync(cells, b->d_mcells, (size_t)b->n * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, ctx->stream));
     FA_HIP(hipStreamSynchronize(ctx->stream));
     return FA_OK;
@@ -2134,6 +2164,7 @@ int fa_mlp_gradient(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *pairs, 
     if (!ctx || !m || !b || !pairs || count <= 0 || count > b->n) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     int rc = ensure_features(ctx, b);
+    if (!rc) rc = ensure_records(ctx, b);  // the cell counts are the labels
     if (rc) return rc;
     const uint32_t blocks = (uint32_t)((count + fa::kMlpBlock - 1) / fa::kMlpBlock);
     if ((rc = mlp_ensure_partial(m, blocks)) || (rc = mlp_set_perm(ctx, m, pairs, count))) return rc;
@@ -2156,6 +2187,7 @@ int fa_mlp_train(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *order, int
     if (!ctx || !m || !b || !order || count <= 0 || count > b->n || minibatch <= 0) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     int rc = ensure_features(ctx, b);
+    if (!rc) rc = ensure_records(ctx, b);  // the cell counts are the labels
     if (rc) return rc;
     const uint32_t mb_blocks = (uint32_t)((minibatch + fa::kMlpBlock - 1) / fa::kMlpBlock);
     if ((rc = mlp_ensure_partial(m, mb_blocks)) || (rc = mlp_set_perm(ctx, m, order, count)))

@@ -71,8 +71,10 @@ struct fa_batch {
     uint32_t *d_order = nullptr;
     uint32_t *d_meta = nullptr;  // per packed slot: pid, qo, to, qlen, tlen, h0, w, 0
     uint32_t *d_params = nullptr;  // the same 8 words per pair in job order (meta_gather's source)
-    int32_t *d_out = nullptr;
-    uint32_t *d_cells = nullptr;
+    int32_t *d_rec = nullptr;    // the extension kernels' 32-byte records (ExtArgs::orec)
+    bool rec_pending = false;    // d_rec is newer than d_out / d_cells
+    int32_t *d_out = nullptr;    // OutScore per pair and cells per pair (job order), split from
+    uint32_t *d_cells = nullptr; // d_rec on demand (download, predictor training)
     uint32_t *d_queue = nullptr;
     void *d_sort = nullptr;      // packing sort: keys in/out, pair ids, scratch
     size_t sort_bytes = 0;
@@ -103,6 +105,7 @@ struct fa_batch {
     bool align2_set = false;
     fa::MateArgs margs{};
     int32_t *d_mout = nullptr;     // 7 int32 per pair
+    uint32_t *d_mcells = nullptr;  // cells per pair of the mate-rescue run
     uint16_t *d_imax_log = nullptr;
     uint32_t log_slots = 0;
     void *d_mrev = nullptr;        // reverse launches: keys in/out, ids, order, histogram, sort scratch

@@ -235,3 +235,27 @@ def test_overflow_count_is_the_32bit_fallbacks():
     db.free()
     ora, oc, orc = O.extend_batch(sw, *flat)
     _assert_same(O.to_outscore_order(ora), oc, gpu, gc, "overflow fallbacks")
+
+
+def test_single_n_bases_among_n_free_pairs():
+    """Pairs without any N read no N-mask words (a per-pair flag set at
+    upload); pairs with a single N at a sequence's first or last base, or at
+    a 16/32-base word edge, sit among them in the same lane groups and must
+    still score their N."""
+    from ferrousalign_amd import synth
+    ps = synth.workload("uneven_mix_4m", n=6000)
+    assert not (ps.q == 4).any() and not (ps.t == 4).any()
+    rng = np.random.default_rng(11)
+    q, t = ps.q.copy(), ps.t.copy()
+    picks = rng.choice(ps.n, 1200, replace=False)
+    for j, k in enumerate(picks):
+        side, where = j % 2, (j // 2) % 6
+        seq, L = (q, int(ps.qlen[k])) if side == 0 else (t, int(ps.tlen[k]))
+        pos = [0, L - 1, 15, 16, 31, 32][where]
+        if pos < L:
+            seq[k, pos] = 4
+    nps = synth.PairSet(q, t, ps.qlen, ps.tlen, ps.h0, ps.w)
+    sw = synth.default_scoring()
+    print("single N bases", flush=True)
+    ora, oc, gpu, gc = _run_both(nps, sw, lanes=0)
+    _assert_same(ora, oc, gpu, gc, "single N")

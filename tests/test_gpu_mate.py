@@ -231,3 +231,33 @@ def test_mate_score2_two_copies(byte_mode):
     xtra = synth.MATE_XTRA | (O.KSW_XBYTE if byte_mode else 0)
     ora = _run_cases(cases, synth.MATE_MATRIX, (6, 1, 6, 1), xtra, "mate two copies")
     assert [tuple(int(x) for x in r) for r in ora] == want
+
+
+def test_extension_and_mate_runs_keep_their_own_results():
+    """One batch, extension and mate-rescue runs in turn: each download gives
+    the results of its own last run (the two keep separate cell counts)."""
+    from ferrousalign_amd import synth, _native as N
+    from ferrousalign_amd.banded_swa import DeviceBatch
+    ps = synth.mate_pairs(512)
+    q, qo, ql, t, to, tl = ps.flat()
+    n = len(ql)
+    h0 = np.full(n, 40, np.int32)
+    w = np.full(n, 60, np.int32)
+    sw = synth.default_scoring()
+    eo, ec, erc = O.extend_batch(sw, q, qo, ql, t, to, tl, h0, w)
+    assert (erc == 0).all()
+    eo = O.to_outscore_order(eo)
+    mo, mc, mrc = O.ksw_align2_batch(q, qo, ql, t, to, tl, synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA)
+    assert (mrc == 0).all()
+    print("extension + mate on one batch", flush=True)
+    db = DeviceBatch(sw, q, qo, ql, t, to, tl, h0, w)
+    db.set_align2(synth.MATE_MATRIX, 6, 1, 6, 1, synth.MATE_XTRA)
+    db.run(0)
+    db.run(N.FA_RUN_ALIGN2)
+    for _ in range(2):
+        out, cells = db.download()
+        assert (out == eo).all() and (cells.astype(np.uint64) == ec).all()
+        mout, mcells = db.download_align2()
+        _check(mout, mcells, mo, mc, "mate after extension")
+        db.run(0)
+    db.free()

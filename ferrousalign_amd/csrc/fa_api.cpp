@@ -336,6 +336,7 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     pa.tlen = b->d_tlen;
     pa.elig = b->d_elig;
     pa.cost = by_cost ? b->d_pred : nullptr;
+    pa.shift = pa.mode == fa::kPlanCostOrder ? fa::kRankShiftCompact : fa::kRankShift;
     // the histogram first: the launches, their ranks and the key width come
     // from it, so the rest of the plan needs no further readback
     FA_HIP(hipMemsetAsync(base, 0, hist_bytes, ctx->stream));
@@ -379,10 +380,10 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
             const int kl = (kinds[x] >> 8) & 0xFF, ks = kinds[x] & 0xFF;
             t.cost_order[rank[x]] = (uint8_t)(kl * ks >= 64 ? 1 : 0);
         }
-    // keys: rank << 27 | order inside the launch; 32 bits hold 32 ranks
+    // keys: rank << shift | order inside the launch
     int rbits = 0;
     while ((1 << rbits) < nranks) ++rbits;
-    const int end_bit = fa::kRankShift + rbits;
+    const int end_bit = pa.shift + rbits;
     if (n > 0) {
         if (end_bit <= 32) {
             uint32_t *kin = (uint32_t *)kbase, *kout = kin + n;
@@ -2117,12 +2118,13 @@ int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32
     FA_HIP(hipSetDevice(ctx->device));
     int rc;
     if (!b->d_pred && (rc = dev_alloc(&b->d_pred, (size_t)b->n))) return rc;
-    // a new batch's features (set_cost & 2, or none yet) are computed in the
-    // same kernel as the forward pass and kept for later calls
-    const bool fresh = (set_cost & 2) || !b->d_feat;
-    if (!b->d_feat && (rc = dev_alloc(&b->d_feat, (size_t)b->n * fa::kMlpFeatures))) return rc;
-    FA_HIP(fa::mlp_launch_predict(b->args, m->wh, m->norm, b->d_feat, !fresh, (uint32_t)b->n, b->d_pred,
-                                  ctx->stream));
+    // features from the batch's feature buffer once training or
+    // fa_batch_features has filled it, unless the caller asks for a new
+    // batch's cost (set_cost & 2); otherwise the forward kernel computes them
+    // itself and stores nothing (only training reads them)
+    const bool cached = b->d_feat && !(set_cost & 2);
+    FA_HIP(fa::mlp_launch_predict(b->args, m->wh, m->norm, cached ? b->d_feat : nullptr, cached,
+                                  (uint32_t)b->n, b->d_pred, ctx->stream));
     if (cells_pred && b->n)
         FA_HIP(hipMemcpyAsync(cells_pred, b->d_pred, (size_t)b->n * sizeof(float), hipMemcpyDeviceToHost,
                               ctx->stream));

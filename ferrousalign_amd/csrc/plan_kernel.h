@@ -17,6 +17,8 @@ struct PlanArgs {
     const uint8_t *elig;  // 16-bit fit level per pair (0..2), 3: long pair
     const float *cost;    // per-pair packing cost (kPlanCost, kPlanCostOrder), else null
     int long_rank;        // launch rank of the long pairs (after every lane-group kernel)
+    int shift;            // bits of the order inside a launch (kRankShift, or kRankShiftCompact
+                          // for kPlanCostOrder)
 };
 
 // Per (fit level, qlen) bin, 3 x 256 entries each; zero before plan_hist.
@@ -34,7 +36,11 @@ struct PlanTable {
 };
 
 // Sort keys: launch rank << kRankShift | a 27-bit order inside the launch.
+// kPlanCostOrder keeps 19 bits (the cost on a log scale in 1/50 octaves,
+// then qlen), so that with up to 32 ranks the keys sort in three 8-bit radix
+// passes instead of four.
 constexpr int kRankShift = 27;
+constexpr int kRankShiftCompact = 19;
 
 // Packing cost in 1/16 cells, non-negative and saturating at 2^24 cells
 // (a pair has at most 256 x 2048), so a sum over 2^31 pairs cannot wrap.

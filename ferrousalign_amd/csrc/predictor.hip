@@ -87,8 +87,26 @@ __device__ __forceinline__ uint32_t window_n(uint32_t m0, uint32_t m1, int d) {
 // neighbouring diagonal), query GC fraction.  Base j mismatches when exactly
 // one of the two is N, or neither is and the 2-bit codes differ.
 __device__ __forceinline__ void pair_features(const ExtArgs &a, uint32_t k, float *o) {
-    const int qlen = a.qlen[k], tlen = a.tlen[k];
-    const uint32_t qo = a.qo[k], to = a.to[k];
+    int qlen, tlen, h0, w;
+    uint32_t qo, to, nfree = 0u;
+    if (a.params) {  // the pair's 32-byte record: one load, with its N-free bits
+        const uint4 *r = reinterpret_cast<const uint4 *>(a.params) + 2 * (size_t)k;
+        const uint4 r0 = r[0], r1 = r[1];
+        qo = r0.y;
+        to = r0.z;
+        qlen = (int)r0.w;
+        tlen = (int)r1.x;
+        h0 = (int)r1.y;
+        w = (int)r1.z;
+        nfree = r1.w;
+    } else {
+        qlen = a.qlen[k];
+        tlen = a.tlen[k];
+        h0 = a.h0[k];
+        w = a.w[k];
+        qo = a.qo[k];
+        to = a.to[k];
+    }
     int lim = qlen < tlen ? qlen : tlen;
     if (lim > 32) lim = 32;
     int mm = 0;
@@ -98,8 +116,11 @@ __device__ __forceinline__ void pair_features(const ExtArgs &a, uint32_t k, floa
         // the next block only where the sequence has one (shifted windows reach
         // at most base 34; below a sequence's end the missing bits are masked)
         const uint32_t q2 = qlen > 32 ? a.seq2[qo + 2] : 0u, t2 = tlen > 32 ? a.seq2[to + 2] : 0u;
-        const uint32_t qn0 = a.seqn[qo >> 1], tn0 = a.seqn[to >> 1];
-        const uint32_t qn1 = qlen > 32 ? a.seqn[(qo >> 1) + 1] : 0u, tn1 = tlen > 32 ? a.seqn[(to >> 1) + 1] : 0u;
+        // N-mask words only of a sequence that has an N
+        const bool qn = !(nfree & 1u), tn = !(nfree & 2u);
+        const uint32_t qn0 = qn ? a.seqn[qo >> 1] : 0u, tn0 = tn ? a.seqn[to >> 1] : 0u;
+        const uint32_t qn1 = qn && qlen > 32 ? a.seqn[(qo >> 1) + 1] : 0u;
+        const uint32_t tn1 = tn && tlen > 32 ? a.seqn[(to >> 1) + 1] : 0u;
         const uint32_t lm = lim == 32 ? 0xFFFFFFFFu : ((1u << lim) - 1u);
         mm = 32;
 #pragma unroll
@@ -123,8 +144,8 @@ __device__ __forceinline__ void pair_features(const ExtArgs &a, uint32_t k, floa
     }
     o[0] = (float)qlen;
     o[1] = (float)tlen;
-    o[2] = (float)a.h0[k];
-    o[3] = (float)a.w[k];
+    o[2] = (float)h0;
+    o[3] = (float)w;
     o[4] = (float)mm;
     o[5] = qlen > 0 ? (float)gc / (float)qlen : 0.f;
 }

@@ -405,7 +405,8 @@ int fa_batch_features(fa_ctx *ctx, fa_batch *b, float *feat);
 /* Predicted cells per pair (to cells_pred when non-NULL).  set_cost bit 0:
  * the predictions become the batch's packing cost (they stay in device
  * memory, where the packing plan reads them); bit 1: compute the batch's
- * features again first (what a new batch costs). */
+ * features again, in the forward kernel (what a new batch costs; features
+ * that fa_batch_features or training stored are used otherwise). */
 int fa_mlp_predict(fa_ctx *ctx, fa_mlp *m, fa_batch *b, float *cells_pred, int32_t set_cost);
 /* Mean loss 0.5*(y - log1p(cells))^2 over pairs[0..count) of the batch and
  * its gradient (labels: the cells recorded by the batch's last run). */

@@ -51,6 +51,33 @@ def test_features_exact(setup):
     assert np.max(np.abs(g[:, 5] - o[:, 5])) <= 1.2e-7
 
 
+def test_features_exact_with_n_bases():
+    """N bases in the first 40 bases of some queries and targets (the
+    mismatch feature's windows reach base 34; other pairs are N-free and
+    read no N-mask words)."""
+    from ferrousalign_amd import synth
+    from ferrousalign_amd.banded_swa import DeviceBatch, Context
+    from ferrousalign_amd.predictor import WorkPredictor
+    ps = synth.workload("uneven_mix_4m", n=3000)
+    rng = np.random.default_rng(4)
+    q, t = ps.q.copy(), ps.t.copy()
+    for k in rng.choice(ps.n, 1500, replace=False):
+        seq, L = (q, int(ps.qlen[k])) if k % 2 else (t, int(ps.tlen[k]))
+        for p in rng.integers(0, min(L, 40), rng.integers(1, 4)):
+            seq[k, p] = 4
+    nps = synth.PairSet(q, t, ps.qlen, ps.tlen, ps.h0, ps.w)
+    sw = synth.default_scoring()
+    ctx = Context(0)
+    db = DeviceBatch(sw, *nps.flat(), ctx=ctx)
+    g = WorkPredictor(ctx, seed=1).features(db)
+    o = _oracle_features(nps, sw)
+    bad = np.nonzero((g[:, :5] != o[:, :5]).any(axis=1))[0]
+    assert bad.size == 0, "pair %d: gpu %s oracle %s" % (bad[0], g[bad[0]].tolist(), o[bad[0]].tolist())
+    assert np.max(np.abs(g[:, 5] - o[:, 5])) <= 1.2e-7
+    db.free()
+    ctx.close()
+
+
 def test_forward_and_gradient_vs_oracle(setup):
     ctx, ps, sw, db, cells, pred = setup
     from ferrousalign_amd.predictor import norm_from_features, init_params
@@ -178,8 +205,8 @@ def _wavefront_idle_cells(db, flags, cells):
 def test_cost_order_packing_lowers_idle_cells():
     """Descending predicted cells per kernel (FA_PACK_COST_ORDER, the learned
     path's packing): on held-out pairs the idle cell-slots of the wavefronts
-    must fall below length sorting's (measured 0.73 of it), and lowest with
-    the true cells."""
+    must fall below length sorting's (measured 0.66 of it), and lowest with
+    the true cells (0.05: the order keeps cost steps of 1.4%)."""
     from ferrousalign_amd import synth, _native as N
     from ferrousalign_amd.banded_swa import DeviceBatch, Context
     from ferrousalign_amd.predictor import WorkPredictor

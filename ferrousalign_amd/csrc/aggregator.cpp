@@ -147,7 +147,11 @@ void worker_main(fa_agg *g) {
                     grp.push_back(take[y]);
                     jobs += take[y]->njobs;
                 }
-            run_group(g, grp);
+            try {
+                run_group(g, grp);
+            } catch (...) {  // host allocation failure: the group's calls get the error
+                for (AggCall *c : grp) c->rc = FA_ERR_ARG;
+            }
             ++runs;
         }
         lk.lock();

@@ -2033,13 +2033,16 @@ struct fa_mlp {
     float *partial = nullptr;
     uint32_t partial_blocks = 0;
     float *grad = nullptr, *loss = nullptr;
+    float *losses = nullptr;  // per-minibatch loss sums of one fa_mlp_train call
+    int64_t losses_n = 0;
     uint32_t *perm = nullptr;
     int64_t perm_n = 0;
     int64_t step = 0;
 };
 
 static void mlp_release(fa_mlp *m) {
-    void *ptrs[] = {m->master, m->m, m->v, m->norm, m->wh, m->partial, m->grad, m->loss, m->perm};
+    void *ptrs[] = {m->master, m->m, m->v, m->norm, m->wh, m->partial, m->grad, m->loss, m->perm,
+                    m->losses};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete m;
@@ -2195,9 +2198,17 @@ int fa_mlp_train(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *order, int
     if ((rc = mlp_ensure_partial(m, mb_blocks)) || (rc = mlp_set_perm(ctx, m, order, count)))
         return rc;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    std::vector<float> losses;
-    double total = 0.0;
-    for (int64_t first = 0; first < count; first += minibatch) {
+    // each minibatch's loss sum goes to its own slot; one read-back at the end
+    const int64_t nmb = (count + minibatch - 1) / minibatch;
+    if (m->losses_n < nmb) {
+        if (m->losses) (void)hipFree(m->losses);
+        m->losses = nullptr;
+        m->losses_n = 0;
+        if ((rc = dev_alloc(&m->losses, (size_t)nmb))) return rc;
+        m->losses_n = nmb;
+    }
+    int64_t mb = 0;
+    for (int64_t first = 0; first < count; first += minibatch, ++mb) {
         const uint32_t cnt = (uint32_t)std::min<int64_t>(minibatch, count - first);
         const uint32_t blocks = (cnt + fa::kMlpBlock - 1) / fa::kMlpBlock;
         m->step += 1;
@@ -2205,13 +2216,15 @@ int fa_mlp_train(fa_ctx *ctx, fa_mlp *m, fa_batch *b, const uint32_t *order, int
         const float bc2 = 1.f - std::pow(b2, (float)m->step);
         FA_HIP(fa::mlp_launch_backward(m->wh, m->norm, b->d_feat, b->d_cells, m->perm,
                                        (uint32_t)first, cnt, m->partial, ctx->stream));
-        FA_HIP(fa::mlp_launch_reduce_adam(m->partial, blocks, nullptr, m->loss, m->master, m->m,
+        FA_HIP(fa::mlp_launch_reduce_adam(m->partial, blocks, nullptr, m->losses + mb, m->master, m->m,
                                           m->v, m->wh, lr, b1, b2, eps, bc1, bc2, 1, ctx->stream));
-        float lsum = 0.f;
-        FA_HIP(hipMemcpyAsync(&lsum, m->loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        FA_HIP(hipStreamSynchronize(ctx->stream));
-        total += lsum;
     }
+    std::vector<float> lsum((size_t)nmb);
+    FA_HIP(hipMemcpyAsync(lsum.data(), m->losses, (size_t)nmb * sizeof(float), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    FA_HIP(hipStreamSynchronize(ctx->stream));
+    double total = 0.0;
+    for (float x : lsum) total += x;
     if (mean_loss) *mean_loss = (float)(total / (double)count);
     return FA_OK;
 }

@@ -364,10 +364,21 @@ __global__ __launch_bounds__(256) void mlp_reduce_adam_kernel(const float *parti
                                                               __half *wh, float lr, float b1,
                                                               float b2, float eps, float bc1,
                                                               float bc2, int apply) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > kMlpParams) return;
+    // 16 parameters per block, 16 threads per parameter: thread j sums the
+    // block partials j, j + 16, ... and the 16 sums are added in j order, a
+    // fixed order, so the result does not depend on timing
+    __shared__ float part[16][16];
+    const int pl = (int)(threadIdx.x & 15), j = (int)(threadIdx.x >> 4);
+    const int p = (int)blockIdx.x * 16 + pl;
+    float s = 0.f;
+    if (p <= kMlpParams)
+        for (uint32_t b = (uint32_t)j; b < nblocks; b += 16u) s += partial[(size_t)b * (kMlpParams + 1) + p];
+    part[j][pl] = s;
+    __syncthreads();
+    if (j != 0 || p > kMlpParams) return;
     float g = 0.f;
-    for (uint32_t b = 0; b < nblocks; ++b) g += partial[(size_t)b * (kMlpParams + 1) + p];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g += part[k][pl];
     if (p == kMlpParams) {
         if (loss_out) *loss_out = g;
         return;
@@ -425,7 +436,7 @@ hipError_t mlp_launch_reduce_adam(const float *partial, uint32_t nblocks, float 
                                   float *loss_out, float *master, float *m, float *v, void *wh,
                                   float lr, float b1, float b2, float eps, float bc1, float bc2,
                                   int apply, hipStream_t st) {
-    hipLaunchKernelGGL(mlp_reduce_adam_kernel, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0,
+    hipLaunchKernelGGL(mlp_reduce_adam_kernel, dim3((kMlpParams + 1 + 15) / 16), dim3(256), 0,
                        st, partial, nblocks, grad_out, loss_out, master, m, v, (__half *)wh, lr,
                        b1, b2, eps, bc1, bc2, apply);
     return hipGetLastError();

@@ -7,6 +7,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -143,12 +144,14 @@ namespace {
 
 using fa_host::check_scoring;
 
+// f(lo, hi, thread) over up to 16 contiguous ranges of [0, n) on threads of
+// their own; serially below `serial_below` items.
 template <class F>
-void parallel_for(int64_t n, F &&f) {
+void parallel_for(int64_t n, F &&f, int64_t serial_below = 65536) {
     unsigned nt = std::thread::hardware_concurrency();
     if (nt < 1) nt = 1;
     if (nt > 16) nt = 16;
-    if (n < 65536 || nt == 1) {
+    if (n < serial_below || nt == 1) {
         f(0, n, 0u);
         return;
     }
@@ -196,13 +199,20 @@ std::vector<uint32_t> radix_order(std::vector<uint64_t> keys) {
 using fa_host::seq_words;
 
 // t == nullptr: the target comes from a device reference (codes 0-3 by construction)
-int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q,
-                 const uint8_t *t) {
+// The checks of validate_job that read no sequence.
+int validate_scalar(int32_t ql, int32_t tl, int32_t h0, int32_t w) {
     if (h0 <= 0) return FA_ERR_H0;
     if (ql <= 0 || ql > FA_MAX_QLEN) return FA_ERR_QLEN;
     if (tl < 0 || tl > FA_MAX_TLEN) return FA_ERR_TLEN;
     if (w < 0) return FA_ERR_BAND;
     if (h0 > (1 << 19)) return FA_ERR_SCORING;
+    return FA_OK;
+}
+
+int validate_job(int32_t ql, int32_t tl, int32_t h0, int32_t w, const uint8_t *q,
+                 const uint8_t *t) {
+    const int r = validate_scalar(ql, tl, h0, w);
+    if (r != FA_OK) return r;
     if (!fa_host::codes_ok(q, ql)) return FA_ERR_CODE;
     if (t && !fa_host::codes_ok(t, tl)) return FA_ERR_CODE;
     return FA_OK;
@@ -217,6 +227,26 @@ int dev_alloc(T **p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
     return hip_fail(hipMalloc((void **)p, count * sizeof(T)), "hipMalloc");
+}
+
+// context slots of the one-call batches (the CIGAR path uses 0-25)
+constexpr int kSlotBatchArena = 28, kSlotBatchSort = 29;
+
+// Grow-only device buffer slot k of the context (CIGAR-regeneration calls,
+// one-call batches); the previous contents are not kept.
+template <class T>
+int ctx_alloc(fa_ctx *ctx, int k, T **p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (ctx->slot_bytes[k] < bytes) {
+        if (ctx->slot[k]) (void)hipFree(ctx->slot[k]);
+        ctx->slot[k] = nullptr;
+        ctx->slot_bytes[k] = 0;
+        const size_t want = bytes + bytes / 8;
+        FA_HIP(hipMalloc(&ctx->slot[k], want));
+        ctx->slot_bytes[k] = want;
+    }
+    *p = (T *)ctx->slot[k];
+    return FA_OK;
 }
 
 template <class T>
@@ -244,6 +274,14 @@ int ensure_records(fa_ctx *ctx, fa_batch *b) {
 void batch_release(fa_batch *b) {
     if (!b) return;
     drop_graph(b);
+    if (b->in_arena) {  // the context keeps these for the next one-call batch
+        void **carved[] = {(void **)&b->d_seq2, (void **)&b->d_seqn, (void **)&b->d_qo, (void **)&b->d_to,
+                           (void **)&b->d_qlen, (void **)&b->d_tlen, (void **)&b->d_h0, (void **)&b->d_w,
+                           (void **)&b->d_elig, (void **)&b->d_order, (void **)&b->d_meta, (void **)&b->d_params,
+                           (void **)&b->d_out, (void **)&b->d_rec, (void **)&b->d_cells, (void **)&b->d_queue,
+                           (void **)&b->d_long, (void **)&b->d_sort, (void **)&b->h_hist};
+        for (void **p : carved) *p = nullptr;
+    }
     if (b->margs.order) (void)hipFree(const_cast<uint32_t *>(b->margs.order));
     void *ptrs[] = {b->d_seq2, b->d_seqn, b->d_qo, b->d_to, b->d_qlen, b->d_tlen, b->d_h0,
                     b->d_w, b->d_order, b->d_meta, b->d_out, b->d_cells, b->d_queue, b->d_feat, b->d_pred,
@@ -310,7 +348,15 @@ int plan_packing(fa_ctx *ctx, fa_batch *b, uint32_t flags) {
     size_t tb = std::max(tb32, tb64);
     const size_t hist_bytes = kBins * (4 + 8 + 8) + 256;
     const size_t need = hist_bytes + 16 * (size_t)n + 4 * (size_t)n + tb + 512;
-    if (b->sort_bytes < need) {
+    if (b->in_arena) {  // a one-call batch: the context's grow-only scratch
+        uint8_t *p = nullptr;
+        int rc0 = ctx_alloc(ctx, kSlotBatchSort, &p, need);
+        if (rc0) return rc0;
+        b->d_sort = p;
+        b->sort_bytes = need;
+        if (!ctx->hist_pin) FA_HIP(hipHostMalloc(&ctx->hist_pin, kBins * 12, hipHostMallocDefault));
+        b->h_hist = ctx->hist_pin;
+    } else if (b->sort_bytes < need) {
         if (b->d_sort) (void)hipFree(b->d_sort);
         b->d_sort = nullptr;
         b->sort_bytes = 0;
@@ -688,9 +734,8 @@ const char *fa_strerror(int code) {
 
 int fa_pack_2bit(const uint8_t *codes, int32_t len, uint32_t *w2, uint32_t *wn) {
     if (len < 0 || (len > 0 && (!codes || !w2 || !wn))) return FA_ERR_ARG;
-    if (!fa_host::codes_ok(codes, len)) return FA_ERR_CODE;
-    pack_seq(codes, len, w2, wn);
-    return FA_OK;
+    // the check runs in the packing pass, as in the batch uploads
+    return fa_host::pack_codes_checked(codes, len, w2, wn) ? FA_OK : FA_ERR_CODE;
 }
 
 int fa_unpack_2bit(const uint32_t *w2, const uint32_t *wn, int32_t len, uint8_t *codes) {
@@ -740,6 +785,7 @@ int fa_ctx_destroy(fa_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     fa_host::chain_workspace_free(ctx);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->hist_pin) (void)hipHostFree(ctx->hist_pin);
     for (void *p : ctx->slot)
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -775,6 +821,10 @@ struct TargetSource {
     const uint8_t *seg_rev = nullptr;
     bool validated = false;  // the caller already ran validate_job on every job
 };
+// Without `validated`, batch_upload runs the checks that read no sequence
+// first and checks the codes while packing (one pass over the bases instead
+// of two); any error then makes it validate every job in full, so that the
+// error reported is the same: the first invalid job's, in job order.
 
 // Grow-only pinned host buffer of the context (staging for uploads; the
 // caller synchronises before the next use).
@@ -789,22 +839,6 @@ int pinned_staging(fa_ctx *ctx, size_t bytes) {
     return FA_OK;
 }
 
-// Grow-only device buffer slot k of the context (CIGAR-regeneration calls);
-// the previous contents are not kept.
-template <class T>
-int ctx_alloc(fa_ctx *ctx, int k, T **p, size_t count) {
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (ctx->slot_bytes[k] < bytes) {
-        if (ctx->slot[k]) (void)hipFree(ctx->slot[k]);
-        ctx->slot[k] = nullptr;
-        ctx->slot_bytes[k] = 0;
-        const size_t want = bytes + bytes / 8;
-        FA_HIP(hipMalloc(&ctx->slot[k], want));
-        ctx->slot_bytes[k] = want;
-    }
-    *p = (T *)ctx->slot[k];
-    return FA_OK;
-}
 
 template <class T>
 int ctx_upload(fa_ctx *ctx, int k, T **p, const T *src, size_t count) {
@@ -814,10 +848,13 @@ int ctx_upload(fa_ctx *ctx, int k, T **p, const T *src, size_t count) {
     return FA_OK;
 }
 
+// in_arena: a batch that lives for one call (fa_execute_batch_simd_scoring):
+// its arrays come from the context's grow-only arena rather than from a
+// hipMalloc each, and are not freed with it.
 int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                  const uint8_t *query_seqs, const int64_t *query_offset, const int32_t *query_len,
                  const TargetSource &ts, const int32_t *ref_len, const int32_t *h0,
-                 const int32_t *band_width, fa_batch **out) {
+                 const int32_t *band_width, fa_batch **out, bool in_arena = false) {
     if (!ctx || !out || njobs < 0) return FA_ERR_ARG;
     *out = nullptr;
     int rc = check_scoring(sw);
@@ -857,10 +894,7 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         int64_t nl = 0;
         double wk = 0.0;
         for (int64_t k = lo; k < hi; ++k) {
-            int r = ts.validated ? FA_OK
-                                 : validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
-                                                query_seqs + query_offset[k],
-                                                from_ref ? nullptr : ts.seqs + ts.offset[k]);
+            int r = ts.validated ? FA_OK : validate_scalar(query_len[k], ref_len[k], h0[k], band_width[k]);
             if (r == FA_OK && from_ref) {
                 // inside [0, 2 l_pac) and on one strand (get_reference_segment
                 // returns an empty segment for one that bridges, bntseq.rs:461-469)
@@ -888,29 +922,73 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         twork[t] += wk;
     });
     for (int v : ttmax) tmax = std::max(tmax, v);
+    if (!ts.validated && std::any_of(vrc.begin(), vrc.end(), [](int8_t r) { return r != FA_OK; })) {
+        // an invalid job: the batch is rejected with the first invalid job's
+        // error, which the full checks (codes included) decide
+        parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
+            for (int64_t k = lo; k < hi; ++k) {
+                int r = validate_job(query_len[k], ref_len[k], h0[k], band_width[k], query_seqs + query_offset[k],
+                                     from_ref ? nullptr : ts.seqs + ts.offset[k]);
+                if (r == FA_OK && from_ref) {
+                    const int64_t b0 = ts.seg_beg[k], e = b0 + ref_len[k], lp = ts.ref->l_pac;
+                    if (b0 < 0 || e > 2 * lp || (b0 < lp && e > lp)) r = FA_ERR_SEGMENT;
+                }
+                vrc[(size_t)k] = (int8_t)r;
+            }
+        });
+    }
     for (int x = 0; x < 16; ++x) {
         b->long_qmax = std::max(b->long_qmax, tqmax[(size_t)x]);
         b->nlong += tnlong[(size_t)x];
         b->long_work += twork[(size_t)x];
     }
-    for (int64_t k = 0; k < n; ++k) {
-        int r = vrc[(size_t)k];
-        if (r != FA_OK) {
-            first_err = r;
+    for (int64_t k = 0; k < n; ++k)
+        if (vrc[(size_t)k] != FA_OK) {
+            first_err = vrc[(size_t)k];
             break;
         }
+    if (first_err == FA_OK && n > 0) {
         // a pair's query and target words, back to back, start where they
         // touch the fewest 128-byte lines (32 words): the kernels read each
-        // pair's words alone, in packed order, and HBM is read in whole lines
-        const uint64_t blk = (uint64_t)seq_words(query_len[k]) + seq_words(ref_len[k]);
-        if (((words & 31u) + blk + 31u) / 32u > (blk + 31u) / 32u) words = (words + 31u) & ~(uint64_t)31u;
-        qo[(size_t)k] = (uint32_t)words;
-        words += seq_words(query_len[k]);
-        to[(size_t)k] = (uint32_t)words;
-        words += seq_words(ref_len[k]);
+        // pair's words alone, in packed order, and HBM is read in whole
+        // lines.  Placed per chunk of jobs in parallel, each chunk from a line
+        // start, then shifted by the prefix of the chunks' sizes.
+        const int64_t nch = std::min<int64_t>(256, (n + 16383) / 16384);
+        std::vector<uint64_t> base((size_t)nch + 1, 0);
+        parallel_for(
+            nch,
+            [&](int64_t c0, int64_t c1, unsigned) {
+                for (int64_t c = c0; c < c1; ++c) {
+                    uint64_t wds = 0;
+                    for (int64_t k = n * c / nch; k < n * (c + 1) / nch; ++k) {
+                        const uint64_t blk = (uint64_t)seq_words(query_len[k]) + seq_words(ref_len[k]);
+                        if (((wds & 31u) + blk + 31u) / 32u > (blk + 31u) / 32u)
+                            wds = (wds + 31u) & ~(uint64_t)31u;
+                        wds = std::min<uint64_t>(wds, 0xFFFFFFFFull);  // a total this big is rejected below
+                        qo[(size_t)k] = (uint32_t)wds;
+                        wds += seq_words(query_len[k]);
+                        to[(size_t)k] = (uint32_t)std::min<uint64_t>(wds, 0xFFFFFFFFull);
+                        wds += seq_words(ref_len[k]);
+                    }
+                    base[(size_t)c + 1] = (wds + 31u) & ~(uint64_t)31u;
+                }
+            },
+            2);
+        for (int64_t c = 0; c < nch; ++c) base[(size_t)c + 1] += base[(size_t)c];
+        words = base[(size_t)nch];
         if (words > 0xFFFFFFF0ull) {
             first_err = FA_ERR_ARG;
-            break;
+        } else {
+            parallel_for(
+                nch,
+                [&](int64_t c0, int64_t c1, unsigned) {
+                    for (int64_t c = c0; c < c1; ++c)
+                        for (int64_t k = n * c / nch; k < n * (c + 1) / nch; ++k) {
+                            qo[(size_t)k] += (uint32_t)base[(size_t)c];
+                            to[(size_t)k] += (uint32_t)base[(size_t)c];
+                        }
+                },
+                2);
         }
     }
     if (first_err != FA_OK) {
@@ -925,7 +1003,12 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         return rc;
     }
     uint32_t *seq2 = (uint32_t *)ctx->pin, *seqn = seq2 + words;
+    // every job passed the checks that read no sequence; unless the caller
+    // validated the codes too, they are checked while packing
+    const bool check = !ts.validated;
+    std::atomic<bool> bad_code{false};
     parallel_for(n, [&](int64_t lo, int64_t hi, unsigned) {
+        bool bad = false;
         for (int64_t k = lo; k < hi; ++k) {
             // the alignment gap before the pair (never read; zeroed so that
             // uploads are deterministic)
@@ -934,40 +1017,89 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                 std::memset(&seq2[gap0], 0, (qo[(size_t)k] - gap0) * sizeof(uint32_t));
                 std::memset(&seqn[gap0 / 2], 0, (qo[(size_t)k] - gap0) / 2 * sizeof(uint32_t));
             }
-            pack_seq(query_seqs + query_offset[k], query_len[k], &seq2[qo[(size_t)k]],
-                     &seqn[qo[(size_t)k] / 2]);
+            const uint8_t *qs = query_seqs + query_offset[k];
+            uint32_t *qw = &seq2[qo[(size_t)k]], *qn = &seqn[qo[(size_t)k] / 2];
+            if (check)
+                bad |= !fa_host::pack_codes_checked(qs, query_len[k], qw, qn);
+            else
+                pack_seq(qs, query_len[k], qw, qn);
             if (!from_ref) {
-                pack_seq(ts.seqs + ts.offset[k], ref_len[k], &seq2[to[(size_t)k]],
-                         &seqn[to[(size_t)k] / 2]);
+                const uint8_t *tsq = ts.seqs + ts.offset[k];
+                uint32_t *tw = &seq2[to[(size_t)k]], *tn = &seqn[to[(size_t)k] / 2];
+                if (check)
+                    bad |= !fa_host::pack_codes_checked(tsq, ref_len[k], tw, tn);
+                else
+                    pack_seq(tsq, ref_len[k], tw, tn);
             } else {
                 const uint32_t nw = seq_words(ref_len[k]);
                 std::memset(&seq2[to[(size_t)k]], 0, nw * sizeof(uint32_t));
                 std::memset(&seqn[to[(size_t)k] / 2], 0, nw / 2 * sizeof(uint32_t));
             }
         }
+        if (bad) bad_code.store(true, std::memory_order_relaxed);
     });
+    if (n > 0) {  // the last chunk's padding to its line
+        const uint64_t end = (uint64_t)to[(size_t)n - 1] + seq_words(ref_len[n - 1]);
+        std::memset(&seq2[end], 0, (size_t)(words - end) * sizeof(uint32_t));
+        std::memset(&seqn[end / 2], 0, (size_t)(words - end) / 2 * sizeof(uint32_t));
+    }
+    if (bad_code.load()) {
+        // every job passed the other checks, so the first invalid one is a
+        // code error
+        delete b;
+        return FA_ERR_CODE;
+    }
 
     hipStream_t st = ctx->stream;
-    if ((rc = dev_upload(&b->d_seq2, seq2, (size_t)words, st)) ||
-        (rc = dev_upload(&b->d_seqn, seqn, (size_t)(words / 2), st)) ||
-        (rc = dev_upload(&b->d_qo, qo.data(), qo.size(), st)) ||
-        (rc = dev_upload(&b->d_to, to.data(), to.size(), st)) ||
-        (rc = dev_upload(&b->d_qlen, b->qlen.data(), b->qlen.size(), st)) ||
-        (rc = dev_upload(&b->d_tlen, b->tlen.data(), b->tlen.size(), st)) ||
-        (rc = dev_upload(&b->d_h0, b->h0.data(), b->h0.size(), st)) ||
-        (rc = dev_upload(&b->d_w, b->w.data(), b->w.size(), st)) ||
-        (rc = dev_upload(&b->d_elig, b->elig16.data(), b->elig16.size(), st)) ||
-        (rc = dev_alloc(&b->d_order, (size_t)n)) || (rc = dev_alloc(&b->d_meta, (size_t)n * 8)) ||
-        (rc = dev_alloc(&b->d_params, (size_t)n * 8)) ||
-        (rc = dev_alloc(&b->d_out, (size_t)n * 6)) || (rc = dev_alloc(&b->d_rec, (size_t)n * 8)) ||
-        (rc = dev_alloc(&b->d_cells, (size_t)n)) || (rc = dev_alloc(&b->d_queue, 64)) ||
-        (b->nlong > 0 &&
-         (rc = dev_alloc((uint8_t **)&b->d_long,
-                         b->long_bytes = fa::extend_long_scratch(b->long_qmax, (uint32_t)b->nlong))))) {
-        (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
-        batch_release(b);
-        return rc;
+    // the batch's device arrays, and the host data the first nine start from
+    const size_t un = (size_t)n;
+    b->long_bytes = b->nlong > 0 ? fa::extend_long_scratch(b->long_qmax, (uint32_t)b->nlong) : 0;
+    struct DevArray {
+        void **p;
+        size_t bytes;
+        const void *src;
+    } arrays[] = {
+        {(void **)&b->d_seq2, (size_t)words * 4, seq2},     {(void **)&b->d_seqn, (size_t)(words / 2) * 4, seqn},
+        {(void **)&b->d_qo, un * 4, qo.data()},             {(void **)&b->d_to, un * 4, to.data()},
+        {(void **)&b->d_qlen, un * 4, b->qlen.data()},      {(void **)&b->d_tlen, un * 4, b->tlen.data()},
+        {(void **)&b->d_h0, un * 4, b->h0.data()},          {(void **)&b->d_w, un * 4, b->w.data()},
+        {(void **)&b->d_elig, un, b->elig16.data()},        {(void **)&b->d_order, un * 4, nullptr},
+        {(void **)&b->d_meta, un * 32, nullptr},            {(void **)&b->d_params, un * 32, nullptr},
+        {(void **)&b->d_out, un * 24, nullptr},             {(void **)&b->d_rec, un * 32, nullptr},
+        {(void **)&b->d_cells, un * 4, nullptr},            {(void **)&b->d_queue, 256, nullptr},
+        {(void **)&b->d_long, b->long_bytes, nullptr},
+    };
+    auto al = [](size_t x) { return (std::max<size_t>(x, 1) + 255) & ~(size_t)255; };
+    if (in_arena) {
+        size_t total = 0;
+        for (const DevArray &d : arrays) total += al(d.bytes);
+        uint8_t *base = nullptr;
+        if ((rc = ctx_alloc(ctx, kSlotBatchArena, &base, total))) {
+            delete b;
+            return rc;
+        }
+        b->in_arena = true;
+        for (const DevArray &d : arrays) {
+            *d.p = base;
+            base += al(d.bytes);
+        }
+    } else {
+        for (const DevArray &d : arrays) {
+            if (d.p == (void **)&b->d_long && !d.bytes) continue;
+            if ((rc = hip_fail(hipMalloc(d.p, std::max<size_t>(d.bytes, 1)), "hipMalloc"))) {
+                batch_release(b);
+                return rc;
+            }
+        }
     }
+    for (const DevArray &d : arrays)
+        if (d.src && d.bytes &&
+            (rc = hip_fail(hipMemcpyAsync(*d.p, d.src, d.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync"))) {
+            (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
+            batch_release(b);
+            return rc;
+        }
+    if (in_arena && !b->nlong) b->d_long = nullptr;
     if (from_ref && n > 0) {
         int64_t *d_beg = nullptr;
         uint8_t *d_rev = nullptr;
@@ -1739,30 +1871,40 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, 
         !h0 || !band_width)
         return FA_ERR_ARG;
     // per-job validation (on the host threads): invalid jobs are reported and
-    // left out of the launch
+    // left out of the launch.  First the checks that read no sequence; the
+    // codes are checked while the upload packs them, and only when one is
+    // out of range are the jobs validated again in full, to find which.
     std::vector<int8_t> vrc((size_t)njobs);
-    parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned) {
-        for (int64_t k = lo; k < hi; ++k)
-            vrc[(size_t)k] = (int8_t)validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
-                                                  query_seqs + query_offset[k],
-                                                  ref_seqs + ref_offset[k]);
-    });
+    auto validate = [&](bool codes) {
+        parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned) {
+            for (int64_t k = lo; k < hi; ++k)
+                vrc[(size_t)k] = (int8_t)(codes ? validate_job(query_len[k], ref_len[k], h0[k], band_width[k],
+                                                               query_seqs + query_offset[k], ref_seqs + ref_offset[k])
+                                                : validate_scalar(query_len[k], ref_len[k], h0[k], band_width[k]));
+        });
+    };
+    validate(false);
     std::vector<int64_t> keep;
-    keep.reserve((size_t)njobs);
     int first_err = FA_OK;
-    for (int64_t k = 0; k < njobs; ++k) {
-        const int r = vrc[(size_t)k];
-        if (per_job_rc) per_job_rc[k] = r;
-        if (r == FA_OK)
-            keep.push_back(k);
-        else if (first_err == FA_OK)
-            first_err = r;
-        if (r != FA_OK) {
-            std::memset(&out[k], 0, sizeof(fa_out_score));
-            if (cells) cells[k] = 0;
+    auto apply = [&]() {
+        keep.clear();
+        keep.reserve((size_t)njobs);
+        first_err = FA_OK;
+        for (int64_t k = 0; k < njobs; ++k) {
+            const int r = vrc[(size_t)k];
+            if (per_job_rc) per_job_rc[k] = r;
+            if (r == FA_OK)
+                keep.push_back(k);
+            else if (first_err == FA_OK)
+                first_err = r;
+            if (r != FA_OK) {
+                std::memset(&out[k], 0, sizeof(fa_out_score));
+                if (cells) cells[k] = 0;
+            }
         }
-    }
-    if (!keep.empty()) {
+    };
+    apply();
+    for (int attempt = 0; attempt < 2 && !keep.empty(); ++attempt) {
         const bool all = (int64_t)keep.size() == njobs;
         std::vector<int64_t> qo2, to2;
         std::vector<int32_t> ql2, tl2, h02, w2;
@@ -1788,8 +1930,13 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, 
         TargetSource ts;
         ts.seqs = ref_seqs;
         ts.offset = to;
-        ts.validated = true;
-        rc = batch_upload(ctx, sw, (int64_t)keep.size(), query_seqs, qo, ql, ts, tl, hh, ww, &b);
+        ts.validated = attempt == 1;  // the first upload checks the codes while packing
+        rc = batch_upload(ctx, sw, (int64_t)keep.size(), query_seqs, qo, ql, ts, tl, hh, ww, &b, true);
+        if (rc == FA_ERR_CODE && attempt == 0) {
+            validate(true);  // which jobs: the full checks, then the upload again without them
+            apply();
+            continue;
+        }
         if (rc) return rc;
         rc = fa_batch_run(ctx, b, flags);
         std::vector<fa_out_score> o2;
@@ -1810,6 +1957,7 @@ int fa_execute_batch_simd_scoring(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, 
         }
         fa_batch_free(ctx, b);
         if (rc) return rc;
+        break;
     }
     return first_err;
 }

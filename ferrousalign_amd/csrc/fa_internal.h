@@ -26,8 +26,9 @@ struct fa_ctx {
     void *chain_ws = nullptr;  // device workspace of the chain-extension path (chain_api.cpp)
     void *pin = nullptr;       // pinned host staging for uploads and downloads (grow-only)
     size_t pin_bytes = 0;
-    void *slot[32] = {};       // grow-only device buffers of the CIGAR-regeneration calls
-    size_t slot_bytes[32] = {};
+    void *slot[32] = {};       // grow-only device buffers of the CIGAR-regeneration calls and of
+    size_t slot_bytes[32] = {};  // the one-call batches of fa_execute_batch_simd_scoring
+    void *hist_pin = nullptr;  // pinned packing histogram of those batches
 };
 
 namespace fa_host {
@@ -86,6 +87,7 @@ struct fa_batch {
     int tab[3][256];
     std::vector<int> kinds;
     fa::ExtArgs args{};
+    bool in_arena = false;      // arrays, sort scratch and histogram belong to the context
     // launch plan for the current packing
     uint32_t packed_flags = 0xFFFFFFFFu;
     std::vector<fa_host::Bucket> buckets;

@@ -41,10 +41,12 @@ void pack_scalar(const uint8_t *s, int32_t len, uint32_t *w2, uint32_t *wn) {
     }
 }
 
-__attribute__((target("bmi2"))) void pack_bmi2(const uint8_t *s, int32_t len, uint32_t *w2,
+template <bool CHECK>
+__attribute__((target("bmi2"))) bool pack_bmi2(const uint8_t *s, int32_t len, uint32_t *w2,
                                                uint32_t *wn) {
     const int32_t full = len & ~31;
     uint8_t tail[32];
+    uint64_t bad = 0;
     for (int32_t p0 = 0; p0 < len; p0 += 32) {
         const uint8_t *src = s + p0;
         if (p0 >= full) {  // last, partial block: zero padding (code 0, not N)
@@ -56,6 +58,7 @@ __attribute__((target("bmi2"))) void pack_bmi2(const uint8_t *s, int32_t len, ui
         uint32_t lo = 0, hi = 0;
         for (int g = 0; g < 4; ++g) {
             const uint64_t x = load8(src + 8 * g);
+            if constexpr (CHECK) bad |= (((x & kLow7) + kAdd) | x) & kHigh;
             const uint32_t c = (uint32_t)_pext_u64(x, 0x0303030303030303ull);
             n |= (uint32_t)_pext_u64(x, 0x0404040404040404ull) << (8 * g);
             if (g < 2)
@@ -67,6 +70,7 @@ __attribute__((target("bmi2"))) void pack_bmi2(const uint8_t *s, int32_t len, ui
         w2[(p0 >> 4) + 1] = hi;
         wn[p0 >> 5] = n;
     }
+    return bad == 0;
 }
 
 bool have_bmi2() {
@@ -96,9 +100,16 @@ bool codes_ok(const uint8_t *p, int64_t len) {
 
 void pack_codes(const uint8_t *s, int32_t len, uint32_t *w2, uint32_t *wn) {
     if (have_bmi2())
-        pack_bmi2(s, len, w2, wn);
+        pack_bmi2<false>(s, len, w2, wn);
     else
         pack_scalar(s, len, w2, wn);
+}
+
+bool pack_codes_checked(const uint8_t *s, int32_t len, uint32_t *w2, uint32_t *wn) {
+    if (have_bmi2()) return pack_bmi2<true>(s, len, w2, wn);
+    if (!codes_ok(s, len)) return false;
+    pack_scalar(s, len, w2, wn);
+    return true;
 }
 
 }  // namespace fa_host

@@ -217,7 +217,8 @@ int fa_simd_banded_swa_batch_int16_soa(fa_ctx *ctx, int32_t lanes, const int16_t
  * (w2[p/16] >> 2*(p%16)) & 3, or N (code 4) when (wn[p/32] >> p%32) & 1.
  * w2 must hold 2*ceil(len/32) words and wn ceil(len/32) words.  The 2-bit
  * order inside a word differs from the reference's .pac (MSB-first, 4 bases
- * per byte, index/bntseq.rs:563-575); N has no 2-bit code there. */
+ * per byte, index/bntseq.rs:563-575); N has no 2-bit code there.
+ * FA_ERR_CODE when a code exceeds 4 (w2 and wn are then undefined). */
 int fa_pack_2bit(const uint8_t *codes, int32_t len, uint32_t *w2, uint32_t *wn);
 int fa_unpack_2bit(const uint32_t *w2, const uint32_t *wn, int32_t len, uint8_t *codes);
 

@@ -172,3 +172,52 @@ def test_reference_right_jobs_h0_zero():
         want_cells=True)
     assert code2 == 0 and (rc2 == 0).all()
     assert (out2 == _oracle(sw, R2)).all()
+
+
+def test_invalid_codes_and_scalar_errors_keep_their_order():
+    """The uploads check the base codes while packing and validate in full
+    only when something is wrong; the per-job codes and the batch's code
+    must be those of the reference's checks in their order (h0, qlen, tlen,
+    w, then the query's codes, then the target's: validate order of
+    ksw_affine_gap.rs:968-985), and every valid job is still scored."""
+    sw = synth.default_scoring()
+    ps = synth.workload("uneven_mix_4m", n=3000)
+    q, qo, ql, t, to, tl, h0, w = [np.array(x) for x in ps.flat()]
+    q, t, h0 = q.copy(), t.copy(), h0.copy()
+    want = np.zeros(ps.n, np.int32)
+    rng = np.random.default_rng(21)
+    picks = rng.choice(ps.n, 60, replace=False)
+    for j, k in enumerate(picks):
+        kind = j % 4
+        if kind in (0, 3):        # a query code above 4 (at a random base, 5..255)
+            q[qo[k] + int(rng.integers(0, ql[k]))] = int(rng.integers(5, 256))
+        if kind == 1 and tl[k] > 0:  # a target code above 4
+            t[to[k] + int(rng.integers(0, tl[k]))] = int(rng.integers(5, 256))
+        if kind in (2, 3):        # h0 = 0: decides before any code
+            h0[k] = 0
+        want[k] = N.FA_ERR_H0 if kind in (2, 3) else (N.FA_ERR_CODE if kind == 0 or tl[k] > 0 else 0)
+    print("codes and scalar errors", flush=True)
+    out, cells, rc, code = B.extend_arrays(sw, q, qo, ql, t, to, tl, h0, w, want_cells=True, per_job_rc=True)
+    assert np.array_equal(rc, want)
+    first = want[np.nonzero(want)[0][0]]
+    assert code == first
+    bad = want != 0
+    assert (out[bad] == 0).all() and (cells[bad] == 0).all()
+    ora, ocells, orc = O.extend_batch(sw, q, qo, ql, t, to, tl, h0, w)
+    assert np.array_equal(orc, want)  # the oracle's checks, in the same order
+    ora = O.to_outscore_order(ora)
+    assert (out[~bad] == ora[~bad]).all() and (cells[~bad] == ocells[~bad]).all()
+    # a whole-batch upload reports the first invalid job's error
+    for kinds in ((0,), (1,), (2,), (0, 2), (2, 0)):
+        qq, tt, hh = ps.q.reshape(-1).copy(), ps.t.reshape(-1).copy(), np.array(ps.h0).copy()
+        for pos, kind in zip((100, 200), kinds):
+            if kind == 0:
+                qq[qo[pos] + 1] = 9
+            elif kind == 1:
+                tt[to[pos] + 2] = 9
+            else:
+                hh[pos] = 0
+        expect = {0: N.FA_ERR_CODE, 1: N.FA_ERR_CODE, 2: N.FA_ERR_H0}[kinds[0]]
+        with pytest.raises(N.FaError) as e:
+            B.DeviceBatch(sw, qq, qo, ql, tt, to, tl, hh, w)
+        assert e.value.code == expect, kinds

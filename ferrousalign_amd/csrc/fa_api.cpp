@@ -7,6 +7,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -165,37 +166,6 @@ void parallel_for(int64_t n, F &&f, int64_t serial_below = 65536) {
     for (auto &x : th) x.join();
 }
 
-// Stable ascending order of keys[0..n) (LSD radix sort, 11-bit digits, the
-// keys carried along; a digit every key shares is skipped): equal keys keep
-// their index order, as std::stable_sort would, at a fraction of its cost on
-// millions of pairs.
-std::vector<uint32_t> radix_order(std::vector<uint64_t> keys) {
-    const size_t n = keys.size();
-    std::vector<uint32_t> a(n), b(n);
-    for (size_t k = 0; k < n; ++k) a[k] = (uint32_t)k;
-    uint64_t any = 0;
-    for (uint64_t x : keys) any |= x;
-    std::vector<uint64_t> kb(n);
-    std::vector<uint32_t> cnt(2049);
-    for (int sh = 0; sh < 64 && (any >> sh) != 0; sh += 11) {
-        std::fill(cnt.begin(), cnt.end(), 0u);
-        for (size_t k = 0; k < n; ++k) ++cnt[((keys[k] >> sh) & 2047u) + 1];
-        bool one = false;
-        for (int d = 1; d <= 2048; ++d)
-            if (cnt[d] == n) one = true;
-        if (one) continue;
-        for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
-        for (size_t k = 0; k < n; ++k) {
-            const uint32_t pos = cnt[(keys[k] >> sh) & 2047u]++;
-            b[pos] = a[k];
-            kb[pos] = keys[k];
-        }
-        a.swap(b);
-        keys.swap(kb);
-    }
-    return a;
-}
-
 using fa_host::seq_words;
 
 // t == nullptr: the target comes from a device reference (codes 0-3 by construction)
@@ -229,7 +199,7 @@ int dev_alloc(T **p, size_t count) {
     return hip_fail(hipMalloc((void **)p, count * sizeof(T)), "hipMalloc");
 }
 
-// context slots of the one-call batches (the CIGAR path uses 0-25)
+// context slots of the one-call batches (the CIGAR path uses 0-11)
 constexpr int kSlotBatchArena = 28, kSlotBatchSort = 29;
 
 // Grow-only device buffer slot k of the context (CIGAR-regeneration calls,
@@ -786,6 +756,7 @@ int fa_ctx_destroy(fa_ctx *ctx) {
     fa_host::chain_workspace_free(ctx);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->hist_pin) (void)hipHostFree(ctx->hist_pin);
+    if (ctx->hscr) (void)hipHostFree(ctx->hscr);
     for (void *p : ctx->slot)
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -1264,6 +1235,75 @@ struct SwtbOut {
     int64_t *cigar_off_out, *ref_off_out, *query_off_out;
 };
 
+// Context slots of the CIGAR-regeneration calls.
+enum SwtbSlot {
+    kSwtbSeq2, kSwtbSeqN, kSwtbBlock, kSwtbSort, kSwtbTrace, kSwtbCigar, kSwtbRefAln,
+    kSwtbQryAln, kSwtbDense, kSwtbOutCigar, kSwtbOutRef, kSwtbOutQry
+};
+static_assert(kSwtbOutQry < kSlotBatchArena, "slots of the CIGAR calls and the one-call batches");
+
+// Per-pair arrays of a CIGAR-regeneration call in one block, laid out alike
+// in the context's pinned host scratch and in one device slot: what the
+// kernels read ([0, up), one upload), the per-pair results ([up, down), one
+// download) and the host's own arrays ([down, end)).  256-byte aligned.
+struct SwtbBlock {
+    size_t qo, to, ql, tl, ww, hh, lsl, tbo, cgo, alo, key, idx, up;
+    size_t out, cells, nc, nr, nq, down;
+    size_t dense, job, end;
+    SwtbBlock(size_t m, bool with_job) {
+        size_t o = 0;
+        auto take = [&o](size_t bytes) {
+            const size_t at = o;
+            o += (bytes + 255) & ~(size_t)255;
+            return at;
+        };
+        qo = take(4 * m);
+        to = take(4 * m);
+        ql = take(4 * m);
+        tl = take(4 * m);
+        ww = take(4 * m);
+        hh = take(4 * m);
+        lsl = take(4 * m);
+        tbo = take(8 * m);
+        cgo = take(8 * m);
+        alo = take(8 * m);
+        key = take(4 * m);
+        idx = take(4 * m);
+        up = o;
+        out = take(24 * m);
+        cells = take(4 * m);
+        nc = take(4 * m);
+        nr = take(4 * m);
+        nq = take(4 * m);
+        down = o;
+        dense = take(24 * m);
+        job = take(with_job ? 8 * m : 0);
+        end = o;
+    }
+};
+
+// Grow-only pinned host scratch of the context for those arrays (a call
+// synchronises its copies before it returns, so the next call may reuse it).
+int host_scratch(fa_ctx *ctx, size_t bytes) {
+    if (ctx->hscr_bytes >= bytes) return FA_OK;
+    if (ctx->hscr) (void)hipHostFree(ctx->hscr);
+    ctx->hscr = nullptr;
+    ctx->hscr_bytes = 0;
+    const size_t want = bytes + bytes / 8 + 4096;
+    FA_HIP(hipHostMalloc(&ctx->hscr, want));
+    ctx->hscr_bytes = want;
+    return FA_OK;
+}
+
+// Bucket code of a pair's kernel (L in {4, 8, 16}, S < 64): band flag,
+// log2(L) - 2 and S, ordered as lsl = kSwtbBand | L << 8 | S is.
+constexpr int kSwtbCodes = 512;
+constexpr int kSwtbKeyBits = 29;  // code (9 bits) << 20 | 255 - qlen << 12 | 4095 - tlen
+inline uint32_t swtb_code(uint32_t lsl) {
+    const uint32_t L = (lsl >> 8) & 0xFFu, lg = L == 4 ? 0u : (L == 8 ? 1u : 2u);
+    return ((lsl & fa::kSwtbBand) ? 256u : 0u) | (lg << 6) | (lsl & 63u);
+}
+
 int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                const uint8_t *query_seqs, const int64_t *query_offset, const int32_t *query_len,
                const uint8_t *ref_seqs, const int64_t *ref_offset, const int32_t *ref_len,
@@ -1271,11 +1311,14 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                const SwtbOut &so) {
     int rc;
     if (njobs > (int64_t)0x7FFFFFFF) return FA_ERR_ARG;
-    // validation (h0 = 0 is allowed here: region.rs:1058 passes 0), in parallel;
-    // the code returned is the one of the first invalid job
+    // validation (h0 = 0 is allowed here: region.rs:1058 passes 0) and the
+    // count of non-empty jobs, in parallel; the code returned is the one of the
+    // first invalid job
+    std::vector<int64_t> filled(16, 0);  // non-empty jobs of each thread's range
     {
         std::vector<std::pair<int64_t, int>> first_bad(16, {njobs, FA_OK});
         parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned t) {
+            int64_t nonempty = 0;
             for (int64_t k = lo; k < hi; ++k) {
                 const int32_t ql = query_len[k], tl = ref_len[k];
                 int c = FA_OK;
@@ -1290,7 +1333,9 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                     first_bad[t] = {k, c};
                     return;
                 }
+                nonempty += ql > 0 && tl > 0;
             }
+            filled[t] = nonempty;
         });
         std::pair<int64_t, int> bad{njobs, FA_OK};
         for (const auto &b : first_bad)
@@ -1298,140 +1343,189 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         if (bad.second != FA_OK) return bad.second;
     }
     FA_HIP(hipSetDevice(ctx->device));
-    // non-empty jobs run on the device; empty ones get the reference's zero
-    // record and an empty CIGAR (implementation.rs:17-30)
-    std::vector<int64_t> job;
-    for (int64_t k = 0; k < njobs; ++k) {
-        if (query_len[k] > 0 && ref_len[k] > 0) {
-            job.push_back(k);
-        } else {
-            std::memset(&out[k], 0, sizeof(fa_out_score));
-            if (cells) cells[k] = 0;
-            if (!so.dense) so.n_cigar[k] = so.n_ref_aligned[k] = so.n_query_aligned[k] = 0;
-        }
+    size_t m = 0;
+    for (int64_t x : filled) m += (size_t)x;
+    const bool all = m == (size_t)njobs;  // no empty job: pair p is job p
+    // per-pair arrays in the context's pinned scratch, reused from call to call
+    const SwtbBlock B(m, !all);
+    if ((rc = host_scratch(ctx, B.end))) return rc;
+    uint8_t *const hb = (uint8_t *)ctx->hscr;
+    uint32_t *const qo = (uint32_t *)(hb + B.qo), *const to = (uint32_t *)(hb + B.to);
+    int32_t *const ql = (int32_t *)(hb + B.ql), *const tl = (int32_t *)(hb + B.tl);
+    int32_t *const ww = (int32_t *)(hb + B.ww), *const hh = (int32_t *)(hb + B.hh);
+    uint32_t *const lsl = (uint32_t *)(hb + B.lsl), *const key = (uint32_t *)(hb + B.key);
+    uint32_t *const idx = (uint32_t *)(hb + B.idx);
+    uint64_t *const tbo = (uint64_t *)(hb + B.tbo), *const cgo = (uint64_t *)(hb + B.cgo);
+    uint64_t *const alo = (uint64_t *)(hb + B.alo), *const dense = (uint64_t *)(hb + B.dense);
+    const int32_t *const hout = (const int32_t *)(hb + B.out);
+    const uint32_t *const hcells = (const uint32_t *)(hb + B.cells);
+    const int32_t *const hnc = (const int32_t *)(hb + B.nc), *const hnr = (const int32_t *)(hb + B.nr);
+    const int32_t *const hnq = (const int32_t *)(hb + B.nq);
+    int64_t *const job = all ? nullptr : (int64_t *)(hb + B.job);
+    if (!all) {
+        // non-empty jobs run on the device, listed in job order (each thread's
+        // from the counts of the ranges before it); empty ones get the
+        // reference's zero record and an empty CIGAR (implementation.rs:17-30)
+        parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned t) {
+            int64_t p = 0;
+            for (unsigned u = 0; u < t; ++u) p += filled[u];
+            for (int64_t k = lo; k < hi; ++k) {
+                if (query_len[k] > 0 && ref_len[k] > 0) {
+                    job[p++] = k;
+                    continue;
+                }
+                std::memset(&out[k], 0, sizeof(fa_out_score));
+                if (cells) cells[k] = 0;
+                if (!so.dense) so.n_cigar[k] = so.n_ref_aligned[k] = so.n_query_aligned[k] = 0;
+            }
+        });
     }
-    const size_t m = job.size();
     if (m == 0) {
         if (so.dense)
             for (int64_t k = 0; k <= njobs; ++k)
                 so.cigar_off_out[k] = so.ref_off_out[k] = so.query_off_out[k] = 0;
         return FA_OK;
     }
-    std::vector<uint32_t> qo(m), to(m), lsl(m), order(m);
-    std::vector<int32_t> ql(m), tl(m), ww(m), hh(m);
-    std::vector<uint64_t> tbo(m), cgo(m), alo(m);
-    uint64_t words = 0, tbw = 0, cgw = 0, alw = 0;
-    int tmax = 0, qmax = 0;
-    std::vector<int> ttm(16, 0), tqm(16, 0);
+    auto jobof = [&](size_t p) -> int64_t { return all ? (int64_t)p : job[p]; };
+    // per pair: kernel, sort key, sizes of its sequence words, traceback block,
+    // CIGAR and aligned-base slots; per thread range: their sums, the longest
+    // target and band-window query, and the pairs of each kernel
+    struct Range {
+        uint64_t words = 0, tbw = 0, cgw = 0, alw = 0;
+        int tmax = 0, qmax = 0;
+        uint32_t hist[kSwtbCodes] = {};
+    };
+    std::vector<Range> part(16);
     parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned t) {
+        Range r;
         for (int64_t p = lo; p < hi; ++p) {
-            const int64_t k = job[(size_t)p];
-            ql[p] = query_len[k];
-            tl[p] = ref_len[k];
+            const int64_t k = jobof((size_t)p);
+            const int32_t q = query_len[k], tt = ref_len[k];
+            ql[p] = q;
+            tl[p] = tt;
             ww[p] = band_width[k];
             hh[p] = h0[k];
             int L = 16;
             for (int c : {4, 8, 16})
-                if (fa::swtb_slot_count(c, ql[p]) > 0) {
+                if (fa::swtb_slot_count(c, q) > 0) {
                     L = c;
                     break;
                 }
-            int S = fa::swtb_slot_count(L, ql[p]);
-            lsl[p] = ((uint32_t)L << 8) | (uint32_t)S;
+            int S = fa::swtb_slot_count(L, q);
+            uint32_t ls = ((uint32_t)L << 8) | (uint32_t)S;
             int bl, bs;
-            if (fa::swtb_band_config(ql[p], ww[p], L * S, &bl, &bs)) {  // band-window kernel
+            if (fa::swtb_band_config(q, ww[p], L * S, &bl, &bs)) {  // band-window kernel
                 L = bl;
                 S = bs;
-                lsl[p] = fa::kSwtbBand | ((uint32_t)L << 8) | (uint32_t)S;
-                tqm[t] = std::max(tqm[t], ql[p]);
+                ls = fa::kSwtbBand | ((uint32_t)L << 8) | (uint32_t)S;
+                r.qmax = std::max(r.qmax, q);
             }
-            tbo[p] = (uint64_t)tl[p] * (uint64_t)L * (uint64_t)((S + 15) / 16);  // size, for now
-            ttm[t] = std::max(ttm[t], tl[p]);
+            r.tmax = std::max(r.tmax, tt);
+            lsl[p] = ls;
+            const uint32_t code = swtb_code(ls);
+            ++r.hist[code];
+            // ascending keys: kernels by descending lsl, then qlen and tlen descending
+            key[p] = ((uint32_t)(kSwtbCodes - 1 - code) << 20) | ((uint32_t)(255 - q) << 12) |
+                     (uint32_t)(4095 - tt);
+            idx[p] = (uint32_t)p;
+            qo[p] = seq_words(q);  // sizes here, offsets in the packing pass
+            to[p] = seq_words(tt);
+            tbo[p] = (uint64_t)tt * (uint64_t)L * (uint64_t)((S + 15) / 16);
+            r.words += (uint64_t)qo[p] + to[p];
+            r.tbw += tbo[p];
+            r.cgw += (uint64_t)(q + tt + 4);
+            r.alw += (uint64_t)(q + tt + 2);
         }
+        part[t] = r;
     });
-    for (int x : ttm) tmax = std::max(tmax, x);
-    for (int x : tqm) qmax = std::max(qmax, x);
-    for (size_t p = 0; p < m; ++p) {
-        qo[p] = (uint32_t)words;
-        words += seq_words(ql[p]);
-        to[p] = (uint32_t)words;
-        words += seq_words(tl[p]);
-        const uint64_t tbs = tbo[p];
-        tbo[p] = tbw;
-        tbw += tbs;
-        cgo[p] = cgw;
-        cgw += (uint64_t)(ql[p] + tl[p] + 4);
-        alo[p] = alw;
-        alw += (uint64_t)(ql[p] + tl[p] + 2);
-        order[p] = (uint32_t)p;
+    uint64_t words = 0, tbw = 0, cgw = 0, alw = 0;
+    int tmax = 0, qmax = 0;
+    std::vector<std::array<uint64_t, 4>> start(16);  // each range's first offsets
+    for (size_t t = 0; t < part.size(); ++t) {
+        start[t] = {words, tbw, cgw, alw};
+        words += part[t].words;
+        tbw += part[t].tbw;
+        cgw += part[t].cgw;
+        alw += part[t].alw;
+        tmax = std::max(tmax, part[t].tmax);
+        qmax = std::max(qmax, part[t].qmax);
     }
     if (words > 0xFFFFFFF0ull) return FA_ERR_ARG;
-    {  // stable order by (kernel key, qlen, tlen), all descending
-        std::vector<uint64_t> k64(m);
-        for (size_t p = 0; p < m; ++p)
-            k64[p] = ((uint64_t)(0x1FFFFu - lsl[p]) << 20) | ((uint64_t)(255 - ql[p]) << 12) |
-                     (uint64_t)(4095 - tl[p]);
-        order = radix_order(std::move(k64));
-    }
-    // packed sequences staged in the context's pinned buffer (every word is
-    // written: sequences are packed back to back)
+    // offsets, then the packed sequences staged in the context's pinned buffer
+    // (every word is written: sequences are packed back to back)
     if ((rc = pinned_staging(ctx, (size_t)words * 4 + (size_t)(words / 2) * 4))) return rc;
     uint32_t *seq2 = (uint32_t *)ctx->pin, *seqn = seq2 + words;
-    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
+    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned t) {
+        uint64_t wo = start[t][0], bo = start[t][1], co = start[t][2], ao = start[t][3];
         for (int64_t p = lo; p < hi; ++p) {
-            const int64_t k = job[(size_t)p];
+            const uint32_t qw = qo[p], tw = to[p];
+            qo[p] = (uint32_t)wo;
+            to[p] = (uint32_t)(wo + qw);
+            wo += (uint64_t)qw + tw;
+            const uint64_t tbs = tbo[p];
+            tbo[p] = bo;
+            bo += tbs;
+            cgo[p] = co;
+            co += (uint64_t)(ql[p] + tl[p] + 4);
+            alo[p] = ao;
+            ao += (uint64_t)(ql[p] + tl[p] + 2);
+            const int64_t k = jobof((size_t)p);
             pack_seq(query_seqs + query_offset[k], ql[p], &seq2[qo[p]], &seqn[qo[p] / 2]);
             pack_seq(ref_seqs + ref_offset[k], tl[p], &seq2[to[p]], &seqn[to[p] / 2]);
         }
     });
     hipStream_t st = ctx->stream;
     // device buffers: grow-only slots of the context (the traceback block
-    // alone is ~2.5 KB per 150-base pair), reused by the next call
-    uint32_t *d_seq2, *d_seqn, *d_qo, *d_to, *d_order, *d_lsl, *d_tb, *d_cells, *d_cig;
-    int32_t *d_ql, *d_tl, *d_w, *d_h0, *d_out, *d_nc, *d_nr, *d_nq;
-    uint64_t *d_tbo, *d_cgo, *d_alo;
-    uint8_t *d_ra, *d_qa;
-    if ((rc = ctx_upload(ctx, 0, &d_seq2, seq2, (size_t)words)) ||
-        (rc = ctx_upload(ctx, 1, &d_seqn, seqn, (size_t)(words / 2))) ||
-        (rc = ctx_upload(ctx, 2, &d_qo, qo.data(), m)) || (rc = ctx_upload(ctx, 3, &d_to, to.data(), m)) ||
-        (rc = ctx_upload(ctx, 4, &d_ql, ql.data(), m)) || (rc = ctx_upload(ctx, 5, &d_tl, tl.data(), m)) ||
-        (rc = ctx_upload(ctx, 6, &d_w, ww.data(), m)) || (rc = ctx_upload(ctx, 7, &d_h0, hh.data(), m)) ||
-        (rc = ctx_upload(ctx, 8, &d_order, order.data(), m)) ||
-        (rc = ctx_upload(ctx, 9, &d_lsl, lsl.data(), m)) ||
-        (rc = ctx_upload(ctx, 10, &d_tbo, tbo.data(), m)) ||
-        (rc = ctx_upload(ctx, 11, &d_cgo, cgo.data(), m)) ||
-        (rc = ctx_upload(ctx, 12, &d_alo, alo.data(), m)) ||
-        (rc = ctx_alloc(ctx, 13, &d_tb, (size_t)tbw)) || (rc = ctx_alloc(ctx, 14, &d_out, m * 6)) ||
-        (rc = ctx_alloc(ctx, 15, &d_cells, m)) || (rc = ctx_alloc(ctx, 16, &d_cig, (size_t)cgw)) ||
-        (rc = ctx_alloc(ctx, 17, &d_nc, m)) || (rc = ctx_alloc(ctx, 18, &d_nr, m)) ||
-        (rc = ctx_alloc(ctx, 19, &d_nq, m)) || (rc = ctx_alloc(ctx, 20, &d_ra, (size_t)alw)) ||
-        (rc = ctx_alloc(ctx, 21, &d_qa, (size_t)alw))) {
-        (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
+    // alone is ~2.5 KB per 150-base pair), reused by the next call; the
+    // per-pair block goes up in one copy from the pinned scratch, and the
+    // order of the pairs is sorted on the device
+    uint32_t *d_seq2, *d_seqn, *d_tb, *d_cig;
+    uint8_t *db, *ds, *d_ra, *d_qa;
+    size_t sort_tmp = 0;
+    const size_t s_order = (4 * m + 255) & ~(size_t)255, s_tmp = 2 * s_order;
+    FA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (const uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (int)m, 0, kSwtbKeyBits, st));
+    if ((rc = ctx_upload(ctx, kSwtbSeq2, &d_seq2, seq2, (size_t)words)) ||
+        (rc = ctx_upload(ctx, kSwtbSeqN, &d_seqn, seqn, (size_t)(words / 2))) ||
+        (rc = ctx_alloc(ctx, kSwtbBlock, &db, B.down)) ||
+        (rc = hip_fail(hipMemcpyAsync(db, hb, B.up, hipMemcpyHostToDevice, st), "hipMemcpyAsync")) ||
+        (rc = ctx_alloc(ctx, kSwtbSort, &ds, s_tmp + sort_tmp)) ||
+        (rc = hip_fail(hipcub::DeviceRadixSort::SortPairs(
+                           ds + s_tmp, sort_tmp, (const uint32_t *)(db + B.key), (uint32_t *)ds,
+                           (const uint32_t *)(db + B.idx), (uint32_t *)(ds + s_order), (int)m, 0,
+                           kSwtbKeyBits, st),
+                       "hipcub::DeviceRadixSort::SortPairs")) ||
+        (rc = ctx_alloc(ctx, kSwtbTrace, &d_tb, (size_t)tbw)) ||
+        (rc = ctx_alloc(ctx, kSwtbCigar, &d_cig, (size_t)cgw)) ||
+        (rc = ctx_alloc(ctx, kSwtbRefAln, &d_ra, (size_t)alw)) ||
+        (rc = ctx_alloc(ctx, kSwtbQryAln, &d_qa, (size_t)alw))) {
+        (void)hipStreamSynchronize(st);  // copies from the staging buffers may be in flight
         return rc;
     }
     fa::SwtbArgs a{};
     a.seq2 = d_seq2;
     a.seqn = d_seqn;
-    a.qo = d_qo;
-    a.to = d_to;
-    a.qlen = d_ql;
-    a.tlen = d_tl;
-    a.w = d_w;
-    a.h0 = d_h0;
-    a.order = d_order;
-    a.tbo = d_tbo;
-    a.lsl = d_lsl;
+    a.qo = (const uint32_t *)(db + B.qo);
+    a.to = (const uint32_t *)(db + B.to);
+    a.qlen = (const int32_t *)(db + B.ql);
+    a.tlen = (const int32_t *)(db + B.tl);
+    a.w = (const int32_t *)(db + B.ww);
+    a.h0 = (const int32_t *)(db + B.hh);
+    a.order = (const uint32_t *)(ds + s_order);
+    a.tbo = (const uint64_t *)(db + B.tbo);
+    a.lsl = (const uint32_t *)(db + B.lsl);
     a.tb = d_tb;
-    a.out = d_out;
-    a.cells = d_cells;
-    a.cig_off = d_cgo;
+    a.out = (int32_t *)(db + B.out);
+    a.cells = (uint32_t *)(db + B.cells);
+    a.cig_off = (const uint64_t *)(db + B.cgo);
     a.cigar = d_cig;
-    a.n_cigar = d_nc;
-    a.aln_off = d_alo;
+    a.n_cigar = (int32_t *)(db + B.nc);
+    a.aln_off = (const uint64_t *)(db + B.alo);
     a.ref_aln = d_ra;
     a.qry_aln = d_qa;
-    a.n_ref = d_nr;
-    a.n_qry = d_nq;
+    a.n_ref = (int32_t *)(db + B.nr);
+    a.n_qry = (int32_t *)(db + B.nq);
     for (int q = 0; q < 5; ++q) {
         uint32_t v = 0;
         for (int t = 0; t < 4; ++t) v |= (uint32_t)(uint8_t)(sw->mat[t * 5 + q] + 128) << (8 * t);
@@ -1447,71 +1541,97 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     a.tpad = ((tmax + 16) / 16) * 16;
     a.qpad = ((qmax + 16) / 16) * 16;
     hipError_t e = hipEventRecord(ctx->ev0, st);
-    for (size_t x = 0; x < m && e == hipSuccess;) {  // one launch per (kernel, L, S) bucket
-        size_t y = x;
-        while (y < m && lsl[order[y]] == lsl[order[x]]) ++y;
-        const uint32_t key = lsl[order[x]];
-        e = fa::swtb_dp_launch((int)((key >> 8) & 0xFFu), (int)(key & 0xFFu),
-                               (key & fa::kSwtbBand) != 0, a, (uint32_t)x, (uint32_t)(y - x), st);
-        x = y;
+    {  // one launch per (kernel, L, S) bucket, in the sorted order: descending codes
+        uint32_t first = 0;
+        for (int code = kSwtbCodes - 1; code >= 0 && e == hipSuccess; --code) {
+            uint32_t cnt = 0;
+            for (const Range &r : part) cnt += r.hist[code];
+            if (cnt == 0) continue;
+            e = fa::swtb_dp_launch(4 << ((code >> 6) & 3), code & 63, (code & 256) != 0, a, first,
+                                   cnt, st);
+            first += cnt;
+        }
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
     if (e == hipSuccess) e = fa::swtb_traceback_launch(a, (uint32_t)m, st);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev2, st);
-    // records and counts first; then the CIGARs and aligned bases are gathered
-    // on the device into dense buffers (prefix sums of the counts) so only the
-    // used entries cross PCIe
-    std::vector<int32_t> hout(m * 6), hnc(m), hnr(m), hnq(m);
-    std::vector<uint32_t> hcells(m);
-    if (e == hipSuccess) e = hipMemcpyAsync(hout.data(), d_out, m * 6 * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hcells.data(), d_cells, m * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hnc.data(), d_nc, m * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hnr.data(), d_nr, m * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hnq.data(), d_nq, m * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    // records and counts first (one copy into the pinned scratch); then the
+    // CIGARs and aligned bases are gathered on the device into dense buffers
+    // (prefix sums of the counts) so only the used entries cross PCIe
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hb + B.out, db + B.out, B.down - B.out, hipMemcpyDeviceToHost, st);
+    {
+        const hipError_t es = hipStreamSynchronize(st);  // uploads from the scratch end here
+        if (e == hipSuccess) e = es;
+    }
     float t_dp = 0.f, t_tb = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&t_dp, ctx->ev0, ctx->ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(&t_tb, ctx->ev1, ctx->ev2);
     if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
-    std::vector<uint64_t> dense(3 * m);  // per pair: CIGAR, reference-base, query-base offsets
+    // dense[3p..3p+2]: pair p's CIGAR, reference-base and query-base offsets
+    // (per-range sums, then each range from its start)
+    std::vector<std::array<uint64_t, 3>> dsum(16, {0, 0, 0});
+    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned t) {
+        uint64_t c = 0, r = 0, q = 0;
+        for (int64_t p = lo; p < hi; ++p) {
+            c += (uint64_t)hnc[p];
+            r += (uint64_t)hnr[p];
+            q += (uint64_t)hnq[p];
+        }
+        dsum[t] = {c, r, q};
+    });
     uint64_t nc_all = 0, nr_all = 0, nq_all = 0;
-    for (size_t p = 0; p < m; ++p) {
-        dense[3 * p] = nc_all;
-        dense[3 * p + 1] = nr_all;
-        dense[3 * p + 2] = nq_all;
-        nc_all += (uint64_t)hnc[p];
-        nr_all += (uint64_t)hnr[p];
-        nq_all += (uint64_t)hnq[p];
+    for (auto &s : dsum) {
+        const std::array<uint64_t, 3> n3 = s;
+        s = {nc_all, nr_all, nq_all};
+        nc_all += n3[0];
+        nr_all += n3[1];
+        nq_all += n3[2];
     }
     if (so.dense && (nc_all > (uint64_t)so.cigar_cap || nr_all > (uint64_t)so.aligned_cap ||
                      nq_all > (uint64_t)so.aligned_cap))
         return FA_ERR_ARG;  // the capacities are too small for this batch
+    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned t) {
+        uint64_t c = dsum[t][0], r = dsum[t][1], q = dsum[t][2];
+        for (int64_t p = lo; p < hi; ++p) {
+            dense[3 * p] = c;
+            dense[3 * p + 1] = r;
+            dense[3 * p + 2] = q;
+            c += (uint64_t)hnc[p];
+            r += (uint64_t)hnr[p];
+            q += (uint64_t)hnq[p];
+        }
+    });
     uint64_t *d_dense;
     uint32_t *d_ocig;
     uint8_t *d_ora, *d_oqa;
-    if ((rc = ctx_upload(ctx, 22, &d_dense, dense.data(), dense.size())) ||
-        (rc = ctx_alloc(ctx, 23, &d_ocig, (size_t)nc_all + 1)) ||
-        (rc = ctx_alloc(ctx, 24, &d_ora, (size_t)nr_all + 1)) ||
-        (rc = ctx_alloc(ctx, 25, &d_oqa, (size_t)nq_all + 1))) {
+    if ((rc = ctx_upload(ctx, kSwtbDense, &d_dense, dense, 3 * m)) ||
+        (rc = ctx_alloc(ctx, kSwtbOutCigar, &d_ocig, (size_t)nc_all + 1)) ||
+        (rc = ctx_alloc(ctx, kSwtbOutRef, &d_ora, (size_t)nr_all + 1)) ||
+        (rc = ctx_alloc(ctx, kSwtbOutQry, &d_oqa, (size_t)nq_all + 1))) {
         (void)hipStreamSynchronize(st);
         return rc;
     }
     // downloads land in the pinned staging buffer (the uploads from it are
     // complete: the stream was synchronised above)
-    if ((rc = pinned_staging(ctx, (size_t)nc_all * 4 + (size_t)nr_all + (size_t)nq_all + 64))) return rc;
+    if ((rc = pinned_staging(ctx, (size_t)nc_all * 4 + (size_t)nr_all + (size_t)nq_all + 64))) {
+        (void)hipStreamSynchronize(st);  // the offsets' upload from the scratch may be in flight
+        return rc;
+    }
     uint32_t *hcig = (uint32_t *)ctx->pin;
     uint8_t *hra = (uint8_t *)(hcig + nc_all), *hqa = hra + nr_all;
     e = fa::swtb_compact_launch(a, (uint32_t)m, d_dense, d_ocig, d_ora, d_oqa, st);
     if (e == hipSuccess && nc_all) e = hipMemcpyAsync(hcig, d_ocig, nc_all * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && nr_all) e = hipMemcpyAsync(hra, d_ora, nr_all, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && nq_all) e = hipMemcpyAsync(hqa, d_oqa, nq_all, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const hipError_t es = hipStreamSynchronize(st);  // also after a failed launch or copy
+    if (e == hipSuccess) e = es;
     if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
     ctx->last_dp_ms = t_dp;
     ctx->last_tb_ms = t_tb;
     parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
         for (int64_t p = lo; p < hi; ++p) {
-            const int64_t k = job[(size_t)p];
+            const int64_t k = jobof((size_t)p);
             const int32_t *o = &hout[(size_t)p * 6];
             out[k].score = o[0];
             out[k].target_end_pos = o[1];
@@ -1531,37 +1651,35 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
         }
     });
     if (so.dense) {
-        // offsets in job order (an empty job adds nothing), then the gathered
+        // offsets in job order (an empty job adds nothing; each thread range
+        // starts at the first pair of a job not before it), then the gathered
         // buffers in one piece each: only the pages they fill are touched
-        size_t p = 0;
-        for (int64_t k = 0; k < njobs; ++k) {
-            if (p < m && job[p] == k) {
-                so.cigar_off_out[k] = (int64_t)dense[3 * p];
-                so.ref_off_out[k] = (int64_t)dense[3 * p + 1];
-                so.query_off_out[k] = (int64_t)dense[3 * p + 2];
-                ++p;
-            } else {
+        parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned) {
+            size_t p = all ? (size_t)lo : (size_t)(std::lower_bound(job, job + m, lo) - job);
+            for (int64_t k = lo; k < hi; ++k) {
+                const bool here = p < m && jobof(p) == k;
                 so.cigar_off_out[k] = p < m ? (int64_t)dense[3 * p] : (int64_t)nc_all;
                 so.ref_off_out[k] = p < m ? (int64_t)dense[3 * p + 1] : (int64_t)nr_all;
                 so.query_off_out[k] = p < m ? (int64_t)dense[3 * p + 2] : (int64_t)nq_all;
+                p += here;
             }
-        }
+        });
         so.cigar_off_out[njobs] = (int64_t)nc_all;
         so.ref_off_out[njobs] = (int64_t)nr_all;
         so.query_off_out[njobs] = (int64_t)nq_all;
         const int64_t pieces = 64;
         parallel_for(pieces, [&](int64_t lo, int64_t hi, unsigned) {
             for (int64_t x = lo; x < hi; ++x) {
-                auto part = [&](uint64_t total, uint64_t &a, uint64_t &b) {
+                auto piece = [&](uint64_t total, uint64_t &a, uint64_t &b) {
                     a = total * (uint64_t)x / (uint64_t)pieces;
                     b = total * (uint64_t)(x + 1) / (uint64_t)pieces;
                 };
                 uint64_t a0, a1;
-                part(nc_all, a0, a1);
+                piece(nc_all, a0, a1);
                 if (a1 > a0) std::memcpy(so.cigar + a0, hcig + a0, (size_t)(a1 - a0) * 4);
-                part(nr_all, a0, a1);
+                piece(nr_all, a0, a1);
                 if (a1 > a0) std::memcpy(so.ref_aligned + a0, hra + a0, (size_t)(a1 - a0));
-                part(nq_all, a0, a1);
+                piece(nq_all, a0, a1);
                 if (a1 > a0) std::memcpy(so.query_aligned + a0, hqa + a0, (size_t)(a1 - a0));
             }
         });

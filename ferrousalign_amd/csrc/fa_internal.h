@@ -29,6 +29,8 @@ struct fa_ctx {
     void *slot[32] = {};       // grow-only device buffers of the CIGAR-regeneration calls and of
     size_t slot_bytes[32] = {};  // the one-call batches of fa_execute_batch_simd_scoring
     void *hist_pin = nullptr;  // pinned packing histogram of those batches
+    void *hscr = nullptr;      // pinned per-pair arrays of the CIGAR-regeneration calls (grow-only)
+    size_t hscr_bytes = 0;
 };
 
 namespace fa_host {

@@ -183,6 +183,55 @@ def test_dense_output_matches_slots():
     assert rc == N.FA_ERR_ARG
 
 
+def test_large_batch_with_empty_jobs():
+    """131,072 region-like pairs, so the host work runs on many threads, with
+    empty queries and targets scattered through every thread's range (the
+    first and last job among them) and wide bands on a share of the pairs, so
+    several kernels run: records, cells and CIGAR lengths of every pair equal
+    the oracle's; both entry points give the same CIGARs and aligned bases;
+    full contents of pairs spread over the batch equal the oracle's."""
+    from ferrousalign_amd import synth
+    from ferrousalign_amd.banded_swa import CIGAR_OPS, scalar_banded_swa_batch, scalar_banded_swa_batch_dense
+    n = 131072
+    qf, qo, ql, tf, to, tl, w = synth.region_pairs(n=n)
+    ql, tl, w = ql.copy(), tl.copy(), w.copy()
+    rng = np.random.default_rng(5)
+    ql[::9973] = 0
+    tl[5::7919] = 0
+    ql[-1] = 0
+    wide = rng.random(n) < 0.1
+    w[wide] = rng.integers(24, 120, int(wide.sum()))
+    h0 = np.where(rng.random(n) < 0.2, rng.integers(1, 60, n), 0).astype(np.int32)
+    sw = synth.default_scoring()
+    print("large batch", n, flush=True)
+    out, cells, cig, coff, nc, ra, qa, aoff, nr, nq = scalar_banded_swa_batch(sw, qf, qo, ql, tf, to, tl, w, h0,
+                                                                             raw=True)
+    ora, ocells, onc = O.scalar_banded_swa_batch(sw, qf, qo, ql, tf, to, tl, w, h0)
+    assert np.array_equal(out, ora)
+    assert np.array_equal(cells, ocells.astype(np.uint32))
+    assert np.array_equal(nc, onc)
+    empty = (ql == 0) | (tl == 0)
+    assert not out[empty].any() and not nc[empty].any() and not nr[empty].any()
+    dout, dcells, dcig, dco, dra, dro, dqa, dqo = scalar_banded_swa_batch_dense(sw, qf, qo, ql, tf, to, tl, w, h0)
+    assert np.array_equal(out, dout) and np.array_equal(cells, dcells)
+    assert dco[0] == dro[0] == dqo[0] == 0
+    assert np.array_equal(np.diff(dco), nc) and np.array_equal(np.diff(dro), nr) and np.array_equal(np.diff(dqo), nq)
+    # the slot buffers gathered in job order equal the dense ones
+    ci = np.concatenate([np.arange(coff[k], coff[k] + nc[k]) for k in range(n)])
+    ai = np.concatenate([np.arange(aoff[k], aoff[k] + nr[k]) for k in range(n)])
+    qi = np.concatenate([np.arange(aoff[k], aoff[k] + nq[k]) for k in range(n)])
+    assert np.array_equal(cig[ci], dcig) and np.array_equal(ra[ai], dra) and np.array_equal(qa[qi], dqa)
+    for k in list(range(0, n, 4099)) + [1, 9973, n - 2]:
+        q = qf[qo[k]:qo[k] + ql[k]]
+        t = tf[to[k]:to[k] + tl[k]]
+        if len(q) == 0 or len(t) == 0:
+            continue
+        rec, c2, r2, q2, _ = O.scalar_banded_swa(q, t, sw, int(w[k]), int(h0[k]))
+        got = [(CIGAR_OPS[int(x) & 15], int(x) >> 4) for x in dcig[dco[k]:dco[k + 1]]]
+        assert got == c2, k
+        assert np.array_equal(dra[dro[k]:dro[k + 1]], r2) and np.array_equal(dqa[dqo[k]:dqo[k + 1]], q2), k
+
+
 def _ctx():
     from ferrousalign_amd.banded_swa import default_context
     return default_context().handle

@@ -739,6 +739,8 @@ int fa_ctx_create(int device, fa_ctx **out) {
     for (int j = 0; ok && j < 3; ++j)
         ok = hipStreamCreateWithFlags(&c->side[j], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_join[j], hipEventDisableTiming) == hipSuccess;
+    for (hipEvent_t &ev : c->ev_part)
+        ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         fa_ctx_destroy(c);
         return FA_ERR_DEVICE;
@@ -768,6 +770,8 @@ int fa_ctx_destroy(fa_ctx *ctx) {
         if (ctx->ev_join[j]) (void)hipEventDestroy(ctx->ev_join[j]);
     }
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (hipEvent_t ev : ctx->ev_part)
+        if (ev) (void)hipEventDestroy(ev);
     delete ctx;
     return FA_OK;
 }
@@ -1299,6 +1303,7 @@ int host_scratch(fa_ctx *ctx, size_t bytes) {
 // log2(L) - 2 and S, ordered as lsl = kSwtbBand | L << 8 | S is.
 constexpr int kSwtbCodes = 512;
 constexpr int kSwtbKeyBits = 29;  // code (9 bits) << 20 | 255 - qlen << 12 | 4095 - tlen
+constexpr int kSwtbParts = 8;     // pieces of the output download (the context's ev_part)
 inline uint32_t swtb_code(uint32_t lsl) {
     const uint32_t L = (lsl >> 8) & 0xFFu, lg = L == 4 ? 0u : (L == 8 ? 1u : 2u);
     return ((lsl & fa::kSwtbBand) ? 256u : 0u) | (lg << 6) | (lsl & 63u);
@@ -1620,68 +1625,86 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
     }
     uint32_t *hcig = (uint32_t *)ctx->pin;
     uint8_t *hra = (uint8_t *)(hcig + nc_all), *hqa = hra + nr_all;
+    // the gathered CIGARs and bases come back in pieces of consecutive pairs
+    // (consecutive in the dense buffers too); the host threads copy each out
+    // as soon as it has landed, while the next ones are still in flight (the
+    // dense call only: the slot call's pair-by-pair copies measured slower so)
+    const int parts =
+        so.dense ? (int)std::max<size_t>(1, std::min<size_t>(kSwtbParts, m / 65536)) : 1;
+    auto first_pair = [&](int x) { return (size_t)((uint64_t)m * (uint64_t)x / (uint64_t)parts); };
+    auto cut = [&](size_t p, int f) -> uint64_t {  // entry of pair p in dense buffer f
+        return p < m ? dense[3 * p + f] : (f == 0 ? nc_all : (f == 1 ? nr_all : nq_all));
+    };
     e = fa::swtb_compact_launch(a, (uint32_t)m, d_dense, d_ocig, d_ora, d_oqa, st);
-    if (e == hipSuccess && nc_all) e = hipMemcpyAsync(hcig, d_ocig, nc_all * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && nr_all) e = hipMemcpyAsync(hra, d_ora, nr_all, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && nq_all) e = hipMemcpyAsync(hqa, d_oqa, nq_all, hipMemcpyDeviceToHost, st);
-    const hipError_t es = hipStreamSynchronize(st);  // also after a failed launch or copy
-    if (e == hipSuccess) e = es;
-    if (e != hipSuccess) return hip_fail(e, "fa_scalar_banded_swa_batch");
+    for (int x = 0; x < parts && e == hipSuccess; ++x) {
+        const size_t p0 = first_pair(x), p1 = first_pair(x + 1);
+        const uint64_t c0 = cut(p0, 0), c1 = cut(p1, 0), r0 = cut(p0, 1), r1 = cut(p1, 1);
+        const uint64_t q0 = cut(p0, 2), q1 = cut(p1, 2);
+        if (c1 > c0) e = hipMemcpyAsync(hcig + c0, d_ocig + c0, (c1 - c0) * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && r1 > r0) e = hipMemcpyAsync(hra + r0, d_ora + r0, r1 - r0, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && q1 > q0) e = hipMemcpyAsync(hqa + q0, d_oqa + q0, q1 - q0, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_part[x], st);
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        return hip_fail(e, "fa_scalar_banded_swa_batch");
+    }
     ctx->last_dp_ms = t_dp;
     ctx->last_tb_ms = t_tb;
-    parallel_for((int64_t)m, [&](int64_t lo, int64_t hi, unsigned) {
-        for (int64_t p = lo; p < hi; ++p) {
-            const int64_t k = jobof((size_t)p);
-            const int32_t *o = &hout[(size_t)p * 6];
-            out[k].score = o[0];
-            out[k].target_end_pos = o[1];
-            out[k].gtarget_end_pos = o[2];
-            out[k].query_end_pos = o[3];
-            out[k].global_score = o[4];
-            out[k].max_offset = o[5];
-            if (cells) cells[k] = hcells[p];
-            if (so.dense) continue;
-            so.n_cigar[k] = hnc[p];
-            so.n_ref_aligned[k] = hnr[p];
-            so.n_query_aligned[k] = hnq[p];
-            std::memcpy(so.cigar + so.cigar_offset[k], hcig + dense[3 * p], (size_t)hnc[p] * 4);
-            std::memcpy(so.ref_aligned + so.aligned_offset[k], hra + dense[3 * p + 1], (size_t)hnr[p]);
-            std::memcpy(so.query_aligned + so.aligned_offset[k], hqa + dense[3 * p + 2],
-                        (size_t)hnq[p]);
-        }
-    });
     if (so.dense) {
         // offsets in job order (an empty job adds nothing; each thread range
-        // starts at the first pair of a job not before it), then the gathered
-        // buffers in one piece each: only the pages they fill are touched
+        // starts at the first pair of a job not before it)
         parallel_for(njobs, [&](int64_t lo, int64_t hi, unsigned) {
             size_t p = all ? (size_t)lo : (size_t)(std::lower_bound(job, job + m, lo) - job);
             for (int64_t k = lo; k < hi; ++k) {
                 const bool here = p < m && jobof(p) == k;
-                so.cigar_off_out[k] = p < m ? (int64_t)dense[3 * p] : (int64_t)nc_all;
-                so.ref_off_out[k] = p < m ? (int64_t)dense[3 * p + 1] : (int64_t)nr_all;
-                so.query_off_out[k] = p < m ? (int64_t)dense[3 * p + 2] : (int64_t)nq_all;
+                so.cigar_off_out[k] = (int64_t)cut(p, 0);
+                so.ref_off_out[k] = (int64_t)cut(p, 1);
+                so.query_off_out[k] = (int64_t)cut(p, 2);
                 p += here;
             }
         });
         so.cigar_off_out[njobs] = (int64_t)nc_all;
         so.ref_off_out[njobs] = (int64_t)nr_all;
         so.query_off_out[njobs] = (int64_t)nq_all;
-        const int64_t pieces = 64;
-        parallel_for(pieces, [&](int64_t lo, int64_t hi, unsigned) {
-            for (int64_t x = lo; x < hi; ++x) {
-                auto piece = [&](uint64_t total, uint64_t &a, uint64_t &b) {
-                    a = total * (uint64_t)x / (uint64_t)pieces;
-                    b = total * (uint64_t)(x + 1) / (uint64_t)pieces;
-                };
-                uint64_t a0, a1;
-                piece(nc_all, a0, a1);
-                if (a1 > a0) std::memcpy(so.cigar + a0, hcig + a0, (size_t)(a1 - a0) * 4);
-                piece(nr_all, a0, a1);
-                if (a1 > a0) std::memcpy(so.ref_aligned + a0, hra + a0, (size_t)(a1 - a0));
-                piece(nq_all, a0, a1);
-                if (a1 > a0) std::memcpy(so.query_aligned + a0, hqa + a0, (size_t)(a1 - a0));
+    }
+    for (int x = 0; x < parts; ++x) {
+        e = hipEventSynchronize(ctx->ev_part[x]);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(st);
+            return hip_fail(e, "fa_scalar_banded_swa_batch");
+        }
+        // records of the piece's pairs, then their CIGARs and bases: into the
+        // caller's slots pair by pair, or (dense) the thread's run of pairs
+        // in one copy per buffer, so only the pages they fill are touched
+        const size_t p0 = first_pair(x), p1 = first_pair(x + 1);
+        parallel_for((int64_t)(p1 - p0), [&](int64_t lo, int64_t hi, unsigned) {
+            const size_t a0 = p0 + (size_t)lo, a1 = p0 + (size_t)hi;
+            for (size_t p = a0; p < a1; ++p) {
+                const int64_t k = jobof(p);
+                const int32_t *o = &hout[p * 6];
+                out[k].score = o[0];
+                out[k].target_end_pos = o[1];
+                out[k].gtarget_end_pos = o[2];
+                out[k].query_end_pos = o[3];
+                out[k].global_score = o[4];
+                out[k].max_offset = o[5];
+                if (cells) cells[k] = hcells[p];
+                if (so.dense) continue;
+                so.n_cigar[k] = hnc[p];
+                so.n_ref_aligned[k] = hnr[p];
+                so.n_query_aligned[k] = hnq[p];
+                std::memcpy(so.cigar + so.cigar_offset[k], hcig + dense[3 * p], (size_t)hnc[p] * 4);
+                std::memcpy(so.ref_aligned + so.aligned_offset[k], hra + dense[3 * p + 1], (size_t)hnr[p]);
+                std::memcpy(so.query_aligned + so.aligned_offset[k], hqa + dense[3 * p + 2],
+                            (size_t)hnq[p]);
             }
+            if (!so.dense) return;
+            const uint64_t c0 = cut(a0, 0), c1 = cut(a1, 0), r0 = cut(a0, 1), r1 = cut(a1, 1);
+            const uint64_t q0 = cut(a0, 2), q1 = cut(a1, 2);
+            if (c1 > c0) std::memcpy(so.cigar + c0, hcig + c0, (size_t)(c1 - c0) * 4);
+            if (r1 > r0) std::memcpy(so.ref_aligned + r0, hra + r0, (size_t)(r1 - r0));
+            if (q1 > q0) std::memcpy(so.query_aligned + q0, hqa + q0, (size_t)(q1 - q0));
         });
     }
     return FA_OK;

@@ -20,6 +20,7 @@ struct fa_ctx {
     hipStream_t side[3] = {nullptr, nullptr, nullptr};  // fork/join streams (default priority)
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t ev_part[8] = {};  // pieces of a download the host consumes as they land
     std::string arch;
     int cus = 0;
     double last_dp_ms = 0.0, last_tb_ms = 0.0;  // kernels of the last fa_scalar_banded_swa_batch

@@ -137,14 +137,6 @@ int grow_side(SideBufs &b, size_t n) {
     return FA_OK;
 }
 
-template <class T>
-int put(DBuf &d, const T *src, int64_t count, hipStream_t st) {
-    int rc = grow(d, (size_t)count * sizeof(T));
-    if (rc) return rc;
-    if (count > 0) FA_HIP(hipMemcpyAsync(d.p, src, (size_t)count * sizeof(T), hipMemcpyHostToDevice, st));
-    return FA_OK;
-}
-
 int upload_into(fa_ctx *ctx, fa_chain_batch *cb, int64_t nreads, const uint8_t *query_seqs,
                 int64_t query_bytes, const int64_t *query_offset, const int32_t *query_len,
                 const int64_t *read_chain_start, const int32_t *read_chain_count, int64_t nchains,
@@ -173,23 +165,28 @@ int upload_into(fa_ctx *ctx, fa_chain_batch *cb, int64_t nreads, const uint8_t *
     FA_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     int rc;
-    // reads with 64 bytes of padding on both sides (word-wide loads of the job windows)
-    if ((rc = grow(cb->qseq, (size_t)query_bytes + 128))) return rc;
-    if (query_bytes > 0)
-        FA_HIP(hipMemcpyAsync((uint8_t *)cb->qseq.p + 64, query_seqs, (size_t)query_bytes,
-                              hipMemcpyHostToDevice, st));
-    if ((rc = put(cb->qoff, query_offset, nreads, st)) || (rc = put(cb->qlen, query_len, nreads, st)) ||
-        (rc = put(cb->rcs, read_chain_start, nreads, st)) ||
-        (rc = put(cb->rcc, read_chain_count, nreads, st)) ||
-        (rc = put(cb->kept, chain_kept, nchains, st)) ||
-        (rc = put(cb->cstart, chain_seed_start, nchains, st)) ||
-        (rc = put(cb->ccount, chain_seed_count, nchains, st)) ||
-        (rc = put(cb->slot_base, cb->slot_base_host.data(), nchains + 1, st)) ||
-        (rc = put(cb->cseeds, chain_seeds, n_chain_seeds, st)) || (rc = put(cb->seeds, seeds, nseeds, st)) ||
+    // reads with 64 bytes of padding on both sides (word-wide loads of the job
+    // windows); every array goes up through the context's pinned staging
+    // buffer, which the host threads fill while earlier pieces are in flight
+    std::vector<fa_host::HostSeg> segs;
+    auto put = [&](DBuf &d, const auto *src, int64_t count, size_t at = 0, size_t extra = 0) {
+        const size_t bytes = (size_t)count * sizeof(*src);
+        const int r = grow(d, bytes + extra);
+        if (r == FA_OK && bytes > 0) segs.push_back({(uint8_t *)d.p + at, src, bytes});
+        return r;
+    };
+    cb->has_frac = chain_frac_rep != nullptr;
+    if ((rc = put(cb->qseq, query_seqs, query_bytes, 64, 128)) ||
+        (rc = put(cb->qoff, query_offset, nreads)) || (rc = put(cb->qlen, query_len, nreads)) ||
+        (rc = put(cb->rcs, read_chain_start, nreads)) || (rc = put(cb->rcc, read_chain_count, nreads)) ||
+        (rc = put(cb->kept, chain_kept, nchains)) || (rc = put(cb->cstart, chain_seed_start, nchains)) ||
+        (rc = put(cb->ccount, chain_seed_count, nchains)) ||
+        (rc = put(cb->slot_base, cb->slot_base_host.data(), nchains + 1)) ||
+        (rc = put(cb->cseeds, chain_seeds, n_chain_seeds)) || (rc = put(cb->seeds, seeds, nseeds)) ||
+        (cb->has_frac && (rc = put(cb->frac, chain_frac_rep, nchains))) ||
         (rc = grow(cb->regions, (size_t)nchains * sizeof(fa_region))))
         return rc;
-    cb->has_frac = chain_frac_rep != nullptr;
-    if (cb->has_frac && (rc = put(cb->frac, chain_frac_rep, nchains, st))) return rc;
+    if ((rc = fa_host::staged_upload(ctx, segs.data(), (int)segs.size(), st))) return rc;
     FA_HIP(hipStreamSynchronize(st));
     cb->device = ctx->device;
     cb->nreads = nreads;
@@ -323,10 +320,8 @@ int fa_chains_download(fa_ctx *ctx, fa_chain_batch *cb, fa_region *regions) {
     if (!ctx || !cb || (cb->nchains > 0 && !regions)) return FA_ERR_ARG;
     if (cb->nchains == 0) return FA_OK;
     FA_HIP(hipSetDevice(ctx->device));
-    FA_HIP(hipMemcpyAsync(regions, cb->regions.p, (size_t)cb->nchains * sizeof(fa_region),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    FA_HIP(hipStreamSynchronize(ctx->stream));
-    return FA_OK;
+    return fa_host::staged_download(ctx, regions, cb->regions.p, (size_t)cb->nchains * sizeof(fa_region),
+                                    ctx->stream);
 }
 
 int fa_chains_extend(fa_ctx *ctx, fa_chain_batch *cb, const fa_ref *ref,

@@ -1135,6 +1135,95 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
 
 }  // namespace
 
+namespace fa_host {
+
+int staged_upload(fa_ctx *ctx, const HostSeg *segs, int nseg, hipStream_t st) {
+    size_t total = 0;
+    for (int k = 0; k < nseg; ++k) total += segs[k].bytes;
+    if (total == 0) return FA_OK;
+    // pieces of the arrays laid end to end, alternately in the two halves
+    const size_t piece = std::min<size_t>(std::max<size_t>(total / 8, (size_t)4 << 20), (size_t)64 << 20);
+    int rc = pinned_staging(ctx, 2 * piece);
+    if (rc) return rc;
+    uint8_t *const half[2] = {(uint8_t *)ctx->pin, (uint8_t *)ctx->pin + piece};
+    bool pending[2] = {false, false};
+    struct Run {
+        const uint8_t *src;
+        uint8_t *dst;
+        size_t bytes, at;  // at: offset in the piece
+    };
+    std::vector<Run> runs;
+    hipError_t e = hipSuccess;
+    int k = 0;
+    size_t koff = 0;  // position in segment k
+    for (size_t x = 0, done = 0; done < total && e == hipSuccess; ++x, done += piece) {
+        const int h = (int)(x & 1);
+        const size_t len = std::min(piece, total - done);
+        if (pending[h] && (e = hipEventSynchronize(ctx->ev_part[h])) != hipSuccess) break;
+        runs.clear();
+        for (size_t got = 0; got < len;) {
+            const size_t take = std::min(segs[k].bytes - koff, len - got);
+            if (take > 0)
+                runs.push_back({(const uint8_t *)segs[k].src + koff, (uint8_t *)segs[k].dst + koff, take, got});
+            got += take;
+            koff += take;
+            if (koff == segs[k].bytes) {
+                ++k;
+                koff = 0;
+            }
+        }
+        parallel_for(16, [&](int64_t lo, int64_t hi, unsigned) {
+            for (int64_t t = lo; t < hi; ++t) {
+                const size_t b0 = len * (size_t)t / 16, b1 = len * (size_t)(t + 1) / 16;
+                for (const Run &r : runs) {
+                    const size_t s0 = std::max(b0, r.at), s1 = std::min(b1, r.at + r.bytes);
+                    if (s1 > s0) std::memcpy(half[h] + s0, r.src + (s0 - r.at), s1 - s0);
+                }
+            }
+        }, 2);
+        for (const Run &r : runs)
+            if (e == hipSuccess) e = hipMemcpyAsync(r.dst, half[h] + r.at, r.bytes, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_part[h], st);
+        pending[h] = true;
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);  // copies from the staging buffer may be in flight
+        return hip_fail(e, "staged_upload");
+    }
+    return FA_OK;
+}
+
+int staged_download(fa_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return FA_OK;
+    int rc = pinned_staging(ctx, bytes);
+    if (rc) return rc;
+    const int parts = (int)std::max<size_t>(1, std::min<size_t>(8, bytes / ((size_t)8 << 20)));
+    uint8_t *const pin = (uint8_t *)ctx->pin;
+    hipError_t e = hipSuccess;
+    for (int x = 0; x < parts && e == hipSuccess; ++x) {
+        const size_t b0 = bytes * (size_t)x / parts, b1 = bytes * (size_t)(x + 1) / parts;
+        e = hipMemcpyAsync(pin + b0, (const uint8_t *)src + b0, b1 - b0, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_part[x], st);
+    }
+    for (int x = 0; x < parts && e == hipSuccess; ++x) {
+        if ((e = hipEventSynchronize(ctx->ev_part[x])) != hipSuccess) break;
+        const size_t b0 = bytes * (size_t)x / parts, b1 = bytes * (size_t)(x + 1) / parts;
+        parallel_for(16, [&](int64_t lo, int64_t hi, unsigned) {
+            for (int64_t t = lo; t < hi; ++t) {
+                const size_t c0 = b0 + (b1 - b0) * (size_t)t / 16, c1 = b0 + (b1 - b0) * (size_t)(t + 1) / 16;
+                if (c1 > c0) std::memcpy((uint8_t *)dst + c0, pin + c0, c1 - c0);
+            }
+        }, 2);
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        return hip_fail(e, "staged_download");
+    }
+    return FA_OK;
+}
+
+}  // namespace fa_host
+
 extern "C" {
 
 int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,

@@ -161,5 +161,19 @@ void chain_workspace_free(fa_ctx *ctx);
 // array of fa_execute_batch_device_noalloc (noalloc.hip); once per device.
 int noalloc_init(int device);
 
+// Host-to-device copies of pageable host arrays through the context's pinned
+// staging buffer on stream st: the host threads fill one half of it while the
+// other half's copy is in flight.  Returns once every copy is enqueued; the
+// staging buffer is in use until the stream reaches them.
+struct HostSeg {
+    void *dst;        // device
+    const void *src;  // host
+    size_t bytes;
+};
+int staged_upload(fa_ctx *ctx, const HostSeg *segs, int nseg, hipStream_t st);
+// Device-to-host copy into pageable memory through the staging buffer, in
+// pieces that the host threads copy out as they land; synchronous.
+int staged_download(fa_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st);
+
 
 }  // namespace fa_host

@@ -1137,10 +1137,17 @@ int batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
 
 namespace fa_host {
 
+constexpr size_t kStagedMin = (size_t)4 << 20;  // below: plain copies
+
 int staged_upload(fa_ctx *ctx, const HostSeg *segs, int nseg, hipStream_t st) {
     size_t total = 0;
     for (int k = 0; k < nseg; ++k) total += segs[k].bytes;
     if (total == 0) return FA_OK;
+    if (total < kStagedMin) {  // small: the runtime's own staging, no host threads
+        for (int k = 0; k < nseg; ++k)
+            if (segs[k].bytes) FA_HIP(hipMemcpyAsync(segs[k].dst, segs[k].src, segs[k].bytes, hipMemcpyHostToDevice, st));
+        return FA_OK;
+    }
     // pieces of the arrays laid end to end, alternately in the two halves
     const size_t piece = std::min<size_t>(std::max<size_t>(total / 8, (size_t)4 << 20), (size_t)64 << 20);
     int rc = pinned_staging(ctx, 2 * piece);
@@ -1195,6 +1202,11 @@ int staged_upload(fa_ctx *ctx, const HostSeg *segs, int nseg, hipStream_t st) {
 
 int staged_download(fa_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st) {
     if (bytes == 0) return FA_OK;
+    if (bytes < kStagedMin) {
+        FA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        FA_HIP(hipStreamSynchronize(st));
+        return FA_OK;
+    }
     int rc = pinned_staging(ctx, bytes);
     if (rc) return rc;
     const int parts = (int)std::max<size_t>(1, std::min<size_t>(8, bytes / ((size_t)8 << 20)));
@@ -2064,13 +2076,12 @@ int fa_batch_download(fa_ctx *ctx, fa_batch *b, fa_out_score *out, uint32_t *cel
     static_assert(sizeof(fa_out_score) == 24, "OutScore layout");
     int rc = ensure_records(ctx, b);
     if (rc) return rc;
-    if (out && b->n)
-        FA_HIP(hipMemcpyAsync(out, b->d_out, (size_t)b->n * sizeof(fa_out_score),
-                              hipMemcpyDeviceToHost, ctx->stream));
-    if (cells && b->n)
-        FA_HIP(hipMemcpyAsync(cells, b->d_cells, (size_t)b->n * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
-    FA_HIP(hipStreamSynchronize(ctx->stream));
+    if (out && b->n &&
+        (rc = fa_host::staged_download(ctx, out, b->d_out, (size_t)b->n * sizeof(fa_out_score), ctx->stream)))
+        return rc;
+    if (cells && b->n &&
+        (rc = fa_host::staged_download(ctx, cells, b->d_cells, (size_t)b->n * sizeof(uint32_t), ctx->stream)))
+        return rc;
     return FA_OK;
 }
 
@@ -2370,13 +2381,13 @@ int fa_batch_download_align2(fa_ctx *ctx, fa_batch *b, fa_kswr *out, uint32_t *c
     if (!ctx || !b || !b->align2_set) return FA_ERR_ARG;
     FA_HIP(hipSetDevice(ctx->device));
     static_assert(sizeof(fa_kswr) == 28, "Kswr layout");
-    if (out && b->n)
-        FA_HIP(hipMemcpyAsync(out, b->d_mout, (size_t)b->n * sizeof(fa_kswr), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    if (cells && b->n)
-        FA_HIP(hipMemcpyAsync(cells, b->d_mcells, (size_t)b->n * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
-    FA_HIP(hipStreamSynchronize(ctx->stream));
+    int rc;
+    if (out && b->n &&
+        (rc = fa_host::staged_download(ctx, out, b->d_mout, (size_t)b->n * sizeof(fa_kswr), ctx->stream)))
+        return rc;
+    if (cells && b->n &&
+        (rc = fa_host::staged_download(ctx, cells, b->d_mcells, (size_t)b->n * sizeof(uint32_t), ctx->stream)))
+        return rc;
     return FA_OK;
 }
 

@@ -1602,7 +1602,7 @@ int swtb_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                            (const uint32_t *)(db + B.idx), (uint32_t *)(ds + s_order), (int)m, 0,
                            kSwtbKeyBits, st),
                        "hipcub::DeviceRadixSort::SortPairs")) ||
-        (rc = ctx_alloc(ctx, kSwtbTrace, &d_tb, (size_t)tbw)) ||
+        (rc = ctx_alloc(ctx, kSwtbTrace, &d_tb, (size_t)tbw + 32)) ||  // + a line: whole-line reads
         (rc = ctx_alloc(ctx, kSwtbCigar, &d_cig, (size_t)cgw)) ||
         (rc = ctx_alloc(ctx, kSwtbRefAln, &d_ra, (size_t)alw)) ||
         (rc = ctx_alloc(ctx, kSwtbQryAln, &d_qa, (size_t)alw))) {

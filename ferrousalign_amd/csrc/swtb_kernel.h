@@ -26,7 +26,7 @@ struct SwtbArgs {
     const uint32_t *order;   // packed position -> pair id
     const uint64_t *tbo;     // traceback block offset (words) per pair
     const uint32_t *lsl;     // (L << 8) | S per pair
-    uint32_t *tb;
+    uint32_t *tb;            // 128-byte aligned, readable 128 bytes past the last block (whole lines)
     int32_t *out;            // 6 int32 per pair, OutScore order
     uint32_t *cells;
     // traceback outputs, per pair at the given offsets

@@ -300,26 +300,51 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
 
 namespace {
 
+// Codes of one pair, read through a copy of the 128-byte line that holds the
+// current word: the walk steps back about one row at a time and a row of a
+// band-window pair is 16 or 32 bytes, so one line serves several steps.  The
+// copy sits in LDS, word k of the line at line[k * kTbLanes] (a column per
+// lane: the lanes' reads of the same k fall in different banks).
+constexpr int kTbLanes = 256;  // threads per block of the traceback kernel
 struct TbView {
-    const uint32_t *tb;
+    const uint32_t *tb0;  // the traceback buffer (256-byte aligned, lines whole)
+    uint64_t base;        // word offset of the pair's block in it
+    uint32_t *line;       // this lane's column of the LDS copy
     int L, S, nwl, band_w;
     bool band;
-    size_t last = ~(size_t)0;  // index of the word last read (runs of I reuse it)
+    uint32_t smul;             // (n * smul) >> 20 == n / S for 0 <= n < 4096 (S <= 32)
+    uint64_t held = ~0ull;     // line in the copy
+    uint64_t last = ~0ull;     // word last read (runs of I reuse it)
     uint32_t word = 0;
     __device__ int code(int ci, int cj) {
         if (ci == 0) return cj > 0 ? TB_INS : TB_MATCH;
         if (cj == 0) return TB_DEL;
         const int i = ci - 1;
         int j = cj - 1;
+        // row i < 2048 and column j < L * S <= 512: divisions by S as products
         if (band) {  // slot in the window of row i; outside it: never in a band
-            j -= (i / S) * S - band_w;
+            j -= (int)(((uint32_t)i * smul) >> 20) * S - band_w;
             if (j < 0 || j >= L * S) return TB_MATCH;
         }
-        const int lane = j / S, s = j % S;
-        const size_t idx = (size_t)i * (size_t)(L * nwl) + (size_t)(lane * nwl + s / 16);
-        if (idx != last) {
-            word = tb[idx];
-            last = idx;
+        const int lane = (int)(((uint32_t)j * smul) >> 20), s = j - lane * S;
+        const uint64_t w = base + (uint64_t)i * (uint64_t)(L * nwl) + (uint64_t)(lane * nwl + s / 16);
+        if (w != last) {
+            if ((w >> 5) != held) {
+                held = w >> 5;
+                const uint4 *src = reinterpret_cast<const uint4 *>(tb0) + held * 8;
+                uint4 v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = src[k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    line[(4 * k + 0) * kTbLanes] = v[k].x;
+                    line[(4 * k + 1) * kTbLanes] = v[k].y;
+                    line[(4 * k + 2) * kTbLanes] = v[k].z;
+                    line[(4 * k + 3) * kTbLanes] = v[k].w;
+                }
+            }
+            word = line[(w & 31u) * kTbLanes];
+            last = w;
         }
         return (int)((word >> (2 * (s % 16))) & 3u);
     }
@@ -327,7 +352,8 @@ struct TbView {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_t n) {
+__global__ __launch_bounds__(kTbLanes) void swtb_traceback_kernel(SwtbArgs a, uint32_t n) {
+    __shared__ uint32_t lines[32 * kTbLanes];
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int qlen = a.qlen[p], tlen = a.tlen[p];
@@ -341,40 +367,42 @@ __global__ __launch_bounds__(256) void swtb_traceback_kernel(SwtbArgs a, uint32_
     const int32_t *o = a.out + (size_t)p * 6;
     const int score = o[0], max_i = o[1] - 1, max_j = o[3] - 1, gscore = o[4];
     const uint32_t lsl = a.lsl[p];
-    TbView v{a.tb + a.tbo[p], (int)((lsl >> 8) & 0xFFu), (int)(lsl & 0xFFu),
-             (int)((lsl & 0xFFu) + 15) / 16, a.w[p], (lsl & kSwtbBand) != 0};
+    TbView v{a.tb, a.tbo[p], lines + threadIdx.x, (int)((lsl >> 8) & 0xFFu), (int)(lsl & 0xFFu),
+             (int)((lsl & 0xFFu) + 15) / 16, a.w[p], (lsl & kSwtbBand) != 0,
+             (1u << 20) / (lsl & 0xFFu) + 1u};
     // runs are produced end to start: keep them from index 1 (cig[0] is kept
     // for the 5' clip), reverse afterwards
+    // The reference's walk (:250-326): an outer iteration reads the code at
+    // (ci, cj) and follows the run of that code (M: while ci, cj > 0; D: while
+    // ci > 0; I: while cj > 0), emitting it when not empty; it stops at (0, 0),
+    // at 10,000 iterations, on a code other than M, D, I, or when a run made no
+    // progress.  Here one loop step is one cell, so the lanes of a wavefront,
+    // whose runs differ, run the same instructions instead of each run kind's
+    // loop in turn.
     int nrc = 0;
     int ci = max_i + 1, cj = max_j + 1, iter = 0;
-    while (ci > 0 || cj > 0) {
-        if (iter >= 10000) break;
-        ++iter;
-        const int pi = ci, pj = cj;
-        const int code = v.code(ci, cj);
-        int cnt = 0;
-        if (code == TB_MATCH) {
-            while (ci > 0 && cj > 0 && v.code(ci, cj) == TB_MATCH) {
-                ++cnt;
-                --ci;
-                --cj;
-            }
-            if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_M;
-        } else if (code == TB_DEL) {
-            while (ci > 0 && v.code(ci, cj) == TB_DEL) {
-                ++cnt;
-                --ci;
-            }
-            if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_D;
-        } else if (code == TB_INS) {
-            while (cj > 0 && v.code(ci, cj) == TB_INS) {
-                ++cnt;
-                --cj;
-            }
-            if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | OP_I;
-        } else {
-            break;
+    int op = -1, cnt = 0, pi = 0, pj = 0;  // op < 0: no run open
+    for (;;) {
+        if (op < 0) {
+            if (ci == 0 && cj == 0) break;
+            if (iter >= 10000) break;
+            ++iter;
+            pi = ci;
+            pj = cj;
+            op = v.code(ci, cj);
+            if (op > TB_INS) break;
+            cnt = 0;
         }
+        const int c = v.code(ci, cj);  // at a run's start the word just read
+        const bool ext = c == op && (op == TB_MATCH ? (ci > 0 && cj > 0) : (op == TB_DEL ? ci > 0 : cj > 0));
+        if (ext) {
+            ++cnt;
+            ci -= op != TB_INS;
+            cj -= op != TB_DEL;
+            continue;
+        }
+        if (cnt > 0) cig[1 + nrc++] = ((uint32_t)cnt << 4) | (op == TB_MATCH ? OP_M : (op == TB_DEL ? OP_D : OP_I));
+        op = -1;
         if (ci == pi && cj == pj) break;
     }
     for (int x = 0, y = nrc - 1; x < y; ++x, --y) {
@@ -499,7 +527,7 @@ hipError_t swtb_compact_launch(const SwtbArgs &a, uint32_t n, const uint64_t *de
 
 hipError_t swtb_traceback_launch(const SwtbArgs &a, uint32_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(swtb_traceback_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    hipLaunchKernelGGL(swtb_traceback_kernel, dim3((n + kTbLanes - 1) / kTbLanes), dim3(kTbLanes), 0, st, a, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(swtb_bases_kernel, dim3((n + 3) / 4), dim3(256), 0, st, a, n);

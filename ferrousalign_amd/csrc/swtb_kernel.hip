@@ -72,10 +72,10 @@ __device__ __forceinline__ int group_allmax(int x) {
     if constexpr (L >= 16) x = imax(x, dpp<kRowMirror>(x, x));
     return x;
 }
-__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
-    const uint32_t lo = (uint32_t)imax((int)(a & 0xFFFFu), (int)(b & 0xFFFFu));
-    const uint32_t hi = (uint32_t)imax((int)(a >> 16), (int)(b >> 16));
-    return lo | (hi << 16);
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {  // one v_pk_max_u16
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u2, a),
+                                                                  __builtin_bit_cast(u2, b)));
 }
 template <int L>
 __device__ __forceinline__ uint32_t group_allmax_pk(uint32_t x) {

@@ -205,11 +205,27 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
         }
         const int carry = group_excl_scan_max<L>(run, l);
 
-        int H[S];
-        int kmax = 0;
+        // pass 2: H, the code, E, the row maximum; then, with H of the column
+        // before (the previous slot's, or for slot 0 the previous lane's last
+        // one, after the loop), the write-back eh[c].h <- H(i, c-1) and the
+        // non-zero columns of the next band, so each slot's band tests serve
+        // all three
+        int kmax = 0, hend = 0, Hprev = 0;
+        uint32_t fl = 0;
         uint32_t codes[NWL];
 #pragma unroll
         for (int k = 0; k < NWL; ++k) codes[k] = 0u;
+        auto write_back = [&](int s, int hp) {
+            const int c = cb + s;
+            const bool ge_rb = c >= rb, le_re = c <= re;
+            const int hv = c <= rb ? h1i : hp;
+            Hh[s] = (ge_rb && le_re) ? hv : Hh[s];
+            hend = c == re ? Hh[s] : hend;
+            const bool nz = (Hh[s] | Ee[s]) != 0;
+            const uint32_t f = (nz && ge_rb && c < re) ? (uint32_t)(1023 - c) : 0u;
+            const uint32_t la = (nz && ge_rb && le_re) ? (uint32_t)(c + 1) : 0u;
+            fl = pk_max_u16(fl, f | (la << 16));
+        };
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int c = cb + s;
@@ -217,36 +233,18 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
             const int m = M[s], e = Ee[s];
             const int h = imax(imax(m, e), F);
             const uint32_t code = (m >= e && m >= F) ? TB_MATCH : (e >= F ? TB_DEL : TB_INS);
-            H[s] = h;
             const int en = imax(imax(e - e_del, m - oe_del), 0);
             const bool inb = c >= rb && c < re;
             kmax = imax(kmax, inb ? ((h << 10) | (1023 - c)) : 0);
             Ee[s] = inb ? en : (c == re ? 0 : e);
             codes[s / 16] |= (inb ? code : 0u) << (2 * (s % 16));
+            if (s > 0) write_back(s, Hprev);
+            Hprev = h;
         }
+        write_back(0, dpp<kRowShr + 1>(0, Hprev));
         if (alive) {
 #pragma unroll
             for (int k = 0; k < NWL; ++k) tbp[(size_t)i * (L * NWL) + k] = codes[k];
-        }
-        const int hlast = dpp<kRowShr + 1>(0, H[S - 1]);
-        int hend = 0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int c = cb + s;
-            const int hp = s > 0 ? H[s - 1] : hlast;
-            const int hv = c <= rb ? h1i : hp;
-            const bool wr = c >= rb && c <= re;
-            Hh[s] = wr ? hv : Hh[s];
-            hend = c == re ? Hh[s] : hend;
-        }
-        uint32_t fl = 0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int c = cb + s;
-            const bool nz = (Hh[s] | Ee[s]) != 0;
-            const uint32_t f = (nz && c >= rb && c < re) ? (uint32_t)(1023 - c) : 0u;
-            const uint32_t la = (nz && c >= rb && c <= re) ? (uint32_t)(c + 1) : 0u;
-            fl = pk_max_u16(fl, f | (la << 16));
         }
         kmax = group_allmax<L>(kmax);
         hend = group_allmax<L>(hend);

@@ -12,9 +12,9 @@ call it, every field against the ksw_extend2 oracle:
   builds them with h0 = 0 (collect_soa.rs:206) and orchestration_soa.rs:170
   passes them to execute_batch_simd_scoring; ksw_extend2 asserts h0 > 0
   (ksw_affine_gap.rs:968), so each such job gets FA_ERR_H0 and a zeroed
-  record while the valid jobs of the same call are scored, and with the
-  h0 substitution of INTEGRATION.md (the left-extended score, or
-  seed_len * a without a left job) every job is scored."""
+  record while the valid jobs of the same call are scored, and when the
+  caller gives each right job the seed's left-extended score as h0 (or
+  seed_len * a without a left job), every job is scored."""
 import numpy as np
 import pytest
 

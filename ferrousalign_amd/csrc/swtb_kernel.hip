@@ -191,19 +191,23 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
         const int h1i = rb == 0 ? imax(h0 - (o_del + e_del * (i + 1)), 0) : 0;
         const uint32_t sel = 0x0C0C0C00u | (uint32_t)tbase;
 
+        // pass 1: M and the lane-local exclusive prefix of G(c) = max(M - oe_ins,
+        // 0) + c * e_ins, kept relative to the lane's cb * e_ins (G - cb * e_ins
+        // = max(M + s * e_ins - oe_ins, s * e_ins): wave-uniform constants per
+        // slot); band tests against the lane-relative bound rb - cb
+        const int cbe = cb * e_ins, trb = rb - cb;
         int M[S], pre[S];
         int run = kNeg;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int c = cb + s;
             const int sb = (int)__builtin_amdgcn_perm(a.nrow, prof[s], sel);
             M[s] = imax(Hh[s] + sb - 128, 0);  // :144-146
-            int gk = imax(M[s] - oe_ins, 0) + c * e_ins;
-            gk = c >= rb ? gk : kNeg;
+            const int gk = imax(M[s] + (s * e_ins - oe_ins), s * e_ins);
             pre[s] = run;
-            run = imax(run, gk);
+            run = imax(run, s >= trb ? gk : kNeg);
         }
-        const int carry = group_excl_scan_max<L>(run, l);
+        // the scan on absolute values; kNeg stays far below any G either way
+        const int carry = group_excl_scan_max<L>(run + cbe, l) - cbe;
 
         // pass 2: H, the code, E, the row maximum; then, with H of the column
         // before (the previous slot's, or for slot 0 the previous lane's last
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int c = cb + s;
-            const int F = imax(carry, pre[s]) - (c - 1) * e_ins;
+            const int F = imax(carry, pre[s]) + (e_ins - s * e_ins);  // - (c - 1) * e_ins, relative
             const int m = M[s], e = Ee[s];
             const int h = imax(imax(m, e), F);
             const uint32_t code = (m >= e && m >= F) ? TB_MATCH : (e >= F ? TB_DEL : TB_INS);

@@ -219,32 +219,36 @@ __global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
         uint32_t codes[NWL];
 #pragma unroll
         for (int k = 0; k < NWL; ++k) codes[k] = 0u;
+        // band tests of column c = cb + s as tests of s against rb - cb and
+        // re - cb; column-valued fields from per-lane constants
+        const int tre = re - cb, k1023 = 1023 - cb, k1 = cb + 1;
         auto write_back = [&](int s, int hp) {
-            const int c = cb + s;
-            const bool ge_rb = c >= rb, le_re = c <= re;
-            const int hv = c <= rb ? h1i : hp;
+            const bool ge_rb = s >= trb, le_re = s <= tre;
+            const int hv = s <= trb ? h1i : hp;
             Hh[s] = (ge_rb && le_re) ? hv : Hh[s];
-            hend = c == re ? Hh[s] : hend;
+            hend = s == tre ? Hh[s] : hend;
             const bool nz = (Hh[s] | Ee[s]) != 0;
-            const uint32_t f = (nz && ge_rb && c < re) ? (uint32_t)(1023 - c) : 0u;
-            const uint32_t la = (nz && ge_rb && le_re) ? (uint32_t)(c + 1) : 0u;
+            const uint32_t f = (nz && ge_rb && s < tre) ? (uint32_t)(k1023 - s) : 0u;  // 1023 - c
+            const uint32_t la = (nz && ge_rb && le_re) ? (uint32_t)(k1 + s) : 0u;      // c + 1
             fl = pk_max_u16(fl, f | (la << 16));
         };
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int c = cb + s;
             const int F = imax(carry, pre[s]) + (e_ins - s * e_ins);  // - (c - 1) * e_ins, relative
             const int m = M[s], e = Ee[s];
             const int h = imax(imax(m, e), F);
             const uint32_t code = (m >= e && m >= F) ? TB_MATCH : (e >= F ? TB_DEL : TB_INS);
             const int en = imax(imax(e - e_del, m - oe_del), 0);
-            const bool inb = c >= rb && c < re;
-            kmax = imax(kmax, inb ? ((h << 10) | (1023 - c)) : 0);
-            Ee[s] = inb ? en : (c == re ? 0 : e);
+            const bool inb = s >= trb && s < tre;
+            // row-argmax key (h << 10) | (1023 - c), here with 1023 - s (cb
+            // subtracted after the slots: every key of the lane shares it)
+            kmax = imax(kmax, inb ? ((h << 10) | (1023 - s)) : 0);
+            Ee[s] = inb ? en : (s == tre ? 0 : e);
             codes[s / 16] |= (inb ? code : 0u) << (2 * (s % 16));
             if (s > 0) write_back(s, Hprev);
             Hprev = h;
         }
+        kmax = kmax ? kmax - cb : 0;  // in-band columns lie in [0, qlen], so 1023 - c >= 0
         write_back(0, dpp<kRowShr + 1>(0, Hprev));
         if (alive) {
 #pragma unroll

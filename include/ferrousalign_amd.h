@@ -250,7 +250,11 @@ int fa_batch_upload(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
  * n_query_aligned[j]).  An empty query or reference gives the reference's
  * zero record and an empty CIGAR.  cells may be NULL.  Limits as for the
  * lane-group kernels (query <= FA_MAX_QLEN_PACKED, reference <=
- * FA_MAX_TLEN_PACKED, uniform N row). */
+ * FA_MAX_TLEN_PACKED, uniform N row).  The call's arrays stay with the
+ * context for the next call (grow-only, freed with it): about 140 bytes of
+ * pinned host memory per job, the packed sequences, and on the device the
+ * traceback codes (about 2.5 KB per 150-base job) and the outputs.  Calls on
+ * one context do not overlap. */
 int fa_scalar_banded_swa_batch(fa_ctx *ctx, const fa_banded_pairwise_sw *sw, int64_t njobs,
                                const uint8_t *query_seqs, const int64_t *query_offset,
                                const int32_t *query_len, const uint8_t *ref_seqs,

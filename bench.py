@@ -222,6 +222,16 @@ def mate_bench(args, ctx, golden, byte_mode=False):
     ms_fused, _, _ = timed(db, flags | N.FA_ALIGN2_FUSED, steps, 1)
     out_f, cells_f = db.download_align2()
     same_fused = bool((out_f == out).all() and (cells_f == cells).all())
+    # the same pairs through the host-buffer call (fa_ksw_align2_batch: checks,
+    # packing, upload, run, download), the first call untimed
+    from ferrousalign_amd.banded_swa import ksw_align2_batch
+    hb = []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        hout, hcells = ksw_align2_batch(*ps.flat(), synth.MATE_MATRIX, 6, 1, 6, 1, xtra, ctx=ctx)
+        if rep:
+            hb.append((time.perf_counter() - t0) * 1e3)
+    host_same = bool(np.array_equal(hout, out) and np.array_equal(hcells, cells))
     from tests.golden.make_golden_mate import digest
     gm = {}
     gpath = os.path.join(ROOT, "tests", "golden", "mate_digests.json")
@@ -240,6 +250,9 @@ def mate_bench(args, ctx, golden, byte_mode=False):
                       "qlen": 151, "tlen": 1500, "xtra": hex(xtra), "launches_per_step": nl},
            "gcups": round(total / (ms / 1e3) / 1e9, 2), "bit_exact_vs_golden": exact,
            "fused_reverse_ms_per_step": round(ms_fused, 3), "identical_to_fused": same_fused,
+           "host_buffer_ms": round(float(np.mean(hb)), 2),
+           "host_buffer_pairs_per_s": round(n / (float(np.mean(hb)) / 1e3), 1),
+           "host_buffer_identical": host_same,
            "roofline": roof}
     if not args.no_cpu:
         threads = os.cpu_count() or 1

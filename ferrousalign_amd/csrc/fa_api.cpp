@@ -2404,8 +2404,12 @@ int fa_ksw_align2_batch(fa_ctx *ctx, int64_t n, const uint8_t *query_seqs,
     sw.e_del = sw.e_ins = 1;
     std::vector<int32_t> ones((size_t)n, 1), zeros((size_t)n, 0);
     fa_batch *b = nullptr;
-    int rc = fa_batch_upload(ctx, &sw, n, query_seqs, query_offset, query_len, target_seqs,
-                             target_offset, target_len, ones.data(), zeros.data(), &b);
+    // a batch for this call only: its arrays come from the context's arena
+    TargetSource ts;
+    ts.seqs = target_seqs;
+    ts.offset = target_offset;
+    int rc = batch_upload(ctx, &sw, n, query_seqs, query_offset, query_len, ts, target_len,
+                          ones.data(), zeros.data(), &b, true);
     if (rc) return rc;
     rc = fa_batch_set_align2(ctx, b, mat, m, o_del, e_del, o_ins, e_ins, xtra);
     if (!rc) rc = fa_batch_run(ctx, b, FA_RUN_ALIGN2);

@@ -108,8 +108,11 @@ __device__ __forceinline__ int seq_code(const uint32_t *seq2, const uint32_t *se
 
 }  // namespace
 
+// The 16-slot band windows at 3 waves per SIMD (168 VGPRs, a 12-byte spill)
+// rather than 2 (204 VGPRs): measured 0.2 ms faster on 1M region-like pairs.
 template <int L, int S, bool BAND>
-__global__ __launch_bounds__(64) void swtb_dp_kernel(SwtbArgs a, uint32_t first, uint32_t count) {
+__global__ __launch_bounds__(64, (BAND && S == 16) ? 3 : 1) void swtb_dp_kernel(SwtbArgs a, uint32_t first,
+                                                                                uint32_t count) {
     extern __shared__ uint8_t lds[];
     constexpr int G = 64 / L;
     constexpr int NWL = (S + 15) / 16;
